@@ -1,0 +1,67 @@
+/*
+ * sdpa_amd.h — host driver of the MI355X SDPA-GMP path.
+ *
+ * sdpa_amd_solve restates the reference driver pinpal() (sdpa_main.cpp:43-461):
+ * it reads an SDPA sparse-format problem and a param.sdpa file, runs the
+ * Mehrotra predictor-corrector interior-point loop with the Newton system on
+ * the GPU (sdpg.h) and the step length / update / stopping tests on the host,
+ * and writes the reference's iteration log and final summary.
+ */
+#ifndef SDPA_AMD_H
+#define SDPA_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int iterations;
+    int status;           /* 0 loop ended normally, 1 Cholesky failure, 2 cannot move */
+    char phase[16];       /* as printed (phase.value after REVERSE_PRIMAL_DUAL) */
+    char objP[96];        /* printed objValPrimal, 41 significant digits */
+    char objD[96];        /* printed objValDual */
+    char relgap[48];      /* relative gap, 17 digits */
+    char pfeas[48], dfeas[48];
+    double mu;
+    double t_schur_ms;    /* Make bMat, device time, all iterations */
+    double t_chol_ms;     /* Cholesky bMat, device time */
+    double t_solve_ms, t_gvec_ms, t_dxdz_ms, t_res_ms;
+    double t_total_s;     /* wall clock of the main loop */
+    int limbs;
+} sdpa_amd_result;
+
+/*
+ * param_path may be NULL (defaults of Parameter::setDefaultParameter,
+ * sdpa_parts.cpp:165-178).  prec_override > 0 replaces the precision line.
+ * log_path: NULL, "-" (stdout) or a file.  gemm_mode: SDPG_GEMM_PLAIN or
+ * SDPG_GEMM_OZAKI.  keep_iterates: if non-NULL, the directory where the point
+ * (X, invZ) of every iteration is saved for replay benchmarks.
+ * Returns 0 on success, < 0 on input or device errors.
+ */
+int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
+                   int gemm_mode, int slices, int device, sdpa_amd_result *res);
+
+/*
+ * Replay benchmark of the hot path on one iterate of the solve: runs the solver
+ * for `warm_iters` iterations, then times `steps` executions of
+ * [Make_bMat + Cholesky] (sdpg_schur + sdpg_potrf) on that iterate after
+ * `warmup` untimed ones.  Returns per-step device milliseconds of each phase.
+ */
+int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_override, int gemm_mode, int slices,
+                   int device, int warm_iters, int warmup, int steps, double *ms_schur, double *ms_chol,
+                   double *ms_wall);
+
+/*
+ * Read a problem, compute its formula table (Newton::computeFormula_SDP,
+ * sdpa_newton.cpp:694-819) and return a committed engine context
+ * (sdpa_main.cpp:197-225 set-up).  *m and *nd receive the number of
+ * constraints and the size of a dense linear space; formula (may be NULL,
+ * else m*n_sdp ints) receives the table.  The caller destroys the context.
+ */
+struct sdpg_ctx;
+int sdpa_amd_ctx_from_file(const char *data_path, int prec_bits, int device, int gemm_mode, struct sdpg_ctx **out,
+                           int *m, long long *nd, int *formula);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

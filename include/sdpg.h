@@ -1,0 +1,145 @@
+/*
+ * sdpg.h — C-ABI of the MI355X Newton engine for SDPA-GMP.
+ *
+ * Drop-in boundary: the reference has no plugin/FFI; its boundary is the C++
+ * class Newton (sdpa_newton.h:49-168), constructed once (sdpa_main.cpp:214),
+ * given a dense Schur matrix (sdpa_main.cpp:221) and the formula table
+ * (sdpa_main.cpp:225), then called as Newton::Mehrotra twice per IPM
+ * iteration (sdpa_main.cpp:309, 342).  These entry points are what a host
+ * Newton::Mehrotra calls, in the reference order:
+ *   Make_gVec -> [PREDICTOR] Make_bMat -> compute_DyVec ([PREDICTOR] Cholesky
+ *   + solve) -> compute_DzMat -> compute_DxMat   (sdpa_newton.cpp:1375-1414).
+ *
+ * Conventions: every function returns int; 0 = success, > 0 = a LAPACK-style
+ * info (only sdpg_potrf: B not positive definite, which the reference maps to
+ * FAILURE, sdpa_linear.cpp:622-630), < 0 = an API error (SDPG_E*).  All
+ * buffers are caller-owned host memory.  A context owns its device buffers
+ * and one HIP stream; calls on one context must not overlap.
+ *
+ * Multiple-precision exchange format ("packed mp"): L+2 32-bit words per
+ * element, L = sdpg_limbs(prec_bits):
+ *   word 0  int32 binary exponent e,  word 1  sign (0 = +, 1 = -),
+ *   words 2..L+1  mantissa limbs, least significant first,
+ *   value = (-1)^sign * M * 2^(e - 32L), bit 32L-1 of M set; zero: M = 0,
+ *   e = SDPG_EZERO, sign = 0.
+ * A "dense linear space" (X, Z, dX, ...) is the concatenation of the SDP
+ * blocks, each n_l x n_l column-major (sdpa_struct.h:111-133 DenseMatrix
+ * de_ele[i + n*j]), followed by the LP diagonal entries.
+ */
+#ifndef SDPG_H
+#define SDPG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDPG_OK 0
+#define SDPG_EARG (-1)
+#define SDPG_EHIP (-2)
+#define SDPG_ESTATE (-3)
+#define SDPG_ENOMEM (-4)
+#define SDPG_EPREC (-5)
+#define SDPG_ENODEV (-6)
+
+#define SDPG_EZERO (-(1 << 29))
+
+/* sdpg_get / sdpg_put selectors */
+#define SDPG_X 1
+#define SDPG_Z 2
+#define SDPG_INVZ 3
+#define SDPG_Y 4
+#define SDPG_RP 5      /* primal residual vector b - A.X            (m)   */
+#define SDPG_RD 6      /* dual residual matrix C - Z - sum y_i A_i  (dls) */
+#define SDPG_GVEC 7    /* right-hand side g                         (m)   */
+#define SDPG_B 8       /* Schur matrix before factorisation         (m*m) */
+#define SDPG_L 9       /* Cholesky factor, lower triangle valid     (m*m) */
+#define SDPG_DY 10     /* (m)   */
+#define SDPG_DZ 11     /* (dls) */
+#define SDPG_DX 12     /* (dls) */
+#define SDPG_RZINV 13  /* r_zinv of the last sdpg_gvec call (dls) */
+
+/* GEMM forms (sdpg_set_gemm) */
+#define SDPG_GEMM_PLAIN 0  /* multi-limb integer MAD on the vector ALUs */
+#define SDPG_GEMM_OZAKI 1  /* INT8 slices on MFMA, exact INT32 accumulation */
+
+/* phase codes (SolveInfo::phaseType, sdpa_parts.h) */
+#define SDPG_NOINFO 0
+#define SDPG_PFEAS 1
+#define SDPG_DFEAS 2
+#define SDPG_PDFEAS 3
+
+/* limbs L for a requested precision in bits (GMP mpf rounding of precision) */
+int sdpg_limbs(int prec_bits);
+
+typedef struct sdpg_ctx sdpg_ctx;
+
+/*
+ * Newton::Newton + Newton::initialize (sdpa_newton.cpp:61-120) and
+ * initialize_dense_bMat (sdpa_newton.cpp:218-222): m constraints, n_sdp SDP
+ * blocks of the given sizes, n_lp LP variables (sdpa_main.cpp:114-131 block
+ * typing).  Device = HIP device ordinal.
+ */
+int sdpg_ctx_create(sdpg_ctx **ctx, int prec_bits, int m, int n_sdp, const int *sdp_sizes, int n_lp, int device);
+void sdpg_ctx_destroy(sdpg_ctx *ctx);
+
+/*
+ * Input data (InputData::A, C; sdpa_io.cpp:626-751 setElement).  con = -1 for
+ * C (already negated as in sdpa_io.cpp:661), 0..m-1 for A_con.  Entries are
+ * 0-based (row, col) as read; dense != 0 means the block was switched to
+ * dense storage (SparseMatrix::changeToDense, sdpa_struct.cpp:484-525) and val
+ * holds n*n column-major values (row/col ignored).  Each (con, block) may be
+ * set once.
+ */
+int sdpg_set_block(sdpg_ctx *ctx, int con, int sdp_block, int nnz, const int *row, const int *col, const uint32_t *val,
+                   int dense);
+int sdpg_set_lp(sdpg_ctx *ctx, int con, int nnz, const int *pos, const uint32_t *val);
+int sdpg_set_b(sdpg_ctx *ctx, const uint32_t *b);
+/*
+ * Formula table of Newton::computeFormula_SDP (sdpa_newton.cpp:694-819):
+ * use_formula[i*n_sdp + l] in {1 = F1, 2 = F2, 3 = F3}, 0 where A_i has no
+ * block l.
+ */
+int sdpg_set_formula(sdpg_ctx *ctx, const int *use_formula);
+/* freeze the input data: build the pair lists (sdpa_newton.cpp:1052-1054 rule) and upload */
+int sdpg_commit(sdpg_ctx *ctx);
+
+int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices);
+
+/* current point (Solutions xMat, zMat, invzMat, yVec) */
+int sdpg_set_point(sdpg_ctx *ctx, const uint32_t *X, const uint32_t *Z, const uint32_t *invZ, const uint32_t *y);
+/*
+ * Residuals::compute (sdpa_dataset.cpp:625-646) and the objective inner
+ * products of SolveInfo (sdpa_parts.cpp:747-760): Rp = b - A.X,
+ * Rd = C - Z - sum y_i A_i, their max-abs norms, objP = C.X, objD = b.y.
+ * Any output pointer may be NULL.
+ */
+int sdpg_residuals(sdpg_ctx *ctx, uint32_t *normRp, uint32_t *normRd, uint32_t *objP, uint32_t *objD);
+/* Newton::Make_gVec incl. compute_rMat (sdpa_newton.cpp:821-895); corrector != 0 for CORRECTOR */
+int sdpg_gvec(sdpg_ctx *ctx, int corrector, const uint32_t *mu, const uint32_t *beta, int phase);
+/* Newton::Make_bMat, dense SDP + LP parts (sdpa_newton.cpp:1007-1096, 1190-1225, 1251-1272) */
+int sdpg_schur(sdpg_ctx *ctx);
+/* Lal::choleskyFactorWithAdjust -> Rpotrf("Lower") (sdpa_linear.cpp:608-631); *info > 0: not PD */
+int sdpg_potrf(sdpg_ctx *ctx, int *info);
+/* Lal::solveSystems: Rtrsv(L,N) then Rtrsv(L,T) on g (sdpa_linear.cpp:683-695) -> dy */
+int sdpg_potrs(sdpg_ctx *ctx);
+/* Newton::compute_DzMat (sdpa_newton.cpp:1346-1356) */
+int sdpg_dz(sdpg_ctx *ctx, int phase);
+/* Newton::compute_DxMat incl. getSymmetrize (sdpa_newton.cpp:1358-1373) */
+int sdpg_dx(sdpg_ctx *ctx);
+
+/* copy a device quantity out / in (packed mp); sizes per selector comment */
+int sdpg_get(sdpg_ctx *ctx, int what, uint32_t *out);
+int sdpg_put(sdpg_ctx *ctx, int what, const uint32_t *in);
+
+/* accumulated device milliseconds: [0] schur, [1] potrf, [2] potrs, [3] gvec, [4] dz+dx, [5] residuals */
+int sdpg_timers(sdpg_ctx *ctx, double *ms, int n);
+int sdpg_reset_timers(sdpg_ctx *ctx);
+int sdpg_sync(sdpg_ctx *ctx);
+/* last error message of this thread */
+const char *sdpg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

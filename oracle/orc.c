@@ -2161,6 +2161,103 @@ int orc_solve(const char *data_path, const orc_param *param, int prec_override, 
     return rc;
 }
 
+/* read a dump file written by dump_arr into v (count must match) */
+static int load_arr(const char *dir, const char *name, F *v, int n) {
+    char path[1024];
+    snprintf(path, sizeof path, "%s/%s.bin", dir, name);
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return -1;
+    int32_t hdr[2];
+    if (fread(hdr, sizeof hdr, 1, fp) != 1 || hdr[0] != n) {
+        fclose(fp);
+        return -2;
+    }
+    int L = hdr[1];
+    int32_t *sg = (int32_t *)xcalloc(n, 4), *ex = (int32_t *)xcalloc(n, 4);
+    uint32_t *lb = (uint32_t *)xcalloc((size_t)n * L, 4);
+    int ok = fread(sg, 4, n, fp) == (size_t)n && fread(ex, 4, n, fp) == (size_t)n &&
+             fread(lb, 4, (size_t)n * L, fp) == (size_t)n * L;
+    fclose(fp);
+    if (ok)
+        for (int i = 0; i < n; i++) ompf_import32(&v[i], L, sg[i], ex[i], &lb[(size_t)i * L]);
+    free(sg);
+    free(ex);
+    free(lb);
+    return ok ? 0 : -3;
+}
+static int load_dls(const char *dir, const char *name, DLS *d) {
+    int tot = d->nlp;
+    for (int l = 0; l < d->nsdp; l++) tot += d->n[l] * d->n[l];
+    F *v = (F *)xcalloc(tot, sizeof(F));
+    int rc = load_arr(dir, name, v, tot);
+    if (!rc) {
+        int o = 0;
+        for (int l = 0; l < d->nsdp; l++)
+            for (int i = 0; i < d->n[l] * d->n[l]; i++) d->b[l][i] = v[o++];
+        for (int i = 0; i < d->nlp; i++) d->lp[i] = v[o++];
+    }
+    free(v);
+    return rc;
+}
+
+/*
+ * One predictor Newton step (Make_gVec, Make_bMat, Cholesky, solve, dZ, dX) of
+ * the reference on the point stored in in_dir (X, Z, invZ, y, mu, beta,
+ * phase.txt as written by a dump), at precision prec_bits; the residuals are
+ * recomputed from the point (Residuals::compute).  Outputs go to out_dir in
+ * the dump format with out_L interchange limbs.
+ */
+int orc_newton_step(const char *data_path, int prec_bits, const char *in_dir, const char *out_dir, int out_L) {
+    ompf_set_prec_bits(prec_bits);
+    Ctx c;
+    memset(&c, 0, sizeof c);
+    int rc = read_problem(data_path, &c.P);
+    if (rc) return rc;
+    Prob *P = &c.P;
+    int m = P->m;
+    newton_alloc(&c.N, P);
+    compute_formula(&c.N, P);
+    dls_init(&c.S.X, P);
+    dls_init(&c.S.Z, P);
+    dls_init(&c.S.invCholX, P);
+    dls_init(&c.S.invCholZ, P);
+    dls_init(&c.S.invZ, P);
+    c.S.y = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.W.DLS1, P);
+    dls_init(&c.W.DLS2, P);
+    c.init.primalVec = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.init.dualMat, P);
+    c.cur.primalVec = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.cur.dualMat, P);
+    F mu, beta;
+    int phase = 0;
+    if ((rc = load_dls(in_dir, "X", &c.S.X)) || (rc = load_dls(in_dir, "Z", &c.S.Z)) ||
+        (rc = load_dls(in_dir, "invZ", &c.S.invZ)) || (rc = load_arr(in_dir, "y", c.S.y, m)) ||
+        (rc = load_arr(in_dir, "mu", &mu, 1)) || (rc = load_arr(in_dir, "beta", &beta, 1))) {
+        ctx_free(&c);
+        return -100 + rc;
+    }
+    {
+        char path[1024];
+        snprintf(path, sizeof path, "%s/phase.txt", in_dir);
+        FILE *fp = fopen(path, "r");
+        if (!fp || fscanf(fp, "%d", &phase) != 1) phase = 0;
+        if (fp) fclose(fp);
+    }
+    residuals(&c.cur, P, &c.S);
+    char cmd[1200];
+    snprintf(cmd, sizeof cmd, "mkdir -p '%s'", out_dir);
+    if (system(cmd) != 0) {
+        ctx_free(&c);
+        return -200;
+    }
+    dump_arr(out_dir, "Rp", c.cur.primalVec, m, out_L);
+    dump_dls(out_dir, "Rd", &c.cur.dualMat, out_L);
+    int ok = mehrotra(&c, 1, &mu, &beta, phase, NULL, NULL, out_dir, out_L);
+    ctx_free(&c);
+    return ok ? 0 : 1;
+}
+
 int orc_time_hotpath(const char *data_path, const orc_param *param, int prec_override, int warm_iters, int reps,
                      int threads, double *t_bmat, double *t_chol) {
     orc_param prm;

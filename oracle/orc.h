@@ -57,6 +57,13 @@ int orc_solve(const char *data_path, const orc_param *param, int prec_override, 
  * iterations; runs `reps` repetitions and returns seconds per repetition of
  * each phase.  Used as the CPU baseline.
  */
+/*
+ * One predictor Newton step on a dumped point (X, Z, invZ, y, mu, beta,
+ * phase) at prec_bits; writes Rp, Rd, g, B, L, dy, dX, dZ (and the inputs) to
+ * out_dir.  Returns 0, 1 on Cholesky failure, < 0 on I/O errors.
+ */
+int orc_newton_step(const char *data_path, int prec_bits, const char *in_dir, const char *out_dir, int out_L);
+
 int orc_time_hotpath(const char *data_path, const orc_param *param, int prec_override, int warm_iters, int reps,
                      int threads, double *t_bmat, double *t_chol);
 

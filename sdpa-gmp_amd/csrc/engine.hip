@@ -1,0 +1,1471 @@
+// engine.hip — MI355X Newton engine for SDPA-GMP: kernels, Engine<L>, C-ABI.
+//
+// Hot path of one IPM iteration (reference Newton::Mehrotra,
+// sdpa_newton.cpp:1375-1414) on the GPU in limb-planar fixed-limb
+// multiple precision: right-hand side (compute_rMat + Make_gVec), Schur
+// complement (Make_bMat: F1/F2/F3 + LP), Cholesky (Rpotrf), triangular solves
+// (Rtrsv x2), dZ (multi_plusToA) and dX (triple product + getSymmetrize), plus
+// the residuals of Residuals::compute.  Dense products go through the GEMM
+// engine of gemm.hip (plain limb MAD or Ozaki INT8 MFMA).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine_common.h"
+#include "gemm.h"
+#include "limbs.h"
+#include "mpx.h"
+#include "sdpg.h"
+
+using namespace mpx;
+
+namespace sdpg {
+
+static thread_local std::string g_err;
+static void set_err(const std::string &s) { g_err = s; }
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                \
+            set_err(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);            \
+            return SDPG_EHIP;                                                                  \
+        }                                                                                      \
+    } while (0)
+
+static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ kernels
+
+template <int L>
+__global__ void k_unpack(const uint32_t *__restrict__ src, PV dst, int64_t off, int64_t cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t *s = src + i * (L + 2);
+    mp<L> x;
+    x.e = (int32_t)s[0];
+    x.s = s[1] & 1u;
+#pragma unroll
+    for (int k = 0; k < L; k++) x.m[k] = s[2 + k];
+    if (x.m[L - 1] == 0) x = zero<L>();
+    stm<L>(dst, off + i, x);
+}
+
+template <int L>
+__global__ void k_pack(PV src, int64_t off, int64_t cnt, uint32_t *__restrict__ dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    mp<L> x = ldm<L>(src, off + i);
+    uint32_t *d = dst + i * (L + 2);
+    d[0] = (uint32_t)x.e;
+    d[1] = x.s;
+#pragma unroll
+    for (int k = 0; k < L; k++) d[2 + k] = x.m[k];
+}
+
+template <int L>
+__global__ void k_zero(PV dst, int64_t off, int64_t cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    stm<L>(dst, off + i, zero<L>());
+}
+
+// out = a*X + b*Y  (a, b scalars; Y may be null => b ignored)
+template <int L>
+__global__ void k_lincomb(PV out, PV X, PV Y, mp<L> a, mp<L> b, int useY, int64_t cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    mp<L> r = mul<L>(a, ldm<L>(X, i));
+    if (useY) r = add<L>(r, mul<L>(b, ldm<L>(Y, i)));
+    stm<L>(out, i, r);
+}
+
+// out = X + s*Y    (Raxpy form: y += s*x as in mplapack/Raxpy_omp.cpp)
+template <int L>
+__global__ void k_axpy(PV out, PV X, PV Y, mp<L> s, int64_t cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    stm<L>(out, i, add<L>(ldm<L>(X, i), mul<L>(s, ldm<L>(Y, i))));
+}
+
+// Lal::getSymmetrize on one n x n block at offset off (sdpa_linear.cpp:457-482):
+// lower = (lower + upper) / 2, upper = lower
+template <int L>
+__global__ void k_symmetrize(PV A, int64_t off, int n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * n) return;
+    int i = (int)(t % n), j = (int)(t / n);
+    if (i <= j) return;  // thread owns the strictly-lower (i, j)
+    const mp<L> half = ldexp2<L>(from_int<L>(1), -1);
+    mp<L> lo = ldm<L>(A, off + i + (int64_t)n * j);
+    mp<L> up = ldm<L>(A, off + j + (int64_t)n * i);
+    mp<L> v = mul<L>(half, add<L>(lo, up));
+    stm<L>(A, off + i + (int64_t)n * j, v);
+    stm<L>(A, off + j + (int64_t)n * i, v);
+}
+
+// Constraint descriptor of block (i, l)
+struct Desc {
+    int32_t kind;  // 0 absent, 1 sparse, 2 dense
+    int32_t n;
+    int64_t off;  // sparse: first entry index; dense: first value index
+    int32_t nnz;  // sparse stored entries
+    int32_t effect;
+    int64_t blkoff;  // offset of block l inside a dense linear space
+};
+
+// ip = A (block) . D (sparse: _OPENMP branch of sdpa_linear.cpp:235-254; dense: Rdot)
+template <int L>
+__device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, PV sv, PV dv, PV D, int64_t doff) {
+    mp<L> ret = zero<L>();
+    if (d.kind == 1) {
+        for (int t = 0; t < d.nnz; t++) {
+            int64_t e = d.off + t;
+            int i = ri[e], j = ci[e];
+            mp<L> v = ldm<L>(sv, e);
+            mp<L> temp;
+            if (i == j) {
+                temp = mul<L>(v, ldm<L>(D, doff + i + (int64_t)d.n * j));
+            } else {
+                temp = add<L>(ldm<L>(D, doff + i + (int64_t)d.n * j), ldm<L>(D, doff + j + (int64_t)d.n * i));
+                temp = mul<L>(temp, v);
+            }
+            ret = add<L>(ret, temp);
+        }
+    } else if (d.kind == 2) {
+        int64_t nn = (int64_t)d.n * d.n;
+        for (int64_t k = 0; k < nn; k++) ret = fma_acc<L>(ret, ldm<L>(dv, d.off + k), ldm<L>(D, doff + k));
+    }
+    return ret;
+}
+
+// For each constraint c (and optionally C at index m): ip_c = A_c . D (SDP blocks + LP),
+// then  mode 0: out_c = -ip + R_c   (Make_gVec: gVec = -(A.DLS1) + primalVec)
+//       mode 1: out_c =  b_c - ip   (Residuals: primalVec = b - A.X)
+//       mode 2: out_c =  ip          (objective C.X, single constraint)
+template <int L>
+__global__ void k_con_inner(int ncon, int nsdp, const Desc *desc, const int32_t *ri, const int32_t *ci, PV sv,
+                            PV dv, const int64_t *lp_ptr, const int32_t *lp_pos, PV lp_val, int64_t lpbase, PV D,
+                            PV R, PV out, int mode) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncon) return;
+    mp<L> ret = zero<L>();
+    for (int l = 0; l < nsdp; l++) {
+        Desc d = desc[(int64_t)c * nsdp + l];
+        if (d.kind == 0) continue;
+        ret = add<L>(ret, blk_inner<L>(d, ri, ci, sv, dv, D, d.blkoff));
+    }
+    for (int64_t t = lp_ptr[c]; t < lp_ptr[c + 1]; t++) ret = fma_acc<L>(ret, ldm<L>(lp_val, t), ldm<L>(D, lpbase + lp_pos[t]));
+    mp<L> o;
+    if (mode == 0) {
+        o = add<L>(neg<L>(ret), ldm<L>(R, c));
+    } else if (mode == 1) {
+        o = sub<L>(ldm<L>(R, c), ret);
+    } else {
+        o = ret;
+    }
+    stm<L>(out, c, o);
+}
+
+// Scatter sum_i s_i A_i into the dense linear space (multi_plusToA, sdpa_dataset.cpp:488-493).
+// pos_ptr/pos_con/pos_ent: per position, the sparse entries touching it (ordered by constraint);
+// dense constraints of the position's block: dcon_ptr/dcon (per block), value offset dcon_off.
+// mode 0 (dZ):  out = -acc + (useR ? R : 0)        (compute_DzMat)
+// mode 1 (Rd):  out = -acc + Cfull - Z             (Residuals::compute)
+template <int L>
+__global__ void k_scatter(int64_t npos, const int64_t *pos_ptr, const int32_t *pos_con, const int64_t *pos_ent,
+                          PV sv, PV svals_dense, const int32_t *pos_blk, const int64_t *pos_in_blk,
+                          const int64_t *dcon_ptr, const int32_t *dcon, const int64_t *dcon_off, PV s, PV R, int useR,
+                          PV Cfull, PV Z, PV out, int mode) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npos) return;
+    mp<L> acc = zero<L>();
+    for (int64_t q = pos_ptr[t]; q < pos_ptr[t + 1]; q++) {
+        mp<L> v = mul<L>(ldm<L>(sv, pos_ent[q]), ldm<L>(s, pos_con[q]));
+        acc = add<L>(acc, v);
+    }
+    int b = pos_blk[t];
+    if (b >= 0) {
+        int64_t k = pos_in_blk[t];
+        for (int64_t q = dcon_ptr[b]; q < dcon_ptr[b + 1]; q++) {
+            acc = add<L>(acc, mul<L>(ldm<L>(s, dcon[q]), ldm<L>(svals_dense, dcon_off[q] + k)));
+        }
+    }
+    mp<L> o;
+    if (mode == 0) {
+        o = neg<L>(acc);
+        if (useR) o = add<L>(o, ldm<L>(R, t));
+    } else {
+        o = sub<L>(sub<L>(ldm<L>(Cfull, t), ldm<L>(Z, t)), acc);
+    }
+    stm<L>(out, t, o);
+}
+
+// max |x_i| over cnt elements: per-block partial maxima (one per workgroup)
+template <int L>
+__global__ void k_maxabs(PV X, int64_t cnt, PV part) {
+    __shared__ uint32_t sbuf[256 * (L + 2)];
+    mp<L> best = zero<L>();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+        mp<L> v = absv<L>(ldm<L>(X, i));
+        if (cmp_abs<L>(v, best) > 0) best = v;
+    }
+    uint32_t *me = sbuf + threadIdx.x * (L + 2);
+    me[0] = (uint32_t)best.e;
+    me[1] = best.s;
+#pragma unroll
+    for (int k = 0; k < L; k++) me[2 + k] = best.m[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mp<L> b = zero<L>();
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            uint32_t *o = sbuf + t * (L + 2);
+            mp<L> v;
+            v.e = (int32_t)o[0];
+            v.s = o[1];
+#pragma unroll
+            for (int k = 0; k < L; k++) v.m[k] = o[2 + k];
+            if (cmp_abs<L>(v, b) > 0) b = v;
+        }
+        stm<L>(part, blockIdx.x, b);
+    }
+}
+
+// sequential dot product (small vectors): out[0] = sum_i x_i y_i
+template <int L>
+__global__ void k_dot_seq(PV x, PV y, int64_t n, PV out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    mp<L> acc = zero<L>();
+    for (int64_t i = 0; i < n; i++) acc = fma_acc<L>(acc, ldm<L>(x, i), ldm<L>(y, i));
+    stm<L>(out, 0, acc);
+}
+
+// ---- Schur complement ----
+
+// F_c = A_c * invZ for sparse A_c by the symmetric row expansion (Lal::multiply(D,S,D),
+// sdpa_linear.cpp:877-955): row r of F = sum over expansion entries (r, q, v) of v * invZ[q, :]
+// grid: x over n*n outputs, y over the constraint list
+template <int L>
+__global__ void k_spmm_rows(int n, const int64_t *exp_rowptr, const int32_t *exp_col, const int64_t *exp_ent, PV sv,
+                            PV invZ, int64_t zoff, PV F, int64_t fstride, const int32_t *sp_list) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blockIdx.y;
+    if (t >= (int64_t)n * n) return;
+    int r = (int)(t % n), c = (int)(t / n);
+    int64_t base = (int64_t)k * (n + 1);
+    mp<L> acc = zero<L>();
+    for (int64_t q = exp_rowptr[base + r]; q < exp_rowptr[base + r + 1]; q++) {
+        acc = fma_acc<L>(acc, ldm<L>(sv, exp_ent[q]), ldm<L>(invZ, zoff + exp_col[q] + (int64_t)n * c));
+    }
+    stm<L>(F, (int64_t)sp_list[k] * fstride + r + (int64_t)n * c, acc);
+}
+
+struct Pair {
+    int32_t i, j;    // owner i (formula of i), partner j
+    int32_t slot;    // workspace slot of F_i / G_i
+    int32_t form;    // 1, 2, 3
+};
+
+// B[i,j] += value, B[j,i] += value (i != j) — one thread per pair of one block
+template <int L>
+__device__ __forceinline__ void bacc(PV B, int m, int i, int j, const mp<L> &v) {
+    int64_t a = i + (int64_t)m * j;
+    stm<L>(B, a, add<L>(ldm<L>(B, a), v));
+    if (i != j) {
+        int64_t b = j + (int64_t)m * i;
+        stm<L>(B, b, add<L>(ldm<L>(B, b), v));
+    }
+}
+
+// F1: value = A_j . G_i (Newton::calF1, sdpa_newton.cpp:897)
+// F2: sparse A_j: sum v * Rdot(n, X[alpha,:], F_i[:,beta]) (+mirror)  (Newton::calF2, sdpa_newton.cpp:899-937)
+//     dense A_j : A_j . G_i with G_i = X F_i
+// F3: Newton::calF3 (sdpa_newton.cpp:939-1005)
+template <int L>
+__global__ void k_pairs(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc, const int32_t *ri,
+                        const int32_t *ci, PV sv, PV dv, PV W, int64_t wstride, PV Fw, PV X, PV invZ, int64_t boff,
+                        PV B, int m) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    Pair p = pairs[t];
+    Desc dj = desc[(int64_t)p.j * nsdp + l];
+    int n = dj.n;
+    mp<L> value = zero<L>();
+    if (p.form == 1 || (p.form == 2 && dj.kind == 2)) {
+        value = blk_inner<L>(dj, ri, ci, sv, dv, W, (int64_t)p.slot * wstride);
+    } else if (p.form == 2) {
+        int64_t foff = (int64_t)p.slot * wstride;
+        for (int e = 0; e < dj.nnz; e++) {
+            int64_t ee = dj.off + e;
+            int alpha = ri[ee], beta = ci[ee];
+            mp<L> v1 = ldm<L>(sv, ee);
+            mp<L> v2 = zero<L>();
+            for (int k = 0; k < n; k++)
+                v2 = fma_acc<L>(v2, ldm<L>(X, boff + alpha + (int64_t)n * k), ldm<L>(Fw, foff + k + (int64_t)n * beta));
+            value = add<L>(value, mul<L>(v1, v2));
+            if (alpha != beta) {
+                v2 = zero<L>();
+                for (int k = 0; k < n; k++)
+                    v2 = fma_acc<L>(v2, ldm<L>(X, boff + beta + (int64_t)n * k),
+                                    ldm<L>(Fw, foff + k + (int64_t)n * alpha));
+                value = add<L>(value, mul<L>(v1, v2));
+            }
+        }
+    } else {
+        Desc di = desc[(int64_t)p.i * nsdp + l];
+        mp<L> local = zero<L>();
+        for (int i1 = 0; i1 < dj.nnz; i1++) {
+            int64_t e1 = dj.off + i1;
+            int alpha = ri[e1], beta = ci[e1];
+            mp<L> v1 = ldm<L>(sv, e1);
+            mp<L> sum = zero<L>();
+            for (int i2 = 0; i2 < di.nnz; i2++) {
+                int64_t e2 = di.off + i2;
+                int gamma = ri[e2], delta = ci[e2];
+                mp<L> v2 = ldm<L>(sv, e2);
+                mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * beta)),
+                                   ldm<L>(X, boff + alpha + (int64_t)n * gamma));
+                sum = add<L>(sum, plu);
+                if (gamma != delta) {
+                    plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * beta)),
+                                 ldm<L>(X, boff + alpha + (int64_t)n * delta));
+                    sum = add<L>(sum, plu);
+                }
+            }
+            local = add<L>(local, mul<L>(v1, sum));
+            if (alpha == beta) continue;
+            sum = zero<L>();
+            for (int i2 = 0; i2 < di.nnz; i2++) {
+                int64_t e2 = di.off + i2;
+                int gamma = ri[e2], delta = ci[e2];
+                mp<L> v2 = ldm<L>(sv, e2);
+                mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * alpha)),
+                                   ldm<L>(X, boff + beta + (int64_t)n * gamma));
+                sum = add<L>(sum, plu);
+                if (gamma != delta) {
+                    plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * alpha)),
+                                 ldm<L>(X, boff + beta + (int64_t)n * delta));
+                    sum = add<L>(sum, plu);
+                }
+            }
+            local = add<L>(local, mul<L>(v1, sum));
+        }
+        value = local;
+    }
+    bacc<L>(B, m, p.i, p.j, value);
+}
+
+// LP part (compute_bMat_dense_LP, sdpa_newton.cpp:1190-1225): pairs within one LP variable
+struct LPPair {
+    int32_t i, j;
+    int32_t lp;        // LP index
+    int32_t pad;
+    int64_t ei, ej;    // value indices
+};
+template <int L>
+__global__ void k_pairs_lp(int npairs, const LPPair *pairs, PV lpv, PV X, PV invZ, int64_t lpbase, PV B, int m) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    LPPair p = pairs[t];
+    mp<L> v = mul<L>(ldm<L>(X, lpbase + p.lp), ldm<L>(invZ, lpbase + p.lp));
+    v = mul<L>(v, ldm<L>(lpv, p.ei));
+    v = mul<L>(v, ldm<L>(lpv, p.ej));
+    bacc<L>(B, m, p.i, p.j, v);
+}
+
+// mirror: B[j,i] = B[i,j] is maintained by bacc; nothing else needed.
+
+// ---- Cholesky (Rpotrf lower) ----
+// One workgroup factors the panel of columns [k, k+nb) over rows [k, m), right-looking inside
+// the panel; diagonal sqrt and reciprocal are full-width limb operations.  info (1-based) on failure.
+template <int L>
+__global__ void k_potrf_panel(PV A, int m, int k, int nb, PV rdiag, int *info) {
+    __shared__ uint32_t srec[L + 2];
+    __shared__ int sfail;
+    for (int j = k; j < k + nb; j++) {
+        if (threadIdx.x == 0) {
+            mp<L> d = ldm<L>(A, j + (int64_t)m * j);
+            if (sgn<L>(d) <= 0) {
+                sfail = j + 1;
+            } else {
+                sfail = 0;
+                mp<L> s = sqrtv<L>(d);
+                stm<L>(A, j + (int64_t)m * j, s);
+                mp<L> r = recip<L>(s);
+                stm<L>(rdiag, j, r);
+                srec[0] = (uint32_t)r.e;
+                srec[1] = r.s;
+#pragma unroll
+                for (int q = 0; q < L; q++) srec[2 + q] = r.m[q];
+            }
+        }
+        __syncthreads();
+        if (sfail) {
+            if (threadIdx.x == 0) *info = sfail;
+            return;
+        }
+        mp<L> r;
+        r.e = (int32_t)srec[0];
+        r.s = srec[1];
+#pragma unroll
+        for (int q = 0; q < L; q++) r.m[q] = srec[2 + q];
+        for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
+            int64_t a = i + (int64_t)m * j;
+            stm<L>(A, a, mul<L>(r, ldm<L>(A, a)));
+        }
+        __syncthreads();
+        // update the remaining panel columns c in (j, k+nb), rows i >= c
+        int ncols = k + nb - (j + 1);
+        int64_t work = (int64_t)ncols * (m - (j + 1));
+        for (int64_t t = threadIdx.x; t < work; t += blockDim.x) {
+            int c = j + 1 + (int)(t % ncols);
+            int i = j + 1 + (int)(t / ncols);
+            if (i < c) continue;
+            int64_t a = i + (int64_t)m * c;
+            mp<L> v = sub<L>(ldm<L>(A, a), mul<L>(ldm<L>(A, i + (int64_t)m * j), ldm<L>(A, c + (int64_t)m * j)));
+            stm<L>(A, a, v);
+        }
+        __syncthreads();
+    }
+}
+
+// Lal::solveSystems: L y = g (forward) then L^T x = y (backward), single workgroup.
+// The division by L_jj is a multiplication by the full-width reciprocal kept by the factorisation.
+template <int L>
+__global__ void k_trsv_both(PV A, int m, PV rdiag, PV x) {
+    __shared__ uint32_t sx[L + 2];
+    for (int j = 0; j < m; j++) {
+        if (threadIdx.x == 0) {
+            mp<L> v = mul<L>(ldm<L>(x, j), ldm<L>(rdiag, j));
+            stm<L>(x, j, v);
+            sx[0] = (uint32_t)v.e;
+            sx[1] = v.s;
+#pragma unroll
+            for (int q = 0; q < L; q++) sx[2 + q] = v.m[q];
+        }
+        __syncthreads();
+        mp<L> xj;
+        xj.e = (int32_t)sx[0];
+        xj.s = sx[1];
+#pragma unroll
+        for (int q = 0; q < L; q++) xj.m[q] = sx[2 + q];
+        for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x)
+            stm<L>(x, i, sub<L>(ldm<L>(x, i), mul<L>(xj, ldm<L>(A, i + (int64_t)m * j))));
+        __syncthreads();
+    }
+    for (int j = m - 1; j >= 0; j--) {
+        if (threadIdx.x == 0) {
+            mp<L> v = mul<L>(ldm<L>(x, j), ldm<L>(rdiag, j));
+            stm<L>(x, j, v);
+            sx[0] = (uint32_t)v.e;
+            sx[1] = v.s;
+#pragma unroll
+            for (int q = 0; q < L; q++) sx[2 + q] = v.m[q];
+        }
+        __syncthreads();
+        mp<L> xj;
+        xj.e = (int32_t)sx[0];
+        xj.s = sx[1];
+#pragma unroll
+        for (int q = 0; q < L; q++) xj.m[q] = sx[2 + q];
+        for (int i = threadIdx.x; i < j; i += blockDim.x)
+            stm<L>(x, i, sub<L>(ldm<L>(x, i), mul<L>(xj, ldm<L>(A, j + (int64_t)m * i))));
+        __syncthreads();
+    }
+}
+
+// LP part of dX / r_zinv / x_rd_zinv : out = a*x*y*z style triple on diagonal entries
+template <int L>
+__global__ void k_lp_triple(PV out, PV A, PV Bm, PV C, int64_t base, int nlp) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlp) return;
+    mp<L> v = mul<L>(ldm<L>(A, base + t), ldm<L>(Bm, base + t));
+    v = mul<L>(v, ldm<L>(C, base + t));
+    stm<L>(out, base + t, v);
+}
+
+// ------------------------------------------------------------------ Engine
+
+struct EngineBase {
+    virtual ~EngineBase() {}
+    int nlimb = 0, prec = 0, m = 0, nsdp = 0, nlp = 0, device = 0;
+    std::vector<int> sizes;
+    std::vector<int64_t> blkoff;
+    int64_t Nd = 0;  // dense linear space elements
+    bool committed = false;
+    int gemm_mode = SDPG_GEMM_PLAIN, gemm_slices = 0;
+    double timers[8] = {0};
+    // staged input
+    struct Blk {
+        bool set = false, dense = false;
+        std::vector<int> r, c;
+        std::vector<uint32_t> v;  // packed mp
+    };
+    std::vector<Blk> blk;  // (m+1)*nsdp, index (con+1)*nsdp + l
+    std::vector<std::vector<int>> lp_pos;
+    std::vector<std::vector<uint32_t>> lp_val;  // per con (m+1)
+    std::vector<uint32_t> bvec;
+    std::vector<int> formula;
+
+    virtual int commit() = 0;
+    virtual int set_point(const uint32_t *X, const uint32_t *Z, const uint32_t *invZ, const uint32_t *y) = 0;
+    virtual int residuals(uint32_t *nrp, uint32_t *nrd, uint32_t *objP, uint32_t *objD) = 0;
+    virtual int gvec(int corrector, const uint32_t *mu, const uint32_t *beta, int phase) = 0;
+    virtual int schur() = 0;
+    virtual int potrf(int *info) = 0;
+    virtual int potrs() = 0;
+    virtual int dz(int phase) = 0;
+    virtual int dx() = 0;
+    virtual int get(int what, uint32_t *out) = 0;
+    virtual int put(int what, const uint32_t *in) = 0;
+    virtual int sync() = 0;
+};
+
+template <int L>
+struct Engine : EngineBase {
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device planar arrays
+    struct Arr {
+        uint32_t *p = nullptr;
+        int64_t n = 0;
+        PV pv() const { return PV{p, n}; }
+    };
+    Arr X, Z, invZ, Rd, rzinv, xrd, dX, dZ, W1, W2, DLS1, Cfull;
+    Arr y, Rp, g, dy, b, rdiag, scal;
+    Arr Bm;
+    Arr sval, dval, lpval;  // constraint data
+    Arr Fw, Gw;             // Schur workspaces
+    int64_t wstride = 0, wslots = 0;
+    // index arrays
+    Desc *d_desc = nullptr, *d_cdesc = nullptr;
+    int32_t *d_ri = nullptr, *d_ci = nullptr;
+    int64_t *d_lp_ptr = nullptr, *d_clp_ptr = nullptr;
+    int32_t *d_lp_pos = nullptr;
+    int64_t lp_c_base = 0;  // C's LP values start
+    // scatter index
+    int64_t *d_pos_ptr = nullptr, *d_pos_ent = nullptr, *d_pos_in_blk = nullptr, *d_dcon_ptr = nullptr,
+            *d_dcon_off = nullptr;
+    int32_t *d_pos_con = nullptr, *d_pos_blk = nullptr, *d_dcon = nullptr;
+    // Schur plan per block
+    struct BlockPlan {
+        std::vector<int> f1_sparse, f1_dense, f2_sparse, f2_dense;  // constraints needing F (and G)
+        std::vector<int> slot_of;  // constraint -> slot (or -1)
+        int nslots = 0;
+        int32_t *d_slot_cons = nullptr;  // per slot: sparse expansion slot index (for spmm)
+        int n_sp_slots = 0;
+        int32_t *d_sp_list = nullptr;    // slots of sparse constraints, in order
+        std::vector<int> sp_slots, dn_slots, g_slots;  // slots by kind; g_slots: need G
+        std::vector<int> dn_cons;
+        Pair *d_pairs = nullptr;
+        int npairs = 0;
+        int64_t *d_exp_rowptr = nullptr, *d_exp_ent = nullptr;
+        int32_t *d_exp_col = nullptr, *d_exp_slot = nullptr;
+    };
+    std::vector<BlockPlan> plan;
+    LPPair *d_lppairs = nullptr;
+    int nlppairs = 0;
+    int *d_info = nullptr;
+    std::vector<void *> allocs;
+    GemmEngine<L> gemm;
+
+    ~Engine() override {
+        if (st) hipStreamSynchronize(st);
+        for (void *p : allocs) hipFree(p);
+        if (stage) hipFree(stage);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (st) hipStreamDestroy(st);
+    }
+
+    template <class T>
+    int dalloc(T **p, int64_t count) {
+        *p = nullptr;
+        if (count <= 0) count = 1;
+        hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+        if (e != hipSuccess) {
+            set_err(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            return SDPG_ENOMEM;
+        }
+        allocs.push_back(*p);
+        return 0;
+    }
+    int aalloc(Arr &a, int64_t n) {
+        a.n = n > 0 ? n : 1;
+        return dalloc(&a.p, (int64_t)(L + 1) * a.n);
+    }
+    template <class T>
+    int upload(T **p, const std::vector<T> &v) {
+        int rc = dalloc(p, (int64_t)v.size());
+        if (rc) return rc;
+        if (!v.empty()) {
+            HIPCHK(hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));  // the host vector may be released by the caller
+        }
+        return 0;
+    }
+    // staging buffer for packed transfers (grown on demand, reused afterwards)
+    uint32_t *stage = nullptr;
+    size_t stage_cap = 0;
+    int ensure_stage(size_t words) {
+        if (words <= stage_cap) return 0;
+        HIPCHK(hipStreamSynchronize(st));
+        if (stage) HIPCHK(hipFree(stage));
+        stage = nullptr;
+        size_t n = words + words / 4 + 1024;
+        HIPCHK(hipMalloc((void **)&stage, sizeof(uint32_t) * n));
+        stage_cap = n;
+        return 0;
+    }
+    // packed host -> planar device (synchronous with respect to the host buffer)
+    int put_packed(Arr &a, int64_t off, const uint32_t *src, int64_t cnt) {
+        if (cnt <= 0) return 0;
+        int rc = ensure_stage((size_t)(L + 2) * cnt);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpyAsync(stage, src, sizeof(uint32_t) * (L + 2) * cnt, hipMemcpyHostToDevice, st));
+        k_unpack<L><<<cdiv(cnt, 256), 256, 0, st>>>(stage, a.pv(), off, cnt);
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    int get_packed(const Arr &a, int64_t off, uint32_t *dst, int64_t cnt) {
+        if (cnt <= 0) return 0;
+        int rc = ensure_stage((size_t)(L + 2) * cnt);
+        if (rc) return rc;
+        k_pack<L><<<cdiv(cnt, 256), 256, 0, st>>>(a.pv(), off, cnt, stage);
+        HIPCHK(hipMemcpyAsync(dst, stage, sizeof(uint32_t) * (L + 2) * cnt, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    void zero_arr(Arr &a, int64_t off, int64_t cnt) {
+        if (cnt > 0) k_zero<L><<<cdiv(cnt, 256), 256, 0, st>>>(a.pv(), off, cnt);
+    }
+    static mp<L> unpack1(const uint32_t *w) {
+        mp<L> x;
+        x.e = (int32_t)w[0];
+        x.s = w[1] & 1u;
+        for (int k = 0; k < L; k++) x.m[k] = w[2 + k];
+        if (x.m[L - 1] == 0) x = zero<L>();
+        return x;
+    }
+    static void pack1(const mp<L> &x, uint32_t *w) {
+        w[0] = (uint32_t)x.e;
+        w[1] = x.s;
+        for (int k = 0; k < L; k++) w[2 + k] = x.m[k];
+    }
+    void tstart() { hipEventRecord(ev0, st); }
+    int tstop(int slot) {
+        hipEventRecord(ev1, st);
+        HIPCHK(hipEventSynchronize(ev1));
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev0, ev1);
+        timers[slot] += ms;
+        return 0;
+    }
+
+    int init() {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+        blkoff.resize(nsdp + 1);
+        Nd = 0;
+        for (int l = 0; l < nsdp; l++) {
+            blkoff[l] = Nd;
+            Nd += (int64_t)sizes[l] * sizes[l];
+        }
+        blkoff[nsdp] = Nd;
+        Nd += nlp;
+        blk.assign((size_t)(m + 1) * nsdp, Blk());
+        lp_pos.assign(m + 1, {});
+        lp_val.assign(m + 1, {});
+        return gemm.init(st);
+    }
+
+    int commit() override {
+        const int W = L + 2;
+        // ---- constraint data: sparse entries, dense values, descriptors ----
+        std::vector<int32_t> ri, ci;
+        std::vector<uint32_t> sv, dvv;
+        std::vector<Desc> desc((size_t)(m + 1) * nsdp);
+        int64_t sn = 0, dn = 0;
+        for (int c = -1; c < m; c++)
+            for (int l = 0; l < nsdp; l++) {
+                Blk &B = blk[(size_t)(c + 1) * nsdp + l];
+                Desc d{};
+                d.n = sizes[l];
+                d.blkoff = blkoff[l];
+                if (!B.set) {
+                    d.kind = 0;
+                } else if (B.dense) {
+                    d.kind = 2;
+                    d.off = dn;
+                    d.nnz = sizes[l] * sizes[l];
+                    d.effect = d.nnz;
+                    dvv.insert(dvv.end(), B.v.begin(), B.v.end());
+                    dn += (int64_t)sizes[l] * sizes[l];
+                } else {
+                    d.kind = 1;
+                    d.off = sn;
+                    d.nnz = (int)B.r.size();
+                    int eff = 0;
+                    for (size_t t = 0; t < B.r.size(); t++) {
+                        ri.push_back(B.r[t]);
+                        ci.push_back(B.c[t]);
+                        eff += (B.r[t] == B.c[t]) ? 1 : 2;
+                    }
+                    d.effect = eff;
+                    sv.insert(sv.end(), B.v.begin(), B.v.end());
+                    sn += (int64_t)B.r.size();
+                }
+                desc[(size_t)(c + 1) * nsdp + l] = d;
+            }
+        int rc;
+        if ((rc = aalloc(sval, sn))) return rc;
+        if ((rc = put_packed(sval, 0, sv.data(), sn))) return rc;
+        if ((rc = aalloc(dval, dn))) return rc;
+        if ((rc = put_packed(dval, 0, dvv.data(), dn))) return rc;
+        if ((rc = upload(&d_ri, ri))) return rc;
+        if ((rc = upload(&d_ci, ci))) return rc;
+        std::vector<Desc> cdesc(desc.begin(), desc.begin() + nsdp), adesc(desc.begin() + nsdp, desc.end());
+        if ((rc = upload(&d_cdesc, cdesc))) return rc;
+        if ((rc = upload(&d_desc, adesc))) return rc;
+        hdesc = adesc;
+        // ---- LP data: constraints 0..m-1 then C ----
+        std::vector<int64_t> lptr(m + 1, 0), clptr(2, 0);
+        std::vector<int32_t> lpos;
+        std::vector<uint32_t> lv;
+        for (int c = 0; c < m; c++) {
+            lptr[c] = (int64_t)lpos.size();
+            for (size_t t = 0; t < lp_pos[c + 1].size(); t++) lpos.push_back(lp_pos[c + 1][t]);
+            lv.insert(lv.end(), lp_val[c + 1].begin(), lp_val[c + 1].end());
+        }
+        lptr[m] = (int64_t)lpos.size();
+        lp_c_base = (int64_t)lpos.size();
+        clptr[0] = lp_c_base;
+        for (size_t t = 0; t < lp_pos[0].size(); t++) lpos.push_back(lp_pos[0][t]);
+        lv.insert(lv.end(), lp_val[0].begin(), lp_val[0].end());
+        clptr[1] = (int64_t)lpos.size();
+        if ((rc = upload(&d_lp_ptr, lptr))) return rc;
+        if ((rc = upload(&d_clp_ptr, clptr))) return rc;
+        if ((rc = upload(&d_lp_pos, lpos))) return rc;
+        if ((rc = aalloc(lpval, (int64_t)lpos.size()))) return rc;
+        if ((rc = put_packed(lpval, 0, lv.data(), (int64_t)lpos.size()))) return rc;
+        // ---- scatter index over the dense linear space (A_i only, ordered by constraint) ----
+        {
+            std::vector<std::vector<std::pair<int, int64_t>>> lists(Nd);
+            std::vector<int64_t> dcon_ptr(nsdp + 1, 0), dcon_off;
+            std::vector<int32_t> dcon;
+            for (int c = 0; c < m; c++)
+                for (int l = 0; l < nsdp; l++) {
+                    const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
+                    if (d.kind == 1) {
+                        for (int t = 0; t < d.nnz; t++) {
+                            int64_t e = d.off + t;
+                            int i = ri[e], j = ci[e];
+                            lists[blkoff[l] + i + (int64_t)sizes[l] * j].push_back({c, e});
+                            if (i != j) lists[blkoff[l] + j + (int64_t)sizes[l] * i].push_back({c, e});
+                        }
+                    }
+                }
+            for (int l = 0; l < nsdp; l++) {
+                dcon_ptr[l] = (int64_t)dcon.size();
+                for (int c = 0; c < m; c++) {
+                    const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
+                    if (d.kind == 2) {
+                        dcon.push_back(c);
+                        dcon_off.push_back(d.off);
+                    }
+                }
+            }
+            dcon_ptr[nsdp] = (int64_t)dcon.size();
+            // LP positions
+            for (int c = 0; c < m; c++)
+                for (int64_t t = lptr[c]; t < lptr[c + 1]; t++) lists[blkoff[nsdp] + lpos[t]].push_back({c, -1 - t});
+            std::vector<int64_t> pptr(Nd + 1, 0), pent, pinb(Nd, 0);
+            std::vector<int32_t> pcon, pblk(Nd, -1);
+            for (int64_t q = 0; q < Nd; q++) {
+                pptr[q] = (int64_t)pcon.size();
+                for (auto &pr : lists[q]) {
+                    pcon.push_back(pr.first);
+                    pent.push_back(pr.second);
+                }
+            }
+            pptr[Nd] = (int64_t)pcon.size();
+            for (int l = 0; l < nsdp; l++)
+                for (int64_t k = 0; k < (int64_t)sizes[l] * sizes[l]; k++) {
+                    pblk[blkoff[l] + k] = l;
+                    pinb[blkoff[l] + k] = k;
+                }
+            // LP entries reference the lp value array: encode with a separate value view (offset by sn)
+            // by copying LP values after the sparse values into a combined scatter-value array.
+            bool has_lp_entries = false;
+            for (auto &e : pent)
+                if (e < 0) has_lp_entries = true;
+            if (has_lp_entries) {
+                // combined array = sval ++ lpval(A part)
+                Arr comb;
+                if ((rc = aalloc(comb, sn + lptr[m]))) return rc;
+                std::vector<uint32_t> allv(sv);
+                allv.insert(allv.end(), lv.begin(), lv.begin() + lptr[m] * W);
+                if ((rc = put_packed(comb, 0, allv.data(), sn + lptr[m]))) return rc;
+                for (auto &e : pent)
+                    if (e < 0) e = sn + (-1 - e);
+                scat_vals = comb;
+            } else {
+                scat_vals = sval;
+            }
+            if ((rc = upload(&d_pos_ptr, pptr))) return rc;
+            if ((rc = upload(&d_pos_con, pcon))) return rc;
+            if ((rc = upload(&d_pos_ent, pent))) return rc;
+            if ((rc = upload(&d_pos_blk, pblk))) return rc;
+            if ((rc = upload(&d_pos_in_blk, pinb))) return rc;
+            if ((rc = upload(&d_dcon_ptr, dcon_ptr))) return rc;
+            if ((rc = upload(&d_dcon, dcon))) return rc;
+            if ((rc = upload(&d_dcon_off, dcon_off))) return rc;
+        }
+        // ---- full C as a dense linear space ----
+        {
+            std::vector<mp<L>> cf(Nd, zero<L>());
+            for (int l = 0; l < nsdp; l++) {
+                const Blk &B = blk[l];
+                if (!B.set) continue;
+                int n = sizes[l];
+                if (B.dense) {
+                    for (int64_t k = 0; k < (int64_t)n * n; k++) cf[blkoff[l] + k] = unpack1(&B.v[k * W]);
+                } else {
+                    for (size_t t = 0; t < B.r.size(); t++) {
+                        mp<L> v = unpack1(&B.v[t * W]);
+                        int i = B.r[t], j = B.c[t];
+                        cf[blkoff[l] + i + (int64_t)n * j] = v;
+                        cf[blkoff[l] + j + (int64_t)n * i] = v;
+                    }
+                }
+            }
+            for (size_t t = 0; t < lp_pos[0].size(); t++) cf[blkoff[nsdp] + lp_pos[0][t]] = unpack1(&lp_val[0][t * W]);
+            std::vector<uint32_t> pk(Nd * W);
+            for (int64_t q = 0; q < Nd; q++) pack1(cf[q], &pk[q * W]);
+            if ((rc = aalloc(Cfull, Nd))) return rc;
+            if ((rc = put_packed(Cfull, 0, pk.data(), Nd))) return rc;
+        }
+        // ---- work arrays ----
+        Arr *dls[] = {&X, &Z, &invZ, &Rd, &rzinv, &xrd, &dX, &dZ, &W1, &W2, &DLS1};
+        for (Arr *a : dls)
+            if ((rc = aalloc(*a, Nd))) return rc;
+        Arr *vec[] = {&y, &Rp, &g, &dy, &b, &rdiag};
+        for (Arr *a : vec)
+            if ((rc = aalloc(*a, m))) return rc;
+        if ((rc = aalloc(scal, 4096))) return rc;
+        if ((rc = aalloc(Bm, (int64_t)m * m))) return rc;
+        if ((rc = dalloc(&d_info, 1))) return rc;
+        for (Arr *a : dls) zero_arr(*a, 0, Nd);
+        if (!bvec.empty() && (rc = put_packed(b, 0, bvec.data(), m))) return rc;
+        // ---- Schur plan: pair rule (sdpa_newton.cpp:1052-1054) and workspace slots ----
+        plan.assign(nsdp, BlockPlan());
+        int64_t maxslots = 0, maxn2 = 1;
+        for (int l = 0; l < nsdp; l++) {
+            BlockPlan &P = plan[l];
+            int n = sizes[l];
+            maxn2 = std::max<int64_t>(maxn2, (int64_t)n * n);
+            std::vector<int> cons;
+            for (int c = 0; c < m; c++)
+                if (desc[(size_t)(c + 1) * nsdp + l].kind) cons.push_back(c);
+            P.slot_of.assign(m, -1);
+            std::vector<Pair> pairs;
+            std::vector<int> slot_con;
+            for (int i : cons) {
+                const Desc &di = desc[(size_t)(i + 1) * nsdp + l];
+                int f = formula.empty() ? 1 : formula[(size_t)i * nsdp + l];
+                if (f < 1 || f > 3) f = 1;
+                bool needG = (f == 1);
+                if (f == 2) {
+                    for (int j : cons) {
+                        const Desc &dj = desc[(size_t)(j + 1) * nsdp + l];
+                        if (!(di.effect < dj.effect || (di.effect == dj.effect && i < j)) && dj.kind == 2) needG = true;
+                    }
+                }
+                if (f == 1 || f == 2) {
+                    P.slot_of[i] = P.nslots++;
+                    slot_con.push_back(i);
+                    if (di.kind == 1)
+                        P.sp_slots.push_back(P.slot_of[i]);
+                    else
+                        P.dn_slots.push_back(P.slot_of[i]);
+                    if (needG) P.g_slots.push_back(P.slot_of[i]);
+                }
+                for (int j : cons) {
+                    const Desc &dj = desc[(size_t)(j + 1) * nsdp + l];
+                    if (di.effect < dj.effect || (di.effect == dj.effect && i < j)) continue;
+                    pairs.push_back(Pair{i, j, P.slot_of[i], f});
+                }
+            }
+            P.npairs = (int)pairs.size();
+            if ((rc = upload(&P.d_pairs, pairs))) return rc;
+            P.dn_cons.clear();
+            for (int s : P.dn_slots) P.dn_cons.push_back(slot_con[s]);
+            // symmetric row expansion of sparse slots (for F = A invZ)
+            std::vector<int64_t> rowptr;
+            std::vector<int32_t> col, slotlist;
+            std::vector<int64_t> ent;
+            int k = 0;
+            for (int s : P.sp_slots) {
+                int c = slot_con[s];
+                const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
+                std::vector<std::vector<std::pair<int, int64_t>>> rows(n);
+                for (int t = 0; t < d.nnz; t++) {
+                    int64_t e = d.off + t;
+                    int i = ri[e], j = ci[e];
+                    rows[i].push_back({j, e});
+                    if (i != j) rows[j].push_back({i, e});
+                }
+                for (int r = 0; r < n; r++) {
+                    rowptr.push_back((int64_t)col.size());
+                    for (auto &pr : rows[r]) {
+                        col.push_back(pr.first);
+                        ent.push_back(pr.second);
+                    }
+                }
+                rowptr.push_back((int64_t)col.size());  // sentinel of this slot (n+1 entries per slot)
+                slotlist.push_back(k++);
+            }
+            P.n_sp_slots = k;
+            if ((rc = upload(&P.d_exp_rowptr, rowptr))) return rc;
+            if ((rc = upload(&P.d_exp_col, col))) return rc;
+            if ((rc = upload(&P.d_exp_ent, ent))) return rc;
+            if ((rc = upload(&P.d_exp_slot, slotlist))) return rc;
+            std::vector<int32_t> spl(P.sp_slots.begin(), P.sp_slots.end());
+            if ((rc = upload(&P.d_sp_list, spl))) return rc;
+            maxslots = std::max<int64_t>(maxslots, P.nslots);
+        }
+        wstride = maxn2;
+        wslots = std::max<int64_t>(maxslots, 1);
+        if ((rc = aalloc(Fw, wstride * wslots))) return rc;
+        if ((rc = aalloc(Gw, wstride * wslots))) return rc;
+        // LP pairs
+        {
+            std::vector<LPPair> lpp;
+            for (int q = 0; q < nlp; q++) {
+                std::vector<std::pair<int, int64_t>> cl;
+                for (int c = 0; c < m; c++)
+                    for (int64_t t = lptr[c]; t < lptr[c + 1]; t++)
+                        if (lpos[t] == q) cl.push_back({c, t});
+                for (size_t a = 0; a < cl.size(); a++)
+                    for (size_t bb = a; bb < cl.size(); bb++)
+                        lpp.push_back(LPPair{cl[a].first, cl[bb].first, q, 0, cl[a].second, cl[bb].second});
+            }
+            nlppairs = (int)lpp.size();
+            if ((rc = upload(&d_lppairs, lpp))) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        // release staging memory
+        blk.clear();
+        blk.shrink_to_fit();
+        committed = true;
+        return 0;
+    }
+    Arr scat_vals;
+
+    int set_point(const uint32_t *Xp, const uint32_t *Zp, const uint32_t *iZp, const uint32_t *yp) override {
+        int rc;
+        if (Xp && (rc = put_packed(X, 0, Xp, Nd))) return rc;
+        if (Zp && (rc = put_packed(Z, 0, Zp, Nd))) return rc;
+        if (iZp && (rc = put_packed(invZ, 0, iZp, Nd))) return rc;
+        if (yp && (rc = put_packed(y, 0, yp, m))) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+
+    // per-block dense products via the GEMM engine
+    int block_gemm(Arr &C, const Arr &A, const Arr &B, int l) {
+        int n = sizes[l];
+        int64_t o = blkoff[l];
+        return gemm.gemm(n, n, n, A.pv(), o, n, false, B.pv(), o, n, false, C.pv(), o, n, GEMM_SET, 1, 0, 0, 0,
+                         gemm_mode, gemm_slices);
+    }
+    // out = A*B*C per block (ns_jordan_triple_product); tmp receives A*B
+    int triple(Arr &out, const Arr &A, const Arr &B, const Arr &C, Arr &tmp) {
+        int rc;
+        for (int l = 0; l < nsdp; l++) {
+            if ((rc = block_gemm(tmp, A, B, l))) return rc;
+            if ((rc = block_gemm(out, tmp, C, l))) return rc;
+        }
+        if (nlp) k_lp_triple<L><<<cdiv(nlp, 128), 128, 0, st>>>(out.pv(), A.pv(), B.pv(), C.pv(), blkoff[nsdp], nlp);
+        return 0;
+    }
+
+    int maxabs(const Arr &A, int64_t cnt, mp<L> &out) {
+        int nb = (int)std::min<int64_t>(64, std::max<int64_t>(1, cdiv(cnt, 256)));
+        k_maxabs<L><<<nb, 256, 0, st>>>(A.pv(), cnt, scal.pv());
+        std::vector<uint32_t> h((size_t)nb * (L + 2));
+        int rc = get_packed(scal, 0, h.data(), nb);
+        if (rc) return rc;
+        out = zero<L>();
+        for (int i = 0; i < nb; i++) {
+            mp<L> v = unpack1(&h[(size_t)i * (L + 2)]);
+            if (cmp_abs<L>(v, out) > 0) out = v;
+        }
+        return 0;
+    }
+
+    int residuals(uint32_t *nrp, uint32_t *nrd, uint32_t *objP, uint32_t *objD) override {
+        tstart();
+        // Rp = b - A.X
+        k_con_inner<L><<<cdiv(m, 64), 64, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
+                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), Rp.pv(), 1);
+        // Rd = C - Z - sum y_i A_i
+        k_scatter<L><<<cdiv(Nd, 128), 128, 0, st>>>(Nd, d_pos_ptr, d_pos_con, d_pos_ent, scat_vals.pv(), dval.pv(),
+                                                     d_pos_blk, d_pos_in_blk, d_dcon_ptr, d_dcon, d_dcon_off, y.pv(),
+                                                     Rd.pv(), 0, Cfull.pv(), Z.pv(), Rd.pv(), 1);
+        int rc = tstop(5);
+        if (rc) return rc;
+        mp<L> v;
+        if (nrp) {
+            if ((rc = maxabs(Rp, m, v))) return rc;
+            pack1(v, nrp);
+        }
+        if (nrd) {
+            if ((rc = maxabs(Rd, Nd, v))) return rc;
+            pack1(v, nrd);
+        }
+        if (objP) {  // C . X
+            k_con_inner<L><<<1, 64, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr,
+                                             d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), PV{scal.p, scal.n}, 2);
+            std::vector<uint32_t> h(L + 2);
+            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
+            memcpy(objP, h.data(), sizeof(uint32_t) * (L + 2));
+        }
+        if (objD) {
+            k_dot_seq<L><<<1, 1, 0, st>>>(b.pv(), y.pv(), m, scal.pv());
+            std::vector<uint32_t> h(L + 2);
+            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
+            memcpy(objD, h.data(), sizeof(uint32_t) * (L + 2));
+        }
+        return 0;
+    }
+
+    int gvec(int corrector, const uint32_t *mu, const uint32_t *beta, int phase) override {
+        tstart();
+        mp<L> target = mul<L>(unpack1(beta), unpack1(mu));
+        mp<L> mone = from_int<L>(-1);
+        int rc;
+        // compute_rMat: r_zinv = (beta mu) invZ - X  [- dX dZ invZ]
+        k_lincomb<L><<<cdiv(Nd, 256), 256, 0, st>>>(rzinv.pv(), invZ.pv(), X.pv(), target, mone, 1, Nd);
+        if (corrector) {
+            if ((rc = triple(DLS1, dX, dZ, invZ, W2))) return rc;
+            k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(rzinv.pv(), rzinv.pv(), DLS1.pv(), mone, Nd);
+        }
+        // Make_gVec: DLS1 = r_zinv - x_rd_zinv (infeasible phases), gVec = -(A . DLS1) + Rp
+        if (phase == SDPG_PFEAS || phase == SDPG_NOINFO) {
+            if (!corrector)
+                if ((rc = triple(xrd, X, Rd, invZ, W2))) return rc;
+            k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(DLS1.pv(), rzinv.pv(), xrd.pv(), mone, Nd);
+        } else {
+            HIPCHK(hipMemcpyAsync(DLS1.p, rzinv.p, sizeof(uint32_t) * (L + 1) * Nd, hipMemcpyDeviceToDevice, st));
+        }
+        k_con_inner<L><<<cdiv(m, 64), 64, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
+                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], DLS1.pv(), Rp.pv(), g.pv(), 0);
+        return tstop(3);
+    }
+
+    int schur() override {
+        tstart();
+        zero_arr(Bm, 0, (int64_t)m * m);
+        int rc;
+        for (int l = 0; l < nsdp; l++) {
+            BlockPlan &P = plan[l];
+            int n = sizes[l];
+            int64_t o = blkoff[l];
+            // F_i = A_i invZ  (sparse: row expansion; dense: GEMM)
+            if (P.n_sp_slots) {
+                dim3 grid(cdiv((int64_t)n * n, 128), P.n_sp_slots);
+                k_spmm_rows<L><<<grid, 128, 0, st>>>(n, P.d_exp_rowptr, P.d_exp_col, P.d_exp_ent, sval.pv(),
+                                                     invZ.pv(), o, Fw.pv(), wstride, P.d_sp_list);
+            }
+            if (!P.dn_slots.empty()) {
+                // dense constraints: batched F_s = A_c invZ; their values are contiguous only if the
+                // constraints are, so launch per constraint run of equal stride
+                for (size_t t = 0; t < P.dn_slots.size(); t++) {
+                    int s = P.dn_slots[t], c = P.dn_cons[t];
+                    const Desc dd = hdesc[(size_t)c * nsdp + l];
+                    if ((rc = gemm.gemm(n, n, n, dval.pv(), dd.off, n, false, invZ.pv(), o, n, false, Fw.pv(),
+                                        (int64_t)s * wstride, n, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
+                        return rc;
+                }
+            }
+            // G_i = X F_i for slots needing it: one batched GEMM with X shared
+            if (!P.g_slots.empty()) {
+                // slots needing G are contiguous ranges in practice; batch over maximal runs
+                size_t a = 0;
+                while (a < P.g_slots.size()) {
+                    size_t e = a + 1;
+                    while (e < P.g_slots.size() && P.g_slots[e] == P.g_slots[e - 1] + 1) e++;
+                    int s0 = P.g_slots[a], cnt = (int)(e - a);
+                    if ((rc = gemm.gemm(n, n, n, X.pv(), o, n, false, Fw.pv(), (int64_t)s0 * wstride, n, false,
+                                        Gw.pv(), (int64_t)s0 * wstride, n, GEMM_SET, cnt, 0, wstride, wstride,
+                                        gemm_mode, gemm_slices)))
+                        return rc;
+                    a = e;
+                }
+            }
+            if (P.npairs)
+                k_pairs<L><<<cdiv(P.npairs, 64), 64, 0, st>>>(P.npairs, P.d_pairs, l, nsdp, d_desc, d_ri, d_ci,
+                                                               sval.pv(), dval.pv(), Gw.pv(), wstride, Fw.pv(), X.pv(),
+                                                               invZ.pv(), o, Bm.pv(), m);
+        }
+        if (nlppairs)
+            k_pairs_lp<L><<<cdiv(nlppairs, 64), 64, 0, st>>>(nlppairs, d_lppairs, lpval.pv(), X.pv(), invZ.pv(),
+                                                             blkoff[nsdp], Bm.pv(), m);
+        HIPCHK(hipGetLastError());
+        return tstop(0);
+    }
+    std::vector<Desc> hdesc;  // host copy of the A descriptors (index c*nsdp + l)
+
+    int potrf(int *info) override {
+        tstart();
+        HIPCHK(hipMemsetAsync(d_info, 0, sizeof(int), st));
+        // keep B (the matrix before factorisation) available: factor in place into Bm, copy of B in W? no:
+        // B is needed afterwards only by tests; they read it before calling potrf.
+        const int NBK = 32;
+        int rc;
+        for (int k = 0; k < m; k += NBK) {
+            int nb = std::min(NBK, m - k);
+            k_potrf_panel<L><<<1, 256, 0, st>>>(Bm.pv(), m, k, nb, rdiag.pv(), d_info);
+            if (k + nb < m) {
+                int r = m - k - nb;
+                // A22 -= P P^T (lower), P = A[k+nb:, k:k+nb]
+                if ((rc = gemm.gemm(r, r, nb, Bm.pv(), (k + nb) + (int64_t)m * k, m, false, Bm.pv(),
+                                    (k + nb) + (int64_t)m * k, m, true, Bm.pv(), (k + nb) + (int64_t)m * (k + nb), m,
+                                    GEMM_SUB, 1, 0, 0, 0, gemm_mode, gemm_slices, true)))
+                    return rc;
+            }
+            // early exit check every panel keeps failures cheap
+            int hinfo = 0;
+            HIPCHK(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (hinfo) {
+                *info = hinfo;
+                tstop(1);
+                return 0;
+            }
+        }
+        *info = 0;
+        return tstop(1);
+    }
+
+    int potrs() override {
+        tstart();
+        HIPCHK(hipMemcpyAsync(dy.p, g.p, sizeof(uint32_t) * (L + 1) * dy.n, hipMemcpyDeviceToDevice, st));
+        k_trsv_both<L><<<1, 256, 0, st>>>(Bm.pv(), m, rdiag.pv(), dy.pv());
+        HIPCHK(hipGetLastError());
+        return tstop(2);
+    }
+
+    int dz(int phase) override {
+        tstart();
+        int useR = (phase == SDPG_PFEAS || phase == SDPG_NOINFO) ? 1 : 0;
+        k_scatter<L><<<cdiv(Nd, 128), 128, 0, st>>>(Nd, d_pos_ptr, d_pos_con, d_pos_ent, scat_vals.pv(), dval.pv(),
+                                                     d_pos_blk, d_pos_in_blk, d_dcon_ptr, d_dcon, d_dcon_off, dy.pv(),
+                                                     Rd.pv(), useR, Cfull.pv(), Z.pv(), dZ.pv(), 0);
+        HIPCHK(hipGetLastError());
+        return tstop(4);
+    }
+
+    int dx() override {
+        tstart();
+        int rc;
+        if ((rc = triple(W1, X, dZ, invZ, W2))) return rc;
+        mp<L> mone = from_int<L>(-1);
+        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(dX.pv(), rzinv.pv(), W1.pv(), mone, Nd);
+        for (int l = 0; l < nsdp; l++) {
+            int n = sizes[l];
+            k_symmetrize<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(dX.pv(), blkoff[l], n);
+        }
+        HIPCHK(hipGetLastError());
+        return tstop(4);
+    }
+
+    Arr *sel(int what, int64_t &cnt) {
+        switch (what) {
+        case SDPG_X: cnt = Nd; return &X;
+        case SDPG_Z: cnt = Nd; return &Z;
+        case SDPG_INVZ: cnt = Nd; return &invZ;
+        case SDPG_Y: cnt = m; return &y;
+        case SDPG_RP: cnt = m; return &Rp;
+        case SDPG_RD: cnt = Nd; return &Rd;
+        case SDPG_GVEC: cnt = m; return &g;
+        case SDPG_B: cnt = (int64_t)m * m; return &Bm;
+        case SDPG_L: cnt = (int64_t)m * m; return &Bm;
+        case SDPG_DY: cnt = m; return &dy;
+        case SDPG_DZ: cnt = Nd; return &dZ;
+        case SDPG_DX: cnt = Nd; return &dX;
+        case SDPG_RZINV: cnt = Nd; return &rzinv;
+        }
+        return nullptr;
+    }
+    int get(int what, uint32_t *out) override {
+        int64_t cnt = 0;
+        Arr *a = sel(what, cnt);
+        if (!a) return SDPG_EARG;
+        return get_packed(*a, 0, out, cnt);
+    }
+    int put(int what, const uint32_t *in) override {
+        int64_t cnt = 0;
+        Arr *a = sel(what, cnt);
+        if (!a) return SDPG_EARG;
+        int rc = put_packed(*a, 0, in, cnt);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+    int sync() override {
+        HIPCHK(hipStreamSynchronize(st));
+        return 0;
+    }
+};
+
+}  // namespace sdpg
+
+// ------------------------------------------------------------------ C-ABI
+
+struct sdpg_ctx {
+    sdpg::EngineBase *eng;
+};
+
+using namespace sdpg;
+
+extern "C" {
+
+const char *sdpg_last_error(void) { return g_err.c_str(); }
+
+int sdpg_limbs(int prec_bits) { return mpx::limbs_for_bits(prec_bits); }
+
+static EngineBase *make_engine(int L) {
+#define MK_(LL) \
+    if (L == LL) return new Engine<LL>();
+    SDPG_FOR_EACH_L(MK_)
+#undef MK_
+    return nullptr;
+}
+
+int sdpg_ctx_create(sdpg_ctx **ctx, int prec_bits, int m, int n_sdp, const int *sdp_sizes, int n_lp, int device) {
+    if (!ctx || m <= 0 || n_sdp < 0 || n_lp < 0 || (n_sdp > 0 && !sdp_sizes)) {
+        set_err("sdpg_ctx_create: bad arguments");
+        return SDPG_EARG;
+    }
+    *ctx = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("sdpg_ctx_create: no HIP device");
+        return SDPG_ENODEV;
+    }
+    if (device < 0 || device >= ndev) {
+        set_err("sdpg_ctx_create: bad device ordinal");
+        return SDPG_EARG;
+    }
+    int L = mpx::limbs_for_bits(prec_bits);
+    EngineBase *e = make_engine(L);
+    if (!e) {
+        set_err("sdpg_ctx_create: unsupported precision (limbs " + std::to_string(L) + ")");
+        return SDPG_EPREC;
+    }
+    e->nlimb = L;
+    e->prec = prec_bits;
+    e->m = m;
+    e->nsdp = n_sdp;
+    e->nlp = n_lp;
+    e->device = device;
+    e->sizes.assign(sdp_sizes, sdp_sizes + n_sdp);
+    for (int s : e->sizes)
+        if (s <= 0) {
+            delete e;
+            set_err("sdpg_ctx_create: bad block size");
+            return SDPG_EARG;
+        }
+    int rc = 0;
+#define IN_(LL) \
+    if (L == LL) rc = static_cast<Engine<LL> *>(e)->init();
+    SDPG_FOR_EACH_L(IN_)
+#undef IN_
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *ctx = new sdpg_ctx{e};
+    return SDPG_OK;
+}
+
+void sdpg_ctx_destroy(sdpg_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->eng->device);
+    delete ctx->eng;
+    delete ctx;
+}
+
+#define CTX_CHECK(need_commit)                                                   \
+    if (!ctx || !ctx->eng) {                                                     \
+        set_err("null context");                                                 \
+        return SDPG_EARG;                                                        \
+    }                                                                            \
+    EngineBase *E = ctx->eng;                                                    \
+    if ((need_commit) != E->committed) {                                         \
+        set_err(need_commit ? "context not committed" : "context already committed"); \
+        return SDPG_ESTATE;                                                      \
+    }                                                                            \
+    hipSetDevice(E->device);
+
+int sdpg_set_block(sdpg_ctx *ctx, int con, int sdp_block, int nnz, const int *row, const int *col, const uint32_t *val,
+                   int dense) {
+    CTX_CHECK(false);
+    if (con < -1 || con >= E->m || sdp_block < 0 || sdp_block >= E->nsdp || nnz < 0 || (nnz && !val)) {
+        set_err("sdpg_set_block: bad arguments");
+        return SDPG_EARG;
+    }
+    int n = E->sizes[sdp_block];
+    auto &B = E->blk[(size_t)(con + 1) * E->nsdp + sdp_block];
+    if (B.set) {
+        set_err("sdpg_set_block: block already set");
+        return SDPG_ESTATE;
+    }
+    const int W = E->nlimb + 2;
+    if (dense) {
+        if (nnz != n * n) {
+            set_err("sdpg_set_block: dense block needs n*n values");
+            return SDPG_EARG;
+        }
+        B.dense = true;
+        B.v.assign(val, val + (size_t)nnz * W);
+    } else {
+        if (nnz && (!row || !col)) {
+            set_err("sdpg_set_block: missing indices");
+            return SDPG_EARG;
+        }
+        for (int t = 0; t < nnz; t++)
+            if (row[t] < 0 || row[t] >= n || col[t] < 0 || col[t] >= n) {
+                set_err("sdpg_set_block: index out of range");
+                return SDPG_EARG;
+            }
+        B.dense = false;
+        B.r.assign(row, row + nnz);
+        B.c.assign(col, col + nnz);
+        B.v.assign(val, val + (size_t)nnz * W);
+    }
+    B.set = true;
+    return SDPG_OK;
+}
+
+int sdpg_set_lp(sdpg_ctx *ctx, int con, int nnz, const int *pos, const uint32_t *val) {
+    CTX_CHECK(false);
+    if (con < -1 || con >= E->m || nnz < 0 || (nnz && (!pos || !val))) {
+        set_err("sdpg_set_lp: bad arguments");
+        return SDPG_EARG;
+    }
+    for (int t = 0; t < nnz; t++)
+        if (pos[t] < 0 || pos[t] >= E->nlp) {
+            set_err("sdpg_set_lp: position out of range");
+            return SDPG_EARG;
+        }
+    const int W = E->nlimb + 2;
+    E->lp_pos[con + 1].assign(pos, pos + nnz);
+    E->lp_val[con + 1].assign(val, val + (size_t)nnz * W);
+    return SDPG_OK;
+}
+
+int sdpg_set_b(sdpg_ctx *ctx, const uint32_t *b) {
+    CTX_CHECK(false);
+    if (!b) return SDPG_EARG;
+    E->bvec.assign(b, b + (size_t)E->m * (E->nlimb + 2));
+    return SDPG_OK;
+}
+
+int sdpg_set_formula(sdpg_ctx *ctx, const int *use_formula) {
+    CTX_CHECK(false);
+    if (!use_formula) return SDPG_EARG;
+    E->formula.assign(use_formula, use_formula + (size_t)E->m * E->nsdp);
+    return SDPG_OK;
+}
+
+int sdpg_commit(sdpg_ctx *ctx) {
+    CTX_CHECK(false);
+    if (E->bvec.empty()) {
+        set_err("sdpg_commit: b not set");
+        return SDPG_ESTATE;
+    }
+    return E->commit();
+}
+
+int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices) {
+    if (!ctx || !ctx->eng) return SDPG_EARG;
+    if (mode < -1 || mode > 1 || slices < 0) {
+        set_err("sdpg_set_gemm: bad mode");
+        return SDPG_EARG;
+    }
+    ctx->eng->gemm_mode = mode;
+    ctx->eng->gemm_slices = slices;
+    return SDPG_OK;
+}
+
+int sdpg_set_point(sdpg_ctx *ctx, const uint32_t *X, const uint32_t *Z, const uint32_t *invZ, const uint32_t *y) {
+    CTX_CHECK(true);
+    return E->set_point(X, Z, invZ, y);
+}
+int sdpg_residuals(sdpg_ctx *ctx, uint32_t *normRp, uint32_t *normRd, uint32_t *objP, uint32_t *objD) {
+    CTX_CHECK(true);
+    return E->residuals(normRp, normRd, objP, objD);
+}
+int sdpg_gvec(sdpg_ctx *ctx, int corrector, const uint32_t *mu, const uint32_t *beta, int phase) {
+    CTX_CHECK(true);
+    if (!mu || !beta) return SDPG_EARG;
+    return E->gvec(corrector, mu, beta, phase);
+}
+int sdpg_schur(sdpg_ctx *ctx) {
+    CTX_CHECK(true);
+    return E->schur();
+}
+int sdpg_potrf(sdpg_ctx *ctx, int *info) {
+    CTX_CHECK(true);
+    if (!info) return SDPG_EARG;
+    return E->potrf(info);
+}
+int sdpg_potrs(sdpg_ctx *ctx) {
+    CTX_CHECK(true);
+    return E->potrs();
+}
+int sdpg_dz(sdpg_ctx *ctx, int phase) {
+    CTX_CHECK(true);
+    return E->dz(phase);
+}
+int sdpg_dx(sdpg_ctx *ctx) {
+    CTX_CHECK(true);
+    return E->dx();
+}
+int sdpg_get(sdpg_ctx *ctx, int what, uint32_t *out) {
+    CTX_CHECK(true);
+    if (!out) return SDPG_EARG;
+    return E->get(what, out);
+}
+int sdpg_put(sdpg_ctx *ctx, int what, const uint32_t *in) {
+    CTX_CHECK(true);
+    if (!in) return SDPG_EARG;
+    return E->put(what, in);
+}
+int sdpg_timers(sdpg_ctx *ctx, double *ms, int n) {
+    if (!ctx || !ctx->eng || !ms) return SDPG_EARG;
+    for (int i = 0; i < n && i < 8; i++) ms[i] = ctx->eng->timers[i];
+    return SDPG_OK;
+}
+int sdpg_reset_timers(sdpg_ctx *ctx) {
+    if (!ctx || !ctx->eng) return SDPG_EARG;
+    for (double &t : ctx->eng->timers) t = 0;
+    return SDPG_OK;
+}
+int sdpg_sync(sdpg_ctx *ctx) {
+    if (!ctx || !ctx->eng) return SDPG_EARG;
+    return ctx->eng->sync();
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// gemm.h — multiple-precision GEMM engine (templated on limb count L).
+//
+// C = op(A) * op(B) (SET), C -= op(A) op(B) (SUB) or C += (ADD), batched with element
+// strides, optionally restricted to the lower triangle of C.  This is the GEMM
+// the reference reaches through Rgemm_NN_omp / Rgemm_NT_omp / Rsyrk
+// (mplapack/Rgemm_NN_omp.cpp:36-74, mplapack/Rgemm_NT_omp.cpp:36-76,
+// mplapack/Rsyrk.cpp:157-177); it serves the F1 precompute, the Cholesky
+// trailing update and the dX / r_zinv triple products.
+//
+// Forms:
+//  - element form (reference): one output per thread, floating limb multiply-add
+//    per term;
+//  - plain limb form (gemm_plain.h): rows of op(A) and columns of op(B) are
+//    brought to block fixed point (one exponent per row / column), products are
+//    exact multi-limb integer MADs (v_mad_u64_u32) into 96-bit column
+//    accumulators, renormalised once per output;
+//  - Ozaki form (gemm_ozaki.h): the same block fixed-point operands are cut into
+//    signed 8-bit slices and every slice-pair product runs as an INT8 MFMA with
+//    exact INT32 accumulation; slice products are recombined into limbs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine_common.h"
+#include "mpx.h"
+
+namespace sdpg {
+
+enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
+
+struct GemmArgs {
+    int M, N, K;
+    PV A;
+    int64_t offA;
+    int lda;
+    int transA;
+    PV B;
+    int64_t offB;
+    int ldb;
+    int transB;
+    PV C;
+    int64_t offC;
+    int ldc;
+    int op;
+    int batch;
+    int64_t sA, sB, sC;
+    int lower;  // only C[i,j] with i >= j
+};
+
+__device__ __forceinline__ int64_t idxA(const GemmArgs &g, int i, int k, int b) {
+    return g.offA + (int64_t)b * g.sA + (g.transA ? (k + (int64_t)g.lda * i) : (i + (int64_t)g.lda * k));
+}
+__device__ __forceinline__ int64_t idxB(const GemmArgs &g, int k, int j, int b) {
+    return g.offB + (int64_t)b * g.sB + (g.transB ? (j + (int64_t)g.ldb * k) : (k + (int64_t)g.ldb * j));
+}
+__device__ __forceinline__ int64_t idxC(const GemmArgs &g, int i, int j, int b) {
+    return g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
+}
+
+// element form: sequential sum over k (the Rgemm summation order)
+template <int L>
+__global__ void k_gemm_elem(GemmArgs g) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int b = blockIdx.z;
+    if (i >= g.M || j >= g.N) return;
+    if (g.lower && i < j) return;
+    mpx::mp<L> acc = mpx::zero<L>();
+    for (int k = 0; k < g.K; k++)
+        acc = mpx::add<L>(acc, mpx::mul<L>(ldm<L>(g.A, idxA(g, i, k, b)), ldm<L>(g.B, idxB(g, k, j, b))));
+    int64_t c = idxC(g, i, j, b);
+    if (g.op == GEMM_SUB)
+        acc = mpx::sub<L>(ldm<L>(g.C, c), acc);
+    else if (g.op == GEMM_ADD)
+        acc = mpx::add<L>(ldm<L>(g.C, c), acc);
+    stm<L>(g.C, c, acc);
+}
+
+}  // namespace sdpg
+
+#include "gemm_plain.h"
+
+namespace sdpg {
+
+template <int L>
+struct GemmEngine {
+    hipStream_t st = nullptr;
+    PlainWork<L> plain;
+    int init(hipStream_t s) {
+        st = s;
+        return 0;
+    }
+    // mode: 0 plain, 1 ozaki, -1 element (reference) form
+    int gemm(int M, int N, int K, PV A, int64_t offA, int lda, bool transA, PV B, int64_t offB, int ldb, bool transB,
+             PV C, int64_t offC, int ldc, int op, int batch, int64_t sA, int64_t sB, int64_t sC, int mode, int slices,
+             bool lower = false) {
+        if (M <= 0 || N <= 0 || batch <= 0) return 0;
+        GemmArgs g{M, N, K, A, offA, lda, transA ? 1 : 0, B, offB, ldb, transB ? 1 : 0, C, offC, ldc, op,
+                   batch, sA, sB, sC, lower ? 1 : 0};
+        if (K <= 0) {
+            if (op != GEMM_SET) return 0;
+        }
+        if (mode == -1 || K <= 0) {
+            dim3 blk(16, 16, 1), grd((M + 15) / 16, (N + 15) / 16, batch);
+            k_gemm_elem<L><<<grd, blk, 0, st>>>(g);
+            return hipGetLastError() == hipSuccess ? 0 : -2;
+        }
+        (void)slices;
+        return plain.run(g, st);
+    }
+};
+
+}  // namespace sdpg

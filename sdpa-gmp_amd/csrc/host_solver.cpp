@@ -1,0 +1,992 @@
+// host_solver.cpp — host driver: SDPA-GMP's IPM control on the host, Newton on the GPU.
+//
+// Follows the reference main loop pinpal() (sdpa_main.cpp:43-461) and the host
+// control classes of sdpa_parts.cpp (StepLength, DirectionParameter, Switch,
+// AverageComplementarity, RatioInitResCurrentRes, SolveInfo, Phase) and
+// sdpa_dataset.cpp (Solutions::initialize / update / computeInverse); the
+// Newton system (Newton::Mehrotra, sdpa_newton.cpp:1375-1414) is a sequence of
+// C-ABI calls into the device engine (include/sdpg.h).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include <string>
+#include <vector>
+
+#include "host_la.h"
+#include "limbs.h"
+#include "mpx.h"
+#include "mpx_host.h"
+#include "sdpa_amd.h"
+#include "sdpg.h"
+
+namespace sdpg {
+namespace host {
+
+using mpx::mp;
+using namespace sdpg::hla;
+
+static double now_s() {
+    struct timeval tv;
+    gettimeofday(&tv, nullptr);
+    return tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// param.sdpa (sdpa_parts.cpp:136-225)
+struct Parameter {
+    int maxIteration = 200;
+    double epsilonStar = 1.0e-30, lambdaStar = 1.0e+4, omegaStar = 2.0, lowerBound = -1.0e+5, upperBound = 1.0e+5;
+    double betaStar = 0.1, betaBar = 0.3, gammaStar = 0.9, epsilonDash = 1.0e-30;
+    int precision = 200;
+    bool read(const char *path) {
+        FILE *fp = fopen(path, "r");
+        if (!fp) return false;
+        bool ok = true;
+        ok &= fscanf(fp, "%d%*[^\n]", &maxIteration) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &epsilonStar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &lambdaStar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &omegaStar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &lowerBound) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &upperBound) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &betaStar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &betaBar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &gammaStar) == 1;
+        ok &= fscanf(fp, "%lf%*[^\n]", &epsilonDash) == 1;
+        ok &= fscanf(fp, "%d%*[^\n]", &precision) == 1;
+        fclose(fp);
+        return ok;
+    }
+};
+
+enum { noINFO, pFEAS, dFEAS, pdFEAS, pdINF, pFEAS_dINF, pINF_dFEAS, pdOPT, pUNBD, dUNBD };
+
+static const char *phase_str(int v) {
+    switch (v) {
+    case noINFO: return "noINFO    ";
+    case pFEAS: return "pFEAS     ";
+    case dFEAS: return "dFEAS     ";
+    case pdFEAS: return "pdFEAS    ";
+    case pdINF: return "pdINF     ";
+    case pFEAS_dINF: return "pFEAS_dINF";
+    case pINF_dFEAS: return "pINF_dFEAS";
+    case pdOPT: return "pdOPT     ";
+    case pUNBD: return "pUNBD     ";
+    case dUNBD: return "dUNBD     ";
+    }
+    return "phase error";
+}
+// Phase::reverse under REVERSE_PRIMAL_DUAL (sdpa_parts.cpp:1013-1048)
+static int phase_reverse(int v) {
+    switch (v) {
+    case pFEAS: return dFEAS;
+    case dFEAS: return pFEAS;
+    case pFEAS_dINF: return pINF_dFEAS;
+    case pINF_dFEAS: return pFEAS_dINF;
+    case pUNBD: return dUNBD;
+    case dUNBD: return pUNBD;
+    }
+    return v;
+}
+
+// ----- input (sdpa_io.cpp:123-142, 410-751) -----
+struct RawEntry {
+    int k, l, i, j;
+    std::string v;
+};
+
+struct RawProblem {
+    int m = 0, nBlock = 0;
+    std::vector<int> blockStruct;
+    std::vector<std::string> b;
+    std::vector<RawEntry> ent;
+};
+
+static int skip_to_num(FILE *fp) {
+    int c;
+    while ((c = fgetc(fp)) != EOF) {
+        if ((c >= '0' && c <= '9') || c == '+' || c == '-') {
+            ungetc(c, fp);
+            return 1;
+        }
+    }
+    return 0;
+}
+static bool read_int(FILE *fp, int *v) { return skip_to_num(fp) && fscanf(fp, "%d", v) == 1; }
+static bool read_tok(FILE *fp, std::string &s) {
+    if (!skip_to_num(fp)) return false;
+    s.clear();
+    int c;
+    while ((c = fgetc(fp)) != EOF) {
+        if ((c >= '0' && c <= '9') || c == '+' || c == '-' || c == '.' || c == 'e' || c == 'E')
+            s.push_back((char)c);
+        else {
+            ungetc(c, fp);
+            break;
+        }
+    }
+    return !s.empty();
+}
+
+static int read_raw(const char *path, RawProblem &R) {
+    FILE *fp = fopen(path, "r");
+    if (!fp) return -1;
+    char line[8192];
+    for (;;) {
+        if (!fgets(line, sizeof line, fp)) {
+            fclose(fp);
+            return -2;
+        }
+        if (line[0] == '*' || line[0] == '"') continue;
+        if (sscanf(line, "%d", &R.m) != 1) {
+            fclose(fp);
+            return -2;
+        }
+        break;
+    }
+    if (fscanf(fp, "%d", &R.nBlock) != 1 || R.nBlock <= 0) {
+        fclose(fp);
+        return -3;
+    }
+    R.blockStruct.resize(R.nBlock);
+    for (int l = 0; l < R.nBlock; l++)
+        if (!read_int(fp, &R.blockStruct[l])) {
+            fclose(fp);
+            return -3;
+        }
+    R.b.resize(R.m);
+    for (int k = 0; k < R.m; k++)
+        if (!read_tok(fp, R.b[k])) {
+            fclose(fp);
+            return -4;
+        }
+    for (;;) {
+        RawEntry e;
+        if (!read_int(fp, &e.k) || !read_int(fp, &e.l) || !read_int(fp, &e.i) || !read_int(fp, &e.j) ||
+            !read_tok(fp, e.v))
+            break;
+        R.ent.push_back(std::move(e));
+    }
+    fclose(fp);
+    return 0;
+}
+
+template <int L>
+struct Solver {
+    using T = mp<L>;
+    static constexpr int W = L + 2;
+    // problem
+    int m = 0, nsdp = 0, nlp = 0, nDim = 0;
+    std::vector<int> sdpStruct, blockType, blockNumber, blockStructAbs;
+    std::vector<T> b;
+    struct SpB {
+        int blk = 0, n = 0;
+        bool dense = false;
+        std::vector<int> r, c;
+        std::vector<T> v;
+        std::vector<T> de;
+        int effect = 0;
+    };
+    struct SLS {
+        std::vector<SpB> s;
+        std::vector<int> lpi;
+        std::vector<T> lpv;
+        SpB *find(int blk) {
+            for (auto &x : s)
+                if (x.blk == blk) return &x;
+            return nullptr;
+        }
+    };
+    SLS C;
+    std::vector<SLS> A;
+    // dense linear spaces
+    std::vector<int64_t> off;
+    int64_t Nd = 0;
+    std::vector<T> X, Z, invCholX, invCholZ, invZ, dX, dZ, y, dy;
+    std::vector<int> useFormula;
+    sdpg_ctx *ctx = nullptr;
+    Parameter prm;
+    FILE *logf = nullptr;
+    std::string err;
+
+    ~Solver() {
+        if (ctx) sdpg_ctx_destroy(ctx);
+    }
+
+    T parse(const std::string &s) {
+        T v;
+        if (mpx::parse_decimal<L>(s.c_str(), v) <= 0) v = mpx::zero<L>();
+        return v;
+    }
+
+    int load(const RawProblem &R) {
+        m = R.m;
+        blockType.assign(R.nBlock, 0);
+        blockNumber.assign(R.nBlock, 0);
+        blockStructAbs = R.blockStruct;
+        // block typing (sdpa_main.cpp:114-131)
+        for (int i = 0; i < R.nBlock; i++) {
+            int bs = R.blockStruct[i];
+            if (bs >= 2) {
+                blockType[i] = 1;
+                blockNumber[i] = nsdp++;
+                sdpStruct.push_back(bs);
+            } else if (bs < 0) {
+                blockType[i] = 3;
+                blockStructAbs[i] = -bs;
+                blockNumber[i] = nlp;
+                nlp += -bs;
+            } else if (bs == 1) {
+                blockType[i] = 3;
+                blockNumber[i] = nlp;
+                nlp += 1;
+            } else {
+                err = "block struct";
+                return -10;
+            }
+        }
+        nDim = nlp;
+        for (int s : sdpStruct) nDim += s;
+        b.resize(m);
+        for (int k = 0; k < m; k++) b[k] = parse(R.b[k]);
+        A.assign(m, SLS());
+        for (const RawEntry &e : R.ent) {
+            if (e.k < 0 || e.k > m || e.l < 1 || e.l > R.nBlock) {
+                err = "entry out of range";
+                return -11;
+            }
+            SLS &t = (e.k == 0) ? C : A[e.k - 1];
+            T v = parse(e.v);
+            if (e.k == 0) v = mpx::neg<L>(v);  // C = -F0 (sdpa_io.cpp:661)
+            if (blockType[e.l - 1] == 1) {
+                int l2 = blockNumber[e.l - 1];
+                SpB *s = t.find(l2);
+                if (!s) {
+                    SpB nb;
+                    nb.blk = l2;
+                    nb.n = sdpStruct[l2];
+                    size_t pos = 0;
+                    while (pos < t.s.size() && t.s[pos].blk < l2) pos++;
+                    t.s.insert(t.s.begin() + pos, nb);
+                    s = t.find(l2);
+                }
+                if (e.i < 1 || e.j < 1 || e.i > s->n || e.j > s->n) {
+                    err = "out of range in input data";
+                    return -12;
+                }
+                s->r.push_back(e.i - 1);
+                s->c.push_back(e.j - 1);
+                s->v.push_back(v);
+                s->effect += (e.i == e.j) ? 1 : 2;
+            } else {
+                if (e.i != e.j) {
+                    err = "LP entry off the diagonal";
+                    return -13;
+                }
+                int pos = blockNumber[e.l - 1] + e.i - 1;
+                bool found = false;
+                for (size_t q = 0; q < t.lpi.size(); q++)
+                    if (t.lpi[q] == pos) {
+                        t.lpv[q] = v;
+                        found = true;
+                    }
+                if (!found) {
+                    size_t q = 0;
+                    while (q < t.lpi.size() && t.lpi[q] < pos) q++;
+                    t.lpi.insert(t.lpi.begin() + q, pos);
+                    t.lpv.insert(t.lpv.begin() + q, v);
+                }
+            }
+        }
+        // SparseMatrix::changeToDense (sdpa_struct.cpp:484-525)
+        auto densify = [&](SpB &s) {
+            int n = s.n;
+            if ((double)s.r.size() < (double)(n * n) * 0.20) return;
+            s.dense = true;
+            s.de.assign((size_t)n * n, mpx::zero<L>());
+            for (size_t t = 0; t < s.r.size(); t++) {
+                s.de[s.r[t] + (size_t)n * s.c[t]] = s.v[t];
+                s.de[s.c[t] + (size_t)n * s.r[t]] = s.v[t];
+            }
+            s.effect = n * n;
+        };
+        for (auto &s : C.s) densify(s);
+        for (auto &a : A)
+            for (auto &s : a.s) densify(s);
+        off.assign(nsdp + 1, 0);
+        Nd = 0;
+        for (int l = 0; l < nsdp; l++) {
+            off[l] = Nd;
+            Nd += (int64_t)sdpStruct[l] * sdpStruct[l];
+        }
+        off[nsdp] = Nd;
+        Nd += nlp;
+        return 0;
+    }
+
+    // Newton::computeFormula_SDP (sdpa_newton.cpp:694-819), KAPPA = 1.2 (sdpa_main.cpp:27).
+    // The comparisons are evaluated exactly with kappa equal to the double 1.2
+    // (= 5404319552844595 / 2^52), the value the reference converts to mpf.
+    void compute_formula() {
+        useFormula.assign((size_t)m * nsdp, 0);
+        const __int128 K = 5404319552844595LL, P52 = ((__int128)1) << 52;
+        for (int l = 0; l < nsdp; l++) {
+            std::vector<int> cons;
+            for (int i = 0; i < m; i++)
+                if (A[i].find(l)) cons.push_back(i);
+            for (int i : cons) {
+                SpB *si = A[i].find(l);
+                long long inz = si->effect, up = inz;
+                for (int j : cons) {
+                    long long jnz = A[j].find(l)->effect;
+                    if (jnz < inz || (jnz == inz && j < i)) up += jnz;
+                }
+                __int128 n = si->n;
+                __int128 f1 = K * n * inz + n * n * n * P52 + K * up;
+                __int128 f2 = K * n * inz + K * (n + 1) * up;
+                __int128 f3 = (2 * K * inz + P52) * up;
+                int fm;
+                if (si->dense)
+                    fm = (f1 < f2) ? 1 : 2;
+                else if (f1 < f2 && f1 < f3)
+                    fm = 1;
+                else if (f2 < f3)
+                    fm = 2;
+                else
+                    fm = 3;
+                useFormula[(size_t)i * nsdp + l] = fm;
+            }
+        }
+    }
+
+    static void pack(const std::vector<T> &v, std::vector<uint32_t> &out) {
+        out.resize(v.size() * W);
+        for (size_t i = 0; i < v.size(); i++) {
+            out[i * W] = (uint32_t)v[i].e;
+            out[i * W + 1] = v[i].s;
+            for (int k = 0; k < L; k++) out[i * W + 2 + k] = v[i].m[k];
+        }
+    }
+    static T unpack1(const uint32_t *w) {
+        T x;
+        x.e = (int32_t)w[0];
+        x.s = w[1] & 1u;
+        for (int k = 0; k < L; k++) x.m[k] = w[2 + k];
+        if (x.m[L - 1] == 0) x = mpx::zero<L>();
+        return x;
+    }
+    static void unpack(const std::vector<uint32_t> &in, std::vector<T> &v) {
+        for (size_t i = 0; i < v.size(); i++) v[i] = unpack1(&in[i * W]);
+    }
+    static std::vector<uint32_t> pack1(const T &x) {
+        std::vector<T> v(1, x);
+        std::vector<uint32_t> o;
+        pack(v, o);
+        return o;
+    }
+
+    int setup_device(int device, int gemm_mode, int slices) {
+        std::vector<int> sizes(sdpStruct);
+        int rc = sdpg_ctx_create(&ctx, prm.precision, m, nsdp, sizes.data(), nlp, device);
+        if (rc) return rc;
+        auto push = [&](int con, SLS &s) -> int {
+            for (auto &blk : s.s) {
+                std::vector<uint32_t> pk;
+                if (blk.dense) {
+                    pack(blk.de, pk);
+                    int r = sdpg_set_block(ctx, con, blk.blk, blk.n * blk.n, nullptr, nullptr, pk.data(), 1);
+                    if (r) return r;
+                } else {
+                    pack(blk.v, pk);
+                    int r = sdpg_set_block(ctx, con, blk.blk, (int)blk.r.size(), blk.r.data(), blk.c.data(),
+                                           pk.data(), 0);
+                    if (r) return r;
+                }
+            }
+            if (!s.lpi.empty()) {
+                std::vector<uint32_t> pk;
+                pack(s.lpv, pk);
+                int r = sdpg_set_lp(ctx, con, (int)s.lpi.size(), s.lpi.data(), pk.data());
+                if (r) return r;
+            }
+            return 0;
+        };
+        if ((rc = push(-1, C))) return rc;
+        for (int i = 0; i < m; i++)
+            if ((rc = push(i, A[i]))) return rc;
+        std::vector<uint32_t> pb;
+        pack(b, pb);
+        if ((rc = sdpg_set_b(ctx, pb.data()))) return rc;
+        if ((rc = sdpg_set_formula(ctx, useFormula.data()))) return rc;
+        if ((rc = sdpg_commit(ctx))) return rc;
+        return sdpg_set_gemm(ctx, gemm_mode, slices);
+    }
+
+    // ----- dense linear space helpers -----
+    T dls_inner(const std::vector<T> &a, const std::vector<T> &c) const {
+        T r = mpx::zero<L>();
+        for (int l = 0; l < nsdp; l++)
+            r = r + dot<L>((int)(off[l + 1] - off[l]), &a[off[l]], 1, &c[off[l]], 1);
+        for (int q = 0; q < nlp; q++) r = r + a[off[nsdp] + q] * c[off[nsdp] + q];
+        return r;
+    }
+    // C . D for the sparse linear space C (getInnerProduct(SparseLinearSpace, DenseLinearSpace))
+    T sls_inner(SLS &S, const std::vector<T> &D) const {
+        T ret = mpx::zero<L>();
+        for (auto &s : S.s) {
+            const T *Dm = &D[off[s.blk]];
+            int n = s.n;
+            T t = mpx::zero<L>();
+            if (s.dense) {
+                t = dot<L>(n * n, s.de.data(), 1, Dm, 1);
+            } else {
+                for (size_t q = 0; q < s.r.size(); q++) {
+                    int i = s.r[q], j = s.c[q];
+                    T tmp;
+                    if (i == j)
+                        tmp = s.v[q] * Dm[i + (size_t)n * j];
+                    else
+                        tmp = (Dm[i + (size_t)n * j] + Dm[j + (size_t)n * i]) * s.v[q];
+                    t = t + tmp;
+                }
+            }
+            ret = ret + t;
+        }
+        for (size_t q = 0; q < S.lpi.size(); q++) ret = ret + S.lpv[q] * D[off[nsdp] + S.lpi[q]];
+        return ret;
+    }
+    void identity(std::vector<T> &d, const T &s) {
+        d.assign(Nd, mpx::zero<L>());
+        for (int l = 0; l < nsdp; l++) {
+            int n = sdpStruct[l];
+            for (int i = 0; i < n; i++) d[off[l] + i + (size_t)n * i] = s;
+        }
+        for (int q = 0; q < nlp; q++) d[off[nsdp] + q] = s;
+    }
+
+    // Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-171)
+    bool compute_inverse() {
+        for (int l = 0; l < nsdp; l++) {
+            int n = sdpStruct[l];
+            std::vector<T> w(X.begin() + off[l], X.begin() + off[l] + (size_t)n * n);
+            if (potrf_lower<L>(n, w.data()) != 0) return false;
+            inv_lower<L>(n, w.data(), &invCholX[off[l]]);
+            std::vector<T> wz(Z.begin() + off[l], Z.begin() + off[l] + (size_t)n * n);
+            if (potrf_lower<L>(n, wz.data()) != 0) return false;
+            inv_lower<L>(n, wz.data(), &invCholZ[off[l]]);
+            for (size_t k = 0; k < (size_t)n * n; k++) invZ[off[l] + k] = invCholZ[off[l] + k];
+            trmm_lltn<L>(n, &invCholZ[off[l]], &invZ[off[l]]);
+        }
+        const T one = M<L>::i(1);
+        for (int q = 0; q < nlp; q++) {
+            int64_t p = off[nsdp] + q;
+            if (mpx::sgn<L>(X[p]) < 0 || mpx::sgn<L>(Z[p]) < 0) return false;
+            invCholX[p] = one / sqrt(X[p]);
+            invCholZ[p] = one / sqrt(Z[p]);
+            invZ[p] = one / Z[p];
+        }
+        return true;
+    }
+
+    // Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:55-88)
+    T min_eigen(const std::vector<T> &lMat, const std::vector<T> &xMat) {
+        T mn = M<L>::d(1.0E50);
+        const T one = M<L>::i(1);
+        for (int l = 0; l < nsdp; l++) {
+            int n = sdpStruct[l];
+            const T *Lm = &lMat[off[l]], *Xm = &xMat[off[l]];
+            T v;
+            if (n > 20) {
+                v = lanczos_min_eig<L>(n, Lm, Xm);
+            } else {
+                std::vector<T> w2((size_t)n * n), w1((size_t)n * n), LT((size_t)n * n);
+                for (int j = 0; j < n; j++)
+                    for (int i = 0; i < n; i++) LT[i + (size_t)n * j] = Lm[j + (size_t)n * i];
+                gemm_nn<L>(n, Xm, LT.data(), w2.data());  // xMat * lMat^T  (multiply_tran)
+                gemm_nn<L>(n, Lm, w2.data(), w1.data());
+                v = jacobi_min_eig<L>(n, w1.data());
+            }
+            if (v < mn) mn = v;
+        }
+        for (int q = 0; q < nlp; q++) {
+            int64_t p = off[nsdp] + q;
+            T v = xMat[p] * lMat[p] * lMat[p];
+            if (v < mn) mn = v;
+        }
+        return mn;
+    }
+
+    std::string fmt(const T &v, int digits) { return mpx::format_decimal<L>(v, digits); }
+    std::string fmt_signed(const T &v, int digits) {
+        std::string s = fmt(v, digits);
+        return (s[0] == '-') ? s : "+" + s;
+    }
+
+    int get_dir() {
+        std::vector<uint32_t> buf(Nd * W), bufy((size_t)m * W);
+        int rc;
+        if ((rc = sdpg_get(ctx, SDPG_DX, buf.data()))) return rc;
+        unpack(buf, dX);
+        if ((rc = sdpg_get(ctx, SDPG_DZ, buf.data()))) return rc;
+        unpack(buf, dZ);
+        if ((rc = sdpg_get(ctx, SDPG_DY, bufy.data()))) return rc;
+        unpack(bufy, dy);
+        return 0;
+    }
+
+    int upload_point() {
+        std::vector<uint32_t> px, pz, piz, py;
+        pack(X, px);
+        pack(Z, pz);
+        pack(invZ, piz);
+        pack(y, py);
+        return sdpg_set_point(ctx, px.data(), pz.data(), piz.data(), py.data());
+    }
+
+    // Residuals::compute on the device; returns the norms
+    int residuals(T &nrp, T &nrd) {
+        std::vector<uint32_t> a(W), c(W);
+        int rc = sdpg_residuals(ctx, a.data(), c.data(), nullptr, nullptr);
+        if (rc) return rc;
+        nrp = unpack1(a.data());
+        nrd = unpack1(c.data());
+        return 0;
+    }
+
+    // Newton::Mehrotra (sdpa_newton.cpp:1375-1414) as C-ABI calls; false on Cholesky failure
+    int mehrotra(bool predictor, const T &mu, const T &beta, int phase, bool &ok) {
+        ok = true;
+        auto pm = pack1(mu), pbeta = pack1(beta);
+        int rc = sdpg_gvec(ctx, predictor ? 0 : 1, pm.data(), pbeta.data(), phase);
+        if (rc) return rc;
+        if (predictor) {
+            if ((rc = sdpg_schur(ctx))) return rc;
+            int info = 0;
+            if ((rc = sdpg_potrf(ctx, &info))) return rc;
+            if (info > 0) {
+                if (logf) fprintf(logf, "cholesky miss condition :: not positive definite :: info = %d\n", info);
+                ok = false;
+                return 0;
+            }
+        }
+        if ((rc = sdpg_potrs(ctx))) return rc;
+        if ((rc = sdpg_dz(ctx, phase))) return rc;
+        if ((rc = sdpg_dx(ctx))) return rc;
+        return get_dir();
+    }
+
+    // SolveInfo (sdpa_parts.cpp:713-830)
+    struct SolveInfo {
+        T objValPrimal, objValDual, rho, etaPrimal, etaDual;
+    };
+
+    bool phase_update(int &phase, const T &nrp, const T &nrd, const SolveInfo &si) {
+        T acc = M<L>::d(prm.epsilonDash), NONZERO = M<L>::d(1.0e-6), one = M<L>::i(1), two = M<L>::i(2);
+        int v = noINFO;
+        if (nrp <= acc) v = (nrd <= acc) ? pdFEAS : pFEAS;
+        if (v == noINFO && nrd <= acc) v = dFEAS;
+        phase = v;
+        T thr = one + NONZERO;
+        if (v == pdFEAS) {
+            T mean = (abs(si.objValPrimal) + abs(si.objValDual)) / two;
+            T gap = abs(si.objValPrimal - si.objValDual);
+            T dom = (mean < one) ? one : mean;
+            if (gap / dom <= M<L>::d(prm.epsilonStar)) {
+                phase = pdOPT;
+                return false;
+            }
+        }
+        if (v == noINFO && si.rho > thr) {
+            phase = pdINF;
+            return false;
+        }
+        if (v == pFEAS) {
+            if (si.objValPrimal <= M<L>::d(-prm.upperBound)) {
+                phase = pUNBD;
+                return false;
+            }
+            if (si.rho > thr) {
+                phase = pFEAS_dINF;
+                return false;
+            }
+        }
+        if (v == dFEAS) {
+            if (si.objValDual >= M<L>::d(-prm.lowerBound)) {
+                phase = dUNBD;
+                return false;
+            }
+            if (si.rho > thr) {
+                phase = pINF_dFEAS;
+                return false;
+            }
+        }
+        return true;
+    }
+
+    void print_iter(int it, const T &mu, const T &thP, const T &thD, const SolveInfo &si, const T &aP, const T &aD,
+                    const T &beta) {
+        if (!logf) return;
+        fprintf(logf, "%2d %s %s %s %s %s %s %s %s\n", it, fmt(mu, 2).c_str(), fmt(thD, 2).c_str(),
+                fmt(thP, 2).c_str(), fmt_signed(-si.objValDual, 3).c_str(), fmt_signed(-si.objValPrimal, 3).c_str(),
+                fmt(aD, 2).c_str(), fmt(aP, 2).c_str(), fmt(beta, 3).c_str());
+        fflush(logf);
+    }
+
+    // pinpal main loop (sdpa_main.cpp:214-398); stop_after >= 0 ends the loop early (bench warm-up)
+    int run(sdpa_amd_result *res, int stop_after) {
+        using MM = M<L>;
+        compute_formula();
+        X.assign(Nd, mpx::zero<L>());
+        const T lam = MM::d(prm.lambdaStar), one = MM::i(1);
+        identity(X, lam);
+        identity(Z, lam);
+        invCholX.assign(Nd, mpx::zero<L>());
+        invCholZ.assign(Nd, mpx::zero<L>());
+        invZ.assign(Nd, mpx::zero<L>());
+        identity(invCholX, one / sqrt(lam));
+        identity(invCholZ, one / sqrt(lam));
+        identity(invZ, one / lam);
+        y.assign(m, mpx::zero<L>());
+        dy.assign(m, mpx::zero<L>());
+        dX.assign(Nd, mpx::zero<L>());
+        dZ.assign(Nd, mpx::zero<L>());
+        int rc = upload_point();
+        if (rc) return rc;
+        T initNrp, initNrd;
+        if ((rc = residuals(initNrp, initNrd))) return rc;
+        T curNrp = initNrp, curNrd = initNrd;
+        T alphaP = MM::i(0), alphaD = MM::i(0), beta = MM::d(prm.betaStar);
+        bool reductionOn = true;
+        T mu_initial = lam * lam, mu_current = mu_initial;
+        T acc = MM::d(prm.epsilonDash);
+        T thetaP = (initNrp < acc) ? MM::i(0) : one;
+        T thetaD = (initNrd < acc) ? MM::i(0) : one;
+        SolveInfo si;
+        const T nD = MM::i(nDim), omega = MM::d(prm.omegaStar);
+        si.rho = one;
+        si.etaPrimal = omega * nD * mu_initial;
+        si.etaDual = si.etaPrimal;
+        si.objValPrimal = sls_inner(C, X);
+        si.objValDual = dot<L>(m, b.data(), 1, y.data(), 1);
+        int phase = noINFO;
+        phase_update(phase, initNrp, initNrd, si);
+        if (logf) fprintf(logf, "   mu      thetaP  thetaD  objP      objD      alphaP  alphaD  beta \n");
+        const T c09 = MM::d(0.9), alphaBD = MM::d(100.0), inv_alphaBD = one / alphaBD, xi = MM::d(3.0),
+                c095 = MM::d(0.95);
+        const T gammaStar = MM::d(prm.gammaStar), betaStar = MM::d(prm.betaStar), betaBar = MM::d(prm.betaBar),
+                nu = MM::d(2.0), c1em6 = MM::d(1.0e-6), c1em4 = MM::d(1.0e-4), five = MM::d(5.0),
+                c1em10 = MM::d(1.0e-10);
+        int pIteration = 0, status = 0;
+        double t0 = now_s();
+        while (phase_update(phase, curNrp, curNrd, si) && pIteration < prm.maxIteration) {
+            if (stop_after >= 0 && pIteration >= stop_after) break;
+            reductionOn = (phase == noINFO || phase == pFEAS || phase == dFEAS);
+            if (phase == pdFEAS)
+                beta = mpx::zero<L>();
+            else
+                beta = reductionOn ? betaBar : nu;
+            bool ok;
+            if ((rc = mehrotra(true, mu_current, beta, phase, ok))) return rc;
+            if (!ok) {
+                status = 1;
+                break;
+            }
+            // StepLength::MehrotraPredictor (sdpa_parts.cpp:362-410)
+            alphaP = c09;
+            alphaD = c09;
+            auto clamp_steps = [&]() {
+                if (phase == noINFO || phase == dFEAS) {
+                    if (alphaP > one) alphaP = one;
+                } else {
+                    T inc = sls_inner(C, dX);
+                    if (mpx::sgn<L>(inc) > 0) {
+                        if (alphaP > alphaD) alphaP = alphaD;
+                        if (alphaP > one) alphaP = one;
+                    }
+                }
+                if (phase == noINFO || phase == pFEAS) {
+                    if (alphaD > one) alphaD = one;
+                } else {
+                    T inc = dot<L>(m, b.data(), 1, dy.data(), 1);
+                    if (mpx::sgn<L>(inc) < 0) {
+                        if (alphaD > alphaP) alphaD = alphaP;
+                        if (alphaD > one) alphaD = one;
+                    }
+                }
+            };
+            clamp_steps();
+            // DirectionParameter::MehrotraCorrector (sdpa_parts.cpp:614-645)
+            {
+                T xv = dls_inner(X, dZ), uz = dls_inner(dX, Z), uv = dls_inner(dX, dZ);
+                T muT = mu_current + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nD;
+                beta = muT / mu_current;
+                if (beta < one) beta = beta * beta;
+                if (phase == pdFEAS) {
+                    if (beta < betaStar) beta = betaStar;
+                    if (beta > one) beta = one;
+                } else {
+                    if (beta < betaBar) beta = betaBar;
+                }
+            }
+            if (phase == pdFEAS && (beta > five || si.objValPrimal < si.objValDual)) break;
+            if ((rc = mehrotra(false, mu_current, beta, phase, ok))) return rc;
+            // StepLength::MehrotraCorrector (sdpa_parts.cpp:412-523)
+            {
+                T mnx = min_eigen(invCholX, dX);
+                alphaP = (-mnx > inv_alphaBD) ? -(one / mnx) : alphaBD;
+                T mnz = min_eigen(invCholZ, dZ);
+                alphaD = (-mnz > inv_alphaBD) ? -(one / mnz) : alphaBD;
+                alphaP = gammaStar * alphaP;
+                alphaD = gammaStar * alphaD;
+                clamp_steps();
+                if (reductionOn && (phase == noINFO || phase == pFEAS || phase == dFEAS)) {
+                    T xv = dls_inner(X, dZ), uz = dls_inner(dX, Z), uv = dls_inner(dX, dZ);
+                    auto thetaMaxF = [&]() {
+                        T a1 = (one - alphaP) * thetaP, a2 = (one - alphaD) * thetaD;
+                        return (a1 > a2) ? a1 : a2;
+                    };
+                    auto muNewF = [&]() { return mu_current + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nD; };
+                    T thetaMax = thetaMaxF(), muNew = muNewF();
+                    while (thetaMax * thetaMax * mu_initial > xi * muNew) {
+                        T amax = c095 * ((alphaP > alphaD) ? alphaP : alphaD);
+                        if (alphaP > amax) alphaP = amax;
+                        if (alphaD > amax) alphaD = amax;
+                        thetaMax = thetaMaxF();
+                        muNew = muNewF();
+                        if (alphaP < c1em6 && alphaD < c1em6) break;
+                    }
+                }
+                if (phase == pdFEAS) {
+                    T oD = dot<L>(m, b.data(), 1, y.data(), 1), oP = sls_inner(C, X);
+                    T iD = dot<L>(m, b.data(), 1, dy.data(), 1) * alphaD, iP = sls_inner(C, dX) * alphaP;
+                    T den = iP - iD;
+                    if (!mpx::is_zero<L>(den)) {
+                        T mr = (oD - oP) / den;
+                        if (mpx::sgn<L>(mr) > 0 && mr < one) {
+                            alphaP = alphaP * mr;
+                            alphaD = alphaD * mr;
+                        }
+                    }
+                }
+            }
+            print_iter(pIteration, mu_current, thetaP, thetaD, si, alphaP, alphaD, beta);
+            // Solutions::update (sdpa_dataset.cpp:120-143)
+            for (int64_t k = 0; k < Nd; k++) X[k] = X[k] + alphaP * dX[k];
+            for (int i = 0; i < m; i++) y[i] = y[i] + alphaD * dy[i];
+            for (int64_t k = 0; k < Nd; k++) Z[k] = Z[k] + alphaD * dZ[k];
+            if (alphaP < c1em4 && alphaD < c1em4) {
+                if (logf) fprintf(logf, "Step length is too small. \ncannot move\n");
+                status = 2;
+                break;
+            }
+            if (!compute_inverse()) {
+                if (logf) fprintf(logf, "cannot move\n");
+                status = 2;
+                break;
+            }
+            if (reductionOn) {
+                thetaP = abs((one - alphaP) * thetaP);
+                thetaD = abs((one - alphaD) * thetaD);
+            }
+            mu_current = dls_inner(X, Z) / nD;
+            if ((rc = upload_point())) return rc;
+            if ((rc = residuals(curNrp, curNrd))) return rc;
+            thetaP = (initNrp > c1em10) ? curNrp / initNrp : MM::i(0);
+            thetaD = (initNrd > c1em10) ? curNrd / initNrd : MM::i(0);
+            // SolveInfo::update(lambda, ...) (sdpa_parts.cpp:774-830)
+            {
+                si.objValPrimal = sls_inner(C, X);
+                si.objValDual = dot<L>(m, b.data(), 1, y.data(), 1);
+                T pr = thetaP, du = thetaD, x0z0 = nD * mu_initial, xz = nD * mu_current;
+                T x0z = mpx::zero<L>(), xz0 = mpx::zero<L>();
+                si.rho = mpx::zero<L>();
+                for (int l = 0; l < nsdp; l++) {
+                    int n = sdpStruct[l];
+                    for (int i = 0; i < n; i++) {
+                        x0z = x0z + lam * Z[off[l] + (size_t)i * n + i];
+                        xz0 = xz0 + lam * X[off[l] + (size_t)i * n + i];
+                    }
+                }
+                for (int q = 0; q < nlp; q++) {
+                    x0z = x0z + lam * Z[off[nsdp] + q];
+                    xz0 = xz0 + lam * X[off[nsdp] + q];
+                }
+                bool pok = curNrp <= acc, dok = curNrd <= acc;
+                if (pok && xz0 < si.etaPrimal) si.etaPrimal = xz0;
+                if (dok && x0z < si.etaDual) si.etaDual = x0z;
+                if (!pok && dok) si.rho = pr * x0z / ((pr + (one - pr) * omega) * si.etaDual + xz);
+                if (pok && !dok) si.rho = du * xz0 / ((du + (one - du) * omega) * si.etaPrimal + xz);
+                if (!pok && !dok)
+                    si.rho = (du * xz0 + pr * x0z) /
+                             ((pr * du + omega * (pr * (one - du) + (one - pr) * du)) * x0z0 + xz);
+            }
+            pIteration++;
+        }
+        double t1 = now_s();
+        if ((rc = residuals(curNrp, curNrd))) return rc;
+        print_iter(pIteration, mu_current, thetaP, thetaD, si, alphaP, alphaD, beta);
+        int rphase = phase_reverse(phase);
+        T mean = (abs(si.objValPrimal) + abs(si.objValDual)) / MM::i(2);
+        T gap = abs(si.objValPrimal - si.objValDual);
+        T relgap = (mean < one) ? gap : gap / mean;
+        if (res) {
+            memset(res, 0, sizeof(*res));
+            res->iterations = pIteration;
+            res->status = status;
+            snprintf(res->phase, sizeof res->phase, "%s", phase_str(rphase));
+            snprintf(res->objP, sizeof res->objP, "%s", fmt(-si.objValDual, 41).c_str());
+            snprintf(res->objD, sizeof res->objD, "%s", fmt(-si.objValPrimal, 41).c_str());
+            snprintf(res->relgap, sizeof res->relgap, "%s", fmt(relgap, 17).c_str());
+            snprintf(res->pfeas, sizeof res->pfeas, "%s", fmt(curNrd, 17).c_str());
+            snprintf(res->dfeas, sizeof res->dfeas, "%s", fmt(curNrp, 17).c_str());
+            res->mu = mpx::to_double<L>(mu_current);
+            double tm[8] = {0};
+            sdpg_timers(ctx, tm, 8);
+            res->t_schur_ms = tm[0];
+            res->t_chol_ms = tm[1];
+            res->t_solve_ms = tm[2];
+            res->t_gvec_ms = tm[3];
+            res->t_dxdz_ms = tm[4];
+            res->t_res_ms = tm[5];
+            res->t_total_s = t1 - t0;
+            res->limbs = L;
+        }
+        if (logf) {
+            fprintf(logf, "\nphase.value = %s\n", phase_str(rphase));
+            fprintf(logf, "   Iteration = %d\n", pIteration);
+            fprintf(logf, "          mu = %s\n", fmt(mu_current, 17).c_str());
+            fprintf(logf, "relative gap = %s\n", fmt(relgap, 17).c_str());
+            fprintf(logf, "objValPrimal = %s\n", fmt(-si.objValDual, 41).c_str());
+            fprintf(logf, "objValDual   = %s\n", fmt(-si.objValPrimal, 41).c_str());
+            fprintf(logf, "p.feas.error = %s\n", fmt(curNrd, 17).c_str());
+            fprintf(logf, "d.feas.error = %s\n", fmt(curNrp, 17).c_str());
+            if (res)
+                fprintf(logf, "Make bMat    = %.3f ms (device)\nCholesky bMat= %.3f ms (device)\nmain loop    = %.3f s\n",
+                        res->t_schur_ms, res->t_chol_ms, res->t_total_s);
+        }
+        return 0;
+    }
+};
+
+template <int L>
+static int solve_L(const RawProblem &R, const Parameter &prm, const char *log_path, int gemm_mode, int slices,
+                   int device, sdpa_amd_result *res) {
+    Solver<L> S;
+    S.prm = prm;
+    int rc = S.load(R);
+    if (rc) return rc;
+    S.compute_formula();
+    if ((rc = S.setup_device(device, gemm_mode, slices))) return rc;
+    if (log_path) S.logf = strcmp(log_path, "-") == 0 ? stdout : fopen(log_path, "w");
+    rc = S.run(res, -1);
+    if (S.logf && S.logf != stdout) fclose(S.logf);
+    return rc;
+}
+
+template <int L>
+static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int slices, int device, int warm_iters,
+                   int warmup, int steps, double *ms_schur, double *ms_chol, double *ms_wall) {
+    Solver<L> S;
+    S.prm = prm;
+    int rc = S.load(R);
+    if (rc) return rc;
+    S.compute_formula();
+    if ((rc = S.setup_device(device, gemm_mode, slices))) return rc;
+    if ((rc = S.run(nullptr, warm_iters))) return rc;
+    // the device holds the point of iteration warm_iters; time [schur + potrf] on it
+    for (int w = 0; w < warmup; w++) {
+        int info = 0;
+        if ((rc = sdpg_schur(S.ctx)) || (rc = sdpg_potrf(S.ctx, &info))) return rc;
+    }
+    sdpg_reset_timers(S.ctx);
+    double t0 = now_s();
+    for (int s = 0; s < steps; s++) {
+        int info = 0;
+        if ((rc = sdpg_schur(S.ctx)) || (rc = sdpg_potrf(S.ctx, &info))) return rc;
+    }
+    sdpg_sync(S.ctx);
+    double t1 = now_s();
+    double tm[8] = {0};
+    sdpg_timers(S.ctx, tm, 8);
+    *ms_schur = tm[0] / (steps > 0 ? steps : 1);
+    *ms_chol = tm[1] / (steps > 0 ? steps : 1);
+    *ms_wall = 1e3 * (t1 - t0) / (steps > 0 ? steps : 1);
+    return 0;
+}
+
+}  // namespace host
+}  // namespace sdpg
+
+using namespace sdpg::host;
+
+#define DISPATCH_ONE_(LL) \
+    if (L_ == LL) return CALL_(LL);
+#define DISPATCH_L(L, CALL)               \
+    {                                     \
+        const int L_ = (L);               \
+        SDPG_FOR_EACH_L(DISPATCH_ONE_)    \
+        return SDPG_EPREC;                \
+    }
+
+extern "C" int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
+                              int gemm_mode, int slices, int device, sdpa_amd_result *res) {
+    Parameter prm;
+    if (param_path && !prm.read(param_path)) return -20;
+    if (prec_override > 0) prm.precision = prec_override;
+    RawProblem R;
+    int rc = read_raw(data_path, R);
+    if (rc) return rc;
+    int L = sdpg_limbs(prm.precision);
+#define CALL_(LL) solve_L<LL>(R, prm, log_path, gemm_mode, slices, device, res)
+    DISPATCH_L(L, CALL_)
+#undef CALL_
+}
+
+namespace sdpg {
+namespace host {
+template <int L>
+static int ctx_from_file_L(const RawProblem &R, const Parameter &prm, int device, int gemm_mode, sdpg_ctx **out,
+                           int *m, long long *nd, int *formula) {
+    Solver<L> S;
+    S.prm = prm;
+    int rc = S.load(R);
+    if (rc) return rc;
+    S.compute_formula();
+    if ((rc = S.setup_device(device, gemm_mode, 0))) return rc;
+    *out = S.ctx;
+    S.ctx = nullptr;  // ownership passes to the caller
+    if (m) *m = S.m;
+    if (nd) *nd = S.Nd;
+    if (formula) std::copy(S.useFormula.begin(), S.useFormula.end(), formula);
+    return 0;
+}
+}  // namespace host
+}  // namespace sdpg
+
+extern "C" int sdpa_amd_ctx_from_file(const char *data_path, int prec_bits, int device, int gemm_mode, sdpg_ctx **out,
+                                      int *m, long long *nd, int *formula) {
+    Parameter prm;
+    prm.precision = prec_bits;
+    RawProblem R;
+    int rc = read_raw(data_path, R);
+    if (rc) return rc;
+    int L = sdpg_limbs(prec_bits);
+#define CALL_(LL) ctx_from_file_L<LL>(R, prm, device, gemm_mode, out, m, nd, formula)
+    DISPATCH_L(L, CALL_)
+#undef CALL_
+}
+
+extern "C" int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_override, int gemm_mode,
+                              int slices, int device, int warm_iters, int warmup, int steps, double *ms_schur,
+                              double *ms_chol, double *ms_wall) {
+    Parameter prm;
+    if (param_path && !prm.read(param_path)) return -20;
+    if (prec_override > 0) prm.precision = prec_override;
+    RawProblem R;
+    int rc = read_raw(data_path, R);
+    if (rc) return rc;
+    int L = sdpg_limbs(prm.precision);
+#define CALL_(LL) bench_L<LL>(R, prm, gemm_mode, slices, device, warm_iters, warmup, steps, ms_schur, ms_chol, ms_wall)
+    DISPATCH_L(L, CALL_)
+#undef CALL_
+}

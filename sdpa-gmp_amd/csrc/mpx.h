@@ -1,0 +1,427 @@
+// mpx.h — fixed-limb multiple-precision floating point, shared by host and device.
+//
+// An element is  value = (-1)^s * M * 2^(e - 32L)  with M = sum_k m[k] 2^(32k)
+// (base-2^32 limbs, little-endian) normalised so that bit 32L-1 of M is set,
+// i.e. |value| in [2^(e-1), 2^e).  Zero has M = 0, e = EZERO, s = 0.
+// L = 32-bit limbs of mantissa; the precision contract is GMP's mpf at the
+// same requested bits (reference mpf_class, sdpa_algebra.h:9-12): a request of
+// p bits maps to prec = (max(53,p)+127)/64 limbs of 64 bits (gmp-impl.h
+// __GMPF_BITS_TO_PREC), i.e. L = 2*prec (a 256-bit request runs with L = 10).
+//
+// Arithmetic rounds to nearest (ties away) from one guard limb; products
+// skip partial-product columns more than two limbs below the guard limb, which
+// bounds their error by L*2^-32 guard units.  All loops have compile-time trip
+// counts and all limb shifts go through barrel-shifter stages, so no register
+// array is ever indexed dynamically on the GPU.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPX_HD __host__ __device__ __forceinline__
+#else
+#define MPX_HD inline
+#endif
+
+namespace mpx {
+
+constexpr int32_t EZERO = -(1 << 29);
+
+template <int L>
+struct mp {
+    int32_t e;
+    uint32_t s;
+    uint32_t m[L];
+};
+
+// Requested precision (bits) -> limbs: GMP's mpf keeps prec = (max(53,p)+127)/64 limbs
+// of 64 bits per operand (results carry one more), so its guaranteed precision for a
+// p-bit request is about 64*prec - 63 bits; L = 2*prec 32-bit limbs (256 -> 10) matches it.
+inline int limbs_for_bits(int bits) {
+    int b = bits < 53 ? 53 : bits;
+    int prec = (b + 127) / 64;
+    return 2 * prec;
+}
+
+MPX_HD int clz32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return x ? __builtin_clz(x) : 32;
+#else
+    return x ? __builtin_clz(x) : 32;
+#endif
+}
+
+// (hi:lo) >> r for 0 <= r < 32
+MPX_HD uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, r);
+#else
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> r);
+#endif
+}
+
+// lo64 += a*b with carry into hi32 (one v_mad_u64_u32 + one v_addc on the GPU)
+MPX_HD void mac(uint64_t &lo, uint32_t &hi, uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32 %1, vcc, 0, %1, vcc"
+                 : "+v"(lo), "+v"(hi)
+                 : "v"(a), "v"(b)
+                 : "vcc");
+#else
+    uint64_t p = (uint64_t)a * b;
+    lo += p;
+    hi += (lo < p);
+#endif
+}
+
+template <int L>
+MPX_HD bool is_zero(const mp<L> &a) {
+    return a.m[L - 1] == 0;
+}
+
+template <int L>
+MPX_HD mp<L> zero() {
+    mp<L> r;
+    r.e = EZERO;
+    r.s = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) r.m[k] = 0;
+    return r;
+}
+
+// x >>= d bits (N limbs); bits shifted out are OR-ed into sticky
+template <int N>
+MPX_HD void shr(uint32_t (&x)[N], uint32_t d, uint32_t &sticky) {
+    if (d >= 32u * N) {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            sticky |= x[k];
+            x[k] = 0;
+        }
+        return;
+    }
+    uint32_t q = d >> 5, r = d & 31;
+#pragma unroll
+    for (int st = 16; st >= 1; st >>= 1) {
+        if (st < N) {
+            bool take = (q & st) != 0;
+            uint32_t lost = 0;
+#pragma unroll
+            for (int k = 0; k < st; k++) lost |= x[k];
+            sticky |= take ? lost : 0u;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                uint32_t v = (k + st < N) ? x[k + st] : 0u;
+                x[k] = take ? v : x[k];
+            }
+        }
+    }
+    if (r) {
+        sticky |= x[0] & ((1u << r) - 1u);
+#pragma unroll
+        for (int k = 0; k < N; k++) x[k] = funnel(k + 1 < N ? x[k + 1] : 0u, x[k], r);
+    }
+}
+
+// x <<= d bits (N limbs), d < 32N
+template <int N>
+MPX_HD void shl(uint32_t (&x)[N], uint32_t d) {
+    uint32_t q = d >> 5, r = d & 31;
+#pragma unroll
+    for (int st = 16; st >= 1; st >>= 1) {
+        if (st < N) {
+            bool take = (q & st) != 0;
+#pragma unroll
+            for (int k = N - 1; k >= 0; k--) {
+                uint32_t v = (k - st >= 0) ? x[k - st] : 0u;
+                x[k] = take ? v : x[k];
+            }
+        }
+    }
+    if (r) {
+#pragma unroll
+        for (int k = N - 1; k >= 0; k--) x[k] = funnel(x[k], k > 0 ? x[k - 1] : 0u, 32 - r);
+    }
+}
+
+template <int N>
+MPX_HD uint32_t lead_zeros(const uint32_t (&x)[N]) {
+    uint32_t lz = 0;
+    bool found = false;
+#pragma unroll
+    for (int k = N - 1; k >= 0; k--) {
+        uint32_t c = clz32(x[k]);
+        lz += found ? 0u : c;
+        found = found || (x[k] != 0);
+    }
+    return lz;
+}
+
+// Round the (L+1)-limb normalised buffer x (x[0] guard) into r.
+template <int L>
+MPX_HD void round_store(mp<L> &r, const uint32_t (&x)[L + 1], int32_t e, uint32_t s) {
+    uint32_t up = x[0] >> 31;
+    uint32_t c = up;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        uint32_t v = x[k + 1] + c;
+        c = (c && v == 0) ? 1u : 0u;
+        r.m[k] = v;
+    }
+    if (c) {  // mantissa overflowed to 2^(32L)
+        r.m[L - 1] = 0x80000000u;
+        e += 1;
+    }
+    r.e = e;
+    r.s = s;
+}
+
+template <int L>
+MPX_HD int cmp_abs(const mp<L> &a, const mp<L> &b) {
+    bool za = is_zero(a), zb = is_zero(b);
+    if (za || zb) return (za ? 0 : 1) - (zb ? 0 : 1);
+    if (a.e != b.e) return a.e < b.e ? -1 : 1;
+    int res = 0;
+#pragma unroll
+    for (int k = L - 1; k >= 0; k--) {
+        int c = (a.m[k] < b.m[k]) ? -1 : (a.m[k] > b.m[k] ? 1 : 0);
+        res = res ? res : c;
+    }
+    return res;
+}
+
+template <int L>
+MPX_HD int sgn(const mp<L> &a) {
+    return is_zero(a) ? 0 : (a.s ? -1 : 1);
+}
+
+template <int L>
+MPX_HD int cmp(const mp<L> &a, const mp<L> &b) {
+    int sa = sgn(a), sb = sgn(b);
+    if (sa != sb) return sa < sb ? -1 : 1;
+    if (sa == 0) return 0;
+    int c = cmp_abs(a, b);
+    return sa > 0 ? c : -c;
+}
+
+template <int L>
+MPX_HD mp<L> neg(mp<L> a) {
+    if (!is_zero(a)) a.s ^= 1u;
+    return a;
+}
+template <int L>
+MPX_HD mp<L> absv(mp<L> a) {
+    a.s = 0;
+    return a;
+}
+
+template <int L>
+MPX_HD mp<L> add(mp<L> a, mp<L> b) {
+    if (is_zero(a)) return b;
+    if (is_zero(b)) return a;
+    if (cmp_abs(b, a) > 0) {
+        mp<L> t = a;
+        a = b;
+        b = t;
+    }
+    uint32_t d = (uint32_t)(a.e - b.e);
+    uint32_t x[L + 1], y[L + 1];
+    x[0] = 0;
+    y[0] = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        x[k + 1] = a.m[k];
+        y[k + 1] = b.m[k];
+    }
+    uint32_t sticky = 0;
+    shr<L + 1>(y, d, sticky);
+    int32_t e = a.e;
+    if (a.s == b.s) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k <= L; k++) {
+            uint64_t t = (uint64_t)x[k] + y[k] + c;
+            x[k] = (uint32_t)t;
+            c = (uint32_t)(t >> 32);
+        }
+        if (c) {
+#pragma unroll
+            for (int k = 0; k <= L; k++) x[k] = funnel(k < L ? x[k + 1] : 1u, x[k], 1);
+            e += 1;
+        }
+    } else {
+        uint32_t br = sticky ? 1u : 0u;  // the exact subtrahend exceeds the truncated one
+#pragma unroll
+        for (int k = 0; k <= L; k++) {
+            uint64_t t = (uint64_t)x[k] - y[k] - br;
+            x[k] = (uint32_t)t;
+            br = (uint32_t)(t >> 63);
+        }
+        uint32_t lz = lead_zeros<L + 1>(x);
+        if (lz >= 32u * (L + 1)) return zero<L>();
+        shl<L + 1>(x, lz);
+        e -= (int32_t)lz;
+    }
+    mp<L> r;
+    round_store<L>(r, x, e, a.s);
+    return r;
+}
+
+template <int L>
+MPX_HD mp<L> sub(const mp<L> &a, const mp<L> &b) {
+    return add<L>(a, neg<L>(b));
+}
+
+// Product of mantissas: P[0..2L) top columns (columns below L-3 are skipped).
+template <int L>
+MPX_HD void mant_mul(const uint32_t (&a)[L], const uint32_t (&b)[L], uint32_t (&P)[2 * L]) {
+    constexpr int S0 = (L >= 3) ? L - 3 : 0;
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+#pragma unroll
+    for (int s = 0; s < 2 * L - 1; s++) {
+        if (s >= S0) {
+#pragma unroll
+            for (int p = 0; p < L; p++) {
+                int q = s - p;
+                if (q >= 0 && q < L) mac(lo, hi, a[p], b[q]);
+            }
+        }
+        P[s] = (uint32_t)lo;
+        lo = (lo >> 32) | ((uint64_t)hi << 32);
+        hi = 0;
+    }
+    P[2 * L - 1] = (uint32_t)lo;
+}
+
+template <int L>
+MPX_HD mp<L> mul(const mp<L> &a, const mp<L> &b) {
+    if (is_zero(a) || is_zero(b)) return zero<L>();
+    uint32_t P[2 * L];
+    mant_mul<L>(a.m, b.m, P);
+    uint32_t x[L + 1];
+#pragma unroll
+    for (int k = 0; k <= L; k++) x[k] = P[L - 1 + k];
+    int32_t e = a.e + b.e;
+    if (!(x[L] & 0x80000000u)) {
+#pragma unroll
+        for (int k = L; k >= 0; k--) x[k] = funnel(x[k], k > 0 ? x[k - 1] : P[L - 2 >= 0 ? L - 2 : 0], 31);
+        e -= 1;
+    }
+    mp<L> r;
+    round_store<L>(r, x, e, a.s ^ b.s);
+    return r;
+}
+
+// value * 2^k
+template <int L>
+MPX_HD mp<L> ldexp2(mp<L> a, int k) {
+    if (!is_zero(a)) a.e += k;
+    return a;
+}
+
+template <int L>
+MPX_HD mp<L> from_double(double x) {
+    mp<L> r = zero<L>();
+    if (x == 0.0 || x != x) return r;
+    uint64_t bits;
+    __builtin_memcpy(&bits, &x, 8);
+    uint32_t s = (uint32_t)(bits >> 63);
+    int ex = (int)((bits >> 52) & 0x7ff);
+    uint64_t mant = bits & ((1ull << 52) - 1);
+    if (ex == 0) {  // subnormal: normalise so that bit 52 is set
+        int lz = __builtin_clzll(mant) - 11;
+        mant <<= lz;
+        ex = 1 - lz;
+    } else {
+        mant |= (1ull << 52);
+    }
+    // value = mant * 2^(ex - 1075); mant has 53 bits -> top bit at 52
+    uint64_t top = mant << 11;  // bit 63 set
+    r.m[L - 1] = (uint32_t)(top >> 32);
+    if (L >= 2) r.m[L - 2] = (uint32_t)top;
+    r.e = ex - 1075 + 53;
+    r.s = s;
+    return r;
+}
+
+template <int L>
+MPX_HD mp<L> from_int(long long v) {
+    mp<L> r = zero<L>();
+    if (v == 0) return r;
+    uint64_t u = v < 0 ? (uint64_t)(-(v + 1)) + 1 : (uint64_t)v;
+    int lz = __builtin_clzll(u);
+    u <<= lz;
+    r.m[L - 1] = (uint32_t)(u >> 32);
+    if (L >= 2) r.m[L - 2] = (uint32_t)u;
+    r.e = 64 - lz;
+    r.s = v < 0 ? 1u : 0u;
+    return r;
+}
+
+template <int L>
+MPX_HD double to_double(const mp<L> &a) {
+    if (is_zero(a)) return 0.0;
+    uint64_t top = ((uint64_t)a.m[L - 1] << 32) | (L >= 2 ? a.m[L - 2] : 0u);
+    int ex = a.e - 64;
+    if (ex > 1100) ex = 1100;
+    if (ex < -1200) ex = -1200;
+    double v = __builtin_ldexp((double)top, ex);
+    return a.s ? -v : v;
+}
+
+// 1/b by Newton iteration x <- x + x(1 - b x)
+template <int L>
+MPX_HD mp<L> recip(const mp<L> &b) {
+    // b = B * 2^(e-32L), B in [2^(32L-1), 2^(32L)); work with bn = b * 2^-e in [1/2,1)
+    mp<L> bn = b;
+    bn.e = 0;
+    bn.s = 0;
+    mp<L> x = from_double<L>(1.0 / to_double<L>(bn));
+    const mp<L> one = from_int<L>(1);
+    constexpr int iters = (32 * L > 400) ? 5 : ((32 * L > 200) ? 4 : 3);
+#pragma unroll
+    for (int it = 0; it < iters; it++) {
+        mp<L> t = sub<L>(one, mul<L>(bn, x));
+        x = add<L>(x, mul<L>(x, t));
+    }
+    x.e -= b.e;
+    x.s = b.s;
+    return x;
+}
+
+template <int L>
+MPX_HD mp<L> div(const mp<L> &a, const mp<L> &b) {
+    if (is_zero(a)) return zero<L>();
+    mp<L> x = recip<L>(b);
+    mp<L> q = mul<L>(a, x);
+    // one correction step: q += x (a - b q)
+    mp<L> r = sub<L>(a, mul<L>(b, q));
+    return add<L>(q, mul<L>(x, r));
+}
+
+// sqrt(a), a >= 0, via y ~ 1/sqrt(a): y <- y + y(1 - a y^2)/2
+template <int L>
+MPX_HD mp<L> sqrtv(const mp<L> &a) {
+    if (is_zero(a)) return zero<L>();
+    mp<L> an = a;
+    int ev = a.e & ~1;  // even part of the exponent
+    an.e = a.e - ev;    // an in [1/2, 2)
+    an.s = 0;
+    mp<L> y = from_double<L>(1.0 / __builtin_sqrt(to_double<L>(an)));
+    const mp<L> one = from_int<L>(1);
+    constexpr int iters = (32 * L > 400) ? 5 : ((32 * L > 200) ? 4 : 3);
+#pragma unroll
+    for (int it = 0; it < iters; it++) {
+        mp<L> t = sub<L>(one, mul<L>(an, mul<L>(y, y)));
+        y = add<L>(y, ldexp2<L>(mul<L>(y, t), -1));
+    }
+    mp<L> s = mul<L>(an, y);
+    // correction: s += y (an - s^2) / 2
+    mp<L> r = sub<L>(an, mul<L>(s, s));
+    s = add<L>(s, ldexp2<L>(mul<L>(y, r), -1));
+    s.e += ev / 2;
+    return s;
+}
+
+}  // namespace mpx

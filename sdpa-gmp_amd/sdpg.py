@@ -1,0 +1,270 @@
+"""ctypes binding of the MI355X Newton engine (include/sdpg.h) and host driver
+(include/sdpa_amd.h).
+
+Mirrors the reference's Newton interface (sdpa_newton.h:49-168): a context is the
+Newton object of one problem; gvec / schur / potrf / potrs / dz / dx are the steps of
+Newton::Mehrotra (sdpa_newton.cpp:1375-1414).  Multiple-precision values cross the
+boundary in the packed format of sdpg.h (L+2 uint32 words per element); helpers here
+convert packed arrays to and from exact Python Fractions for tests.
+
+The library must have been built (build.py); there is no fallback: every entry
+point raises if the shared library or a HIP device is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from fractions import Fraction
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(PKG, "lib")
+
+SDPG_X, SDPG_Z, SDPG_INVZ, SDPG_Y, SDPG_RP, SDPG_RD, SDPG_GVEC = 1, 2, 3, 4, 5, 6, 7
+SDPG_B, SDPG_L, SDPG_DY, SDPG_DZ, SDPG_DX, SDPG_RZINV = 8, 9, 10, 11, 12, 13
+GEMM_ELEM, GEMM_PLAIN, GEMM_OZAKI = -1, 0, 1
+EZERO = -(1 << 29)
+
+# every exported symbol of include/sdpg.h and include/sdpa_amd.h
+EXPORTS = [
+    "sdpg_limbs", "sdpg_ctx_create", "sdpg_ctx_destroy", "sdpg_set_block", "sdpg_set_lp", "sdpg_set_b",
+    "sdpg_set_formula", "sdpg_commit", "sdpg_set_gemm", "sdpg_set_point", "sdpg_residuals", "sdpg_gvec",
+    "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
+    "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error",
+    "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file",
+]
+
+
+class SdpgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load libsdpg.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or os.path.join(LIBDIR, "libsdpg.so")
+    if not os.path.exists(path):
+        raise SdpgError(f"{path} not built; run sdpa-gmp_amd/build.py")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    u32p = P(ctypes.c_uint32)
+    lib.sdpg_last_error.restype = ctypes.c_char_p
+    lib.sdpg_limbs.argtypes = [ctypes.c_int]
+    lib.sdpg_ctx_destroy.restype = None
+    lib.sdpg_ctx_destroy.argtypes = [ctypes.c_void_p]
+    for name in ("sdpg_schur", "sdpg_potrs", "sdpg_dx", "sdpg_sync", "sdpg_reset_timers", "sdpg_commit"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+    lib.sdpg_potrf.argtypes = [ctypes.c_void_p, P(ctypes.c_int)]
+    lib.sdpg_dz.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.sdpg_gvec.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p, u32p, ctypes.c_int]
+    lib.sdpg_set_point.argtypes = [ctypes.c_void_p, u32p, u32p, u32p, u32p]
+    lib.sdpg_residuals.argtypes = [ctypes.c_void_p, u32p, u32p, u32p, u32p]
+    lib.sdpg_get.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
+    lib.sdpg_put.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
+    lib.sdpg_set_gemm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.sdpg_timers.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]
+    lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           P(ctypes.c_void_p), P(ctypes.c_int), P(ctypes.c_longlong),
+                                           P(ctypes.c_int)]
+    lib.sdpa_amd_solve.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    lib.sdpa_amd_bench.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+    _lib = lib
+    return lib
+
+
+def limbs_for_bits(bits: int) -> int:
+    """32-bit limbs for a precision request: GMP's mpf prec = (max(53,p)+127)//64 64-bit limbs."""
+    b = max(53, bits)
+    return 2 * ((b + 127) // 64)
+
+
+# ----------------------------------------------------------------- packed <-> exact
+
+def frac_to_words(x: Fraction, L: int) -> list:
+    """Round x to nearest (ties away) into packed words [e, s, m0..m_{L-1}]."""
+    if x == 0:
+        return [EZERO & 0xFFFFFFFF, 0] + [0] * L
+    s = 1 if x < 0 else 0
+    x = abs(x)
+    num, den = x.numerator, x.denominator
+    e = num.bit_length() - den.bit_length()
+    if Fraction(1 << e if e >= 0 else 1, 1 if e >= 0 else 1 << -e) <= x:
+        e += 1
+    # now 2^(e-1) <= x < 2^e  (adjust once more if needed)
+    while (Fraction(2) ** (e - 1)) > x:
+        e -= 1
+    while (Fraction(2) ** e) <= x:
+        e += 1
+    sh = 32 * L - e
+    q = (num << sh) // den if sh >= 0 else num // (den << -sh)
+    r2 = ((num << (sh + 1)) // den if sh + 1 >= 0 else num // (den << -(sh + 1))) & 1
+    M = q + r2
+    if M >> (32 * L):
+        M >>= 1
+        e += 1
+    return [e & 0xFFFFFFFF, s] + [(M >> (32 * k)) & 0xFFFFFFFF for k in range(L)]
+
+
+def words_to_frac(w, L: int) -> Fraction:
+    M = 0
+    for k in range(L - 1, -1, -1):
+        M = (M << 32) | int(w[2 + k])
+    if M == 0:
+        return Fraction(0)
+    e = int(np.int32(np.uint32(w[0])))
+    sh = e - 32 * L
+    v = Fraction(M << sh) if sh >= 0 else Fraction(M, 1 << -sh)
+    return -v if int(w[1]) & 1 else v
+
+
+def pack(values, L: int) -> np.ndarray:
+    """Sequence of Fractions -> packed uint32 array of shape (n, L+2)."""
+    out = np.zeros((len(values), L + 2), dtype=np.uint32)
+    for i, v in enumerate(values):
+        out[i] = frac_to_words(Fraction(v), L)
+    return out
+
+
+def unpack(arr: np.ndarray, L: int) -> list:
+    arr = np.asarray(arr, dtype=np.uint32).reshape(-1, L + 2)
+    return [words_to_frac(row, L) for row in arr]
+
+
+def repack(arr: np.ndarray, L_from: int, L_to: int) -> np.ndarray:
+    """Exact value in L_from limbs -> rounded into L_to limbs."""
+    return pack(unpack(arr, L_from), L_to)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+# ----------------------------------------------------------------- context
+
+class Context:
+    """Newton engine of one problem (the Newton object of sdpa_main.cpp:214-225)."""
+
+    def __init__(self, data_path: str, prec_bits: int = 256, device: int = 0, gemm: int = GEMM_PLAIN):
+        self.lib = load()
+        self.L = self.lib.sdpg_limbs(prec_bits)
+        h = ctypes.c_void_p()
+        m = ctypes.c_int()
+        nd = ctypes.c_longlong()
+        rc = self.lib.sdpa_amd_ctx_from_file(data_path.encode(), prec_bits, device, gemm, ctypes.byref(h),
+                                             ctypes.byref(m), ctypes.byref(nd), None)
+        if rc:
+            raise SdpgError(f"sdpa_amd_ctx_from_file rc={rc}: {self.lib.sdpg_last_error().decode()}")
+        self.h, self.m, self.nd = h, m.value, nd.value
+
+    def close(self):
+        if self.h:
+            self.lib.sdpg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise SdpgError(f"{what} rc={rc}: {self.lib.sdpg_last_error().decode()}")
+        return rc
+
+    def size_of(self, what: int) -> int:
+        if what in (SDPG_Y, SDPG_RP, SDPG_GVEC, SDPG_DY):
+            return self.m
+        if what in (SDPG_B, SDPG_L):
+            return self.m * self.m
+        return self.nd
+
+    def set_gemm(self, mode: int, slices: int = 0):
+        self._chk(self.lib.sdpg_set_gemm(self.h, mode, slices), "sdpg_set_gemm")
+
+    def set_point(self, X, Z, invZ, y):
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (X, Z, invZ, y)]
+        self._chk(self.lib.sdpg_set_point(self.h, *[_ptr(a) for a in arrs]), "sdpg_set_point")
+
+    def residuals(self):
+        outs = [np.zeros(self.L + 2, dtype=np.uint32) for _ in range(4)]
+        self._chk(self.lib.sdpg_residuals(self.h, *[_ptr(a) for a in outs]), "sdpg_residuals")
+        return outs
+
+    def gvec(self, corrector: bool, mu_words, beta_words, phase: int):
+        mu = np.ascontiguousarray(mu_words, dtype=np.uint32)
+        be = np.ascontiguousarray(beta_words, dtype=np.uint32)
+        self._chk(self.lib.sdpg_gvec(self.h, 1 if corrector else 0, _ptr(mu), _ptr(be), phase), "sdpg_gvec")
+
+    def schur(self):
+        self._chk(self.lib.sdpg_schur(self.h), "sdpg_schur")
+
+    def potrf(self) -> int:
+        info = ctypes.c_int()
+        self._chk(self.lib.sdpg_potrf(self.h, ctypes.byref(info)), "sdpg_potrf")
+        return info.value
+
+    def potrs(self):
+        self._chk(self.lib.sdpg_potrs(self.h), "sdpg_potrs")
+
+    def dz(self, phase: int):
+        self._chk(self.lib.sdpg_dz(self.h, phase), "sdpg_dz")
+
+    def dx(self):
+        self._chk(self.lib.sdpg_dx(self.h), "sdpg_dx")
+
+    def get(self, what: int) -> np.ndarray:
+        out = np.zeros((self.size_of(what), self.L + 2), dtype=np.uint32)
+        self._chk(self.lib.sdpg_get(self.h, what, _ptr(out)), "sdpg_get")
+        return out
+
+    def put(self, what: int, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint32)
+        self._chk(self.lib.sdpg_put(self.h, what, _ptr(a)), "sdpg_put")
+
+    def timers(self):
+        t = (ctypes.c_double * 8)()
+        self.lib.sdpg_timers(self.h, t, 8)
+        return list(t)
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("phase", ctypes.c_char * 16),
+                ("objP", ctypes.c_char * 96), ("objD", ctypes.c_char * 96), ("relgap", ctypes.c_char * 48),
+                ("pfeas", ctypes.c_char * 48), ("dfeas", ctypes.c_char * 48), ("mu", ctypes.c_double),
+                ("t_schur_ms", ctypes.c_double), ("t_chol_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
+                ("t_gvec_ms", ctypes.c_double), ("t_dxdz_ms", ctypes.c_double), ("t_res_ms", ctypes.c_double),
+                ("t_total_s", ctypes.c_double), ("limbs", ctypes.c_int)]
+
+
+def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path: str | None = None,
+          gemm: int = GEMM_PLAIN, slices: int = 0, device: int = 0) -> Result:
+    """Whole SDPA solve (pinpal) with the Newton system on the GPU."""
+    lib = load()
+    r = Result()
+    rc = lib.sdpa_amd_solve(data_path.encode(), param_path.encode() if param_path else None, prec,
+                            log_path.encode() if log_path else None, gemm, slices, device, ctypes.byref(r))
+    if rc:
+        raise SdpgError(f"sdpa_amd_solve rc={rc}: {lib.sdpg_last_error().decode()}")
+    return r
+
+
+def bench(data_path: str, param_path: str | None, prec: int, gemm: int, slices: int, warm_iters: int,
+          warmup: int, steps: int, device: int = 0):
+    """Per-step device ms of [schur, potrf] and wall ms on the iterate after warm_iters iterations."""
+    lib = load()
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = lib.sdpa_amd_bench(data_path.encode(), param_path.encode() if param_path else None, prec, gemm, slices,
+                            device, warm_iters, warmup, steps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    if rc:
+        raise SdpgError(f"sdpa_amd_bench rc={rc}: {lib.sdpg_last_error().decode()}")
+    return a.value, b.value, c.value
