@@ -378,60 +378,258 @@ __global__ void k_pairs_lp(int npairs, const LPPair *pairs, PV lpv, PV X, PV inv
     bacc<L>(B, m, p.i, p.j, v);
 }
 
+// ---- F1 inner products in block fixed point ----
+// Constraint values of the sparse blocks, fixed once per problem: one exponent per (i, l)
+// block (its largest entry), offset-binary L-limb integers (W = 32L - 2) per entry, and the
+// offset sum RA = sum a' (L+1 limbs).  One thread per descriptor.
+template <int L>
+__global__ void k_fix_entries(int ndesc, const Desc *desc, PV sv, uint32_t *afx, int64_t sn, int32_t *ea,
+                              uint32_t *ra) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ndesc) return;
+    Desc d = desc[t];
+    int32_t emax = mpx::EZERO;
+    if (d.kind == 1)
+        for (int e = 0; e < d.nnz; e++) {
+            mp<L> x = ldm<L>(sv, d.off + e);
+            if (!is_zero<L>(x) && x.e > emax) emax = x.e;
+        }
+    if (emax == mpx::EZERO) emax = 0;
+    uint32_t rs[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) rs[q] = 0;
+    if (d.kind == 1)
+        for (int e = 0; e < d.nnz; e++) {
+            uint32_t a[L];
+            fx_convert<L>(ldm<L>(sv, d.off + e), emax, 2, a);
+            uint32_t c = 0;
+#pragma unroll
+            for (int q = 0; q < L; q++) {
+                uint64_t s = (uint64_t)rs[q] + a[q] + c;
+                rs[q] = (uint32_t)s;
+                c = (uint32_t)(s >> 32);
+                afx[(int64_t)q * sn + d.off + e] = a[q];
+            }
+            rs[L] += c;
+        }
+    ea[t] = emax;
+#pragma unroll
+    for (int q = 0; q <= L; q++) ra[(int64_t)q * ndesc + t] = rs[q];
+}
+
+// G_s -> symmetric fixed form Gs[p,q] = G[p,q] + G[q,p] (p != q), G[p,p] on the diagonal,
+// exact integer sums of W = 32L - 3 fixed values (one exponent per slot), offset-binary.
+// One workgroup per slot (slot = blockIdx.x + s0).
+template <int L>
+__global__ __launch_bounds__(256) void k_fix_G(int n, int s0, PV G, int64_t wstride, uint32_t *gfx, int64_t gplane,
+                                               int32_t *eg) {
+    __shared__ int32_t smax[256];
+    int s = s0 + blockIdx.x;
+    int64_t base = (int64_t)s * wstride;
+    int64_t nn = (int64_t)n * n;
+    int32_t em = mpx::EZERO;
+    for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
+        mp<L> x = ldm<L>(G, base + t);
+        if (!is_zero<L>(x) && x.e > em) em = x.e;
+    }
+    smax[threadIdx.x] = em;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && smax[threadIdx.x + w] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + w];
+        __syncthreads();
+    }
+    int32_t emax = smax[0] == mpx::EZERO ? 0 : smax[0];
+    for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
+        int p = (int)(t % n), q = (int)(t / n);
+        if (p > q) continue;
+        uint32_t a[L], b[L];
+        fx_convert<L>(ldm<L>(G, base + p + (int64_t)n * q), emax, 3, a, false);
+        if (p != q) {
+            fx_convert<L>(ldm<L>(G, base + q + (int64_t)n * p), emax, 3, b, false);
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < L; k++) {
+                uint64_t sum = (uint64_t)a[k] + b[k] + c;
+                a[k] = (uint32_t)sum;
+                c = (uint32_t)(sum >> 32);
+            }
+        }
+        a[L - 1] += 0x80000000u;
+#pragma unroll
+        for (int k = 0; k < L; k++) {
+            gfx[(int64_t)k * gplane + base + p + (int64_t)n * q] = a[k];
+            gfx[(int64_t)k * gplane + base + q + (int64_t)n * p] = a[k];
+        }
+    }
+    if (threadIdx.x == 0) eg[s] = emax;
+}
+
+// value = A_j . G_i = sum over the stored entries (p, q, v) of A_j of v * Gs_i[p, q]
+// (Newton::calF1 -> getInnerProduct(Sparse, Dense), sdpa_linear.cpp:235-254), exact in
+// offset-binary fixed point: multi-limb MADs into 96-bit columns, one normalisation per pair.
+template <int L>
+__global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc,
+                                                  const int32_t *ri, const int32_t *ci, const uint32_t *afx,
+                                                  int64_t sn, const int32_t *ea, const uint32_t *ra, int ndesc,
+                                                  const uint32_t *gfx, int64_t gplane, int64_t wstride,
+                                                  const int32_t *eg, PV B, int m) {
+    constexpr int S0 = (L >= 2) ? L - 2 : 0;
+    constexpr int NC = 2 * L - 1 - S0;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    Pair p = pairs[t];
+    int di = p.j * nsdp + l;
+    Desc dj = desc[di];
+    int n = dj.n;
+    int64_t gb = (int64_t)p.slot * wstride;
+    uint64_t lo[NC];
+    uint32_t hi[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        lo[c] = 0;
+        hi[c] = 0;
+    }
+    uint32_t rg[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) rg[q] = 0;
+    for (int e = 0; e < dj.nnz; e++) {
+        int64_t ee = dj.off + e;
+        int64_t gi = gb + ri[ee] + (int64_t)n * ci[ee];
+        uint32_t a[L], g[L];
+#pragma unroll
+        for (int q = 0; q < L; q++) {
+            a[q] = afx[(int64_t)q * sn + ee];
+            g[q] = gfx[(int64_t)q * gplane + gi];
+        }
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < L; q++) {
+            uint64_t s = (uint64_t)rg[q] + g[q] + c;
+            rg[q] = (uint32_t)s;
+            c = (uint32_t)(s >> 32);
+        }
+        rg[L] += c;
+#pragma unroll
+        for (int s = S0; s <= 2 * L - 2; s++) {
+#pragma unroll
+            for (int q = 0; q < L; q++) {
+                int r = s - q;
+                if (r >= 0 && r < L) mpx::mac(lo[s - S0], hi[s - S0], a[q], g[r]);
+            }
+        }
+    }
+    uint32_t ras[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) ras[q] = ra[(int64_t)q * ndesc + di];
+    int32_t E = ea[di] + eg[p.slot] - (32 * L - 2) - (32 * L - 3);
+    mp<L> v = fx_finish<L>(lo, hi, dj.nnz, ras, rg, E);
+    bacc<L>(B, m, p.i, p.j, v);
+}
+
 // mirror: B[j,i] = B[i,j] is maintained by bacc; nothing else needed.
 
-// ---- Cholesky (Rpotrf lower) ----
-// One workgroup factors the panel of columns [k, k+nb) over rows [k, m), right-looking inside
-// the panel; diagonal sqrt and reciprocal are full-width limb operations.  info (1-based) on failure.
+// ---- Cholesky (Rpotrf lower), blocked right-looking ----
+// k_potrf_diag: one workgroup factors the nb x nb diagonal block at (k, k) (nb <= 64) and
+// forms its inverse Linv (nb x nb, leading dimension nb) by forward substitution on the
+// identity.  Pivots use a full-width reciprocal square root r = 1/sqrt(d): L_jj = d r and the
+// column below is scaled by r (the reference divides by L_jj, mplapack/Rtrsm.cpp:264-270).
+// info receives the first failing global pivot (1-based, as Rpotrf2's info,
+// mplapack/Rpotrf2.cpp:86-97); once info is set, later launches return at once.
 template <int L>
-__global__ void k_potrf_panel(PV A, int m, int k, int nb, PV rdiag, int *info) {
-    __shared__ uint32_t srec[L + 2];
+__device__ __forceinline__ mp<L> lds_ld(const uint32_t *sh, int plane, int t) {
+    mp<L> x;
+#pragma unroll
+    for (int q = 0; q < L; q++) x.m[q] = sh[q * plane + t];
+    int32_t h = (int32_t)sh[L * plane + t];
+    x.e = h >> 1;
+    x.s = (uint32_t)h & 1u;
+    return x;
+}
+template <int L>
+__device__ __forceinline__ void lds_st(uint32_t *sh, int plane, int t, const mp<L> &x) {
+#pragma unroll
+    for (int q = 0; q < L; q++) sh[q * plane + t] = x.m[q];
+    sh[L * plane + t] = (uint32_t)((x.e << 1) | (int32_t)(x.s & 1u));
+}
+
+// The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
+// serial pivot chain runs; only the finished factor, Linv and the pivots go back to HBM.
+template <int L>
+__global__ __launch_bounds__(256) void k_potrf_diag(PV A, int m, int k, int nb, PV rdiag, PV Linv, int *info) {
+    extern __shared__ uint32_t shm[];
+    __shared__ uint32_t sr[L + 2];
     __shared__ int sfail;
-    for (int j = k; j < k + nb; j++) {
-        if (threadIdx.x == 0) {
-            mp<L> d = ldm<L>(A, j + (int64_t)m * j);
+    if (*info) return;
+    const int tid = threadIdx.x;
+    const int P = nb * nb;
+    uint32_t *sA = shm, *sX = shm + (L + 1) * P;
+    auto at = [&](int i, int j) -> int64_t { return (int64_t)(k + i) + (int64_t)m * (k + j); };
+    for (int t = tid; t < P; t += blockDim.x) {
+        int i = t % nb, j = t / nb;
+        lds_st<L>(sA, P, t, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
+        lds_st<L>(sX, P, t, i == j ? from_int<L>(1) : zero<L>());
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        if (tid == 0) {
+            mp<L> d = lds_ld<L>(sA, P, j + nb * j);
             if (sgn<L>(d) <= 0) {
-                sfail = j + 1;
+                sfail = k + j + 1;
             } else {
                 sfail = 0;
-                mp<L> s = sqrtv<L>(d);
-                stm<L>(A, j + (int64_t)m * j, s);
-                mp<L> r = recip<L>(s);
-                stm<L>(rdiag, j, r);
-                srec[0] = (uint32_t)r.e;
-                srec[1] = r.s;
+                mp<L> r = mpx::rsqrt<L>(d);
+                lds_st<L>(sA, P, j + nb * j, mul<L>(d, r));
+                stm<L>(rdiag, k + j, r);
+                sr[0] = (uint32_t)r.e;
+                sr[1] = r.s;
 #pragma unroll
-                for (int q = 0; q < L; q++) srec[2 + q] = r.m[q];
+                for (int q = 0; q < L; q++) sr[2 + q] = r.m[q];
             }
         }
         __syncthreads();
         if (sfail) {
-            if (threadIdx.x == 0) *info = sfail;
+            if (tid == 0) *info = sfail;
             return;
         }
         mp<L> r;
-        r.e = (int32_t)srec[0];
-        r.s = srec[1];
+        r.e = (int32_t)sr[0];
+        r.s = sr[1];
 #pragma unroll
-        for (int q = 0; q < L; q++) r.m[q] = srec[2 + q];
-        for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
-            int64_t a = i + (int64_t)m * j;
-            stm<L>(A, a, mul<L>(r, ldm<L>(A, a)));
-        }
+        for (int q = 0; q < L; q++) r.m[q] = sr[2 + q];
+        for (int i = j + 1 + tid; i < nb; i += blockDim.x) lds_st<L>(sA, P, i + nb * j, mul<L>(r, lds_ld<L>(sA, P, i + nb * j)));
+        // inverse, row j: X[j, 0..j] *= r  (X holds the partially reduced identity)
+        for (int c = tid; c <= j; c += blockDim.x) lds_st<L>(sX, P, j + nb * c, mul<L>(r, lds_ld<L>(sX, P, j + nb * c)));
         __syncthreads();
-        // update the remaining panel columns c in (j, k+nb), rows i >= c
-        int ncols = k + nb - (j + 1);
-        int64_t work = (int64_t)ncols * (m - (j + 1));
-        for (int64_t t = threadIdx.x; t < work; t += blockDim.x) {
-            int c = j + 1 + (int)(t % ncols);
-            int i = j + 1 + (int)(t / ncols);
+        int nc = nb - (j + 1);
+        // trailing update of the block: A[i, c] -= A[i, j] A[c, j] for j < c <= i
+        for (int t = tid; t < nc * nc; t += blockDim.x) {
+            int c = j + 1 + t % nc, i = j + 1 + t / nc;
             if (i < c) continue;
-            int64_t a = i + (int64_t)m * c;
-            mp<L> v = sub<L>(ldm<L>(A, a), mul<L>(ldm<L>(A, i + (int64_t)m * j), ldm<L>(A, c + (int64_t)m * j)));
-            stm<L>(A, a, v);
+            mp<L> v = sub<L>(lds_ld<L>(sA, P, i + nb * c), mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sA, P, c + nb * j)));
+            lds_st<L>(sA, P, i + nb * c, v);
+        }
+        // forward substitution for the inverse: X[i, c] -= L[i, j] X[j, c] for i > j, c <= j
+        for (int t = tid; t < nc * (j + 1); t += blockDim.x) {
+            int i = j + 1 + t % nc, c = t / nc;
+            mp<L> v = sub<L>(lds_ld<L>(sX, P, i + nb * c), mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sX, P, j + nb * c)));
+            lds_st<L>(sX, P, i + nb * c, v);
         }
         __syncthreads();
     }
+    for (int t = tid; t < P; t += blockDim.x) {
+        int i = t % nb, j = t / nb;
+        if (i >= j) stm<L>(A, at(i, j), lds_ld<L>(sA, P, t));
+        stm<L>(Linv, t, lds_ld<L>(sX, P, t));
+    }
+}
+
+// dst[i + ldd*j] = src[i + lds*j] for an r x c sub-matrix
+template <int L>
+__global__ void k_copy2d(PV dst, int64_t doff, int ldd, PV src, int64_t soff, int lds, int r, int c) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)r * c) return;
+    int i = (int)(t % r), j = (int)(t / r);
+    stm<L>(dst, doff + i + (int64_t)ldd * j, ldm<L>(src, soff + i + (int64_t)lds * j));
 }
 
 // Lal::solveSystems: L y = g (forward) then L^T x = y (backward), single workgroup.
@@ -541,6 +739,9 @@ struct Engine : EngineBase {
     Arr Bm;
     Arr sval, dval, lpval;  // constraint data
     Arr Fw, Gw;             // Schur workspaces
+    uint32_t *afx = nullptr, *rax = nullptr, *gfx = nullptr;  // fixed-point constraint values, G slots
+    int32_t *eax = nullptr, *egx = nullptr;
+    int64_t sn_total = 0, gplane = 0;
     int64_t wstride = 0, wslots = 0;
     // index arrays
     Desc *d_desc = nullptr, *d_cdesc = nullptr;
@@ -562,8 +763,8 @@ struct Engine : EngineBase {
         int32_t *d_sp_list = nullptr;    // slots of sparse constraints, in order
         std::vector<int> sp_slots, dn_slots, g_slots;  // slots by kind; g_slots: need G
         std::vector<int> dn_cons;
-        Pair *d_pairs = nullptr;
-        int npairs = 0;
+        Pair *d_pairs = nullptr, *d_pairs_fx = nullptr;
+        int npairs = 0, npairs_fx = 0;
         int64_t *d_exp_rowptr = nullptr, *d_exp_ent = nullptr;
         int32_t *d_exp_col = nullptr, *d_exp_slot = nullptr;
     };
@@ -673,6 +874,11 @@ struct Engine : EngineBase {
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
+        {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+                lds_max = (size_t)v;
+        }
         blkoff.resize(nsdp + 1);
         Nd = 0;
         for (int l = 0; l < nsdp; l++) {
@@ -876,19 +1082,16 @@ struct Engine : EngineBase {
             for (int c = 0; c < m; c++)
                 if (desc[(size_t)(c + 1) * nsdp + l].kind) cons.push_back(c);
             P.slot_of.assign(m, -1);
-            std::vector<Pair> pairs;
+            std::vector<Pair> pairs, pairs_fx;
             std::vector<int> slot_con;
             for (int i : cons) {
                 const Desc &di = desc[(size_t)(i + 1) * nsdp + l];
                 int f = formula.empty() ? 1 : formula[(size_t)i * nsdp + l];
                 if (f < 1 || f > 3) f = 1;
-                bool needG = (f == 1);
-                if (f == 2) {
-                    for (int j : cons) {
-                        const Desc &dj = desc[(size_t)(j + 1) * nsdp + l];
-                        if (!(di.effect < dj.effect || (di.effect == dj.effect && i < j)) && dj.kind == 2) needG = true;
-                    }
-                }
+                // F2 (calF2, sdpa_newton.cpp:899-937) sums v * X[alpha,:] F[:,beta] per entry of A_j; that is
+                // A_j . (X F_i), so F2 owners also get G_i = X F_i from the batched GEMM and share the F1
+                // inner-product kernels (n^3 extra multiply-adds per F2 constraint instead of nnz_j * n per pair).
+                bool needG = (f == 1 || f == 2);
                 if (f == 1 || f == 2) {
                     P.slot_of[i] = P.nslots++;
                     slot_con.push_back(i);
@@ -901,11 +1104,16 @@ struct Engine : EngineBase {
                 for (int j : cons) {
                     const Desc &dj = desc[(size_t)(j + 1) * nsdp + l];
                     if (di.effect < dj.effect || (di.effect == dj.effect && i < j)) continue;
-                    pairs.push_back(Pair{i, j, P.slot_of[i], f});
+                    if ((f == 1 || f == 2) && dj.kind == 1)
+                        pairs_fx.push_back(Pair{i, j, P.slot_of[i], f});
+                    else
+                        pairs.push_back(Pair{i, j, P.slot_of[i], f});
                 }
             }
             P.npairs = (int)pairs.size();
             if ((rc = upload(&P.d_pairs, pairs))) return rc;
+            P.npairs_fx = (int)pairs_fx.size();
+            if ((rc = upload(&P.d_pairs_fx, pairs_fx))) return rc;
             P.dn_cons.clear();
             for (int s : P.dn_slots) P.dn_cons.push_back(slot_con[s]);
             // symmetric row expansion of sparse slots (for F = A invZ)
@@ -946,6 +1154,19 @@ struct Engine : EngineBase {
         wslots = std::max<int64_t>(maxslots, 1);
         if ((rc = aalloc(Fw, wstride * wslots))) return rc;
         if ((rc = aalloc(Gw, wstride * wslots))) return rc;
+        gplane = wstride * wslots;
+        if ((rc = dalloc(&gfx, (int64_t)L * gplane))) return rc;
+        if ((rc = dalloc(&egx, wslots))) return rc;
+        {  // fixed-point constraint values for the F1 inner products
+            int ndesc = m * nsdp;
+            sn_total = std::max<int64_t>(sn, 1);
+            if ((rc = dalloc(&afx, (int64_t)L * sn_total))) return rc;
+            if ((rc = dalloc(&eax, std::max(ndesc, 1)))) return rc;
+            if ((rc = dalloc(&rax, (int64_t)(L + 1) * std::max(ndesc, 1)))) return rc;
+            if (ndesc)
+                k_fix_entries<L><<<cdiv(ndesc, 64), 64, 0, st>>>(ndesc, d_desc, sval.pv(), afx, sn_total, eax, rax);
+            HIPCHK(hipGetLastError());
+        }
         // LP pairs
         {
             std::vector<LPPair> lpp;
@@ -1112,6 +1333,19 @@ struct Engine : EngineBase {
                     a = e;
                 }
             }
+            if (P.npairs_fx) {
+                // fixed symmetric G for the F1 slots (runs of consecutive slots)
+                size_t a = 0;
+                while (a < P.g_slots.size()) {
+                    size_t e = a + 1;
+                    while (e < P.g_slots.size() && P.g_slots[e] == P.g_slots[e - 1] + 1) e++;
+                    k_fix_G<L><<<(int)(e - a), 256, 0, st>>>(n, P.g_slots[a], Gw.pv(), wstride, gfx, gplane, egx);
+                    a = e;
+                }
+                k_pairs_fx<L><<<cdiv(P.npairs_fx, 128), 128, 0, st>>>(
+                    P.npairs_fx, P.d_pairs_fx, l, nsdp, d_desc, d_ri, d_ci, afx, sn_total, eax, rax, m * nsdp, gfx,
+                    gplane, wstride, egx, Bm.pv(), m);
+            }
             if (P.npairs)
                 k_pairs<L><<<cdiv(P.npairs, 64), 64, 0, st>>>(P.npairs, P.d_pairs, l, nsdp, d_desc, d_ri, d_ci,
                                                                sval.pv(), dval.pv(), Gw.pv(), wstride, Fw.pv(), X.pv(),
@@ -1125,35 +1359,44 @@ struct Engine : EngineBase {
     }
     std::vector<Desc> hdesc;  // host copy of the A descriptors (index c*nsdp + l)
 
+    Arr Linv, Pw;  // Cholesky workspaces: inverse of the diagonal block, panel below it
+    size_t lds_max = 65536;
     int potrf(int *info) override {
         tstart();
         HIPCHK(hipMemsetAsync(d_info, 0, sizeof(int), st));
-        // keep B (the matrix before factorisation) available: factor in place into Bm, copy of B in W? no:
-        // B is needed afterwards only by tests; they read it before calling potrf.
-        const int NBK = 32;
+        // diagonal blocks of NBK columns; the block and its inverse must fit in the LDS of one workgroup
+        int NBK = 32;
+        while (NBK > 8 && (size_t)2 * (L + 1) * NBK * NBK * 4 > lds_max) NBK /= 2;
+        const size_t shbytes = (size_t)2 * (L + 1) * NBK * NBK * 4;
         int rc;
+        if (!Linv.p) {
+            HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shbytes));
+            if ((rc = aalloc(Linv, (int64_t)NBK * NBK))) return rc;
+            if ((rc = aalloc(Pw, (int64_t)m * NBK))) return rc;
+        }
         for (int k = 0; k < m; k += NBK) {
             int nb = std::min(NBK, m - k);
-            k_potrf_panel<L><<<1, 256, 0, st>>>(Bm.pv(), m, k, nb, rdiag.pv(), d_info);
+            k_potrf_diag<L><<<1, 256, shbytes, st>>>(Bm.pv(), m, k, nb, rdiag.pv(), Linv.pv(), d_info);
             if (k + nb < m) {
                 int r = m - k - nb;
-                // A22 -= P P^T (lower), P = A[k+nb:, k:k+nb]
-                if ((rc = gemm.gemm(r, r, nb, Bm.pv(), (k + nb) + (int64_t)m * k, m, false, Bm.pv(),
-                                    (k + nb) + (int64_t)m * k, m, true, Bm.pv(), (k + nb) + (int64_t)m * (k + nb), m,
-                                    GEMM_SUB, 1, 0, 0, 0, gemm_mode, gemm_slices, true)))
+                // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N)
+                if ((rc = gemm.gemm(r, nb, nb, Bm.pv(), (k + nb) + (int64_t)m * k, m, false, Linv.pv(), 0, nb, true,
+                                    Pw.pv(), 0, r, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
+                    return rc;
+                k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(Bm.pv(), (k + nb) + (int64_t)m * k, m,
+                                                                         Pw.pv(), 0, r, r, nb);
+                // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
+                if ((rc = gemm.gemm(r, r, nb, Pw.pv(), 0, r, false, Pw.pv(), 0, r, true, Bm.pv(),
+                                    (k + nb) + (int64_t)m * (k + nb), m, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
+                                    gemm_slices, true)))
                     return rc;
             }
-            // early exit check every panel keeps failures cheap
-            int hinfo = 0;
-            HIPCHK(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (hinfo) {
-                *info = hinfo;
-                tstop(1);
-                return 0;
-            }
         }
-        *info = 0;
+        int hinfo = 0;
+        HIPCHK(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        *info = hinfo;
         return tstop(1);
     }
 

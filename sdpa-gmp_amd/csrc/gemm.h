@@ -76,6 +76,15 @@ __global__ void k_gemm_elem(GemmArgs g) {
     stm<L>(g.C, c, acc);
 }
 
+// element (r, k) of an operand seen as R rows over K (which = 0: op(A)[r, k]; 1: op(B)[k, r])
+__device__ __forceinline__ int64_t fx_at(const GemmArgs &g, int which, int b, int r, int k) {
+    return which == 0 ? idxA(g, r, k, b) : idxB(g, k, r, b);
+}
+// true when consecutive r are consecutive in memory
+__device__ __forceinline__ bool fx_rcontig(const GemmArgs &g, int which) {
+    return which == 0 ? !g.transA : g.transB;
+}
+
 }  // namespace sdpg
 
 #include "gemm_plain.h"

@@ -23,92 +23,185 @@ namespace sdpg {
 
 struct GemmArgs;
 
-// Fixed-point conversion of the "rows" of an operand.
-//  which = 0: rows i of op(A) over k;  which = 1: columns j of op(B) over k.
-//  out layout per batch instance: limb l, k, r  ->  out[((l * K) + k) * R + r]
+// Offset-binary block fixed point: finish sum_k a_k b_k from the column accumulators of
+// sum_k a'_k b'_k (columns S0 = L-2 .. 2L-2, 96 bits each), the offset sums RA = sum a',
+// RB = sum b' (L+1 limbs each) and the count K:  S = T - O (RA + RB) + K O^2, O = 2^(32L-1);
+// returns S * 2^E as a limb float (round to nearest).
 template <int L>
-__global__ void k_fix_rows(GemmArgs g, int which, int R, int nb, uint32_t *out, int32_t *eo, uint32_t *rsum) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = blockIdx.y;
-    if (r >= R || b >= nb) return;
-    const int K = g.K;
-    auto at = [&](int k) -> int64_t {
-        if (which == 0)
-            return g.offA + (int64_t)b * g.sA + (g.transA ? (k + (int64_t)g.lda * r) : (r + (int64_t)g.lda * k));
-        return g.offB + (int64_t)b * g.sB + (g.transB ? (r + (int64_t)g.ldb * k) : (k + (int64_t)g.ldb * r));
-    };
-    PV src = which == 0 ? g.A : g.B;
-    int32_t emax = mpx::EZERO;
-    for (int k = 0; k < K; k++) {
-        int32_t h = (int32_t)src.p[(int64_t)L * src.n + at(k)];
-        uint32_t top = src.p[(int64_t)(L - 1) * src.n + at(k)];
-        int32_t e = h >> 1;
-        if (top != 0 && e > emax) emax = e;
-    }
-    if (emax == mpx::EZERO) emax = 0;
-    uint32_t rs[L + 1];
+__device__ __forceinline__ mpx::mp<L> fx_finish(const uint64_t (&lo)[2 * L - 1 - ((L >= 2) ? L - 2 : 0)],
+                                                const uint32_t (&hi)[2 * L - 1 - ((L >= 2) ? L - 2 : 0)], int K,
+                                                const uint32_t (&RA)[L + 1], const uint32_t (&RB)[L + 1], int32_t E) {
+    constexpr int S0 = (L >= 2) ? L - 2 : 0;
+    constexpr int NC = 2 * L - 1 - S0;
+    constexpr int NB = 2 * L + 2 - S0;
+    int64_t buf[NB];
+    {
+        uint64_t clo = 0;
 #pragma unroll
-    for (int q = 0; q <= L; q++) rs[q] = 0;
-    int64_t base = (int64_t)b * L * K * R;
-    for (int k = 0; k < K; k++) {
-        mpx::mp<L> x = ldm<L>(src, at(k));
-        uint32_t a[L];
-        if (mpx::is_zero<L>(x)) {
-#pragma unroll
-            for (int q = 0; q < L; q++) a[q] = 0;
-        } else {
-            uint32_t t[L + 1];
-            t[0] = 0;
-#pragma unroll
-            for (int q = 0; q < L; q++) t[q + 1] = x.m[q];
-            uint32_t sticky = 0;
-            uint32_t d = 2u + (uint32_t)(emax - x.e);
-            mpx::shr<L + 1>(t, d, sticky);
-            uint32_t c = t[0] >> 31;  // round to nearest
-#pragma unroll
-            for (int q = 0; q < L; q++) {
-                uint32_t v = t[q + 1] + c;
-                c = (c && v == 0) ? 1u : 0u;
-                a[q] = v;
-            }
-            if (x.s) {  // two's complement negate
-                uint32_t br = 1;
-#pragma unroll
-                for (int q = 0; q < L; q++) {
-                    uint32_t v = ~a[q] + br;
-                    br = (br && v == 0) ? 1u : 0u;
-                    a[q] = v;
-                }
-            }
+        for (int c = 0; c < NC; c++) {
+            uint64_t s = clo + lo[c];
+            uint32_t cy = s < clo;
+            uint32_t h2 = hi[c] + cy;
+            buf[c] = (int64_t)(uint32_t)s;
+            clo = (s >> 32) | ((uint64_t)h2 << 32);
         }
-        a[L - 1] += 0x80000000u;  // offset binary: a' = a + 2^(32L-1)
+        buf[NC] = (int64_t)(uint32_t)clo;
+        buf[NC + 1] = (int64_t)(uint32_t)(clo >> 32);
+#pragma unroll
+        for (int c = NC + 2; c < NB; c++) buf[c] = 0;
+    }
+    {  // + K * 2^(64L-2): K << 30 at limb 2L-1
+        uint64_t kk = (uint64_t)(uint32_t)K << 30;
+        buf[2 * L - 1 - S0] += (int64_t)(uint32_t)kk;
+        buf[2 * L - S0] += (int64_t)(uint32_t)(kk >> 32);
+    }
+    {  // - (RA + RB) * 2^(32L-1)
+        uint32_t u[L + 2];
         uint32_t c = 0;
 #pragma unroll
-        for (int q = 0; q < L; q++) {
-            uint64_t s = (uint64_t)rs[q] + a[q] + c;
-            rs[q] = (uint32_t)s;
+        for (int q = 0; q <= L; q++) {
+            uint64_t s = (uint64_t)RA[q] + RB[q] + c;
+            u[q] = (uint32_t)s;
             c = (uint32_t)(s >> 32);
-            out[base + ((int64_t)q * K + k) * R + r] = a[q];
         }
-        rs[L] += c;
-    }
-    eo[(int64_t)b * R + r] = emax;
+        u[L + 1] = c;
 #pragma unroll
-    for (int q = 0; q <= L; q++) rsum[((int64_t)b * (L + 1) + q) * R + r] = rs[q];
+        for (int t = 0; t <= L + 2; t++) {
+            uint32_t cur = t <= L + 1 ? u[t] : 0u;
+            uint32_t prev = t >= 1 ? u[t - 1] : 0u;
+            uint32_t w = (cur << 31) | (prev >> 1);
+            int pos = L - 1 + t - S0;
+            if (pos >= 0 && pos < NB) buf[pos] -= (int64_t)w;
+        }
+    }
+    uint32_t x[NB];
+    {
+        int64_t carry = 0;
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            int64_t v = buf[c] + carry;
+            x[c] = (uint32_t)v;
+            carry = v >> 32;
+        }
+    }
+    uint32_t neg = x[NB - 1] >> 31;
+    if (neg) {
+        uint32_t br = 1;
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            uint32_t v = ~x[c] + br;
+            br = (br && v == 0) ? 1u : 0u;
+            x[c] = v;
+        }
+    }
+    mpx::mp<L> res;
+    uint32_t lz = mpx::lead_zeros<NB>(x);
+    if (lz >= 32u * NB) return mpx::zero<L>();
+    mpx::shl<NB>(x, lz);
+    uint32_t y[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) y[q] = x[NB - 1 - L + q];
+    int32_t e = 32 * (2 * L + 2) - (int32_t)lz + E;
+    mpx::round_store<L>(res, y, e, neg);
+    return res;
+}
+
+// x -> offset-binary fixed a' = round(x * 2^(W - emax)) + 2^(32L-1), with W = 32L - head
+// (|x| < 2^emax); two's complement before the offset.  Returns the L limbs in a.
+template <int L>
+__device__ __forceinline__ void fx_convert(const mpx::mp<L> &x, int32_t emax, int head, uint32_t (&a)[L],
+                                           bool offset = true) {
+    if (mpx::is_zero<L>(x)) {
+#pragma unroll
+        for (int q = 0; q < L; q++) a[q] = 0;
+    } else {
+        uint32_t t[L + 1];
+        t[0] = 0;
+#pragma unroll
+        for (int q = 0; q < L; q++) t[q + 1] = x.m[q];
+        uint32_t sticky = 0;
+        mpx::shr<L + 1>(t, (uint32_t)head + (uint32_t)(emax - x.e), sticky);
+        uint32_t c = t[0] >> 31;
+#pragma unroll
+        for (int q = 0; q < L; q++) {
+            uint32_t v = t[q + 1] + c;
+            c = (c && v == 0) ? 1u : 0u;
+            a[q] = v;
+        }
+        if (x.s) {
+            uint32_t br = 1;
+#pragma unroll
+            for (int q = 0; q < L; q++) {
+                uint32_t v = ~a[q] + br;
+                br = (br && v == 0) ? 1u : 0u;
+                a[q] = v;
+            }
+        }
+    }
+    if (offset) a[L - 1] += 0x80000000u;
+}
+
+// Element (r, k) of the operand seen as R "rows" over K: which = 0 -> op(A)[r, k],
+// which = 1 -> op(B)[k, r].  contig() tells whether r (true) or k is the memory-contiguous index,
+// so that consecutive threads walk consecutive addresses.
+
+// pass 1: per-row largest exponent (atomicMax over the row's elements)
+template <int L>
+__global__ void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int b = blockIdx.y;
+    int K = g.K;
+    if (t >= (int64_t)R * K) return;
+    bool rc = fx_rcontig(g, which);
+    int r = rc ? (int)(t % R) : (int)(t / K);
+    int k = rc ? (int)(t / R) : (int)(t % K);
+    PV src = which == 0 ? g.A : g.B;
+    int64_t idx = fx_at(g, which, b, r, k);
+    uint32_t top = src.p[(int64_t)(L - 1) * src.n + idx];
+    if (top == 0) return;
+    int32_t e = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
+    atomicMax(&eo[(int64_t)b * R + r], e);
+}
+
+// pass 2: a' = round(x * 2^(W - e_r)) + 2^(32L-1), W = 32L - 2; out[((q * K) + k) * R + r] per batch
+template <int L>
+__global__ void k_fix_conv(GemmArgs g, int which, int R, const int32_t *eo, uint32_t *out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int b = blockIdx.y;
+    int K = g.K;
+    if (t >= (int64_t)R * K) return;
+    bool rc = fx_rcontig(g, which);
+    int r = rc ? (int)(t % R) : (int)(t / K);
+    int k = rc ? (int)(t / R) : (int)(t % K);
+    PV src = which == 0 ? g.A : g.B;
+    int32_t emax = eo[(int64_t)b * R + r];
+    if (emax == mpx::EZERO) emax = 0;
+    uint32_t a[L];
+    fx_convert<L>(ldm<L>(src, fx_at(g, which, b, r, k)), emax, 2, a);
+    int64_t base = (int64_t)b * L * K * R;
+#pragma unroll
+    for (int q = 0; q < L; q++) out[base + ((int64_t)q * K + k) * R + r] = a[q];
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
 }
 
 constexpr int PT = 16;  // output tile edge
 constexpr int PK = 16;  // K chunk
 
+// The offset sums RA_i = sum_k a'_ik and RB_j = sum_k b'_kj are accumulated from the LDS tiles by
+// the threads of tile rows ty = 0 (A rows) and ty = 1 (B columns).
 template <int L>
 __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *__restrict__ af,
                                                     const uint32_t *__restrict__ bf, const int32_t *__restrict__ ea,
-                                                    const int32_t *__restrict__ eb, const uint32_t *__restrict__ ra,
-                                                    const uint32_t *__restrict__ rb, int nbA, int nbB) {
+                                                    const int32_t *__restrict__ eb, int nbA, int nbB) {
     constexpr int S0 = (L >= 2) ? L - 2 : 0;
-    constexpr int NC = 2 * L - 1 - S0;  // accumulated columns S0 .. 2L-2
+    constexpr int NC = 2 * L - 1 - S0;
     __shared__ uint32_t As[L][PK][PT];
     __shared__ uint32_t Bs[L][PK][PT];
+    __shared__ uint32_t Ssum[2][PT][L + 1];
     const int M = g.M, N = g.N, K = g.K;
     int tx = threadIdx.x % PT, ty = threadIdx.x / PT;
     int i0 = blockIdx.x * PT, j0 = blockIdx.y * PT;
@@ -125,8 +218,10 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
         lo[c] = 0;
         hi[c] = 0;
     }
+    uint32_t rsum[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) rsum[q] = 0;
     for (int k0 = 0; k0 < K; k0 += PK) {
-        // cooperative tile loads: 256 threads, L*PK*PT words per operand
         for (int w = threadIdx.x; w < L * PK * PT; w += 256) {
             int q = w / (PK * PT), rem = w % (PK * PT), kk = rem / PT, rr = rem % PT;
             int k = k0 + kk;
@@ -138,6 +233,18 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
         }
         __syncthreads();
         int kc = (K - k0) < PK ? (K - k0) : PK;
+        if (ty < 2) {
+            for (int kk = 0; kk < kc; kk++) {
+                uint32_t c = 0;
+#pragma unroll
+                for (int q = 0; q < L; q++) {
+                    uint64_t sm = (uint64_t)rsum[q] + (ty == 0 ? As[q][kk][tx] : Bs[q][kk][tx]) + c;
+                    rsum[q] = (uint32_t)sm;
+                    c = (uint32_t)(sm >> 32);
+                }
+                rsum[L] += c;
+            }
+        }
         for (int kk = 0; kk < kc; kk++) {
             uint32_t a[L], bq[L];
 #pragma unroll
@@ -156,92 +263,22 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
         }
         __syncthreads();
     }
+    if (ty < 2) {
+#pragma unroll
+        for (int q = 0; q <= L; q++) Ssum[ty][tx][q] = rsum[q];
+    }
+    __syncthreads();
     if (i >= M || j >= N) return;
     if (g.lower && i < j) return;
-    // ---- epilogue: limbs S0 .. 2L+1 of  T + K*O^2 - O*(RA_i + RB_j) ----
-    constexpr int NB = 2 * L + 2 - S0;
-    int64_t buf[NB];  // signed limb accumulators (value < 2^34 in magnitude per limb before carry)
-    {
-        uint64_t clo = 0;
-        uint32_t chi = 0;
+    uint32_t ras[L + 1], rbs[L + 1];
 #pragma unroll
-        for (int c = 0; c < NC; c++) {
-            // (chi:clo) += (hi[c]:lo[c]) ; emit low 32 bits; shift right 32
-            uint64_t s = clo + lo[c];
-            uint32_t cy = s < clo;
-            uint32_t h2 = chi + hi[c] + cy;
-            buf[c] = (int64_t)(uint32_t)s;
-            clo = (s >> 32) | ((uint64_t)h2 << 32);
-            chi = 0;
-        }
-        buf[NC] = (int64_t)(uint32_t)clo;
-        buf[NC + 1] = (int64_t)(uint32_t)(clo >> 32);
-#pragma unroll
-        for (int c = NC + 2; c < NB; c++) buf[c] = 0;
+    for (int q = 0; q <= L; q++) {
+        ras[q] = Ssum[0][tx][q];
+        rbs[q] = Ssum[1][ty][q];
     }
-    // + K * 2^(64L-2): K << 30 at limb 2L-1
-    {
-        uint64_t kk = (uint64_t)K << 30;
-        buf[2 * L - 1 - S0] += (int64_t)(uint32_t)kk;
-        buf[2 * L - S0] += (int64_t)(uint32_t)(kk >> 32);
-    }
-    // - (RA_i + RB_j) * 2^(32L-1)
-    {
-        uint32_t u[L + 2];
-        uint32_t c = 0;
-        int bAi = nbA > 1 ? b : 0, bBj = nbB > 1 ? b : 0;
-#pragma unroll
-        for (int q = 0; q <= L; q++) {
-            uint64_t s = (uint64_t)ra[((int64_t)bAi * (L + 1) + q) * M + i] + rb[((int64_t)bBj * (L + 1) + q) * N + j] + c;
-            u[q] = (uint32_t)s;
-            c = (uint32_t)(s >> 32);
-        }
-        u[L + 1] = c;
-        // shift left by 31 bits: w[t] = (u[t] << 31) | (u[t-1] >> 1), t = 0..L+2
-#pragma unroll
-        for (int t = 0; t <= L + 2; t++) {
-            uint32_t cur = t <= L + 1 ? u[t] : 0u;
-            uint32_t prev = t >= 1 ? u[t - 1] : 0u;
-            uint32_t w = (cur << 31) | (prev >> 1);
-            int pos = L - 1 + t - S0;
-            if (pos >= 0 && pos < NB) buf[pos] -= (int64_t)w;
-        }
-    }
-    // carry-normalise to 32-bit limbs (two's complement, top limb carries the sign)
-    uint32_t x[NB];
-    {
-        int64_t carry = 0;
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            int64_t v = buf[c] + carry;
-            x[c] = (uint32_t)v;
-            carry = v >> 32;  // arithmetic shift
-        }
-    }
-    uint32_t neg = x[NB - 1] >> 31;
-    if (neg) {
-        uint32_t br = 1;
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            uint32_t v = ~x[c] + br;
-            br = (br && v == 0) ? 1u : 0u;
-            x[c] = v;
-        }
-    }
-    mpx::mp<L> res;
-    uint32_t lz = mpx::lead_zeros<NB>(x);
-    if (lz >= 32u * NB) {
-        res = mpx::zero<L>();
-    } else {
-        mpx::shl<NB>(x, lz);
-        uint32_t y[L + 1];
-#pragma unroll
-        for (int q = 0; q <= L; q++) y[q] = x[NB - 1 - L + q];
-        int bAi = nbA > 1 ? b : 0, bBj = nbB > 1 ? b : 0;
-        int32_t E = ea[(int64_t)bAi * M + i] + eb[(int64_t)bBj * N + j] - 2 * (32 * L - 2);
-        int32_t e = 32 * (2 * L + 2) - (int32_t)lz + E;
-        mpx::round_store<L>(res, y, e, neg);
-    }
+    int32_t E = ea[(int64_t)bA * M + i] + eb[(int64_t)bB * N + j];
+    E = (E < -(1 << 28) ? 0 : E) - 2 * (32 * L - 2);
+    mpx::mp<L> res = fx_finish<L>(lo, hi, K, ras, rbs, E);
     int64_t cidx = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
     if (g.op == 1)
         res = mpx::sub<L>(ldm<L>(g.C, cidx), res);
@@ -252,14 +289,12 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
 
 template <int L>
 struct PlainWork {
-    uint32_t *af = nullptr, *bf = nullptr, *ra = nullptr, *rb = nullptr;
+    uint32_t *af = nullptr, *bf = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
-    size_t cap_af = 0, cap_bf = 0, cap_ra = 0, cap_rb = 0, cap_ea = 0, cap_eb = 0;
+    size_t cap_af = 0, cap_bf = 0, cap_ea = 0, cap_eb = 0;
     ~PlainWork() {
         hipFree(af);
         hipFree(bf);
-        hipFree(ra);
-        hipFree(rb);
         hipFree(ea);
         hipFree(eb);
     }
@@ -280,17 +315,25 @@ struct PlainWork {
         int nbA = g.sA ? g.batch : 1, nbB = g.sB ? g.batch : 1;
         size_t KK = (size_t)(g.K > 0 ? g.K : 1);
         int rc = 0;
+        if ((size_t)nbA * L * KK * g.M > cap_af || (size_t)nbB * L * KK * g.N > cap_bf ||
+            (size_t)nbA * g.M > cap_ea || (size_t)nbB * g.N > cap_eb)
+            if (hipStreamSynchronize(st) != hipSuccess) return -2;  // queued kernels may still read the old buffers
         rc |= grow(&af, cap_af, (size_t)nbA * L * KK * g.M);
         rc |= grow(&bf, cap_bf, (size_t)nbB * L * KK * g.N);
-        rc |= grow(&ra, cap_ra, (size_t)nbA * (L + 1) * g.M);
-        rc |= grow(&rb, cap_rb, (size_t)nbB * (L + 1) * g.N);
         rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
         rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
         if (rc) return -4;
-        k_fix_rows<L><<<dim3((g.M + 63) / 64, nbA), 64, 0, st>>>(g, 0, g.M, nbA, af, ea, ra);
-        k_fix_rows<L><<<dim3((g.N + 63) / 64, nbB), 64, 0, st>>>(g, 1, g.N, nbB, bf, eb, rb);
+        int64_t nea = (int64_t)nbA * g.M, neb = (int64_t)nbB * g.N;
+        k_fill_i32<<<(int)((nea + 255) / 256), 256, 0, st>>>(ea, nea, mpx::EZERO);
+        k_fill_i32<<<(int)((neb + 255) / 256), 256, 0, st>>>(eb, neb, mpx::EZERO);
+        int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
+        dim3 ga((unsigned)((ta + 255) / 256), nbA), gb((unsigned)((tb + 255) / 256), nbB);
+        k_fix_max<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea);
+        k_fix_max<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb);
+        k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea, af);
+        k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb, bf);
         dim3 grd((g.M + PT - 1) / PT, (g.N + PT - 1) / PT, g.batch);
-        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, bf, ea, eb, ra, rb, nbA, nbB);
+        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, bf, ea, eb, nbA, nbB);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
 };

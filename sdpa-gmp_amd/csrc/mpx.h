@@ -424,4 +424,47 @@ MPX_HD mp<L> sqrtv(const mp<L> &a) {
     return s;
 }
 
+// Change the limb count: keeps the most significant limbs (truncates when narrowing).
+template <int LO, int L>
+MPX_HD mp<LO> resize(const mp<L> &a) {
+    mp<LO> r;
+    r.e = a.e;
+    r.s = a.s;
+#pragma unroll
+    for (int k = 0; k < LO; k++) {
+        int src = k - LO + L;
+        r.m[k] = (src >= 0 && src < L) ? a.m[src] : 0u;
+    }
+    return r;
+}
+
+// one Newton step y <- y + y (1 - d y^2) / 2 for 1/sqrt(d) at LW limbs
+template <int LW>
+MPX_HD mp<LW> rsqrt_step(const mp<LW> &d, const mp<LW> &y) {
+    const mp<LW> one = from_int<LW>(1);
+    mp<LW> t = sub<LW>(one, mul<LW>(d, mul<LW>(y, y)));
+    return add<LW>(y, ldexp2<LW>(mul<LW>(y, t), -1));
+}
+
+// 1/sqrt(a), a > 0, by Newton iteration with precision doubling: a 53-bit double start,
+// then steps at 4, 8 and L limbs (about 106, 212, 424 correct bits), plus one more step
+// at L limbs when 32L > 400.  Used for the Cholesky pivots, where the chain of m
+// dependent square roots is the serial critical path.
+template <int L>
+MPX_HD mp<L> rsqrt(const mp<L> &a) {
+    mp<L> an = a;
+    int ev = a.e & ~1;
+    an.e = a.e - ev;  // an in [1/2, 2)
+    an.s = 0;
+    constexpr int LA = L < 4 ? L : 4;
+    constexpr int LB = L < 8 ? L : 8;
+    mp<LA> yA = from_double<LA>(1.0 / __builtin_sqrt(to_double<L>(an)));
+    yA = rsqrt_step<LA>(resize<LA, L>(an), yA);
+    mp<LB> yB = rsqrt_step<LB>(resize<LB, L>(an), resize<LB, LA>(yA));
+    mp<L> y = rsqrt_step<L>(an, resize<L, LB>(yB));
+    if (32 * L > 400) y = rsqrt_step<L>(an, y);
+    y.e -= ev / 2;
+    return y;
+}
+
 }  // namespace mpx
