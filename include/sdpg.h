@@ -62,6 +62,7 @@ extern "C" {
 /* GEMM forms (sdpg_set_gemm) */
 #define SDPG_GEMM_PLAIN 0  /* multi-limb integer MAD on the vector ALUs */
 #define SDPG_GEMM_OZAKI 1  /* INT8 slices on MFMA, exact INT32 accumulation */
+#define SDPG_GEMM_ELEMENT (-1) /* one output per thread, floating limb MAC per term */
 
 /* phase codes (SolveInfo::phaseType, sdpa_parts.h) */
 #define SDPG_NOINFO 0
@@ -104,6 +105,12 @@ int sdpg_set_formula(sdpg_ctx *ctx, const int *use_formula);
 /* freeze the input data: build the pair lists (sdpa_newton.cpp:1052-1054 rule) and upload */
 int sdpg_commit(sdpg_ctx *ctx);
 
+/*
+ * GEMM form used by the Schur precompute, the Cholesky panel / trailing updates
+ * and dX: SDPG_GEMM_PLAIN (limb MADs on the vector ALUs), SDPG_GEMM_OZAKI (INT8
+ * slices on MFMA), SDPG_GEMM_ELEMENT (one output per thread, the Rgemm order).
+ * slices (Ozaki only): 0 = default 4L-4, or 4L; other counts return SDPG_EARG.
+ */
 int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices);
 
 /* current point (Solutions xMat, zMat, invzMat, yVec) */
@@ -136,6 +143,17 @@ int sdpg_put(sdpg_ctx *ctx, int what, const uint32_t *in);
 int sdpg_timers(sdpg_ctx *ctx, double *ms, int n);
 int sdpg_reset_timers(sdpg_ctx *ctx);
 int sdpg_sync(sdpg_ctx *ctx);
+/*
+ * Rgemm (mplapack/Rgemm_omp.cpp:41-127 -> Rgemm_NN_omp / Rgemm_NT_omp) on host
+ * buffers, one call = one device round trip: C = op(A) op(B) (op_c 0),
+ * C - op(A) op(B) (1) or C + op(A) op(B) (2); column-major packed mp, op(X) = X^T
+ * when trans != 0.  lower != 0 writes only C[i, j] with i >= j (the Rsyrk half
+ * used by the Cholesky update, mplapack/Rsyrk.cpp:157-177).  mode / slices as
+ * sdpg_set_gemm; *ms (may be NULL) receives the device time of the product.
+ */
+int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
+               const uint32_t *B, int ldb, uint32_t *C, int ldc, int op_c, int mode, int slices, int lower,
+               double *ms);
 /* last error message of this thread */
 const char *sdpg_last_error(void);
 

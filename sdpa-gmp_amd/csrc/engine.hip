@@ -722,6 +722,7 @@ struct EngineBase {
     virtual int get(int what, uint32_t *out) = 0;
     virtual int put(int what, const uint32_t *in) = 0;
     virtual int sync() = 0;
+    virtual bool slices_ok(int s) const = 0;
 };
 
 template <int L>
@@ -1469,7 +1470,94 @@ struct Engine : EngineBase {
         HIPCHK(hipStreamSynchronize(st));
         return 0;
     }
+    bool slices_ok(int s) const override { return OzakiWork<L>::slices_ok(s) != 0; }
 };
+
+// Standalone multiple-precision GEMM on host buffers (packed words), one call = one stream:
+// C = op(A) op(B), C - op(A) op(B) or C + op(A) op(B), column-major as Rgemm.
+template <int L>
+static int rgemm_L(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
+                   const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower,
+                   double *ms) {
+    const int W = L + 2;
+    int64_t na = (int64_t)lda * (transa ? m : k), nb = (int64_t)ldb * (transb ? k : n), nc = (int64_t)ldc * n;
+    if (na <= 0) na = 1;
+    if (nb <= 0) nb = 1;
+    hipStream_t st = nullptr;
+    uint32_t *dA = nullptr, *dB = nullptr, *dC = nullptr, *stg = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SDPG_OK;
+    int64_t stg_n = std::max(na, std::max(nb, nc)) * W;
+    auto fail = [&](const char *what) {
+        set_err(std::string("sdpg_rgemm: ") + what);
+        rc = SDPG_EHIP;
+    };
+    {
+        GemmEngine<L> ge;
+        if (hipSetDevice(device) != hipSuccess) {
+            set_err("sdpg_rgemm: bad device");
+            return SDPG_EARG;
+        }
+        if (hipStreamCreate(&st) != hipSuccess) {
+            fail("stream");
+            return rc;
+        }
+        if (hipMalloc((void **)&dA, sizeof(uint32_t) * (L + 1) * na) != hipSuccess ||
+            hipMalloc((void **)&dB, sizeof(uint32_t) * (L + 1) * nb) != hipSuccess ||
+            hipMalloc((void **)&dC, sizeof(uint32_t) * (L + 1) * nc) != hipSuccess ||
+            hipMalloc((void **)&stg, sizeof(uint32_t) * stg_n) != hipSuccess) {
+            set_err("sdpg_rgemm: out of device memory");
+            rc = SDPG_ENOMEM;
+        }
+        PV pa{dA, na}, pb{dB, nb}, pc{dC, nc};
+        auto upload = [&](const uint32_t *src, int64_t cnt, PV dst) {
+            if (rc) return;
+            if (hipMemcpyAsync(stg, src, sizeof(uint32_t) * W * cnt, hipMemcpyHostToDevice, st) != hipSuccess)
+                return fail("copy");
+            k_unpack<L><<<cdiv(cnt, 256), 256, 0, st>>>(stg, dst, 0, cnt);
+            if (hipStreamSynchronize(st) != hipSuccess) fail("unpack");
+        };
+        if (k > 0) {
+            upload(A, (int64_t)lda * (transa ? m : k), pa);
+            upload(B, (int64_t)ldb * (transb ? k : n), pb);
+        }
+        upload(C, nc, pc);
+        if (!rc) {
+            ge.init(st);
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            (void)hipEventRecord(e0, st);
+            int g = ge.gemm(m, n, k, pa, 0, lda, transa != 0, pb, 0, ldb, transb != 0, pc, 0, ldc, op, 1, 0, 0, 0,
+                            mode, slices, lower != 0);
+            (void)hipEventRecord(e1, st);
+            if (g == -1) {
+                set_err("sdpg_rgemm: slice count not available at this precision");
+                rc = SDPG_EARG;
+            } else if (g == -5) {
+                set_err("sdpg_rgemm: the Ozaki form is built for up to 384-bit limbs");
+                rc = SDPG_EPREC;
+            } else if (g) {
+                fail("gemm launch");
+            }
+        }
+        if (!rc) {
+            k_pack<L><<<cdiv(nc, 256), 256, 0, st>>>(pc, 0, nc, stg);
+            if (hipMemcpyAsync(C, stg, sizeof(uint32_t) * W * nc, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                fail("download");
+            float t = 0;
+            if (!rc && ms && hipEventElapsedTime(&t, e0, e1) == hipSuccess) *ms = t;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+    (void)hipFree(stg);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
 
 }  // namespace sdpg
 
@@ -1647,6 +1735,12 @@ int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices) {
         set_err("sdpg_set_gemm: bad mode");
         return SDPG_EARG;
     }
+    if (mode == SDPG_GEMM_OZAKI && !ctx->eng->slices_ok(slices ? slices : 4 * ctx->eng->nlimb - 4)) {
+        set_err(ctx->eng->nlimb > OZ_LMAX
+                    ? "sdpg_set_gemm: the Ozaki form is built for up to 384-bit limbs"
+                    : "sdpg_set_gemm: slice count not available at this precision (use 0, 4L-4 or 4L)");
+        return ctx->eng->nlimb > OZ_LMAX ? SDPG_EPREC : SDPG_EARG;
+    }
     ctx->eng->gemm_mode = mode;
     ctx->eng->gemm_slices = slices;
     return SDPG_OK;
@@ -1709,6 +1803,35 @@ int sdpg_reset_timers(sdpg_ctx *ctx) {
 int sdpg_sync(sdpg_ctx *ctx) {
     if (!ctx || !ctx->eng) return SDPG_EARG;
     return ctx->eng->sync();
+}
+
+int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
+               const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower,
+               double *ms) {
+    if (m < 0 || n < 0 || k < 0 || op < 0 || op > 2 || mode < -1 || mode > 1 || slices < 0 || !C ||
+        (k > 0 && (!A || !B)) || ldc < std::max(1, m) || lda < std::max(1, transa ? k : m) ||
+        ldb < std::max(1, transb ? n : k)) {
+        set_err("sdpg_rgemm: bad arguments");
+        return SDPG_EARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("sdpg_rgemm: no HIP device");
+        return SDPG_ENODEV;
+    }
+    if (device < 0 || device >= ndev) {
+        set_err("sdpg_rgemm: bad device ordinal");
+        return SDPG_EARG;
+    }
+    if (m == 0 || n == 0) return SDPG_OK;
+    int L = mpx::limbs_for_bits(prec_bits);
+#define RG_(LL)                                                                                                    \
+    if (L == LL) return rgemm_L<LL>(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, \
+                                    ms);
+    SDPG_FOR_EACH_L(RG_)
+#undef RG_
+    set_err("sdpg_rgemm: unsupported precision");
+    return SDPG_EPREC;
 }
 
 }  // extern "C"

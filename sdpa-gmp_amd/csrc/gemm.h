@@ -88,6 +88,7 @@ __device__ __forceinline__ bool fx_rcontig(const GemmArgs &g, int which) {
 }  // namespace sdpg
 
 #include "gemm_plain.h"
+#include "gemm_ozaki.h"
 
 namespace sdpg {
 
@@ -95,6 +96,7 @@ template <int L>
 struct GemmEngine {
     hipStream_t st = nullptr;
     PlainWork<L> plain;
+    OzakiWork<L> ozaki;
     int init(hipStream_t s) {
         st = s;
         return 0;
@@ -114,7 +116,7 @@ struct GemmEngine {
             k_gemm_elem<L><<<grd, blk, 0, st>>>(g);
             return hipGetLastError() == hipSuccess ? 0 : -2;
         }
-        (void)slices;
+        if (mode == 1) return ozaki.run(g, st, slices);
         return plain.run(g, st);
     }
 };

@@ -31,7 +31,7 @@ EXPORTS = [
     "sdpg_limbs", "sdpg_ctx_create", "sdpg_ctx_destroy", "sdpg_set_block", "sdpg_set_lp", "sdpg_set_b",
     "sdpg_set_formula", "sdpg_commit", "sdpg_set_gemm", "sdpg_set_point", "sdpg_residuals", "sdpg_gvec",
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
-    "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error",
+    "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file",
 ]
 
@@ -69,6 +69,8 @@ def load(path: str | None = None):
     lib.sdpg_put.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
     lib.sdpg_set_gemm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.sdpg_timers.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]
+    lib.sdpg_rgemm.argtypes = [ctypes.c_int] * 7 + [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p] + \
+        [ctypes.c_int] * 5 + [P(ctypes.c_double)]
     lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            P(ctypes.c_void_p), P(ctypes.c_int), P(ctypes.c_longlong),
                                            P(ctypes.c_int)]
@@ -146,6 +148,35 @@ def repack(arr: np.ndarray, L_from: int, L_to: int) -> np.ndarray:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+# ----------------------------------------------------------------- Rgemm
+
+def rgemm(prec_bits: int, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = None, transa: bool = False,
+          transb: bool = False, op: int = 0, gemm: int = GEMM_PLAIN, slices: int = 0, lower: bool = False,
+          device: int = 0):
+    """C op= op(A) op(B) on the device.  A, B, C: packed arrays of shape (rows, cols, L+2), column-major
+    in the sense of Rgemm (element (i, j) = X[i, j]).  Returns (C, device milliseconds)."""
+    lib = load()
+    L = limbs_for_bits(prec_bits)
+    ra, ca = A.shape[:2]
+    rb, cb = B.shape[:2]
+    m, k = (ca, ra) if transa else (ra, ca)
+    k2, n = (cb, rb) if transb else (rb, cb)
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    colmaj = lambda X: np.ascontiguousarray(np.transpose(X, (1, 0, 2))).astype(np.uint32)  # noqa: E731
+    a, b = colmaj(A), colmaj(B)
+    if C is None:
+        C = np.zeros((m, n, L + 2), dtype=np.uint32)
+        C[:, :, 0] = EZERO & 0xFFFFFFFF
+    c = colmaj(C)
+    ms = ctypes.c_double(0)
+    rc = lib.sdpg_rgemm(prec_bits, device, int(transa), int(transb), m, n, k, _ptr(a), max(1, ra), _ptr(b),
+                        max(1, rb), _ptr(c), max(1, m), op, gemm, slices, int(lower), ctypes.byref(ms))
+    if rc:
+        raise SdpgError(f"sdpg_rgemm: {rc}: {lib.sdpg_last_error().decode()}")
+    return np.ascontiguousarray(np.transpose(c, (1, 0, 2))), ms.value
 
 
 # ----------------------------------------------------------------- context
