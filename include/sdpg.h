@@ -58,6 +58,20 @@ extern "C" {
 #define SDPG_DZ 11     /* (dls) */
 #define SDPG_DX 12     /* (dls) */
 #define SDPG_RZINV 13  /* r_zinv of the last sdpg_gvec call (dls) */
+#define SDPG_INVCHOLX 14 /* inv(chol(X)) of the last sdpg_factor_point, lower (dls) */
+#define SDPG_INVCHOLZ 15 /* inv(chol(Z)) of the last sdpg_factor_point, lower (dls) */
+
+/* sdpg_inner selectors: scalar inner products over the dense linear space / vectors */
+#define SDPG_IP_XZ 1    /* X . Z   */
+#define SDPG_IP_XDZ 2   /* X . dZ  */
+#define SDPG_IP_DXZ 3   /* dX . Z  */
+#define SDPG_IP_DXDZ 4  /* dX . dZ */
+#define SDPG_IP_CX 5    /* C . X   */
+#define SDPG_IP_CDX 6   /* C . dX  */
+#define SDPG_IP_BY 7    /* b . y   */
+#define SDPG_IP_BDY 8   /* b . dy  */
+#define SDPG_IP_TRX 9   /* trace X (SDP diagonals and LP entries) */
+#define SDPG_IP_TRZ 10  /* trace Z */
 
 /* GEMM forms (sdpg_set_gemm) */
 #define SDPG_GEMM_PLAIN 0  /* multi-limb integer MAD on the vector ALUs */
@@ -135,11 +149,38 @@ int sdpg_dz(sdpg_ctx *ctx, int phase);
 /* Newton::compute_DxMat incl. getSymmetrize (sdpa_newton.cpp:1358-1373) */
 int sdpg_dx(sdpg_ctx *ctx);
 
+/*
+ * Solutions::update (sdpa_dataset.cpp:120-133): X += alphaP dX, y += alphaD dy,
+ * Z += alphaD dZ on the device (the step lengths are packed mp scalars).
+ */
+int sdpg_update(sdpg_ctx *ctx, const uint32_t *alphaP, const uint32_t *alphaD);
+/*
+ * Solutions::computeInverse (sdpa_dataset.cpp:96-118): invCholX = inv(chol(X)),
+ * invCholZ = inv(chol(Z)), invZ = invCholZ^T invCholZ (Jal::getInvChol /
+ * getInvCholAndInv, sdpa_jordan.cpp:123-171), blocked Cholesky and recursive
+ * triangular inverse on the device.  *info = 1 when X or Z is not positive
+ * definite (the reference's "cannot move").
+ */
+int sdpg_factor_point(sdpg_ctx *ctx, int *info);
+/* scalar inner products of StepLength / DirectionParameter / SolveInfo (sdpa_parts.cpp:362-860) */
+int sdpg_inner(sdpg_ctx *ctx, int what, uint32_t *out);
+/*
+ * Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:55-88): smallest eigenvalue of
+ * lMat xMat lMat^T over all blocks, which 0 = (invCholX, dX), 1 = (invCholZ, dZ).
+ * Blocks > 20: Lanczos (Lal::getMinEigen, sdpa_linear.cpp:27-124) with device
+ * matrix-vector products; blocks <= 20: the product on the device, the
+ * symmetric eigenproblem (Rsyev there) on the host.
+ */
+int sdpg_min_eigen(sdpg_ctx *ctx, int which, uint32_t *out);
+
 /* copy a device quantity out / in (packed mp); sizes per selector comment */
 int sdpg_get(sdpg_ctx *ctx, int what, uint32_t *out);
 int sdpg_put(sdpg_ctx *ctx, int what, const uint32_t *in);
 
-/* accumulated device milliseconds: [0] schur, [1] potrf, [2] potrs, [3] gvec, [4] dz+dx, [5] residuals */
+/*
+ * accumulated device milliseconds: [0] schur, [1] potrf, [2] potrs, [3] gvec,
+ * [4] dz+dx, [5] residuals, [6] factor_point, [7] min_eigen
+ */
 int sdpg_timers(sdpg_ctx *ctx, double *ms, int n);
 int sdpg_reset_timers(sdpg_ctx *ctx);
 int sdpg_sync(sdpg_ctx *ctx);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -18,6 +19,7 @@
 
 #include "engine_common.h"
 #include "gemm.h"
+#include "host_la.h"
 #include "limbs.h"
 #include "mpx.h"
 #include "sdpg.h"
@@ -554,8 +556,11 @@ __device__ __forceinline__ void lds_st(uint32_t *sh, int plane, int t, const mp<
 
 // The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
 // serial pivot chain runs; only the finished factor, Linv and the pivots go back to HBM.
+// The matrix is A[off + i + ld*j]; the block starts at (k, k); rdiag[roff + k + j] receives the
+// reciprocal pivots, Linv[loff + i + nb*j] the block inverse.
 template <int L>
-__global__ __launch_bounds__(256) void k_potrf_diag(PV A, int m, int k, int nb, PV rdiag, PV Linv, int *info) {
+__global__ __launch_bounds__(256) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
+                                                    PV Linv, int64_t loff, int *info) {
     extern __shared__ uint32_t shm[];
     __shared__ uint32_t sr[L + 2];
     __shared__ int sfail;
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(PV A, int m, int k, int nb, 
     const int tid = threadIdx.x;
     const int P = nb * nb;
     uint32_t *sA = shm, *sX = shm + (L + 1) * P;
-    auto at = [&](int i, int j) -> int64_t { return (int64_t)(k + i) + (int64_t)m * (k + j); };
+    auto at = [&](int i, int j) -> int64_t { return off + (int64_t)(k + i) + (int64_t)ld * (k + j); };
     for (int t = tid; t < P; t += blockDim.x) {
         int i = t % nb, j = t / nb;
         lds_st<L>(sA, P, t, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(PV A, int m, int k, int nb, 
                 sfail = 0;
                 mp<L> r = mpx::rsqrt<L>(d);
                 lds_st<L>(sA, P, j + nb * j, mul<L>(d, r));
-                stm<L>(rdiag, k + j, r);
+                stm<L>(rdiag, roff + k + j, r);
                 sr[0] = (uint32_t)r.e;
                 sr[1] = r.s;
 #pragma unroll
@@ -619,7 +624,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(PV A, int m, int k, int nb, 
     for (int t = tid; t < P; t += blockDim.x) {
         int i = t % nb, j = t / nb;
         if (i >= j) stm<L>(A, at(i, j), lds_ld<L>(sA, P, t));
-        stm<L>(Linv, t, lds_ld<L>(sX, P, t));
+        stm<L>(Linv, loff + t, lds_ld<L>(sX, P, t));
     }
 }
 
@@ -687,6 +692,95 @@ __global__ void k_lp_triple(PV out, PV A, PV Bm, PV C, int64_t base, int nlp) {
     stm<L>(out, base + t, v);
 }
 
+// ---- reductions, matrix-vector products and point factorisation (Solutions / Jal / Lal) ----
+
+// block-tree sum of one mp value per thread (blockDim.x a power of two, LDS planes of blockDim.x)
+template <int L>
+__device__ mp<L> block_sum(mp<L> v, uint32_t *sh) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    lds_st<L>(sh, nt, t, v);
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (t < h) lds_st<L>(sh, nt, t, add<L>(lds_ld<L>(sh, nt, t), lds_ld<L>(sh, nt, t + h)));
+        __syncthreads();
+    }
+    return lds_ld<L>(sh, nt, 0);
+}
+
+// part[blockIdx.x] = sum over this block's share of x_i * y_i (y.p == nullptr: of x_i); when idx is
+// given, element t is x[xo + idx[t]]
+template <int L>
+__global__ __launch_bounds__(256) void k_dot_part(PV x, int64_t xo, PV y, int64_t yo, const int64_t *idx, int64_t n,
+                                                  PV part) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    mp<L> acc = zero<L>();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t ix = idx ? idx[i] : i;
+        mp<L> a = ldm<L>(x, xo + ix);
+        acc = add<L>(acc, y.p ? mul<L>(a, ldm<L>(y, yo + i)) : a);
+    }
+    mp<L> r = block_sum<L>(acc, sh);
+    if (threadIdx.x == 0) stm<L>(part, blockIdx.x, r);
+}
+
+// y[yo + j] = sum_i A[ao + i + n j] x[xo + i] over i in [0, n) (tri 0), [j, n) (tri 1) or [0, j] (tri 2);
+// one wave per output
+template <int L>
+__global__ __launch_bounds__(256) void k_gemv_t(PV A, int64_t ao, int n, int tri, PV x, int64_t xo, PV y, int64_t yo) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + w;
+    mp<L> acc = zero<L>();
+    if (j < n) {
+        int i0 = tri == 1 ? j : 0, i1 = tri == 2 ? j + 1 : n;
+        for (int i = i0 + lane; i < i1; i += 64)
+            acc = add<L>(acc, mul<L>(ldm<L>(A, ao + i + (int64_t)n * j), ldm<L>(x, xo + i)));
+    }
+    uint32_t *s = sh;  // per-wave tree inside the block planes
+    lds_st<L>(s, 256, threadIdx.x, acc);
+    __syncthreads();
+    for (int h = 32; h >= 1; h >>= 1) {
+        if (lane < h)
+            lds_st<L>(s, 256, threadIdx.x, add<L>(lds_ld<L>(s, 256, threadIdx.x), lds_ld<L>(s, 256, threadIdx.x + h)));
+        __syncthreads();
+    }
+    if (lane == 0 && j < n) stm<L>(y, yo + j, lds_ld<L>(s, 256, threadIdx.x));
+}
+
+// dst[do + j + n i] = src[so + i + n j]
+template <int L>
+__global__ void k_transpose(PV src, int64_t so, PV dst, int64_t dof, int n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * n) return;
+    int i = (int)(t % n), j = (int)(t / n);
+    stm<L>(dst, dof + j + (int64_t)n * i, ldm<L>(src, so + t));
+}
+
+// LP part of Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:140-171): 1/sqrt(x), 1/sqrt(z), 1/z
+template <int L>
+__global__ void k_lp_inverse(PV X, PV Z, PV icx, PV icz, PV iz, int64_t base, int nlp, int *info) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlp) return;
+    mp<L> x = ldm<L>(X, base + t), z = ldm<L>(Z, base + t);
+    if (sgn<L>(x) < 0 || sgn<L>(z) < 0) {
+        atomicMax(info, 1);
+        return;
+    }
+    mp<L> rx = mpx::rsqrt<L>(x), rz = mpx::rsqrt<L>(z);
+    stm<L>(icx, base + t, rx);
+    stm<L>(icz, base + t, rz);
+    stm<L>(iz, base + t, mul<L>(rz, rz));
+}
+
+// LP part of Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:80-86): x * l * l per entry
+template <int L>
+__global__ void k_lp_triple_val(PV Xm, PV Lm, int64_t base, int nlp, PV out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlp) return;
+    mp<L> l = ldm<L>(Lm, base + t);
+    stm<L>(out, t, mul<L>(mul<L>(ldm<L>(Xm, base + t), l), l));
+}
+
 // ------------------------------------------------------------------ Engine
 
 struct EngineBase {
@@ -719,6 +813,10 @@ struct EngineBase {
     virtual int potrs() = 0;
     virtual int dz(int phase) = 0;
     virtual int dx() = 0;
+    virtual int factor_point(int *info) = 0;
+    virtual int update(const uint32_t *alphaP, const uint32_t *alphaD) = 0;
+    virtual int inner(int what, uint32_t *out) = 0;
+    virtual int min_eigen(int which, uint32_t *out) = 0;
     virtual int get(int what, uint32_t *out) = 0;
     virtual int put(int what, const uint32_t *in) = 0;
     virtual int sync() = 0;
@@ -1071,6 +1169,7 @@ struct Engine : EngineBase {
         if ((rc = aalloc(Bm, (int64_t)m * m))) return rc;
         if ((rc = dalloc(&d_info, 1))) return rc;
         for (Arr *a : dls) zero_arr(*a, 0, Nd);
+        if ((rc = alloc_factor_space())) return rc;
         if (!bvec.empty() && (rc = put_packed(b, 0, bvec.data(), m))) return rc;
         // ---- Schur plan: pair rule (sdpa_newton.cpp:1052-1054) and workspace slots ----
         plan.assign(nsdp, BlockPlan());
@@ -1360,45 +1459,320 @@ struct Engine : EngineBase {
     }
     std::vector<Desc> hdesc;  // host copy of the A descriptors (index c*nsdp + l)
 
-    Arr Linv, Pw;  // Cholesky workspaces: inverse of the diagonal block, panel below it
+    // Cholesky workspaces: inverses of every diagonal block (LinvB for the Schur matrix, LinvW for
+    // the blocks of X and Z), the panel below the current block, pivots of X / Z blocks
+    Arr LinvB, LinvW, Pw, rdW;
+    Arr icX, icZ, Wc, Wt;        // inverse Cholesky factors of X and Z, factor / inverse workspaces
+    Arr lq, lqold, lr, lw, ltmp;  // Lanczos vectors
+    int64_t *d_diag_idx = nullptr;
+    int64_t ndiag = 0, maxn = 1;
     size_t lds_max = 65536;
-    int potrf(int *info) override {
-        tstart();
-        HIPCHK(hipMemsetAsync(d_info, 0, sizeof(int), st));
-        // diagonal blocks of NBK columns; the block and its inverse must fit in the LDS of one workgroup
-        int NBK = 32;
+    int NBK = 32;
+    size_t potrf_shbytes = 0;
+
+    int alloc_factor_space() {
+        NBK = 32;
         while (NBK > 8 && (size_t)2 * (L + 1) * NBK * NBK * 4 > lds_max) NBK /= 2;
-        const size_t shbytes = (size_t)2 * (L + 1) * NBK * NBK * 4;
+        potrf_shbytes = (size_t)2 * (L + 1) * NBK * NBK * 4;
+        HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)potrf_shbytes));
+        maxn = 1;
+        for (int l = 0; l < nsdp; l++) maxn = std::max<int64_t>(maxn, sizes[l]);
         int rc;
-        if (!Linv.p) {
-            HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)shbytes));
-            if ((rc = aalloc(Linv, (int64_t)NBK * NBK))) return rc;
-            if ((rc = aalloc(Pw, (int64_t)m * NBK))) return rc;
-        }
-        for (int k = 0; k < m; k += NBK) {
-            int nb = std::min(NBK, m - k);
-            k_potrf_diag<L><<<1, 256, shbytes, st>>>(Bm.pv(), m, k, nb, rdiag.pv(), Linv.pv(), d_info);
-            if (k + nb < m) {
-                int r = m - k - nb;
+        int64_t nbl_m = (m + NBK - 1) / NBK, nbl_n = (maxn + NBK - 1) / NBK;
+        if ((rc = aalloc(LinvB, nbl_m * NBK * NBK))) return rc;
+        if ((rc = aalloc(LinvW, nbl_n * NBK * NBK))) return rc;
+        if ((rc = aalloc(Pw, std::max<int64_t>(m, maxn) * NBK))) return rc;
+        if ((rc = aalloc(rdW, maxn))) return rc;
+        if ((rc = aalloc(icX, Nd)) || (rc = aalloc(icZ, Nd))) return rc;
+        if ((rc = aalloc(Wc, maxn * maxn)) || (rc = aalloc(Wt, maxn * maxn))) return rc;
+        Arr *lv[] = {&lq, &lqold, &lr, &lw, &ltmp};
+        for (Arr *v : lv)
+            if ((rc = aalloc(*v, maxn))) return rc;
+        std::vector<int64_t> di;
+        for (int l = 0; l < nsdp; l++)
+            for (int i = 0; i < sizes[l]; i++) di.push_back(blkoff[l] + i + (int64_t)sizes[l] * i);
+        for (int q = 0; q < nlp; q++) di.push_back(blkoff[nsdp] + q);
+        ndiag = (int64_t)di.size();
+        if ((rc = upload(&d_diag_idx, di))) return rc;
+        return 0;
+    }
+
+    // Rpotrf("Lower") of the n x n matrix A[off + i + ld j], blocked right-looking: per block of NBK
+    // columns a one-workgroup factorisation of the diagonal block (which also forms its inverse,
+    // kept in linv at block index k / NBK), the panel below it as a GEMM with that inverse, and the
+    // lower-triangular trailing update as a GEMM.  Pivot failures set d_info (checked by the caller).
+    int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
+        int rc;
+        for (int k = 0; k < n; k += NBK) {
+            int nb = std::min(NBK, n - k);
+            int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
+            k_potrf_diag<L><<<1, 256, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
+            if (k + nb < n) {
+                int r = n - k - nb;
+                int64_t a21 = off + (k + nb) + (int64_t)ld * k;
                 // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N)
-                if ((rc = gemm.gemm(r, nb, nb, Bm.pv(), (k + nb) + (int64_t)m * k, m, false, Linv.pv(), 0, nb, true,
-                                    Pw.pv(), 0, r, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
+                if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, Pw.pv(), 0, r, GEMM_SET, 1,
+                                    0, 0, 0, gemm_mode, gemm_slices)))
                     return rc;
-                k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(Bm.pv(), (k + nb) + (int64_t)m * k, m,
-                                                                         Pw.pv(), 0, r, r, nb);
+                k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(A, a21, ld, Pw.pv(), 0, r, r, nb);
                 // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
-                if ((rc = gemm.gemm(r, r, nb, Pw.pv(), 0, r, false, Pw.pv(), 0, r, true, Bm.pv(),
-                                    (k + nb) + (int64_t)m * (k + nb), m, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
+                if ((rc = gemm.gemm(r, r, nb, Pw.pv(), 0, r, false, Pw.pv(), 0, r, true, A,
+                                    off + (k + nb) + (int64_t)ld * (k + nb), ld, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
                                     gemm_slices, true)))
                     return rc;
             }
         }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int reset_info() {
+        k_fill_i32<<<1, 1, 0, st>>>(d_info, 1, 0);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+    int read_info(int *info) {
         int hinfo = 0;
         HIPCHK(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         *info = hinfo;
+        return 0;
+    }
+
+    int potrf(int *info) override {
+        tstart();
+        int rc;
+        if ((rc = reset_info())) return rc;
+        if ((rc = chol(Bm.pv(), 0, m, m, rdiag.pv(), 0, LinvB))) return rc;
+        if ((rc = read_info(info))) return rc;
         return tstop(1);
+    }
+
+    // Lal::getInvLowTriangularMatrix: out[oo + i + ldo j] = inv(L) for the Cholesky factor in
+    // Lm[lo + i + ld j] whose diagonal-block inverses are in linv (block index s / NBK).  Recursive
+    // on halves: inv([A 0; C B]) = [inv(A) 0; -inv(B) C inv(A)  inv(B)].  The strictly upper part of
+    // out must be zero on entry.
+    int trtri(PV Lm, int64_t lo, int ld, int s, int len, Arr &linv, PV out, int64_t oo, int ldo) {
+        int rc;
+        if (len <= NBK) {
+            k_copy2d<L><<<cdiv((int64_t)len * len, 256), 256, 0, st>>>(out, oo + s + (int64_t)ldo * s, ldo, linv.pv(),
+                                                                     (int64_t)(s / NBK) * NBK * NBK, len, len, len);
+            return 0;
+        }
+        int nblk = (len + NBK - 1) / NBK;
+        int n1 = ((nblk + 1) / 2) * NBK, n2 = len - n1;
+        if ((rc = trtri(Lm, lo, ld, s, n1, linv, out, oo, ldo))) return rc;
+        if ((rc = trtri(Lm, lo, ld, s + n1, n2, linv, out, oo, ldo))) return rc;
+        // Wt = C inv(A)   (n2 x n1)
+        if ((rc = gemm.gemm(n2, n1, n1, Lm, lo + (s + n1) + (int64_t)ld * s, ld, false, out, oo + s + (int64_t)ldo * s,
+                            ldo, false, Wt.pv(), 0, n2, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
+            return rc;
+        // out21 = 0 - inv(B) Wt
+        return gemm.gemm(n2, n1, n2, out, oo + (s + n1) + (int64_t)ldo * (s + n1), ldo, false, Wt.pv(), 0, n2, false,
+                         out, oo + (s + n1) + (int64_t)ldo * s, ldo, GEMM_SUB, 1, 0, 0, 0, gemm_mode, gemm_slices);
+    }
+
+    // Solutions::computeInverse (sdpa_dataset.cpp:96-118): invCholX = inv(chol(X)),
+    // invCholZ = inv(chol(Z)), invZ = invCholZ^T invCholZ (Jal::getInvChol / getInvCholAndInv,
+    // sdpa_jordan.cpp:123-171); *info = 1 when X or Z is not positive definite.
+    int factor_point(int *info) override {
+        tstart();
+        int rc;
+        if ((rc = reset_info())) return rc;
+        for (int l = 0; l < nsdp; l++) {
+            int n = sizes[l];
+            int64_t o = blkoff[l];
+            for (int which = 0; which < 2; which++) {
+                Arr &src = which ? Z : X;
+                Arr &ic = which ? icZ : icX;
+                k_copy2d<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(Wc.pv(), 0, n, src.pv(), o, n, n, n);
+                if ((rc = chol(Wc.pv(), 0, n, n, rdW.pv(), 0, LinvW))) return rc;
+                zero_arr(ic, o, (int64_t)n * n);
+                if ((rc = trtri(Wc.pv(), 0, n, 0, n, LinvW, ic.pv(), o, n))) return rc;
+            }
+            if ((rc = gemm.gemm(n, n, n, icZ.pv(), o, n, true, icZ.pv(), o, n, false, invZ.pv(), o, n, GEMM_SET, 1, 0,
+                                0, 0, gemm_mode, gemm_slices)))
+                return rc;
+        }
+        if (nlp)
+            k_lp_inverse<L><<<cdiv(nlp, 128), 128, 0, st>>>(X.pv(), Z.pv(), icX.pv(), icZ.pv(), invZ.pv(),
+                                                            blkoff[nsdp], nlp, d_info);
+        HIPCHK(hipGetLastError());
+        int hinfo = 0;
+        if ((rc = read_info(&hinfo))) return rc;
+        *info = hinfo ? 1 : 0;
+        return tstop(6);
+    }
+
+    // Solutions::update (sdpa_dataset.cpp:120-133): X += alphaP dX, y += alphaD dy, Z += alphaD dZ
+    int update(const uint32_t *alphaP, const uint32_t *alphaD) override {
+        mp<L> aP = unpack1(alphaP), aD = unpack1(alphaD);
+        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(X.pv(), X.pv(), dX.pv(), aP, Nd);
+        k_axpy<L><<<cdiv(m, 256), 256, 0, st>>>(y.pv(), y.pv(), dy.pv(), aD, m);
+        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(Z.pv(), Z.pv(), dZ.pv(), aD, Nd);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    // sum of x_i y_i (or of x[idx[i]]) with a two-level reduction; the partial sums are added on the host
+    int dot(const Arr &xa, int64_t xo, const Arr *ya, int64_t yo, const int64_t *idx, int64_t n, mp<L> &out) {
+        int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, cdiv(n, 1024)));
+        k_dot_part<L><<<nb, 256, 0, st>>>(xa.pv(), xo, ya ? ya->pv() : PV{nullptr, 0}, yo, idx, n, scal.pv());
+        HIPCHK(hipGetLastError());
+        std::vector<uint32_t> h((size_t)nb * (L + 2));
+        int rc = get_packed(scal, 0, h.data(), nb);
+        if (rc) return rc;
+        out = zero<L>();
+        for (int i = 0; i < nb; i++) out = add<L>(out, unpack1(&h[(size_t)i * (L + 2)]));
+        return 0;
+    }
+
+    // scalar inner products of the step-length / SolveInfo logic (sdpa_parts.cpp:362-860)
+    int inner(int what, uint32_t *out) override {
+        mp<L> v = zero<L>();
+        int rc = 0;
+        switch (what) {
+        case SDPG_IP_XZ: rc = dot(X, 0, &Z, 0, nullptr, Nd, v); break;
+        case SDPG_IP_XDZ: rc = dot(X, 0, &dZ, 0, nullptr, Nd, v); break;
+        case SDPG_IP_DXZ: rc = dot(dX, 0, &Z, 0, nullptr, Nd, v); break;
+        case SDPG_IP_DXDZ: rc = dot(dX, 0, &dZ, 0, nullptr, Nd, v); break;
+        case SDPG_IP_BY: rc = dot(b, 0, &y, 0, nullptr, m, v); break;
+        case SDPG_IP_BDY: rc = dot(b, 0, &dy, 0, nullptr, m, v); break;
+        case SDPG_IP_TRX: rc = dot(X, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
+        case SDPG_IP_TRZ: rc = dot(Z, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
+        case SDPG_IP_CX:
+        case SDPG_IP_CDX: {
+            const Arr &M_ = what == SDPG_IP_CX ? X : dX;
+            k_con_inner<L><<<1, 64, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr, d_lp_pos,
+                                             lpval.pv(), blkoff[nsdp], M_.pv(), b.pv(), PV{scal.p, scal.n}, 2);
+            std::vector<uint32_t> h(L + 2);
+            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
+            v = unpack1(h.data());
+            break;
+        }
+        default: set_err("sdpg_inner: unknown selector"); return SDPG_EARG;
+        }
+        if (rc) return rc;
+        pack1(v, out);
+        return 0;
+    }
+
+    // Lal::getMinEigen (sdpa_linear.cpp:27-124) for one block: Lanczos on lMat xMat lMat^T with the
+    // matrix-vector products on the device; the tridiagonal eigenproblem (Rsteqr there) on the host
+    int lanczos(int n, int64_t o, const Arr &Lm, const Arr &Xm, mp<L> &res) {
+        using hla::operator+;
+        using hla::operator-;
+        using hla::operator*;
+        using hla::operator/;
+        using hla::operator<;
+        using hla::operator>;
+        using hla::operator>=;
+        using MM = hla::M<L>;
+        int rc;
+        // U = lMat^T so that lMat x is a column-contiguous product
+        k_transpose<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(Lm.pv(), o, Wt.pv(), 0, n);
+        mp<L> mn = MM::d(1.0e+51), mn_old = MM::d(1.0e+52), mn_min = MM::d(1.0e+50);
+        mp<L> error = MM::d(1.0e+10), one = MM::i(1);
+        std::vector<mp<L>> diagVec(n, mn_min), diagVec2(n, zero<L>()), out(n), bb(n);
+        // the loop ends by k = sqrt(n) + 10, so the tridiagonal eigenvectors need that many rows
+        const int ldq = std::min(n, (int)std::sqrt((double)n) + 12);
+        std::vector<mp<L>> Q((size_t)ldq * ldq);
+        mp<L> nD = MM::i(n);
+        mp<L> beta = hla::sqrt(nD);
+        const mp<L> sqrtn10 = hla::sqrt(nD) + MM::i(10), sqrtn9 = hla::sqrt(MM::i(n + 9));
+        const mp<L> c16 = MM::d(1.0e-16), c5 = MM::d(1.0e-5), c8 = MM::d(1.0e-8), c2 = MM::d(1.0e-2),
+                    c4 = MM::d(1.0e-4);
+        zero_arr(lq, 0, n);
+        {
+            std::vector<uint32_t> ones((size_t)n * (L + 2));
+            for (int i = 0; i < n; i++) pack1(one, &ones[(size_t)i * (L + 2)]);
+            if ((rc = put_packed(lr, 0, ones.data(), n))) return rc;
+        }
+        int k = 0, kk = 0;
+        const mp<L> zr = zero<L>();
+        for (;;) {
+            mp<L> kf = MM::i(k);
+            if (!(k < n) || !(kf < sqrtn10) || !(beta > c16)) break;
+            bool condA = hla::abs(mn - mn_old) > c5 * hla::abs(mn) + c8;
+            bool condB = hla::abs(error * beta) > c2 * hla::abs(mn) + c4;
+            if (!(condA || condB)) break;
+            HIPCHK(hipMemcpyAsync(lqold.p, lq.p, sizeof(uint32_t) * (L + 1) * lq.n, hipMemcpyDeviceToDevice, st));
+            mp<L> value = one / beta;
+            k_lincomb<L><<<cdiv(n, 256), 256, 0, st>>>(lq.pv(), lr.pv(), lr.pv(), value, zr, 0, n);
+            // w = lMat^T q ; tmp = xMat w ; w = lMat tmp
+            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Lm.pv(), o, n, 1, lq.pv(), 0, lw.pv(), 0);
+            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Xm.pv(), o, n, 0, lw.pv(), 0, ltmp.pv(), 0);
+            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Wt.pv(), 0, n, 2, ltmp.pv(), 0, lw.pv(), 0);
+            mp<L> alpha;
+            if ((rc = dot(lq, 0, &lw, 0, nullptr, n, alpha))) return rc;
+            diagVec[k] = alpha;
+            // r = w - alpha q - beta qold
+            k_lincomb<L><<<cdiv(n, 256), 256, 0, st>>>(lr.pv(), lw.pv(), lq.pv(), one, mpx::neg<L>(alpha), 1, n);
+            k_axpy<L><<<cdiv(n, 256), 256, 0, st>>>(lr.pv(), lr.pv(), lqold.pv(), mpx::neg<L>(beta), n);
+            if (MM::i(kk) >= hla::sqrt(MM::i(k)) || k == n - 1 || kf > sqrtn9) {
+                kk = 0;
+                out = diagVec;
+                bb = diagVec2;
+                out[n - 1] = diagVec[k];
+                bb[n - 1] = zr;
+                int info = hla::tridiag_ql<L>(k + 1, out.data(), bb.data(), Q.data(), ldq);
+                if (info > 0) break;
+                mn_old = mn;
+                mn = out[0];
+                error = Q[k];
+            }
+            mp<L> rr;
+            if ((rc = dot(lr, 0, &lr, 0, nullptr, n, rr))) return rc;
+            beta = hla::sqrt(rr);
+            diagVec2[k] = beta;
+            ++k;
+            ++kk;
+        }
+        res = mn - hla::abs(error * beta);
+        return 0;
+    }
+
+    // Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:55-88): which 0 = (invCholX, dX), 1 = (invCholZ, dZ)
+    int min_eigen(int which, uint32_t *out) override {
+        tstart();
+        using hla::operator<;
+        const Arr &Lm = which ? icZ : icX;
+        const Arr &Xm = which ? dZ : dX;
+        mp<L> mn = mpx::from_double<L>(1.0E50);
+        int rc;
+        for (int l = 0; l < nsdp; l++) {
+            int n = sizes[l];
+            int64_t o = blkoff[l];
+            mp<L> v;
+            if (n > 20) {
+                if ((rc = lanczos(n, o, Lm, Xm, v))) return rc;
+            } else {
+                // Lal::let(DLS2 = xMat lMat^T), DLS1 = lMat DLS2, Rsyev on DLS1 (host, cyclic Jacobi)
+                if ((rc = gemm.gemm(n, n, n, Xm.pv(), o, n, false, Lm.pv(), o, n, true, Wt.pv(), 0, n, GEMM_SET, 1, 0, 0,
+                                    0, gemm_mode, gemm_slices)))
+                    return rc;
+                if ((rc = gemm.gemm(n, n, n, Lm.pv(), o, n, false, Wt.pv(), 0, n, false, Wc.pv(), 0, n, GEMM_SET, 1, 0,
+                                    0, 0, gemm_mode, gemm_slices)))
+                    return rc;
+                std::vector<uint32_t> h((size_t)n * n * (L + 2));
+                if ((rc = get_packed(Wc, 0, h.data(), (int64_t)n * n))) return rc;
+                std::vector<mp<L>> w((size_t)n * n);
+                for (size_t q = 0; q < w.size(); q++) w[q] = unpack1(&h[q * (L + 2)]);
+                v = hla::jacobi_min_eig<L>(n, w.data());
+            }
+            if (v < mn) mn = v;
+        }
+        if (nlp) {
+            k_lp_triple_val<L><<<cdiv(nlp, 128), 128, 0, st>>>(Xm.pv(), Lm.pv(), blkoff[nsdp], nlp, Wt.pv());
+            std::vector<uint32_t> h((size_t)nlp * (L + 2));
+            if ((rc = get_packed(Wt, 0, h.data(), nlp))) return rc;
+            for (int q = 0; q < nlp; q++) {
+                mp<L> v = unpack1(&h[(size_t)q * (L + 2)]);
+                if (v < mn) mn = v;
+            }
+        }
+        pack1(mn, out);
+        return tstop(7);
     }
 
     int potrs() override {
@@ -1448,6 +1822,8 @@ struct Engine : EngineBase {
         case SDPG_DZ: cnt = Nd; return &dZ;
         case SDPG_DX: cnt = Nd; return &dX;
         case SDPG_RZINV: cnt = Nd; return &rzinv;
+        case SDPG_INVCHOLX: cnt = Nd; return &icX;
+        case SDPG_INVCHOLZ: cnt = Nd; return &icZ;
         }
         return nullptr;
     }
@@ -1779,6 +2155,26 @@ int sdpg_dz(sdpg_ctx *ctx, int phase) {
 int sdpg_dx(sdpg_ctx *ctx) {
     CTX_CHECK(true);
     return E->dx();
+}
+int sdpg_factor_point(sdpg_ctx *ctx, int *info) {
+    CTX_CHECK(true);
+    if (!info) return SDPG_EARG;
+    return E->factor_point(info);
+}
+int sdpg_update(sdpg_ctx *ctx, const uint32_t *alphaP, const uint32_t *alphaD) {
+    CTX_CHECK(true);
+    if (!alphaP || !alphaD) return SDPG_EARG;
+    return E->update(alphaP, alphaD);
+}
+int sdpg_inner(sdpg_ctx *ctx, int what, uint32_t *out) {
+    CTX_CHECK(true);
+    if (!out) return SDPG_EARG;
+    return E->inner(what, out);
+}
+int sdpg_min_eigen(sdpg_ctx *ctx, int which, uint32_t *out) {
+    CTX_CHECK(true);
+    if (!out || (which != 0 && which != 1)) return SDPG_EARG;
+    return E->min_eigen(which, out);
 }
 int sdpg_get(sdpg_ctx *ctx, int what, uint32_t *out) {
     CTX_CHECK(true);

@@ -202,7 +202,7 @@ struct Solver {
     // dense linear spaces
     std::vector<int64_t> off;
     int64_t Nd = 0;
-    std::vector<T> X, Z, invCholX, invCholZ, invZ, dX, dZ, y, dy;
+    std::vector<T> X, Z, invZ, y;  // initial point only; the iterates live on the device
     std::vector<int> useFormula;
     sdpg_ctx *ctx = nullptr;
     Parameter prm;
@@ -422,116 +422,10 @@ struct Solver {
         return sdpg_set_gemm(ctx, gemm_mode, slices);
     }
 
-    // ----- dense linear space helpers -----
-    T dls_inner(const std::vector<T> &a, const std::vector<T> &c) const {
-        T r = mpx::zero<L>();
-        for (int l = 0; l < nsdp; l++)
-            r = r + dot<L>((int)(off[l + 1] - off[l]), &a[off[l]], 1, &c[off[l]], 1);
-        for (int q = 0; q < nlp; q++) r = r + a[off[nsdp] + q] * c[off[nsdp] + q];
-        return r;
-    }
-    // C . D for the sparse linear space C (getInnerProduct(SparseLinearSpace, DenseLinearSpace))
-    T sls_inner(SLS &S, const std::vector<T> &D) const {
-        T ret = mpx::zero<L>();
-        for (auto &s : S.s) {
-            const T *Dm = &D[off[s.blk]];
-            int n = s.n;
-            T t = mpx::zero<L>();
-            if (s.dense) {
-                t = dot<L>(n * n, s.de.data(), 1, Dm, 1);
-            } else {
-                for (size_t q = 0; q < s.r.size(); q++) {
-                    int i = s.r[q], j = s.c[q];
-                    T tmp;
-                    if (i == j)
-                        tmp = s.v[q] * Dm[i + (size_t)n * j];
-                    else
-                        tmp = (Dm[i + (size_t)n * j] + Dm[j + (size_t)n * i]) * s.v[q];
-                    t = t + tmp;
-                }
-            }
-            ret = ret + t;
-        }
-        for (size_t q = 0; q < S.lpi.size(); q++) ret = ret + S.lpv[q] * D[off[nsdp] + S.lpi[q]];
-        return ret;
-    }
-    void identity(std::vector<T> &d, const T &s) {
-        d.assign(Nd, mpx::zero<L>());
-        for (int l = 0; l < nsdp; l++) {
-            int n = sdpStruct[l];
-            for (int i = 0; i < n; i++) d[off[l] + i + (size_t)n * i] = s;
-        }
-        for (int q = 0; q < nlp; q++) d[off[nsdp] + q] = s;
-    }
-
-    // Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:123-171)
-    bool compute_inverse() {
-        for (int l = 0; l < nsdp; l++) {
-            int n = sdpStruct[l];
-            std::vector<T> w(X.begin() + off[l], X.begin() + off[l] + (size_t)n * n);
-            if (potrf_lower<L>(n, w.data()) != 0) return false;
-            inv_lower<L>(n, w.data(), &invCholX[off[l]]);
-            std::vector<T> wz(Z.begin() + off[l], Z.begin() + off[l] + (size_t)n * n);
-            if (potrf_lower<L>(n, wz.data()) != 0) return false;
-            inv_lower<L>(n, wz.data(), &invCholZ[off[l]]);
-            for (size_t k = 0; k < (size_t)n * n; k++) invZ[off[l] + k] = invCholZ[off[l] + k];
-            trmm_lltn<L>(n, &invCholZ[off[l]], &invZ[off[l]]);
-        }
-        const T one = M<L>::i(1);
-        for (int q = 0; q < nlp; q++) {
-            int64_t p = off[nsdp] + q;
-            if (mpx::sgn<L>(X[p]) < 0 || mpx::sgn<L>(Z[p]) < 0) return false;
-            invCholX[p] = one / sqrt(X[p]);
-            invCholZ[p] = one / sqrt(Z[p]);
-            invZ[p] = one / Z[p];
-        }
-        return true;
-    }
-
-    // Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:55-88)
-    T min_eigen(const std::vector<T> &lMat, const std::vector<T> &xMat) {
-        T mn = M<L>::d(1.0E50);
-        const T one = M<L>::i(1);
-        for (int l = 0; l < nsdp; l++) {
-            int n = sdpStruct[l];
-            const T *Lm = &lMat[off[l]], *Xm = &xMat[off[l]];
-            T v;
-            if (n > 20) {
-                v = lanczos_min_eig<L>(n, Lm, Xm);
-            } else {
-                std::vector<T> w2((size_t)n * n), w1((size_t)n * n), LT((size_t)n * n);
-                for (int j = 0; j < n; j++)
-                    for (int i = 0; i < n; i++) LT[i + (size_t)n * j] = Lm[j + (size_t)n * i];
-                gemm_nn<L>(n, Xm, LT.data(), w2.data());  // xMat * lMat^T  (multiply_tran)
-                gemm_nn<L>(n, Lm, w2.data(), w1.data());
-                v = jacobi_min_eig<L>(n, w1.data());
-            }
-            if (v < mn) mn = v;
-        }
-        for (int q = 0; q < nlp; q++) {
-            int64_t p = off[nsdp] + q;
-            T v = xMat[p] * lMat[p] * lMat[p];
-            if (v < mn) mn = v;
-        }
-        return mn;
-    }
-
     std::string fmt(const T &v, int digits) { return mpx::format_decimal<L>(v, digits); }
     std::string fmt_signed(const T &v, int digits) {
         std::string s = fmt(v, digits);
         return (s[0] == '-') ? s : "+" + s;
-    }
-
-    int get_dir() {
-        std::vector<uint32_t> buf(Nd * W), bufy((size_t)m * W);
-        int rc;
-        if ((rc = sdpg_get(ctx, SDPG_DX, buf.data()))) return rc;
-        unpack(buf, dX);
-        if ((rc = sdpg_get(ctx, SDPG_DZ, buf.data()))) return rc;
-        unpack(buf, dZ);
-        if ((rc = sdpg_get(ctx, SDPG_DY, bufy.data()))) return rc;
-        unpack(bufy, dy);
-        return 0;
     }
 
     int upload_point() {
@@ -571,9 +465,18 @@ struct Solver {
         }
         if ((rc = sdpg_potrs(ctx))) return rc;
         if ((rc = sdpg_dz(ctx, phase))) return rc;
-        if ((rc = sdpg_dx(ctx))) return rc;
-        return get_dir();
+        return sdpg_dx(ctx);
     }
+    T ip(int what) {
+        std::vector<uint32_t> w(W);
+        int rc = sdpg_inner(ctx, what, w.data());
+        if (rc) {
+            ip_err = rc;
+            return mpx::zero<L>();
+        }
+        return unpack1(w.data());
+    }
+    int ip_err = 0;
 
     // SolveInfo (sdpa_parts.cpp:713-830)
     struct SolveInfo {
@@ -636,22 +539,32 @@ struct Solver {
     int run(sdpa_amd_result *res, int stop_after) {
         using MM = M<L>;
         compute_formula();
-        X.assign(Nd, mpx::zero<L>());
         const T lam = MM::d(prm.lambdaStar), one = MM::i(1);
+        // initial point (sdpa_main.cpp / Solutions::initialize): X = Z = lambda I, y = 0
+        auto identity = [&](std::vector<T> &d, const T &v) {
+            d.assign(Nd, mpx::zero<L>());
+            for (int l = 0; l < nsdp; l++) {
+                int n = sdpStruct[l];
+                for (int i = 0; i < n; i++) d[off[l] + i + (size_t)n * i] = v;
+            }
+            for (int q = 0; q < nlp; q++) d[off[nsdp] + q] = v;
+        };
         identity(X, lam);
         identity(Z, lam);
-        invCholX.assign(Nd, mpx::zero<L>());
-        invCholZ.assign(Nd, mpx::zero<L>());
-        invZ.assign(Nd, mpx::zero<L>());
-        identity(invCholX, one / sqrt(lam));
-        identity(invCholZ, one / sqrt(lam));
         identity(invZ, one / lam);
         y.assign(m, mpx::zero<L>());
-        dy.assign(m, mpx::zero<L>());
-        dX.assign(Nd, mpx::zero<L>());
-        dZ.assign(Nd, mpx::zero<L>());
         int rc = upload_point();
         if (rc) return rc;
+        X.clear();
+        Z.clear();
+        invZ.clear();
+        X.shrink_to_fit();
+        Z.shrink_to_fit();
+        invZ.shrink_to_fit();
+        {  // Solutions::initialize: the inverse factors of lambda I, on the device
+            int info = 0;
+            if ((rc = sdpg_factor_point(ctx, &info))) return rc;
+        }
         T initNrp, initNrd;
         if ((rc = residuals(initNrp, initNrd))) return rc;
         T curNrp = initNrp, curNrd = initNrd;
@@ -666,8 +579,8 @@ struct Solver {
         si.rho = one;
         si.etaPrimal = omega * nD * mu_initial;
         si.etaDual = si.etaPrimal;
-        si.objValPrimal = sls_inner(C, X);
-        si.objValDual = dot<L>(m, b.data(), 1, y.data(), 1);
+        si.objValPrimal = ip(SDPG_IP_CX);
+        si.objValDual = ip(SDPG_IP_BY);
         int phase = noINFO;
         phase_update(phase, initNrp, initNrd, si);
         if (logf) fprintf(logf, "   mu      thetaP  thetaD  objP      objD      alphaP  alphaD  beta \n");
@@ -698,7 +611,7 @@ struct Solver {
                 if (phase == noINFO || phase == dFEAS) {
                     if (alphaP > one) alphaP = one;
                 } else {
-                    T inc = sls_inner(C, dX);
+                    T inc = ip(SDPG_IP_CDX);
                     if (mpx::sgn<L>(inc) > 0) {
                         if (alphaP > alphaD) alphaP = alphaD;
                         if (alphaP > one) alphaP = one;
@@ -707,7 +620,7 @@ struct Solver {
                 if (phase == noINFO || phase == pFEAS) {
                     if (alphaD > one) alphaD = one;
                 } else {
-                    T inc = dot<L>(m, b.data(), 1, dy.data(), 1);
+                    T inc = ip(SDPG_IP_BDY);
                     if (mpx::sgn<L>(inc) < 0) {
                         if (alphaD > alphaP) alphaD = alphaP;
                         if (alphaD > one) alphaD = one;
@@ -717,7 +630,7 @@ struct Solver {
             clamp_steps();
             // DirectionParameter::MehrotraCorrector (sdpa_parts.cpp:614-645)
             {
-                T xv = dls_inner(X, dZ), uz = dls_inner(dX, Z), uv = dls_inner(dX, dZ);
+                T xv = ip(SDPG_IP_XDZ), uz = ip(SDPG_IP_DXZ), uv = ip(SDPG_IP_DXDZ);
                 T muT = mu_current + (alphaP * uz + alphaD * xv + alphaP * alphaD * uv) / nD;
                 beta = muT / mu_current;
                 if (beta < one) beta = beta * beta;
@@ -732,15 +645,21 @@ struct Solver {
             if ((rc = mehrotra(false, mu_current, beta, phase, ok))) return rc;
             // StepLength::MehrotraCorrector (sdpa_parts.cpp:412-523)
             {
-                T mnx = min_eigen(invCholX, dX);
+                T mnx, mnz;
+                {
+                    std::vector<uint32_t> w(W);
+                    if ((rc = sdpg_min_eigen(ctx, 0, w.data()))) return rc;
+                    mnx = unpack1(w.data());
+                    if ((rc = sdpg_min_eigen(ctx, 1, w.data()))) return rc;
+                    mnz = unpack1(w.data());
+                }
                 alphaP = (-mnx > inv_alphaBD) ? -(one / mnx) : alphaBD;
-                T mnz = min_eigen(invCholZ, dZ);
                 alphaD = (-mnz > inv_alphaBD) ? -(one / mnz) : alphaBD;
                 alphaP = gammaStar * alphaP;
                 alphaD = gammaStar * alphaD;
                 clamp_steps();
                 if (reductionOn && (phase == noINFO || phase == pFEAS || phase == dFEAS)) {
-                    T xv = dls_inner(X, dZ), uz = dls_inner(dX, Z), uv = dls_inner(dX, dZ);
+                    T xv = ip(SDPG_IP_XDZ), uz = ip(SDPG_IP_DXZ), uv = ip(SDPG_IP_DXDZ);
                     auto thetaMaxF = [&]() {
                         T a1 = (one - alphaP) * thetaP, a2 = (one - alphaD) * thetaD;
                         return (a1 > a2) ? a1 : a2;
@@ -757,8 +676,8 @@ struct Solver {
                     }
                 }
                 if (phase == pdFEAS) {
-                    T oD = dot<L>(m, b.data(), 1, y.data(), 1), oP = sls_inner(C, X);
-                    T iD = dot<L>(m, b.data(), 1, dy.data(), 1) * alphaD, iP = sls_inner(C, dX) * alphaP;
+                    T oD = ip(SDPG_IP_BY), oP = ip(SDPG_IP_CX);
+                    T iD = ip(SDPG_IP_BDY) * alphaD, iP = ip(SDPG_IP_CDX) * alphaP;
                     T den = iP - iD;
                     if (!mpx::is_zero<L>(den)) {
                         T mr = (oD - oP) / den;
@@ -770,47 +689,38 @@ struct Solver {
                 }
             }
             print_iter(pIteration, mu_current, thetaP, thetaD, si, alphaP, alphaD, beta);
-            // Solutions::update (sdpa_dataset.cpp:120-143)
-            for (int64_t k = 0; k < Nd; k++) X[k] = X[k] + alphaP * dX[k];
-            for (int i = 0; i < m; i++) y[i] = y[i] + alphaD * dy[i];
-            for (int64_t k = 0; k < Nd; k++) Z[k] = Z[k] + alphaD * dZ[k];
+            // Solutions::update (sdpa_dataset.cpp:120-143), on the device
+            if ((rc = sdpg_update(ctx, pack1(alphaP).data(), pack1(alphaD).data()))) return rc;
             if (alphaP < c1em4 && alphaD < c1em4) {
                 if (logf) fprintf(logf, "Step length is too small. \ncannot move\n");
                 status = 2;
                 break;
             }
-            if (!compute_inverse()) {
-                if (logf) fprintf(logf, "cannot move\n");
-                status = 2;
-                break;
+            {
+                int info = 0;
+                if ((rc = sdpg_factor_point(ctx, &info))) return rc;
+                if (info) {
+                    if (logf) fprintf(logf, "cannot move\n");
+                    status = 2;
+                    break;
+                }
             }
             if (reductionOn) {
                 thetaP = abs((one - alphaP) * thetaP);
                 thetaD = abs((one - alphaD) * thetaD);
             }
-            mu_current = dls_inner(X, Z) / nD;
-            if ((rc = upload_point())) return rc;
+            mu_current = ip(SDPG_IP_XZ) / nD;
             if ((rc = residuals(curNrp, curNrd))) return rc;
             thetaP = (initNrp > c1em10) ? curNrp / initNrp : MM::i(0);
             thetaD = (initNrd > c1em10) ? curNrd / initNrd : MM::i(0);
             // SolveInfo::update(lambda, ...) (sdpa_parts.cpp:774-830)
             {
-                si.objValPrimal = sls_inner(C, X);
-                si.objValDual = dot<L>(m, b.data(), 1, y.data(), 1);
+                si.objValPrimal = ip(SDPG_IP_CX);
+                si.objValDual = ip(SDPG_IP_BY);
                 T pr = thetaP, du = thetaD, x0z0 = nD * mu_initial, xz = nD * mu_current;
-                T x0z = mpx::zero<L>(), xz0 = mpx::zero<L>();
+                // lambda I . Z and X . lambda I (the reference sums lambda * diagonal entries)
+                T x0z = lam * ip(SDPG_IP_TRZ), xz0 = lam * ip(SDPG_IP_TRX);
                 si.rho = mpx::zero<L>();
-                for (int l = 0; l < nsdp; l++) {
-                    int n = sdpStruct[l];
-                    for (int i = 0; i < n; i++) {
-                        x0z = x0z + lam * Z[off[l] + (size_t)i * n + i];
-                        xz0 = xz0 + lam * X[off[l] + (size_t)i * n + i];
-                    }
-                }
-                for (int q = 0; q < nlp; q++) {
-                    x0z = x0z + lam * Z[off[nsdp] + q];
-                    xz0 = xz0 + lam * X[off[nsdp] + q];
-                }
                 bool pok = curNrp <= acc, dok = curNrd <= acc;
                 if (pok && xz0 < si.etaPrimal) si.etaPrimal = xz0;
                 if (dok && x0z < si.etaDual) si.etaDual = x0z;
@@ -820,8 +730,10 @@ struct Solver {
                     si.rho = (du * xz0 + pr * x0z) /
                              ((pr * du + omega * (pr * (one - du) + (one - pr) * du)) * x0z0 + xz);
             }
+            if (ip_err) return ip_err;
             pIteration++;
         }
+        if (ip_err) return ip_err;
         double t1 = now_s();
         if ((rc = residuals(curNrp, curNrd))) return rc;
         print_iter(pIteration, mu_current, thetaP, thetaD, si, alphaP, alphaD, beta);

@@ -23,6 +23,9 @@ LIBDIR = os.path.join(PKG, "lib")
 
 SDPG_X, SDPG_Z, SDPG_INVZ, SDPG_Y, SDPG_RP, SDPG_RD, SDPG_GVEC = 1, 2, 3, 4, 5, 6, 7
 SDPG_B, SDPG_L, SDPG_DY, SDPG_DZ, SDPG_DX, SDPG_RZINV = 8, 9, 10, 11, 12, 13
+SDPG_INVCHOLX, SDPG_INVCHOLZ = 14, 15
+(SDPG_IP_XZ, SDPG_IP_XDZ, SDPG_IP_DXZ, SDPG_IP_DXDZ, SDPG_IP_CX, SDPG_IP_CDX, SDPG_IP_BY, SDPG_IP_BDY,
+ SDPG_IP_TRX, SDPG_IP_TRZ) = range(1, 11)
 GEMM_ELEM, GEMM_PLAIN, GEMM_OZAKI = -1, 0, 1
 EZERO = -(1 << 29)
 
@@ -31,7 +34,8 @@ EXPORTS = [
     "sdpg_limbs", "sdpg_ctx_create", "sdpg_ctx_destroy", "sdpg_set_block", "sdpg_set_lp", "sdpg_set_b",
     "sdpg_set_formula", "sdpg_commit", "sdpg_set_gemm", "sdpg_set_point", "sdpg_residuals", "sdpg_gvec",
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
-    "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm",
+    "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
+    "sdpg_inner", "sdpg_min_eigen",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file",
 ]
 
@@ -69,6 +73,10 @@ def load(path: str | None = None):
     lib.sdpg_put.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
     lib.sdpg_set_gemm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     lib.sdpg_timers.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]
+    lib.sdpg_update.argtypes = [ctypes.c_void_p, u32p, u32p]
+    lib.sdpg_factor_point.argtypes = [ctypes.c_void_p, P(ctypes.c_int)]
+    lib.sdpg_inner.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
+    lib.sdpg_min_eigen.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
     lib.sdpg_rgemm.argtypes = [ctypes.c_int] * 7 + [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p] + \
         [ctypes.c_int] * 5 + [P(ctypes.c_double)]
     lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -211,6 +219,26 @@ class Context:
         if rc < 0:
             raise SdpgError(f"{what} rc={rc}: {self.lib.sdpg_last_error().decode()}")
         return rc
+
+    def factor_point(self) -> int:
+        info = ctypes.c_int()
+        self._chk(self.lib.sdpg_factor_point(self.h, ctypes.byref(info)), "sdpg_factor_point")
+        return info.value
+
+    def update(self, alphaP_words, alphaD_words):
+        a = np.ascontiguousarray(alphaP_words, dtype=np.uint32)
+        d = np.ascontiguousarray(alphaD_words, dtype=np.uint32)
+        self._chk(self.lib.sdpg_update(self.h, _ptr(a), _ptr(d)), "sdpg_update")
+
+    def inner(self, what: int) -> np.ndarray:
+        out = np.zeros(self.L + 2, dtype=np.uint32)
+        self._chk(self.lib.sdpg_inner(self.h, what, _ptr(out)), "sdpg_inner")
+        return out
+
+    def min_eigen(self, which: int) -> np.ndarray:
+        out = np.zeros(self.L + 2, dtype=np.uint32)
+        self._chk(self.lib.sdpg_min_eigen(self.h, which, _ptr(out)), "sdpg_min_eigen")
+        return out
 
     def size_of(self, what: int) -> int:
         if what in (SDPG_Y, SDPG_RP, SDPG_GVEC, SDPG_DY):
