@@ -1,24 +1,30 @@
-"""Benchmark: Schur complement build + Cholesky per IPM iteration at 256 bits (gap <= 1e-60).
+"""Benchmark: Schur complement build + Cholesky per IPM iteration at 256 bits (gap <= 1e-60),
+Ozaki INT8-MFMA form against the plain limb form.
 
-Workload (workload 1 of gen_problems.py): block-diagonal SDP with blocks (50, 25), m = 351
-constraints, ~8% dense N(0,1) constraint blocks, feasible by construction; precision 256
-bits (GMP rounding: 10 x 32-bit limbs), epsilonStar = epsilonDash = 1e-60.
+Workload (workload 2 of gen_problems.py): max-cut relaxation, n = m = 1000, one SDP block,
+A_i = e_i e_i^T, b = 1, C = -Laplacian/4 of an Erdos-Renyi G(1000, 0.01) graph with weights
++-1; precision 256 bits (GMP limb rounding: 10 x 32-bit limbs), epsilonStar = epsilonDash = 1e-60.
+The Schur matrix is dense 1000 x 1000 (F3 formula, B = X o inv(Z)).
 
-A "step" is one Make_bMat + Cholesky (sdpg_schur + sdpg_potrf) on one iterate resident in
-HBM: the solver is first run to convergence (untimed; its per-iteration device time over the
-whole trajectory is reported too), then `--warmup` untimed and `--steps` timed steps are
-replayed on the iterate of iteration `--at` of that trajectory.  Both GEMM forms are run:
-the Ozaki INT8-MFMA form (the value) and the plain limb form (reported beside it).
+For each GEMM form the solver first runs to convergence (untimed here; its per-iteration device
+times over the whole trajectory are reported), then a second run stops at iteration `--at`
+and a "step" = one Make_bMat + Cholesky (sdpg_schur + sdpg_potrf) on that iterate, resident in
+HBM, is run `--warmup` times untimed and `--steps` times timed.  value = wall seconds per step of
+the Ozaki form.
 
-The CPU baseline is the oracle (a C restatement of the reference's mpf path) timed on the
-host on the same iterate.
+roofline: the Ozaki MFMA product kernel, INT8 operations = 2 * S(S+1)/2 per mp multiply-add of the
+reference loop (S slices), over its event-timed average launch duration in the timed steps, against
+the 5.0 POP/s dense INT8 peak.  The per-kernel breakdown of the step is reported beside it.
+
+cpu_baseline: the oracle (C restatement of the reference's mpf path, serial Cholesky as the
+reference's Rpotrf) on the same iterate, dumped by the GPU run: Make_bMat in full and the Cholesky of
+the leading --cpu-n rows/columns of B, scaled by (m / cpu_n)^3 to the full factorisation.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -29,57 +35,35 @@ sys.path.insert(0, PKG)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 PARAM = os.path.join(PKG, "data", "param_256_gap1e-60.sdpa")
-MP_MAC_W1 = None  # filled from the problem structure
+INT8_PEAK = 5.0e15  # MI355X dense INT8 MFMA, ops/s
+HBM_PEAK = 8.0e12
 
 
-def mp_macs(path: str) -> dict:
-    """Algorithmic multiple-precision multiply-adds of one Schur build + Cholesky (reference loop
-    structure: F1 precompute X (A_i Z^-1), F1/F2 inner products, Rpotrf m^3/6)."""
-    import numpy as np  # noqa: F401
-
+def structure(path: str) -> dict:
+    """m, block sizes and the reference's mp multiply-add count of one Make_bMat + Cholesky."""
     with open(path) as f:
         lines = [ln for ln in f if not ln.startswith('"') and not ln.startswith("*")]
     m = int(lines[0].split()[0])
     nb = int(lines[1].split()[0])
     blocks = [int(x) for x in lines[2].replace("=", " ").split() if x.lstrip("-").isdigit()][:nb]
-    nnz = {}
-    for ln in lines[4:]:
-        p = ln.split()
-        if len(p) < 5:
-            continue
-        k, l, i, j = int(p[0]), int(p[1]), int(p[2]), int(p[3])
-        if k == 0:
-            continue
-        nnz.setdefault((k, l), [0, 0])
-        nnz[(k, l)][0] += 1
-        nnz[(k, l)][1] += 1 if i == j else 2
-    pre = 0
-    inner = 0
-    for l, n in enumerate(blocks, start=1):
-        eff = sorted(v[1] for (k, ll), v in nnz.items() if ll == l)
-        cnt = len(eff)
-        # F1 precompute per constraint: sparse A_i Z^-1 (effect * n) + dense X * work1 (n^3)
-        pre += sum(e * n + n ** 3 for e in eff)
-        # pair rule: each unordered pair once; inner product cost ~ partner's stored nnz
-        inner += sum(e * (cnt - r) for r, e in enumerate(eff))
-    chol = m ** 3 // 6
-    return {"m": m, "blocks": blocks, "schur_mp_mac": pre + inner, "chol_mp_mac": chol,
-            "total_mp_mac": pre + inner + chol}
+    return {"m": m, "blocks": blocks, "chol_mp_mac": m ** 3 / 6.0}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--at", type=int, default=5, help="iterate of the trajectory the steps replay")
-    ap.add_argument("--workload", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--at", type=int, default=20, help="iteration whose iterate the timed steps replay")
+    ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--forms", default="ozaki,plain")
+    ap.add_argument("--no-solve", action="store_true", help="skip the full trajectories")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=500, help="Cholesky sample size of the CPU baseline")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path shards by "
-                                   "constraint rows across GPUs only in the design notes)"}))
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding of the "
+                                   "Schur rows and a 2D block-cyclic Cholesky are design notes only)"}))
         sys.exit(2)
 
     import gen_problems
@@ -88,26 +72,37 @@ def main():
     tmp = tempfile.mkdtemp(prefix="sdpg_bench_")
     data = os.path.join(tmp, f"w{a.workload}.dat-s")
     sha = gen_problems.generate(a.workload, data)
-    work = mp_macs(data)
+    info = structure(data)
     L = sdpg.limbs_for_bits(256)
-
     forms = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}
     res = {}
-    for name in a.forms.split(","):
+    dump = os.path.join(tmp, "it_")
+    for idx, name in enumerate(a.forms.split(",")):
         mode = forms[name]
-        t0 = time.time()
-        r = sdpg.solve(data, PARAM, 0, None, mode)
-        t_solve = time.time() - t0
-        iters = max(r.iterations, 1)
-        print(f"[bench] {name}: solve {r.iterations} iterations, phase {r.phase.decode().strip()}, "
-              f"relgap {r.relgap.decode()}, {t_solve:.1f}s", flush=True)
-        ms_s, ms_c, ms_w = sdpg.bench(data, PARAM, 0, mode, 0, a.at, a.warmup, a.steps)
-        print(f"[bench] {name}: replay at iteration {a.at}: schur {ms_s:.3f} ms, potrf {ms_c:.3f} ms, "
-              f"wall {ms_w:.3f} ms per step", flush=True)
-        res[name] = {"iterations": r.iterations, "phase": r.phase.decode().strip(), "relgap": r.relgap.decode(),
-                     "objP": r.objP.decode(), "solve_s": t_solve,
-                     "traj_schur_ms_per_iter": r.t_schur_ms / iters, "traj_chol_ms_per_iter": r.t_chol_ms / iters,
-                     "step_schur_ms": ms_s, "step_chol_ms": ms_c, "step_wall_ms": ms_w}
+        r_out = {}
+        if not a.no_solve:
+            t0 = time.time()
+            r = sdpg.solve(data, PARAM, 0, None, mode)
+            t_solve = time.time() - t0
+            it = max(r.iterations, 1)
+            r_out = {"iterations": r.iterations, "phase": r.phase.decode().strip(), "relgap": r.relgap.decode(),
+                     "objP": r.objP.decode(), "objD": r.objD.decode(), "solve_wall_s": t_solve,
+                     "traj_ms_per_iter": {"schur": r.t_schur_ms / it, "chol": r.t_chol_ms / it,
+                                          "solve": r.t_solve_ms / it, "gvec": r.t_gvec_ms / it,
+                                          "dxdz": r.t_dxdz_ms / it, "inverse": r.t_inv_ms / it,
+                                          "min_eigen": r.t_eig_ms / it}}
+            print(f"[bench] {name}: solve {r.iterations} iterations, {r_out['phase']}, relgap {r_out['relgap']}, "
+                  f"{t_solve:.1f} s; schur+chol {(r.t_schur_ms + r.t_chol_ms) / it:.2f} ms/iter over the trajectory",
+                  flush=True)
+        ms_s, ms_c, ms_w, kst = sdpg.bench(data, PARAM, 0, mode, 0, a.at, a.warmup, a.steps,
+                                           dump_prefix=dump if idx == 0 else None)
+        print(f"[bench] {name}: {a.steps} steps on iterate {a.at}: schur {ms_s:.3f} ms, potrf {ms_c:.3f} ms, "
+              f"wall {ms_w:.3f} ms per step; kernels {kst}", flush=True)
+        r_out.update({"step_schur_ms": ms_s, "step_chol_ms": ms_c, "step_wall_ms": ms_w,
+                      "kernels": {k: {"launches": v[0], "ms_per_step": v[1] / a.steps,
+                                      "avg_launch_us": 1e3 * v[1] / max(v[0], 1)} for k, v in kst.items()},
+                      "_kst": kst})
+        res[name] = r_out
 
     main_form = "ozaki" if "ozaki" in res else list(res)[0]
     step_ms = res[main_form]["step_wall_ms"]
@@ -117,25 +112,46 @@ def main():
         "unit": "s/iter",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": step_ms,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"mp{32 * L}", "data": "synthetic (seeded feasible SDP, gen_problems.py)",
-        "config": {"workload": f"blocks {work['blocks']}, m={work['m']}, 8% sparse A_i, 256-bit, "
+        "dtype": f"mp{32 * L}",
+        "data": "synthetic (seeded max-cut instance from gen_problems.py; no external data)",
+        "config": {"workload": f"max-cut n=m={info['m']}, one block, A_i=e_i e_i^T, 256-bit, "
                                f"epsilonStar=epsilonDash=1e-60", "form": main_form, "replay_iteration": a.at,
                    "input_sha256": sha},
-        "forms": res,
-        "work": work,
     }
+    kst = res[main_form]["_kst"]
+    oz = kst["ozaki_mfma"] if main_form == "ozaki" else None
+    if oz and oz[0]:
+        avg_s = oz[1] / 1e3 / oz[0]
+        ach = oz[2] / oz[0] / avg_s
+        out["roofline"] = {"kernel": "k_gemm_oz (Ozaki INT8 slice products, v_mfma_i32_32x32x32_i8)",
+                           "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TFLOP/s",
+                           "frac": ach / INT8_PEAK, "traffic": None,
+                           "ops_per_launch": oz[2] / oz[0], "avg_launch_us": avg_s * 1e6,
+                           "note": "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices"}
+    shares = {k: v[1] / a.steps / step_ms for k, v in kst.items()}
+    out["step_breakdown"] = {"share_of_step": shares, "dominant": max(shares, key=shares.get)}
+    for f in res.values():
+        f.pop("_kst", None)
+    out["forms"] = res
     if "plain" in res and "ozaki" in res:
         out["ozaki_speedup_vs_plain"] = res["plain"]["step_wall_ms"] / max(res["ozaki"]["step_wall_ms"], 1e-9)
-    if not a.no_cpu:
+        if "objP" in res["plain"] and "objP" in res["ozaki"]:
+            from decimal import Decimal
+            pa, po = Decimal(res["plain"]["objP"]), Decimal(res["ozaki"]["objP"])
+            out["objective_rel_diff_ozaki_vs_plain"] = float(abs(pa - po) / max(abs(pa), Decimal(1)))
+    if not a.no_cpu and os.path.exists(dump + "X.bin"):
         import oracle_lib
 
-        oracle_lib.build_oracle() if not os.path.exists(oracle_lib.ORC_LIB) else None
-        threads = min(os.cpu_count() or 1, 16)
-        tb, tc = oracle_lib.time_hotpath(data, PARAM, 0, a.at, 1, threads)
-        out["cpu_baseline"] = {"value": tb + tc, "unit": "s/iter", "cores": threads, "kind": "port",
-                               "sample": f"one Make_bMat + Cholesky on iterate {a.at} of the same workload "
-                                         f"(oracle, GMP-mpf-semantics C restatement, OpenMP NN GEMM)",
-                               "schur_s": tb, "chol_s": tc}
+        if not os.path.exists(oracle_lib.ORC_LIB):
+            oracle_lib.build_oracle()
+        n = min(a.cpu_n, info["m"])
+        tb, tc = oracle_lib.time_point(data, 256, dump + "X.bin", dump + "invZ.bin", L, n, 1)
+        scale = (info["m"] / n) ** 3
+        out["cpu_baseline"] = {"value": tb + tc * scale, "unit": "s/iter", "cores": 1, "kind": "port",
+                               "sample": f"oracle (GMP-mpf-semantics C restatement, serial Rpotrf structure) on "
+                                         f"the GPU's iterate {a.at}: Make_bMat in full ({tb:.2f} s) and the "
+                                         f"Cholesky of B's leading {n}x{n} block ({tc:.2f} s) scaled by "
+                                         f"({info['m']}/{n})^3"}
     print(json.dumps(out), flush=True)
 
 

@@ -27,15 +27,15 @@ typedef struct {
     double t_solve_ms, t_gvec_ms, t_dxdz_ms, t_res_ms;
     double t_total_s;     /* wall clock of the main loop */
     int limbs;
+    double t_inv_ms;      /* Solutions::computeInverse (sdpg_factor_point), device time */
+    double t_eig_ms;      /* step-length minimum eigenvalues (sdpg_min_eigen), device time */
 } sdpa_amd_result;
 
 /*
  * param_path may be NULL (defaults of Parameter::setDefaultParameter,
  * sdpa_parts.cpp:165-178).  prec_override > 0 replaces the precision line.
  * log_path: NULL, "-" (stdout) or a file.  gemm_mode: SDPG_GEMM_PLAIN or
- * SDPG_GEMM_OZAKI.  keep_iterates: if non-NULL, the directory where the point
- * (X, invZ) of every iteration is saved for replay benchmarks.
- * Returns 0 on success, < 0 on input or device errors.
+ * SDPG_GEMM_OZAKI.  Returns 0 on success, < 0 on input or device errors.
  */
 int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
                    int gemm_mode, int slices, int device, sdpa_amd_result *res);
@@ -45,10 +45,14 @@ int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_overr
  * for `warm_iters` iterations, then times `steps` executions of
  * [Make_bMat + Cholesky] (sdpg_schur + sdpg_potrf) on that iterate after
  * `warmup` untimed ones.  Returns per-step device milliseconds of each phase.
+ * kstats (may be NULL, else 12 doubles): sdpg_profile figures over the timed
+ * steps.  dump_prefix (may be NULL): the iterate is written to
+ * <dump_prefix>X.bin and <dump_prefix>invZ.bin (packed mp, raw uint32) so that a
+ * CPU implementation can time the same step on the same data.
  */
 int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_override, int gemm_mode, int slices,
                    int device, int warm_iters, int warmup, int steps, double *ms_schur, double *ms_chol,
-                   double *ms_wall);
+                   double *ms_wall, double *kstats, const char *dump_prefix);
 
 /*
  * Read a problem, compute its formula table (Newton::computeFormula_SDP,

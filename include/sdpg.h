@@ -185,6 +185,15 @@ int sdpg_timers(sdpg_ctx *ctx, double *ms, int n);
 int sdpg_reset_timers(sdpg_ctx *ctx);
 int sdpg_sync(sdpg_ctx *ctx);
 /*
+ * Per-kernel device time (no reference counterpart; measurement support):
+ * on = 1 clears and starts recording events around each launch of the product
+ * kernels, on = 0 stops.  out (may be NULL; n >= 12) receives, for kind k =
+ * 0 Ozaki MFMA product, 1 plain limb product, 2 Cholesky diagonal block:
+ * out[3k] launches, out[3k+1] milliseconds, out[3k+2] algorithmic work
+ * (k = 0: INT8 operations; 1: mp multiply-adds; 2: mp multiply-adds).
+ */
+int sdpg_profile(sdpg_ctx *ctx, int on, double *out, int n);
+/*
  * Rgemm (mplapack/Rgemm_omp.cpp:41-127 -> Rgemm_NN_omp / Rgemm_NT_omp) on host
  * buffers, one call = one device round trip: C = op(A) op(B) (op_c 0),
  * C - op(A) op(B) (1) or C + op(A) op(B) (2); column-major packed mp, op(X) = X^T

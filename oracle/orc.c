@@ -2297,6 +2297,85 @@ int orc_time_hotpath(const char *data_path, const orc_param *param, int prec_ove
     return rc;
 }
 
+/* packed sdpg words (L+2 per element: e, sign 0/1, limbs) of a DLS from a raw file */
+static int load_packed_dls(const char *path, int L, DLS *d) {
+    int64_t tot = d->nlp;
+    for (int l = 0; l < d->nsdp; l++) tot += (int64_t)d->n[l] * d->n[l];
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return -1;
+    uint32_t *w = (uint32_t *)xcalloc((size_t)tot * (L + 2), 4);
+    size_t got = fread(w, 4, (size_t)tot * (L + 2), fp);
+    fclose(fp);
+    if (got != (size_t)tot * (L + 2)) {
+        free(w);
+        return -2;
+    }
+    int64_t o = 0;
+    for (int l = 0; l < d->nsdp; l++)
+        for (int64_t i = 0; i < (int64_t)d->n[l] * d->n[l]; i++, o++) {
+            const uint32_t *e = &w[o * (L + 2)];
+            int z = e[2 + L - 1] == 0;
+            ompf_import32(&d->b[l][i], L, z ? 0 : ((e[1] & 1) ? -1 : 1), (int32_t)e[0], e + 2);
+        }
+    for (int i = 0; i < d->nlp; i++, o++) {
+        const uint32_t *e = &w[o * (L + 2)];
+        int z = e[2 + L - 1] == 0;
+        ompf_import32(&d->lp[i], L, z ? 0 : ((e[1] & 1) ? -1 : 1), (int32_t)e[0], e + 2);
+    }
+    free(w);
+    return 0;
+}
+
+/*
+ * CPU time of the hot path on a given iterate: Make_bMat on (X, invZ) read from
+ * packed files (L limbs), then the reference's Cholesky (potrf_l, the Rpotrf
+ * structure) on the leading chol_n x chol_n block of the Schur matrix (chol_n <= 0:
+ * all of it).  Used to time a bounded sample of a large workload.
+ */
+int orc_time_point(const char *data_path, int prec_bits, const char *x_path, const char *invz_path, int L,
+                   int chol_n, int threads, double *t_bmat, double *t_chol) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    ompf_set_prec_bits(prec_bits);
+    Ctx c;
+    memset(&c, 0, sizeof c);
+    int rc = read_problem(data_path, &c.P);
+    if (rc) return rc;
+    Prob *P = &c.P;
+    int m = P->m;
+    newton_alloc(&c.N, P);
+    compute_formula(&c.N, P);
+    dls_init(&c.S.X, P);
+    dls_init(&c.S.Z, P);
+    dls_init(&c.S.invCholX, P);
+    dls_init(&c.S.invCholZ, P);
+    dls_init(&c.S.invZ, P);
+    c.S.y = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.W.DLS1, P);
+    dls_init(&c.W.DLS2, P);
+    c.init.primalVec = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.init.dualMat, P);
+    c.cur.primalVec = (F *)xcalloc(m, sizeof(F));
+    dls_init(&c.cur.dualMat, P);
+    if ((rc = load_packed_dls(x_path, L, &c.S.X)) || (rc = load_packed_dls(invz_path, L, &c.S.invZ))) {
+        ctx_free(&c);
+        return -100 + rc;
+    }
+    double t0 = now_s();
+    make_bmat(&c.N, P, &c.S, &c.W);
+    double t1 = now_s();
+    int n = (chol_n > 0 && chol_n < m) ? chol_n : m;
+    (void)potrf_l(n, c.N.bMat, m);
+    double t2 = now_s();
+    *t_bmat = t1 - t0;
+    *t_chol = t2 - t1;
+    ctx_free(&c);
+    return 0;
+}
+
 /* ================= arithmetic probes for tests ================= */
 
 int orc_arith(int prec_bits, int op, int L, int32_t as, int32_t ae, const uint32_t *al, int32_t bs, int32_t be,

@@ -52,20 +52,27 @@ int orc_solve(const char *data_path, const orc_param *param, int prec_override, 
               const char *dump_dir, const int *dump_iters, int n_dump, int dump_L, orc_result *res);
 
 /*
- * Time the reference hot path (Make_bMat + Cholesky of B) on a problem at
- * the default starting point X = Z = lambda*I advanced `warm_iters` IPM
- * iterations; runs `reps` repetitions and returns seconds per repetition of
- * each phase.  Used as the CPU baseline.
- */
-/*
  * One predictor Newton step on a dumped point (X, Z, invZ, y, mu, beta,
  * phase) at prec_bits; writes Rp, Rd, g, B, L, dy, dX, dZ (and the inputs) to
  * out_dir.  Returns 0, 1 on Cholesky failure, < 0 on I/O errors.
  */
 int orc_newton_step(const char *data_path, int prec_bits, const char *in_dir, const char *out_dir, int out_L);
 
+/*
+ * Time the reference hot path (Make_bMat + Cholesky of B) on a problem at
+ * the default starting point X = Z = lambda*I advanced `warm_iters` IPM
+ * iterations; runs `reps` repetitions and returns seconds per repetition of
+ * each phase.  Used as the CPU baseline.
+ */
 int orc_time_hotpath(const char *data_path, const orc_param *param, int prec_override, int warm_iters, int reps,
                      int threads, double *t_bmat, double *t_chol);
+/*
+ * Time Make_bMat on an iterate (X, invZ) given as packed files of L-limb
+ * words, then the Cholesky of the leading chol_n x chol_n block of B
+ * (chol_n <= 0: all of it): a bounded CPU sample of a large workload.
+ */
+int orc_time_point(const char *data_path, int prec_bits, const char *x_path, const char *invz_path, int L,
+                   int chol_n, int threads, double *t_bmat, double *t_chol);
 
 /* arithmetic probe for tests: op codes 0 add,1 sub,2 mul,3 div,4 sqrt,5 set_str(a as text) */
 int orc_arith(int prec_bits, int op, int L, int32_t as, int32_t ae, const uint32_t *al, int32_t bs, int32_t be,

@@ -121,48 +121,79 @@ struct Desc {
     int64_t blkoff;  // offset of block l inside a dense linear space
 };
 
-// ip = A (block) . D (sparse: _OPENMP branch of sdpa_linear.cpp:235-254; dense: Rdot)
+// block-tree sum of one mp value per thread (blockDim.x a power of two, LDS planes of blockDim.x)
 template <int L>
-__device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, PV sv, PV dv, PV D, int64_t doff) {
-    mp<L> ret = zero<L>();
-    if (d.kind == 1) {
-        for (int t = 0; t < d.nnz; t++) {
-            int64_t e = d.off + t;
-            int i = ri[e], j = ci[e];
-            mp<L> v = ldm<L>(sv, e);
-            mp<L> temp;
-            if (i == j) {
-                temp = mul<L>(v, ldm<L>(D, doff + i + (int64_t)d.n * j));
-            } else {
-                temp = add<L>(ldm<L>(D, doff + i + (int64_t)d.n * j), ldm<L>(D, doff + j + (int64_t)d.n * i));
-                temp = mul<L>(temp, v);
-            }
-            ret = add<L>(ret, temp);
-        }
-    } else if (d.kind == 2) {
-        int64_t nn = (int64_t)d.n * d.n;
-        for (int64_t k = 0; k < nn; k++) ret = fma_acc<L>(ret, ldm<L>(dv, d.off + k), ldm<L>(D, doff + k));
+__device__ mp<L> block_sum(mp<L> v, uint32_t *sh) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    lds_st<L>(sh, nt, t, v);
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (t < h) lds_st<L>(sh, nt, t, add<L>(lds_ld<L>(sh, nt, t), lds_ld<L>(sh, nt, t + h)));
+        __syncthreads();
     }
-    return ret;
+    return lds_ld<L>(sh, nt, 0);
 }
 
-// For each constraint c (and optionally C at index m): ip_c = A_c . D (SDP blocks + LP),
-// then  mode 0: out_c = -ip + R_c   (Make_gVec: gVec = -(A.DLS1) + primalVec)
-//       mode 1: out_c =  b_c - ip   (Residuals: primalVec = b - A.X)
-//       mode 2: out_c =  ip          (objective C.X, single constraint)
+// one wave-wide sum (64 lanes, LDS planes of the block)
 template <int L>
-__global__ void k_con_inner(int ncon, int nsdp, const Desc *desc, const int32_t *ri, const int32_t *ci, PV sv,
-                            PV dv, const int64_t *lp_ptr, const int32_t *lp_pos, PV lp_val, int64_t lpbase, PV D,
-                            PV R, PV out, int mode) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncon) return;
-    mp<L> ret = zero<L>();
-    for (int l = 0; l < nsdp; l++) {
-        Desc d = desc[(int64_t)c * nsdp + l];
-        if (d.kind == 0) continue;
-        ret = add<L>(ret, blk_inner<L>(d, ri, ci, sv, dv, D, d.blkoff));
+__device__ mp<L> wave_sum64(mp<L> v, uint32_t *sh) {
+    const int t = threadIdx.x, lane = t & 63;
+    lds_st<L>(sh, blockDim.x, t, v);
+    __syncthreads();
+    for (int h = 32; h >= 1; h >>= 1) {
+        if (lane < h) lds_st<L>(sh, blockDim.x, t, add<L>(lds_ld<L>(sh, blockDim.x, t), lds_ld<L>(sh, blockDim.x, t + h)));
+        __syncthreads();
     }
-    for (int64_t t = lp_ptr[c]; t < lp_ptr[c + 1]; t++) ret = fma_acc<L>(ret, ldm<L>(lp_val, t), ldm<L>(D, lpbase + lp_pos[t]));
+    return lds_ld<L>(sh, blockDim.x, t - lane);
+}
+
+// For each constraint c: ip_c = A_c . D (SDP blocks + LP), one wave per constraint (lanes stride
+// over the stored entries, then a tree sum), then
+//       mode 0: out_c = -ip + R_c   (Make_gVec: gVec = -(A.DLS1) + primalVec)
+//       mode 1: out_c =  b_c - ip   (Residuals: primalVec = b - A.X)
+//       mode 2: out_c =  ip          (objective C.X)
+// Sparse blocks follow the _OPENMP branch of sdpa_linear.cpp:235-254 per entry (v D_ii, or
+// v (D_ij + D_ji)), dense blocks Rdot over n^2 values; the summation order is the tree's.
+template <int L>
+__global__ __launch_bounds__(256) void k_con_inner(int ncon, int nsdp, const Desc *desc, const int32_t *ri,
+                                                   const int32_t *ci, PV sv, PV dv, const int64_t *lp_ptr,
+                                                   const int32_t *lp_pos, PV lp_val, int64_t lpbase, PV D, PV R,
+                                                   PV out, int mode, int whole_block) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    // whole_block: the workgroup works on constraint blockIdx.x; otherwise one wave per constraint
+    const int lane = whole_block ? threadIdx.x : (threadIdx.x & 63);
+    const int G = whole_block ? blockDim.x : 64;
+    const int c = whole_block ? blockIdx.x : blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    mp<L> acc = zero<L>();
+    if (c < ncon) {
+        for (int l = 0; l < nsdp; l++) {
+            Desc d = desc[(int64_t)c * nsdp + l];
+            if (d.kind == 1) {
+                for (int t = lane; t < d.nnz; t += G) {
+                    int64_t e = d.off + t;
+                    int i = ri[e], j = ci[e];
+                    mp<L> v = ldm<L>(sv, e);
+                    mp<L> temp;
+                    if (i == j) {
+                        temp = mul<L>(v, ldm<L>(D, d.blkoff + i + (int64_t)d.n * j));
+                    } else {
+                        temp = add<L>(ldm<L>(D, d.blkoff + i + (int64_t)d.n * j),
+                                      ldm<L>(D, d.blkoff + j + (int64_t)d.n * i));
+                        temp = mul<L>(temp, v);
+                    }
+                    acc = add<L>(acc, temp);
+                }
+            } else if (d.kind == 2) {
+                int64_t nn = (int64_t)d.n * d.n;
+                for (int64_t k = lane; k < nn; k += G)
+                    acc = fma_acc<L>(acc, ldm<L>(dv, d.off + k), ldm<L>(D, d.blkoff + k));
+            }
+        }
+        for (int64_t t = lp_ptr[c] + lane; t < lp_ptr[c + 1]; t += G)
+            acc = fma_acc<L>(acc, ldm<L>(lp_val, t), ldm<L>(D, lpbase + lp_pos[t]));
+    }
+    mp<L> ret = whole_block ? block_sum<L>(acc, sh) : wave_sum64<L>(acc, sh);
+    if (c >= ncon || lane != 0) return;
     mp<L> o;
     if (mode == 0) {
         o = add<L>(neg<L>(ret), ldm<L>(R, c));
@@ -238,15 +269,6 @@ __global__ void k_maxabs(PV X, int64_t cnt, PV part) {
     }
 }
 
-// sequential dot product (small vectors): out[0] = sum_i x_i y_i
-template <int L>
-__global__ void k_dot_seq(PV x, PV y, int64_t n, PV out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    mp<L> acc = zero<L>();
-    for (int64_t i = 0; i < n; i++) acc = fma_acc<L>(acc, ldm<L>(x, i), ldm<L>(y, i));
-    stm<L>(out, 0, acc);
-}
-
 // ---- Schur complement ----
 
 // F_c = A_c * invZ for sparse A_c by the symmetric row expansion (Lal::multiply(D,S,D),
@@ -282,6 +304,31 @@ __device__ __forceinline__ void bacc(PV B, int m, int i, int j, const mp<L> &v) 
         int64_t b = j + (int64_t)m * i;
         stm<L>(B, b, add<L>(ldm<L>(B, b), v));
     }
+}
+
+// A (block) . D by one thread (sparse: the _OPENMP branch of sdpa_linear.cpp:235-254; dense: Rdot)
+template <int L>
+__device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, PV sv, PV dv, PV D, int64_t doff) {
+    mp<L> ret = zero<L>();
+    if (d.kind == 1) {
+        for (int t = 0; t < d.nnz; t++) {
+            int64_t e = d.off + t;
+            int i = ri[e], j = ci[e];
+            mp<L> v = ldm<L>(sv, e);
+            mp<L> temp;
+            if (i == j) {
+                temp = mul<L>(v, ldm<L>(D, doff + i + (int64_t)d.n * j));
+            } else {
+                temp = add<L>(ldm<L>(D, doff + i + (int64_t)d.n * j), ldm<L>(D, doff + j + (int64_t)d.n * i));
+                temp = mul<L>(temp, v);
+            }
+            ret = add<L>(ret, temp);
+        }
+    } else if (d.kind == 2) {
+        int64_t nn = (int64_t)d.n * d.n;
+        for (int64_t k = 0; k < nn; k++) ret = fma_acc<L>(ret, ldm<L>(dv, d.off + k), ldm<L>(D, doff + k));
+    }
+    return ret;
 }
 
 // F1: value = A_j . G_i (Newton::calF1, sdpa_newton.cpp:897)
@@ -537,23 +584,6 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 // column below is scaled by r (the reference divides by L_jj, mplapack/Rtrsm.cpp:264-270).
 // info receives the first failing global pivot (1-based, as Rpotrf2's info,
 // mplapack/Rpotrf2.cpp:86-97); once info is set, later launches return at once.
-template <int L>
-__device__ __forceinline__ mp<L> lds_ld(const uint32_t *sh, int plane, int t) {
-    mp<L> x;
-#pragma unroll
-    for (int q = 0; q < L; q++) x.m[q] = sh[q * plane + t];
-    int32_t h = (int32_t)sh[L * plane + t];
-    x.e = h >> 1;
-    x.s = (uint32_t)h & 1u;
-    return x;
-}
-template <int L>
-__device__ __forceinline__ void lds_st(uint32_t *sh, int plane, int t, const mp<L> &x) {
-#pragma unroll
-    for (int q = 0; q < L; q++) sh[q * plane + t] = x.m[q];
-    sh[L * plane + t] = (uint32_t)((x.e << 1) | (int32_t)(x.s & 1u));
-}
-
 // The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
 // serial pivot chain runs; only the finished factor, Linv and the pivots go back to HBM.
 // The matrix is A[off + i + ld*j]; the block starts at (k, k); rdiag[roff + k + j] receives the
@@ -637,48 +667,62 @@ __global__ void k_copy2d(PV dst, int64_t doff, int ldd, PV src, int64_t soff, in
     stm<L>(dst, doff + i + (int64_t)ldd * j, ldm<L>(src, soff + i + (int64_t)lds * j));
 }
 
-// Lal::solveSystems: L y = g (forward) then L^T x = y (backward), single workgroup.
-// The division by L_jj is a multiplication by the full-width reciprocal kept by the factorisation.
+// Lal::solveSystems (sdpa_linear.cpp:683-695): L y = g then L^T x = y, blocked by the
+// factorisation's diagonal blocks.  Diagonal step: x_k = inv(L_kk) x_k (trans: inv(L_kk)^T x_k)
+// with the stored block inverse; off-diagonal step: the remaining entries of x lose the block's
+// contribution.  One workgroup per diagonal step; 8 threads share each output's dot product.
 template <int L>
-__global__ void k_trsv_both(PV A, int m, PV rdiag, PV x) {
-    __shared__ uint32_t sx[L + 2];
-    for (int j = 0; j < m; j++) {
-        if (threadIdx.x == 0) {
-            mp<L> v = mul<L>(ldm<L>(x, j), ldm<L>(rdiag, j));
-            stm<L>(x, j, v);
-            sx[0] = (uint32_t)v.e;
-            sx[1] = v.s;
-#pragma unroll
-            for (int q = 0; q < L; q++) sx[2 + q] = v.m[q];
+__global__ __launch_bounds__(256) void k_trsv_diag(PV linv, int64_t loff, int nb, PV x, int64_t xo, int trans) {
+    __shared__ uint32_t sx[(L + 1) * 32];
+    __shared__ uint32_t sp[(L + 1) * 256];
+    const int tid = threadIdx.x, i = tid >> 3, q = tid & 7;
+    if (tid < nb) lds_st<L>(sx, 32, tid, ldm<L>(x, xo + tid));
+    __syncthreads();
+    mp<L> acc = zero<L>();
+    if (i < nb)
+        for (int j = q; j < nb; j += 8) {
+            if (trans ? (j < i) : (j > i)) continue;
+            mp<L> a = ldm<L>(linv, loff + (trans ? (j + (int64_t)nb * i) : (i + (int64_t)nb * j)));
+            acc = add<L>(acc, mul<L>(a, lds_ld<L>(sx, 32, j)));
         }
-        __syncthreads();
-        mp<L> xj;
-        xj.e = (int32_t)sx[0];
-        xj.s = sx[1];
-#pragma unroll
-        for (int q = 0; q < L; q++) xj.m[q] = sx[2 + q];
-        for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x)
-            stm<L>(x, i, sub<L>(ldm<L>(x, i), mul<L>(xj, ldm<L>(A, i + (int64_t)m * j))));
+    lds_st<L>(sp, 256, tid, acc);
+    __syncthreads();
+    for (int h = 4; h >= 1; h >>= 1) {
+        if (q < h) lds_st<L>(sp, 256, tid, add<L>(lds_ld<L>(sp, 256, tid), lds_ld<L>(sp, 256, tid + h)));
         __syncthreads();
     }
-    for (int j = m - 1; j >= 0; j--) {
-        if (threadIdx.x == 0) {
-            mp<L> v = mul<L>(ldm<L>(x, j), ldm<L>(rdiag, j));
-            stm<L>(x, j, v);
-            sx[0] = (uint32_t)v.e;
-            sx[1] = v.s;
-#pragma unroll
-            for (int q = 0; q < L; q++) sx[2 + q] = v.m[q];
+    if (q == 0 && i < nb) stm<L>(x, xo + i, lds_ld<L>(sp, 256, tid));
+}
+
+// forward (trans 0): x[r] -= sum_t A[r, k0+t] x[k0+t] for r >= k0+nb;
+// backward (trans 1): x[r] -= sum_t A[k0+t, r] x[k0+t] for r < k0
+template <int L>
+__global__ __launch_bounds__(256) void k_trsv_update(PV A, int m, int k0, int nb, PV x, int trans) {
+    __shared__ uint32_t sx[(L + 1) * 32];
+    __shared__ uint32_t sp[(L + 1) * 256];
+    const int tid = threadIdx.x, q = tid & 7;
+    const int r0 = trans ? 0 : k0 + nb;
+    const int nrows = trans ? k0 : m - k0 - nb;
+    const int r = blockIdx.x * 32 + (tid >> 3);
+    if (tid < nb) lds_st<L>(sx, 32, tid, ldm<L>(x, k0 + tid));
+    __syncthreads();
+    mp<L> acc = zero<L>();
+    if (r < nrows) {
+        int row = r0 + r;
+        for (int t = q; t < nb; t += 8) {
+            int64_t idx = trans ? ((int64_t)(k0 + t) + (int64_t)m * row) : ((int64_t)row + (int64_t)m * (k0 + t));
+            acc = add<L>(acc, mul<L>(ldm<L>(A, idx), lds_ld<L>(sx, 32, t)));
         }
+    }
+    lds_st<L>(sp, 256, tid, acc);
+    __syncthreads();
+    for (int h = 4; h >= 1; h >>= 1) {
+        if (q < h) lds_st<L>(sp, 256, tid, add<L>(lds_ld<L>(sp, 256, tid), lds_ld<L>(sp, 256, tid + h)));
         __syncthreads();
-        mp<L> xj;
-        xj.e = (int32_t)sx[0];
-        xj.s = sx[1];
-#pragma unroll
-        for (int q = 0; q < L; q++) xj.m[q] = sx[2 + q];
-        for (int i = threadIdx.x; i < j; i += blockDim.x)
-            stm<L>(x, i, sub<L>(ldm<L>(x, i), mul<L>(xj, ldm<L>(A, j + (int64_t)m * i))));
-        __syncthreads();
+    }
+    if (q == 0 && r < nrows) {
+        int row = r0 + r;
+        stm<L>(x, row, sub<L>(ldm<L>(x, row), lds_ld<L>(sp, 256, tid)));
     }
 }
 
@@ -693,19 +737,6 @@ __global__ void k_lp_triple(PV out, PV A, PV Bm, PV C, int64_t base, int nlp) {
 }
 
 // ---- reductions, matrix-vector products and point factorisation (Solutions / Jal / Lal) ----
-
-// block-tree sum of one mp value per thread (blockDim.x a power of two, LDS planes of blockDim.x)
-template <int L>
-__device__ mp<L> block_sum(mp<L> v, uint32_t *sh) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    lds_st<L>(sh, nt, t, v);
-    __syncthreads();
-    for (int h = nt / 2; h >= 1; h >>= 1) {
-        if (t < h) lds_st<L>(sh, nt, t, add<L>(lds_ld<L>(sh, nt, t), lds_ld<L>(sh, nt, t + h)));
-        __syncthreads();
-    }
-    return lds_ld<L>(sh, nt, 0);
-}
 
 // part[blockIdx.x] = sum over this block's share of x_i * y_i (y.p == nullptr: of x_i); when idx is
 // given, element t is x[xo + idx[t]]
@@ -821,6 +852,7 @@ struct EngineBase {
     virtual int put(int what, const uint32_t *in) = 0;
     virtual int sync() = 0;
     virtual bool slices_ok(int s) const = 0;
+    virtual int profile(int on, double *out, int n) = 0;
 };
 
 template <int L>
@@ -872,6 +904,7 @@ struct Engine : EngineBase {
     int nlppairs = 0;
     int *d_info = nullptr;
     std::vector<void *> allocs;
+    KProf kprof;
     GemmEngine<L> gemm;
 
     ~Engine() override {
@@ -989,7 +1022,7 @@ struct Engine : EngineBase {
         blk.assign((size_t)(m + 1) * nsdp, Blk());
         lp_pos.assign(m + 1, {});
         lp_val.assign(m + 1, {});
-        return gemm.init(st);
+        return gemm.init(st, &kprof);
     }
 
     int commit() override {
@@ -1336,8 +1369,8 @@ struct Engine : EngineBase {
     int residuals(uint32_t *nrp, uint32_t *nrd, uint32_t *objP, uint32_t *objD) override {
         tstart();
         // Rp = b - A.X
-        k_con_inner<L><<<cdiv(m, 64), 64, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
-                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), Rp.pv(), 1);
+        k_con_inner<L><<<cdiv(m, 4), 256, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
+                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), Rp.pv(), 1, 0);
         // Rd = C - Z - sum y_i A_i
         k_scatter<L><<<cdiv(Nd, 128), 128, 0, st>>>(Nd, d_pos_ptr, d_pos_con, d_pos_ent, scat_vals.pv(), dval.pv(),
                                                      d_pos_blk, d_pos_in_blk, d_dcon_ptr, d_dcon, d_dcon_off, y.pv(),
@@ -1354,17 +1387,16 @@ struct Engine : EngineBase {
             pack1(v, nrd);
         }
         if (objP) {  // C . X
-            k_con_inner<L><<<1, 64, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr,
-                                             d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), PV{scal.p, scal.n}, 2);
+            k_con_inner<L><<<1, 256, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr,
+                                              d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), PV{scal.p, scal.n}, 2,
+                                              1);
             std::vector<uint32_t> h(L + 2);
             if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
             memcpy(objP, h.data(), sizeof(uint32_t) * (L + 2));
         }
         if (objD) {
-            k_dot_seq<L><<<1, 1, 0, st>>>(b.pv(), y.pv(), m, scal.pv());
-            std::vector<uint32_t> h(L + 2);
-            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
-            memcpy(objD, h.data(), sizeof(uint32_t) * (L + 2));
+            if ((rc = dot(b, 0, &y, 0, nullptr, m, v))) return rc;
+            pack1(v, objD);
         }
         return 0;
     }
@@ -1388,8 +1420,8 @@ struct Engine : EngineBase {
         } else {
             HIPCHK(hipMemcpyAsync(DLS1.p, rzinv.p, sizeof(uint32_t) * (L + 1) * Nd, hipMemcpyDeviceToDevice, st));
         }
-        k_con_inner<L><<<cdiv(m, 64), 64, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
-                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], DLS1.pv(), Rp.pv(), g.pv(), 0);
+        k_con_inner<L><<<cdiv(m, 4), 256, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
+                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], DLS1.pv(), Rp.pv(), g.pv(), 0, 0);
         return tstop(3);
     }
 
@@ -1507,7 +1539,9 @@ struct Engine : EngineBase {
         for (int k = 0; k < n; k += NBK) {
             int nb = std::min(NBK, n - k);
             int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
+            hipEvent_t ev = kprof.begin(st);
             k_potrf_diag<L><<<1, 256, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
+            kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0);
             if (k + nb < n) {
                 int r = n - k - nb;
                 int64_t a21 = off + (k + nb) + (int64_t)ld * k;
@@ -1643,8 +1677,8 @@ struct Engine : EngineBase {
         case SDPG_IP_CX:
         case SDPG_IP_CDX: {
             const Arr &M_ = what == SDPG_IP_CX ? X : dX;
-            k_con_inner<L><<<1, 64, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr, d_lp_pos,
-                                             lpval.pv(), blkoff[nsdp], M_.pv(), b.pv(), PV{scal.p, scal.n}, 2);
+            k_con_inner<L><<<1, 256, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr, d_lp_pos,
+                                              lpval.pv(), blkoff[nsdp], M_.pv(), b.pv(), PV{scal.p, scal.n}, 2, 1);
             std::vector<uint32_t> h(L + 2);
             if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
             v = unpack1(h.data());
@@ -1778,7 +1812,17 @@ struct Engine : EngineBase {
     int potrs() override {
         tstart();
         HIPCHK(hipMemcpyAsync(dy.p, g.p, sizeof(uint32_t) * (L + 1) * dy.n, hipMemcpyDeviceToDevice, st));
-        k_trsv_both<L><<<1, 256, 0, st>>>(Bm.pv(), m, rdiag.pv(), dy.pv());
+        const int nblk = (m + NBK - 1) / NBK;
+        for (int kb = 0; kb < nblk; kb++) {  // L y = g
+            int k0 = kb * NBK, nb = std::min(NBK, m - k0);
+            k_trsv_diag<L><<<1, 256, 0, st>>>(LinvB.pv(), (int64_t)kb * NBK * NBK, nb, dy.pv(), k0, 0);
+            if (k0 + nb < m) k_trsv_update<L><<<cdiv(m - k0 - nb, 32), 256, 0, st>>>(Bm.pv(), m, k0, nb, dy.pv(), 0);
+        }
+        for (int kb = nblk - 1; kb >= 0; kb--) {  // L^T x = y
+            int k0 = kb * NBK, nb = std::min(NBK, m - k0);
+            k_trsv_diag<L><<<1, 256, 0, st>>>(LinvB.pv(), (int64_t)kb * NBK * NBK, nb, dy.pv(), k0, 1);
+            if (k0 > 0) k_trsv_update<L><<<cdiv(k0, 32), 256, 0, st>>>(Bm.pv(), m, k0, nb, dy.pv(), 1);
+        }
         HIPCHK(hipGetLastError());
         return tstop(2);
     }
@@ -1847,6 +1891,20 @@ struct Engine : EngineBase {
         return 0;
     }
     bool slices_ok(int s) const override { return OzakiWork<L>::slices_ok(s) != 0; }
+    // on: 1 start (reset), 0 stop; out (n >= 3 * KProf::NK): count, ms, work per kernel kind
+    int profile(int on, double *out, int n) override {
+        HIPCHK(hipStreamSynchronize(st));
+        kprof.flush();
+        if (out)
+            for (int k = 0; k < KProf::NK && 3 * k + 2 < n; k++) {
+                out[3 * k] = (double)kprof.cnt[k];
+                out[3 * k + 1] = kprof.ms[k];
+                out[3 * k + 2] = kprof.work[k];
+            }
+        if (on) kprof.reset();
+        kprof.on = on != 0;
+        return 0;
+    }
 };
 
 // Standalone multiple-precision GEMM on host buffers (packed words), one call = one stream:
@@ -2199,6 +2257,12 @@ int sdpg_reset_timers(sdpg_ctx *ctx) {
 int sdpg_sync(sdpg_ctx *ctx) {
     if (!ctx || !ctx->eng) return SDPG_EARG;
     return ctx->eng->sync();
+}
+
+int sdpg_profile(sdpg_ctx *ctx, int on, double *out, int n) {
+    if (!ctx || !ctx->eng) return SDPG_EARG;
+    hipSetDevice(ctx->eng->device);
+    return ctx->eng->profile(on, out, n);
 }
 
 int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "mpx.h"
 
 namespace sdpg {
@@ -36,6 +38,91 @@ __device__ __forceinline__ void stm(PV v, int64_t i, const mp<L> &x) {
 #pragma unroll
     for (int k = 0; k < L; k++) v.p[(int64_t)k * v.n + i] = x.m[k];
     v.p[(int64_t)L * v.n + i] = (uint32_t)((x.e << 1) | (int32_t)(x.s & 1u));
+}
+
+// Per-kernel device time over a measured region: events recorded on the launching stream around
+// each launch of the kernels of interest, summed after the region ends.
+// kinds: 0 Ozaki MFMA product kernel, 1 plain limb product kernel, 2 Cholesky diagonal block
+struct KProf {
+    static constexpr int NK = 4;
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+        double work;
+    };
+    std::vector<Rec> live;
+    std::vector<hipEvent_t> pool;
+    long long cnt[NK] = {0};
+    double ms[NK] = {0}, work[NK] = {0};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t begin(hipStream_t st) {
+        if (!on) return nullptr;
+        hipEvent_t e = get();
+        (void)hipEventRecord(e, st);
+        return e;
+    }
+    void end(hipStream_t st, hipEvent_t a, int kind, double w) {
+        if (!on || !a) return;
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, st);
+        live.push_back(Rec{kind, a, b, w});
+        if (live.size() > 4096) flush();
+    }
+    void flush() {
+        for (Rec &r : live) {
+            float t = 0;
+            if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+                ms[r.kind] += t;
+                cnt[r.kind] += 1;
+                work[r.kind] += r.work;
+            }
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        live.clear();
+    }
+    void reset() {
+        flush();
+        for (int k = 0; k < NK; k++) {
+            cnt[k] = 0;
+            ms[k] = work[k] = 0;
+        }
+    }
+    ~KProf() {
+        for (Rec &r : live) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// mp values in LDS planes: word q of element t at sh[q * plane + t], head at sh[L * plane + t]
+template <int L>
+__device__ __forceinline__ mp<L> lds_ld(const uint32_t *sh, int plane, int t) {
+    mp<L> x;
+#pragma unroll
+    for (int q = 0; q < L; q++) x.m[q] = sh[q * plane + t];
+    int32_t h = (int32_t)sh[L * plane + t];
+    x.e = h >> 1;
+    x.s = (uint32_t)h & 1u;
+    return x;
+}
+template <int L>
+__device__ __forceinline__ void lds_st(uint32_t *sh, int plane, int t, const mp<L> &x) {
+#pragma unroll
+    for (int q = 0; q < L; q++) sh[q * plane + t] = x.m[q];
+    sh[L * plane + t] = (uint32_t)((x.e << 1) | (int32_t)(x.s & 1u));
 }
 
 template <int L>

@@ -97,8 +97,10 @@ struct GemmEngine {
     hipStream_t st = nullptr;
     PlainWork<L> plain;
     OzakiWork<L> ozaki;
-    int init(hipStream_t s) {
+    int init(hipStream_t s, KProf *prof = nullptr) {
         st = s;
+        plain.prof = prof;
+        ozaki.prof = prof;
         return 0;
     }
     // mode: 0 plain, 1 ozaki, -1 element (reference) form

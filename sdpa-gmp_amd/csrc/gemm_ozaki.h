@@ -203,6 +203,7 @@ struct OzSlices {
 
 template <int L>
 struct OzakiWork {
+    KProf *prof = nullptr;
     int8_t *as = nullptr, *bs = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
     size_t cap_as = 0, cap_bs = 0, cap_ea = 0, cap_eb = 0;
@@ -240,17 +241,16 @@ struct OzakiWork {
         rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
         rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
         if (rc) return -4;
-        int64_t nea = (int64_t)nbA * g.M, neb = (int64_t)nbB * g.N;
-        k_fill_i32<<<(int)((nea + 255) / 256), 256, 0, st>>>(ea, nea, mpx::EZERO);
-        k_fill_i32<<<(int)((neb + 255) / 256), 256, 0, st>>>(eb, neb, mpx::EZERO);
-        int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
-        k_fix_max<L><<<dim3((unsigned)((ta + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, ea);
-        k_fix_max<L><<<dim3((unsigned)((tb + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, eb);
+        k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
+        k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
         int64_t sa_n = (int64_t)Mp * nch * OZ_KC, sb_n = (int64_t)Np * nch * OZ_KC;
         k_oz_slice<L, S><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, ea, as);
         k_oz_slice<L, S><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, eb, bs);
         dim3 grd(Mp / OZ_T, Np / OZ_T, g.batch);
+        hipEvent_t ev = prof ? prof->begin(st) : nullptr;
         k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, as, bs, ea, eb, nbA, nbB, Mp, Np, nch);
+        // algorithmic INT8 operations: 2 per multiply-add, S(S+1)/2 slice pairs per mp multiply-add
+        if (prof) prof->end(st, ev, 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     int run(const GemmArgs &g, hipStream_t st, int slices) {

@@ -145,22 +145,47 @@ __device__ __forceinline__ void fx_convert(const mpx::mp<L> &x, int32_t emax, in
 // which = 1 -> op(B)[k, r].  contig() tells whether r (true) or k is the memory-contiguous index,
 // so that consecutive threads walk consecutive addresses.
 
-// pass 1: per-row largest exponent (atomicMax over the row's elements)
+// pass 1: per-row largest exponent e_r (EZERO for an all-zero row), written once per row:
+// a thread per row when consecutive rows are consecutive in memory, else a wave per row
 template <int L>
 __global__ void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = blockIdx.y;
-    int K = g.K;
-    if (t >= (int64_t)R * K) return;
-    bool rc = fx_rcontig(g, which);
-    int r = rc ? (int)(t % R) : (int)(t / K);
-    int k = rc ? (int)(t / R) : (int)(t % K);
     PV src = which == 0 ? g.A : g.B;
-    int64_t idx = fx_at(g, which, b, r, k);
-    uint32_t top = src.p[(int64_t)(L - 1) * src.n + idx];
-    if (top == 0) return;
-    int32_t e = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
-    atomicMax(&eo[(int64_t)b * R + r], e);
+    const int K = g.K;
+    if (fx_rcontig(g, which)) {
+        int r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r >= R) return;
+        int32_t e = mpx::EZERO;
+        for (int k = 0; k < K; k++) {
+            int64_t idx = fx_at(g, which, b, r, k);
+            if (src.p[(int64_t)(L - 1) * src.n + idx] == 0) continue;
+            int32_t x = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
+            e = x > e ? x : e;
+        }
+        eo[(int64_t)b * R + r] = e;
+    } else {
+        int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        int lane = threadIdx.x & 63;
+        int32_t e = mpx::EZERO;
+        if (r < R)
+            for (int k = lane; k < K; k += 64) {
+                int64_t idx = fx_at(g, which, b, r, k);
+                if (src.p[(int64_t)(L - 1) * src.n + idx] == 0) continue;
+                int32_t x = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
+                e = x > e ? x : e;
+            }
+        for (int off = 32; off >= 1; off >>= 1) {
+            int32_t o = __shfl_xor(e, off, 64);
+            e = o > e ? o : e;
+        }
+        if (r < R && lane == 0) eo[(int64_t)b * R + r] = e;
+    }
+}
+
+// grid of k_fix_max for R rows
+inline dim3 fix_max_grid(const GemmArgs &g, int which, int R, int nb) {
+    bool rc = which == 0 ? !g.transA : g.transB;
+    return rc ? dim3((unsigned)((R + 255) / 256), nb) : dim3((unsigned)((R + 3) / 4), nb);
 }
 
 // pass 2: a' = round(x * 2^(W - e_r)) + 2^(32L-1), W = 32L - 2; out[((q * K) + k) * R + r] per batch
@@ -287,8 +312,19 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
     stm<L>(g.C, cidx, res);
 }
 
+// outputs a GEMM call computes (the lower triangle only when g.lower), times the batch
+inline double gemm_outputs(const GemmArgs &g) {
+    double o = (double)g.M * g.N;
+    if (g.lower) {
+        double mn = g.M < g.N ? g.M : g.N;
+        o = mn * (mn + 1) / 2 + (g.M > g.N ? (double)(g.M - g.N) * g.N : 0.0);
+    }
+    return o * g.batch;
+}
+
 template <int L>
 struct PlainWork {
+    KProf *prof = nullptr;
     uint32_t *af = nullptr, *bf = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
     size_t cap_af = 0, cap_bf = 0, cap_ea = 0, cap_eb = 0;
@@ -323,17 +359,16 @@ struct PlainWork {
         rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
         rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
         if (rc) return -4;
-        int64_t nea = (int64_t)nbA * g.M, neb = (int64_t)nbB * g.N;
-        k_fill_i32<<<(int)((nea + 255) / 256), 256, 0, st>>>(ea, nea, mpx::EZERO);
-        k_fill_i32<<<(int)((neb + 255) / 256), 256, 0, st>>>(eb, neb, mpx::EZERO);
         int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
         dim3 ga((unsigned)((ta + 255) / 256), nbA), gb((unsigned)((tb + 255) / 256), nbB);
-        k_fix_max<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea);
-        k_fix_max<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb);
+        k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
+        k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
         k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea, af);
         k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb, bf);
         dim3 grd((g.M + PT - 1) / PT, (g.N + PT - 1) / PT, g.batch);
+        hipEvent_t ev = prof ? prof->begin(st) : nullptr;
         k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, bf, ea, eb, nbA, nbB);
+        if (prof) prof->end(st, ev, 1, gemm_outputs(g) * (double)g.K);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
 };

@@ -760,6 +760,8 @@ struct Solver {
             res->t_gvec_ms = tm[3];
             res->t_dxdz_ms = tm[4];
             res->t_res_ms = tm[5];
+            res->t_inv_ms = tm[6];
+            res->t_eig_ms = tm[7];
             res->t_total_s = t1 - t0;
             res->limbs = L;
         }
@@ -797,7 +799,8 @@ static int solve_L(const RawProblem &R, const Parameter &prm, const char *log_pa
 
 template <int L>
 static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int slices, int device, int warm_iters,
-                   int warmup, int steps, double *ms_schur, double *ms_chol, double *ms_wall) {
+                   int warmup, int steps, double *ms_schur, double *ms_chol, double *ms_wall, double *kstats,
+                   const char *dump_prefix) {
     Solver<L> S;
     S.prm = prm;
     int rc = S.load(R);
@@ -805,12 +808,28 @@ static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int
     S.compute_formula();
     if ((rc = S.setup_device(device, gemm_mode, slices))) return rc;
     if ((rc = S.run(nullptr, warm_iters))) return rc;
+    if (dump_prefix) {  // the iterate (X, invZ) for a CPU replay of the same step
+        const int W = L + 2;
+        std::vector<uint32_t> buf((size_t)S.Nd * W);
+        const int sel[2] = {SDPG_X, SDPG_INVZ};
+        const char *nm[2] = {"X.bin", "invZ.bin"};
+        for (int q = 0; q < 2; q++) {
+            if ((rc = sdpg_get(S.ctx, sel[q], buf.data()))) return rc;
+            std::string path = std::string(dump_prefix) + nm[q];
+            FILE *f = fopen(path.c_str(), "wb");
+            if (!f) return -21;
+            size_t wr = fwrite(buf.data(), sizeof(uint32_t), buf.size(), f);
+            fclose(f);
+            if (wr != buf.size()) return -21;
+        }
+    }
     // the device holds the point of iteration warm_iters; time [schur + potrf] on it
     for (int w = 0; w < warmup; w++) {
         int info = 0;
         if ((rc = sdpg_schur(S.ctx)) || (rc = sdpg_potrf(S.ctx, &info))) return rc;
     }
     sdpg_reset_timers(S.ctx);
+    if (kstats) sdpg_profile(S.ctx, 1, nullptr, 0);
     double t0 = now_s();
     for (int s = 0; s < steps; s++) {
         int info = 0;
@@ -818,6 +837,7 @@ static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int
     }
     sdpg_sync(S.ctx);
     double t1 = now_s();
+    if (kstats) sdpg_profile(S.ctx, 0, kstats, 12);
     double tm[8] = {0};
     sdpg_timers(S.ctx, tm, 8);
     *ms_schur = tm[0] / (steps > 0 ? steps : 1);
@@ -890,7 +910,7 @@ extern "C" int sdpa_amd_ctx_from_file(const char *data_path, int prec_bits, int 
 
 extern "C" int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_override, int gemm_mode,
                               int slices, int device, int warm_iters, int warmup, int steps, double *ms_schur,
-                              double *ms_chol, double *ms_wall) {
+                              double *ms_chol, double *ms_wall, double *kstats, const char *dump_prefix) {
     Parameter prm;
     if (param_path && !prm.read(param_path)) return -20;
     if (prec_override > 0) prm.precision = prec_override;
@@ -898,7 +918,9 @@ extern "C" int sdpa_amd_bench(const char *data_path, const char *param_path, int
     int rc = read_raw(data_path, R);
     if (rc) return rc;
     int L = sdpg_limbs(prm.precision);
-#define CALL_(LL) bench_L<LL>(R, prm, gemm_mode, slices, device, warm_iters, warmup, steps, ms_schur, ms_chol, ms_wall)
+#define CALL_(LL)                                                                                                 \
+    bench_L<LL>(R, prm, gemm_mode, slices, device, warm_iters, warmup, steps, ms_schur, ms_chol, ms_wall, kstats, \
+                dump_prefix)
     DISPATCH_L(L, CALL_)
 #undef CALL_
 }

@@ -35,7 +35,7 @@ EXPORTS = [
     "sdpg_set_formula", "sdpg_commit", "sdpg_set_gemm", "sdpg_set_point", "sdpg_residuals", "sdpg_gvec",
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
-    "sdpg_inner", "sdpg_min_eigen",
+    "sdpg_inner", "sdpg_min_eigen", "sdpg_profile",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file",
 ]
 
@@ -86,7 +86,9 @@ def load(path: str | None = None):
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
     lib.sdpa_amd_bench.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double)]
+                                   P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
+                                   ctypes.c_char_p]
+    lib.sdpg_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
     _lib = lib
     return lib
 
@@ -302,7 +304,8 @@ class Result(ctypes.Structure):
                 ("pfeas", ctypes.c_char * 48), ("dfeas", ctypes.c_char * 48), ("mu", ctypes.c_double),
                 ("t_schur_ms", ctypes.c_double), ("t_chol_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_gvec_ms", ctypes.c_double), ("t_dxdz_ms", ctypes.c_double), ("t_res_ms", ctypes.c_double),
-                ("t_total_s", ctypes.c_double), ("limbs", ctypes.c_int)]
+                ("t_total_s", ctypes.c_double), ("limbs", ctypes.c_int), ("t_inv_ms", ctypes.c_double),
+                ("t_eig_ms", ctypes.c_double)]
 
 
 def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path: str | None = None,
@@ -318,12 +321,17 @@ def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path
 
 
 def bench(data_path: str, param_path: str | None, prec: int, gemm: int, slices: int, warm_iters: int,
-          warmup: int, steps: int, device: int = 0):
-    """Per-step device ms of [schur, potrf] and wall ms on the iterate after warm_iters iterations."""
+          warmup: int, steps: int, device: int = 0, dump_prefix: str | None = None):
+    """Per-step device ms of [schur, potrf], wall ms, and the per-kernel figures of sdpg_profile
+    (dict kind -> (launches, ms, work)) on the iterate after warm_iters iterations."""
     lib = load()
     a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    ks = (ctypes.c_double * 12)()
     rc = lib.sdpa_amd_bench(data_path.encode(), param_path.encode() if param_path else None, prec, gemm, slices,
-                            device, warm_iters, warmup, steps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+                            device, warm_iters, warmup, steps, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ks,
+                            dump_prefix.encode() if dump_prefix else None)
     if rc:
         raise SdpgError(f"sdpa_amd_bench rc={rc}: {lib.sdpg_last_error().decode()}")
-    return a.value, b.value, c.value
+    kinds = ("ozaki_mfma", "plain_limb", "chol_diag")
+    kst = {k: (int(ks[3 * i]), ks[3 * i + 1], ks[3 * i + 2]) for i, k in enumerate(kinds)}
+    return a.value, b.value, c.value, kst

@@ -58,6 +58,9 @@ def lib():
         L.orc_param_read.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.orc_time_hotpath.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.orc_time_point.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -94,6 +97,16 @@ def time_hotpath(data: str, param: str | None, prec: int, warm: int, reps: int, 
                                 ctypes.byref(b))
     if rc < 0:
         raise RuntimeError(f"orc_time_hotpath rc={rc}")
+    return a.value, b.value
+
+
+def time_point(data: str, prec: int, x_path: str, invz_path: str, L: int, chol_n: int, threads: int):
+    """seconds of Make_bMat on a dumped iterate and of the Cholesky of B's leading chol_n block"""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    rc = lib().orc_time_point(data.encode(), prec, x_path.encode(), invz_path.encode(), L, chol_n, threads,
+                              ctypes.byref(a), ctypes.byref(b))
+    if rc < 0:
+        raise RuntimeError(f"orc_time_point rc={rc}")
     return a.value, b.value
 
 
