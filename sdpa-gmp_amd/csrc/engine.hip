@@ -578,10 +578,13 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 // mirror: B[j,i] = B[i,j] is maintained by bacc; nothing else needed.
 
 // ---- Cholesky (Rpotrf lower), blocked right-looking ----
-// k_potrf_diag: one workgroup factors the nb x nb diagonal block at (k, k) (nb <= 64) and
-// forms its inverse Linv (nb x nb, leading dimension nb) by forward substitution on the
-// identity.  Pivots use a full-width reciprocal square root r = 1/sqrt(d): L_jj = d r and the
-// column below is scaled by r (the reference divides by L_jj, mplapack/Rtrsm.cpp:264-270).
+// k_potrf_diag: one workgroup of 1024 threads factors the nb x nb diagonal block at (k, k)
+// (nb <= 32) and forms its inverse Linv (nb x nb, leading dimension nb) by forward substitution
+// on the identity.  Pivot j: r_j = 1/sqrt(d_j) from one fixed-point Newton iteration
+// (mpx::rsqrt_fx); L_jj = d_j r_j and the column below is scaled by r_j (the reference divides
+// by L_jj, mplapack/Rtrsm.cpp:264-270).  Look-ahead: while waves 1-15 apply column j to the
+// trailing block and to the inverse, wave 0 forms the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2
+// and its r_{j+1}, so the serial chain per column is one multiply-subtract plus one rsqrt.
 // info receives the first failing global pivot (1-based, as Rpotrf2's info,
 // mplapack/Rpotrf2.cpp:86-97); once info is set, later launches return at once.
 // The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
@@ -589,69 +592,98 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 // The matrix is A[off + i + ld*j]; the block starts at (k, k); rdiag[roff + k + j] receives the
 // reciprocal pivots, Linv[loff + i + nb*j] the block inverse.
 template <int L>
-__global__ __launch_bounds__(256) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
-                                                    PV Linv, int64_t loff, int *info) {
+__device__ __forceinline__ void sr_put(uint32_t *sr, const mp<L> &r) {
+    sr[0] = (uint32_t)r.e;
+    sr[1] = r.s;
+#pragma unroll
+    for (int q = 0; q < L; q++) sr[2 + q] = r.m[q];
+}
+template <int L>
+__device__ __forceinline__ mp<L> sr_get(const uint32_t *sr) {
+    mp<L> r;
+    r.e = (int32_t)sr[0];
+    r.s = sr[1];
+#pragma unroll
+    for (int q = 0; q < L; q++) r.m[q] = sr[2 + q];
+    return r;
+}
+
+template <int L>
+__global__ __launch_bounds__(1024) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
+                                                     PV Linv, int64_t loff, int *info) {
     extern __shared__ uint32_t shm[];
     __shared__ uint32_t sr[L + 2];
     __shared__ int sfail;
     if (*info) return;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int P = nb * nb;
     uint32_t *sA = shm, *sX = shm + (L + 1) * P;
     auto at = [&](int i, int j) -> int64_t { return off + (int64_t)(k + i) + (int64_t)ld * (k + j); };
-    for (int t = tid; t < P; t += blockDim.x) {
+    for (int t = tid; t < P; t += nt) {
         int i = t % nb, j = t / nb;
         lds_st<L>(sA, P, t, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
         lds_st<L>(sX, P, t, i == j ? from_int<L>(1) : zero<L>());
     }
     __syncthreads();
+    if (tid == 0) {  // first pivot
+        mp<L> d = lds_ld<L>(sA, P, 0);
+        sfail = sgn<L>(d) <= 0 ? k + 1 : 0;
+        if (!sfail) sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
+    }
+    __syncthreads();
     for (int j = 0; j < nb; j++) {
-        if (tid == 0) {
-            mp<L> d = lds_ld<L>(sA, P, j + nb * j);
-            if (sgn<L>(d) <= 0) {
-                sfail = k + j + 1;
-            } else {
-                sfail = 0;
-                mp<L> r = mpx::rsqrt<L>(d);
-                lds_st<L>(sA, P, j + nb * j, mul<L>(d, r));
-                stm<L>(rdiag, roff + k + j, r);
-                sr[0] = (uint32_t)r.e;
-                sr[1] = r.s;
-#pragma unroll
-                for (int q = 0; q < L; q++) sr[2 + q] = r.m[q];
-            }
-        }
-        __syncthreads();
         if (sfail) {
             if (tid == 0) *info = sfail;
             return;
         }
-        mp<L> r;
-        r.e = (int32_t)sr[0];
-        r.s = sr[1];
-#pragma unroll
-        for (int q = 0; q < L; q++) r.m[q] = sr[2 + q];
-        for (int i = j + 1 + tid; i < nb; i += blockDim.x) lds_st<L>(sA, P, i + nb * j, mul<L>(r, lds_ld<L>(sA, P, i + nb * j)));
-        // inverse, row j: X[j, 0..j] *= r  (X holds the partially reduced identity)
-        for (int c = tid; c <= j; c += blockDim.x) lds_st<L>(sX, P, j + nb * c, mul<L>(r, lds_ld<L>(sX, P, j + nb * c)));
-        __syncthreads();
-        int nc = nb - (j + 1);
-        // trailing update of the block: A[i, c] -= A[i, j] A[c, j] for j < c <= i
-        for (int t = tid; t < nc * nc; t += blockDim.x) {
-            int c = j + 1 + t % nc, i = j + 1 + t / nc;
-            if (i < c) continue;
-            mp<L> v = sub<L>(lds_ld<L>(sA, P, i + nb * c), mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sA, P, c + nb * j)));
-            lds_st<L>(sA, P, i + nb * c, v);
+        const int nc = nb - (j + 1);
+        // phase 1: scale column j and inverse row j by r_j; L_jj = d_j r_j
+        if (tid < nc || (tid >= 32 && tid <= 32 + j) || tid == 64) {
+            mp<L> r = sr_get<L>(sr);
+            if (tid == 64) {
+                lds_st<L>(sA, P, j + nb * j, mul<L>(lds_ld<L>(sA, P, j + nb * j), r));
+                stm<L>(rdiag, roff + k + j, r);
+            } else {
+                int t = tid < nc ? (j + 1 + tid) + nb * j : j + nb * (tid - 32);
+                uint32_t *base = tid < nc ? sA : sX;
+                lds_st<L>(base, P, t, mul<L>(r, lds_ld<L>(base, P, t)));
+            }
         }
-        // forward substitution for the inverse: X[i, c] -= L[i, j] X[j, c] for i > j, c <= j
-        for (int t = tid; t < nc * (j + 1); t += blockDim.x) {
-            int i = j + 1 + t % nc, c = t / nc;
-            mp<L> v = sub<L>(lds_ld<L>(sX, P, i + nb * c), mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sX, P, j + nb * c)));
-            lds_st<L>(sX, P, i + nb * c, v);
+        __syncthreads();
+        if (tid == 0) {
+            // wave 0: the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2 and its r_{j+1}
+            if (nc > 0) {
+                mp<L> l = lds_ld<L>(sA, P, (j + 1) + nb * j);
+                mp<L> d = sub<L>(lds_ld<L>(sA, P, (j + 1) + nb * (j + 1)), mul<L>(l, l));
+                lds_st<L>(sA, P, (j + 1) + nb * (j + 1), d);
+                if (sgn<L>(d) <= 0)
+                    sfail = k + j + 2;
+                else
+                    sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
+            }
+        } else if (tid >= 64) {
+            // waves 1-15: trailing update A[i, c] -= L[i, j] L[c, j] (j < c <= i, except the next
+            // pivot) and the inverse X[i, c] -= L[i, j] X[j, c] (i > j, c <= j)
+            const int items = nc * nc + nc * (j + 1);
+            for (int t = tid - 64; t < items; t += nt - 64) {
+                if (t < nc * nc) {
+                    int c = j + 1 + t % nc, i = j + 1 + t / nc;
+                    if (i < c || (i == j + 1 && c == j + 1)) continue;
+                    mp<L> v = sub<L>(lds_ld<L>(sA, P, i + nb * c),
+                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sA, P, c + nb * j)));
+                    lds_st<L>(sA, P, i + nb * c, v);
+                } else {
+                    int u = t - nc * nc;
+                    int i = j + 1 + u % nc, c = u / nc;
+                    mp<L> v = sub<L>(lds_ld<L>(sX, P, i + nb * c),
+                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sX, P, j + nb * c)));
+                    lds_st<L>(sX, P, i + nb * c, v);
+                }
+            }
         }
         __syncthreads();
     }
-    for (int t = tid; t < P; t += blockDim.x) {
+    for (int t = tid; t < P; t += nt) {
         int i = t % nb, j = t / nb;
         if (i >= j) stm<L>(A, at(i, j), lds_ld<L>(sA, P, t));
         stm<L>(Linv, loff + t, lds_ld<L>(sX, P, t));
@@ -1540,7 +1572,7 @@ struct Engine : EngineBase {
             int nb = std::min(NBK, n - k);
             int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
             hipEvent_t ev = kprof.begin(st);
-            k_potrf_diag<L><<<1, 256, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
+            k_potrf_diag<L><<<1, 1024, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
             kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0);
             if (k + nb < n) {
                 int r = n - k - nb;

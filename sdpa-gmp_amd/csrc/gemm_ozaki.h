@@ -8,8 +8,8 @@
 //    s[chunk][p][row][16]: 16 consecutive k of one row for one slice.
 // 3. Products (k_gemm_oz): for the output digit position d = p + q the slice products
 //    T_d = sum_{p+q=d} sum_k a_p[i,k] b_q[k,j] are exact INT32 sums; every
-//    v_mfma_i32_32x32x32_i8 issues two (p, q) pairs at once (lanes 0-31 carry pair 2u,
-//    lanes 32-63 pair 2u+1, each over the same 16 k of a K chunk).  Pairs with
+//    v_mfma_i32_16x16x64_i8 issues four (p, q) pairs at once (lane group g = lane / 16 carries
+//    pair 4u + g, each over the same 16 k of a K chunk).  Pairs with
 //    p + q >= S lie below 2^-(8S-8) of the row/column scale and are not formed
 //    (S(S+1)/2 slice products instead of S^2).
 // 4. Recombination: sum_d T_d 256^-d is carried digit by digit from d = S-1 up to 0
@@ -28,7 +28,7 @@ namespace sdpg {
 typedef int oz_v4i __attribute__((ext_vector_type(4)));
 typedef int oz_v16i __attribute__((ext_vector_type(16)));
 
-constexpr int OZ_T = 64;   // output tile edge of one workgroup (2 x 2 waves of 32 x 32)
+constexpr int OZ_T = 32;   // output tile edge of one workgroup (2 x 2 waves of 16 x 16)
 constexpr int OZ_KC = 16;  // k per chunk (one 16-byte operand per lane and slice)
 
 // slice planes: ((((b * nch + ch) * S + p) * Rp + r) * 16 + kk), Rp = R rounded up to OZ_T
@@ -75,11 +75,11 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
 template <int S, int D>
 struct OzDigit {
     static __device__ __forceinline__ void run(const oz_v4i *__restrict__ lA, const oz_v4i *__restrict__ lB, int ar,
-                                               int bc, int hi, uint32_t (&R)[16][S / 4], int64_t (&c)[16]) {
-        oz_v16i acc = {0};
+                                               int bc, int g, uint32_t (&R)[4][S / 4], int64_t (&c)[4]) {
+        oz_v4i acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int u = 0; u < (D + 2) / 2; u++) {
-            int p = 2 * u + hi;
+        for (int u = 0; u < (D + 4) / 4; u++) {
+            int p = 4 * u + g;
             int pc = p <= D ? p : D;
             oz_v4i av = lA[pc * OZ_T + ar];
             oz_v4i bv = lB[(D - pc) * OZ_T + bc];
@@ -87,22 +87,22 @@ struct OzDigit {
                 av = oz_v4i{0, 0, 0, 0};
                 bv = oz_v4i{0, 0, 0, 0};
             }
-            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
         }
         constexpr int w = D / 4, sh = 8 * (3 - D % 4);
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
+        for (int e = 0; e < 4; e++) {
             int64_t v = (int64_t)acc[e] + c[e] + (int64_t)((R[e][w] >> sh) & 255u);
             R[e][w] = (R[e][w] & ~(255u << sh)) | (((uint32_t)v & 255u) << sh);
             c[e] = v >> 8;
         }
-        OzDigit<S, D - 1>::run(lA, lB, ar, bc, hi, R, c);
+        OzDigit<S, D - 1>::run(lA, lB, ar, bc, g, R, c);
     }
 };
 template <int S>
 struct OzDigit<S, -1> {
-    static __device__ __forceinline__ void run(const oz_v4i *, const oz_v4i *, int, int, int, uint32_t (&)[16][S / 4],
-                                               int64_t (&)[16]) {}
+    static __device__ __forceinline__ void run(const oz_v4i *, const oz_v4i *, int, int, int, uint32_t (&)[4][S / 4],
+                                               int64_t (&)[4]) {}
 };
 
 template <int L, int S>
@@ -117,13 +117,13 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
     const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T, b = blockIdx.z;
     if (g.lower && i0 + OZ_T - 1 < j0) return;  // tile strictly above the diagonal
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wr = wv & 1, wc = wv >> 1, hi = lane >> 5;
-    const int ar = wr * 32 + (lane & 31), bc = wc * 32 + (lane & 31);
+    const int wr = wv & 1, wc = wv >> 1, grp = lane >> 4;
+    const int ar = wr * 16 + (lane & 15), bc = wc * 16 + (lane & 15);
     const int bA = nbA > 1 ? b : 0, bB = nbB > 1 ? b : 0;
-    uint32_t R[16][NW];
-    int64_t RI[16];
+    uint32_t R[4][NW];
+    int64_t RI[4];
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
+    for (int e = 0; e < 4; e++) {
         RI[e] = 0;
 #pragma unroll
         for (int w = 0; w < NW; w++) R[e][w] = 0;
@@ -138,19 +138,19 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
             lB[t] = pb[(int64_t)p * Np + r];
         }
         __syncthreads();
-        int64_t c[16];
+        int64_t c[4];
 #pragma unroll
-        for (int e = 0; e < 16; e++) c[e] = 0;
-        OzDigit<S, S - 1>::run(lA, lB, ar, bc, hi, R, c);
+        for (int e = 0; e < 4; e++) c[e] = 0;
+        OzDigit<S, S - 1>::run(lA, lB, ar, bc, grp, R, c);
 #pragma unroll
-        for (int e = 0; e < 16; e++) RI[e] += c[e];
+        for (int e = 0; e < 4; e++) RI[e] += c[e];
         __syncthreads();
     }
     int32_t Ebase = -12;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        int i = i0 + wr * 32 + 8 * (e >> 2) + 4 * hi + (e & 3);
-        int j = j0 + wc * 32 + (lane & 31);
+    for (int e = 0; e < 4; e++) {
+        int i = i0 + wr * 16 + 4 * grp + e;
+        int j = j0 + wc * 16 + (lane & 15);
         if (i >= M || j >= N) continue;
         if (g.lower && i < j) continue;
         // Z = RI * 2^(32 NW) + sum_w R[w] 2^(32 (NW-1-w)), two's complement over NW + 2 words

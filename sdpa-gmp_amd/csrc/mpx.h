@@ -467,4 +467,150 @@ MPX_HD mp<L> rsqrt(const mp<L> &a) {
     return y;
 }
 
+// ---- fixed-point reciprocal square root for the Cholesky pivots ----
+// Fixed point with 2 integer bits: an N-word unsigned X stands for X / 2^(32N-2).
+
+// r = (a * b) >> (32N - 2), product columns below N-2 skipped (error below 2^-(32N-36))
+template <int N>
+MPX_HD void fx2_mulhi(const uint32_t (&a)[N], const uint32_t (&b)[N], uint32_t (&r)[N]) {
+    constexpr int S0 = N >= 2 ? N - 2 : 0;
+    uint32_t P[2 * N + 1];
+    uint64_t lo = 0;
+    uint32_t hi = 0;
+#pragma unroll
+    for (int s = 0; s < 2 * N - 1; s++) {
+        if (s >= S0) {
+#pragma unroll
+            for (int p = 0; p < N; p++) {
+                int q = s - p;
+                if (q >= 0 && q < N) mac(lo, hi, a[p], b[q]);
+            }
+        }
+        P[s] = (uint32_t)lo;
+        lo = (lo >> 32) | ((uint64_t)hi << 32);
+        hi = 0;
+    }
+    P[2 * N - 1] = (uint32_t)lo;
+    P[2 * N] = (uint32_t)(lo >> 32);
+#pragma unroll
+    for (int k = 0; k < N; k++) r[k] = (P[N + k] << 2) | (P[N + k - 1] >> 30);
+}
+
+// one Newton step y <- y + y (1 - d y^2) / 2 at N words (d, y in the 2-integer-bit format)
+template <int N>
+MPX_HD void fx2_rsqrt_step(const uint32_t (&d)[N], uint32_t (&y)[N]) {
+    uint32_t u[N], v[N], t[N], w[N];
+    fx2_mulhi<N>(y, y, u);
+    fx2_mulhi<N>(d, u, v);
+    // t = |1 - v|, one = 2^(32N-2)
+    uint32_t neg = 0;
+    {
+        uint32_t br = 0;
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            uint32_t one = (k == N - 1) ? 0x40000000u : 0u;
+            uint64_t x = (uint64_t)one - v[k] - br;
+            t[k] = (uint32_t)x;
+            br = (uint32_t)(x >> 63);
+        }
+        neg = br;  // v > 1
+        if (neg) {
+            uint32_t c = 1;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                uint32_t x = ~t[k] + c;
+                c = (c && x == 0) ? 1u : 0u;
+                t[k] = x;
+            }
+        }
+    }
+    fx2_mulhi<N>(y, t, w);
+    // w /= 2
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = (w[k] >> 1) | (k + 1 < N ? (w[k + 1] << 31) : 0u);
+    uint32_t c = 0;
+    if (!neg) {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            uint64_t x = (uint64_t)y[k] + w[k] + c;
+            y[k] = (uint32_t)x;
+            c = (uint32_t)(x >> 32);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            uint64_t x = (uint64_t)y[k] - w[k] - c;
+            y[k] = (uint32_t)x;
+            c = (uint32_t)(x >> 63);
+        }
+    }
+}
+
+// widen / narrow an N-word fixed value to M words (keeps the most significant words)
+template <int M, int N>
+MPX_HD void fx2_resize(const uint32_t (&a)[N], uint32_t (&r)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        int src = k - M + N;
+        r[k] = (src >= 0 && src < N) ? a[src] : 0u;
+    }
+}
+
+// 1/sqrt(a), a > 0, by Newton iteration in 2-integer-bit fixed point with precision doubling:
+// a 53-bit double start, then steps at 4, 8 and L+1 words (106, 212 and the full width).
+// Used on the serial pivot chain of the Cholesky factorisation, where floating-point adds
+// (variable alignment shifts) would dominate.  sq (optional) receives sqrt(a) = a / sqrt(a).
+template <int L>
+MPX_HD mp<L> rsqrt_fx(const mp<L> &a, mp<L> *sq = nullptr) {
+    constexpr int N = L + 1;
+    const int ev = a.e & ~1;
+    const int sh = (a.e - ev) ? 31 : 30;  // dn = a 2^-ev in [1/2, 2)
+    uint32_t d[N];
+    // d = M << sh over N words (M = a.m, L words)
+    d[0] = a.m[0] << sh;
+#pragma unroll
+    for (int k = 1; k < N; k++) d[k] = (k < L ? (a.m[k] << sh) : 0u) | (a.m[k - 1] >> (32 - sh));
+    double dn = 0;
+    {
+        uint64_t top = ((uint64_t)d[N - 1] << 32) | d[N - 2];
+        dn = (double)top * 0x1p-62;
+    }
+    double y0 = 1.0 / __builtin_sqrt(dn);
+    uint32_t y4[4];
+    {
+        uint64_t Y = (uint64_t)(y0 * 0x1p62);  // y0 < 2: fits 64 bits
+        y4[3] = (uint32_t)(Y >> 32);
+        y4[2] = (uint32_t)Y;
+        y4[1] = 0;
+        y4[0] = 0;
+    }
+    constexpr int NA = N < 4 ? N : 4;
+    constexpr int NB = N < 8 ? N : 8;
+    uint32_t dA[NA], yA[NA];
+    fx2_resize<NA, N>(d, dA);
+    fx2_resize<NA, 4>(y4, yA);
+    fx2_rsqrt_step<NA>(dA, yA);
+    uint32_t dB[NB], yB[NB];
+    fx2_resize<NB, N>(d, dB);
+    fx2_resize<NB, NA>(yA, yB);
+    fx2_rsqrt_step<NB>(dB, yB);
+    uint32_t y[N];
+    fx2_resize<N, NB>(yB, y);
+    fx2_rsqrt_step<N>(d, y);
+    if (32 * L > 400) fx2_rsqrt_step<N>(d, y);
+    // y in (1/sqrt(2), sqrt(2)]: normalise the N words (top bit at 2^(32N-1) or 2^(32N-2) or lower)
+    if (sq) {  // sqrt(a) = a * (1/sqrt(a)) in the same fixed point
+        uint32_t s[N];
+        fx2_mulhi<N>(d, y, s);
+        uint32_t lzs = lead_zeros<N>(s);
+        shl<N>(s, lzs);
+        round_store<L>(*sq, s, 2 - (int32_t)lzs + ev / 2, 0u);
+    }
+    uint32_t lz = lead_zeros<N>(y);
+    shl<N>(y, lz);
+    mp<L> r;
+    round_store<L>(r, y, 2 - (int32_t)lz - ev / 2, 0u);
+    return r;
+}
+
 }  // namespace mpx

@@ -33,6 +33,7 @@ static int op_L(int op, const unsigned *a, const unsigned *b, unsigned *r) {
     case 4: z = mpx::sqrtv<L>(x); break;
     case 5: z = mpx::recip<L>(x); break;
     case 6: z = mpx::rsqrt<L>(x); break;
+    case 7: z = mpx::rsqrt_fx<L>(x); break;
     default: return -1;
     }
     st<L>(z, r);
