@@ -29,6 +29,8 @@ typedef struct {
     int limbs;
     double t_inv_ms;      /* Solutions::computeInverse (sdpg_factor_point), device time */
     double t_eig_ms;      /* step-length minimum eigenvalues (sdpg_min_eigen), device time */
+    char objP70[112];     /* objValPrimal with 70 significant digits */
+    char objD70[112];
 } sdpa_amd_result;
 
 /*

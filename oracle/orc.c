@@ -2098,6 +2098,8 @@ static int run(Ctx *c, const orc_param *prm, FILE *logf, const char *dump_dir, c
         ompf_neg(&t2, &si.objValPrimal);
         ompf_get_str(res->objP, sizeof res->objP, &t1, 41);
         ompf_get_str(res->objD, sizeof res->objD, &t2, 41);
+        ompf_get_str(res->objP70, sizeof res->objP70, &t1, 70);
+        ompf_get_str(res->objD70, sizeof res->objD70, &t2, 70);
         ompf_get_str(res->relgap_s, sizeof res->relgap_s, &relgap, 17);
         ompf_get_str(res->pfeas_s, sizeof res->pfeas_s, &c->cur.normDualMat, 17);
         ompf_get_str(res->dfeas_s, sizeof res->dfeas_s, &c->cur.normPrimalVec, 17);

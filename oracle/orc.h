@@ -38,6 +38,7 @@ typedef struct {
     char objP[80], objD[80]; /* printed objValPrimal / objValDual (40 digits) */
     char relgap_s[80], pfeas_s[80], dfeas_s[80];
     double t_bmat, t_chol, t_total; /* seconds: Make bMat, Cholesky bMat, main loop */
+    char objP70[112], objD70[112];  /* the same objectives with 70 significant digits */
 } orc_result;
 
 /*

@@ -748,6 +748,8 @@ struct Solver {
             snprintf(res->phase, sizeof res->phase, "%s", phase_str(rphase));
             snprintf(res->objP, sizeof res->objP, "%s", fmt(-si.objValDual, 41).c_str());
             snprintf(res->objD, sizeof res->objD, "%s", fmt(-si.objValPrimal, 41).c_str());
+            snprintf(res->objP70, sizeof res->objP70, "%s", fmt(-si.objValDual, 70).c_str());
+            snprintf(res->objD70, sizeof res->objD70, "%s", fmt(-si.objValPrimal, 70).c_str());
             snprintf(res->relgap, sizeof res->relgap, "%s", fmt(relgap, 17).c_str());
             snprintf(res->pfeas, sizeof res->pfeas, "%s", fmt(curNrd, 17).c_str());
             snprintf(res->dfeas, sizeof res->dfeas, "%s", fmt(curNrp, 17).c_str());

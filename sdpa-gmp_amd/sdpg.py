@@ -305,7 +305,7 @@ class Result(ctypes.Structure):
                 ("t_schur_ms", ctypes.c_double), ("t_chol_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_gvec_ms", ctypes.c_double), ("t_dxdz_ms", ctypes.c_double), ("t_res_ms", ctypes.c_double),
                 ("t_total_s", ctypes.c_double), ("limbs", ctypes.c_int), ("t_inv_ms", ctypes.c_double),
-                ("t_eig_ms", ctypes.c_double)]
+                ("t_eig_ms", ctypes.c_double), ("objP70", ctypes.c_char * 112), ("objD70", ctypes.c_char * 112)]
 
 
 def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path: str | None = None,

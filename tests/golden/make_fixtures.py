@@ -8,7 +8,10 @@ Produces
   newton_w{0,1}_it*.npz   Newton-step fixtures: the oracle's 256-bit iterate (inputs, exact
                           in 12 interchange limbs) and the 512-bit oracle outputs on that same
                           iterate (16 limbs), with the 256-bit oracle's normwise error per output
-  solves.json             end-to-end oracle summaries of workloads 0 and 1 at 256 bits, 1e-60
+  solves.json             end-to-end oracle summaries of workloads 0 and 1 at epsilonStar =
+                          epsilonDash = 1e-60: at 256 bits (GMP truncation: the primal residual
+                          stalls near 1e-49, so the run ends pFEAS) and at 512 bits (pdOPT; the
+                          reference trajectory for the GPU's 256-bit run, objectives to 70 digits)
   sdplib/*.dat-s.xz, sdplib_golden.json
                           inputs and recorded results of the reference's own regression runs
                           (SDPLIB/data/*.result.xz, example1.result): iteration count, phase,
@@ -114,6 +117,19 @@ def newton_fixtures(tmp, name, iters):
     return summary
 
 
+def solve_summary(tmp, name, prec):
+    data = os.path.join(tmp, f"{name}.dat-s")
+    log = os.path.join(tmp, f"{name}_{prec}.log")
+    res = oracle_lib.solve(data, PARAM, prec, log)
+    out = {"precision": prec, "iterations": res.iterations, "phase": res.phase_name.decode().strip(),
+           "objP": res.objP70.decode(), "objD": res.objD70.decode(), "relgap": res.relgap_s.decode(),
+           "pfeas": res.pfeas_s.decode(), "dfeas": res.dfeas_s.decode()}
+    with open(log) as f:
+        out["log"] = [ln.rstrip("\n") for ln in f if re.match(r"^\s*\d+ ", ln)]
+    print(f"{name} at {prec} bits: {out['iterations']} iterations, {out['phase']}, relgap {out['relgap']}", flush=True)
+    return out
+
+
 def parse_result(text):
     g = {}
     m = re.search(r"phase.value = (\S+)", text)
@@ -146,8 +162,20 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-sdplib", action="store_true")
     ap.add_argument("--skip-newton", action="store_true")
+    ap.add_argument("--solves-only", action="store_true", help="recompute solves.json only")
     a = ap.parse_args()
     oracle_lib.build_oracle()
+    if a.solves_only:
+        tmp = tempfile.mkdtemp(prefix="sdpg_fix_")
+        try:
+            gen_inputs(tmp)
+            solves = {w: {"oracle256": solve_summary(tmp, w, 256), "oracle512": solve_summary(tmp, w, 512)}
+                      for w in ("w0", "w1")}
+            with open(os.path.join(HERE, "solves.json"), "w") as f:
+                json.dump(solves, f, indent=1)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
     if not a.skip_sdplib:
         sdplib_fixtures()
     if a.skip_newton:
@@ -157,9 +185,10 @@ def main():
         hashes = gen_inputs(tmp)
         with open(os.path.join(HERE, "inputs.json"), "w") as f:
             json.dump(hashes, f, indent=1)
-        solves = {}
-        solves["w0"] = newton_fixtures(tmp, "w0", [0, 1, 5, 20, 40, 60])
-        solves["w1"] = newton_fixtures(tmp, "w1", [0, 10, 45])
+        newton_fixtures(tmp, "w0", [0, 1, 5, 20, 40, 60])
+        newton_fixtures(tmp, "w1", [0, 10, 45])
+        solves = {w: {"oracle256": solve_summary(tmp, w, 256), "oracle512": solve_summary(tmp, w, 512)}
+                  for w in ("w0", "w1")}
         with open(os.path.join(HERE, "solves.json"), "w") as f:
             json.dump(solves, f, indent=1)
     finally:

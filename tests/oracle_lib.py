@@ -20,7 +20,8 @@ class OrcResult(ctypes.Structure):
                 ("mu", ctypes.c_double), ("relgap", ctypes.c_double), ("gap", ctypes.c_double),
                 ("objP", ctypes.c_char * 80), ("objD", ctypes.c_char * 80), ("relgap_s", ctypes.c_char * 80),
                 ("pfeas_s", ctypes.c_char * 80), ("dfeas_s", ctypes.c_char * 80), ("t_bmat", ctypes.c_double),
-                ("t_chol", ctypes.c_double), ("t_total", ctypes.c_double)]
+                ("t_chol", ctypes.c_double), ("t_total", ctypes.c_double), ("objP70", ctypes.c_char * 112),
+                ("objD70", ctypes.c_char * 112)]
 
 
 class OrcParam(ctypes.Structure):

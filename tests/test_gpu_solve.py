@@ -3,8 +3,12 @@
 - example1 and SDPLIB problems at the reference's defaults (200-bit request = 256-bit mpf):
   the reference's own recorded results (example1.result, SDPLIB/data/*.result.xz) — same
   phase, iteration count within 1, objectives to the 16 printed digits.
-- workload 0 at 256 bits with epsilonStar = epsilonDash = 1e-60 against the oracle's solve:
-  iteration count within 1, objectives within 1e-50 relative.
+- workloads 0 and 1 at 256 bits with epsilonStar = epsilonDash = 1e-60 against the oracle's
+  solves (tests/golden/solves.json).  The 512-bit oracle reaches pdOPT (relgap ~1e-61); the GPU's
+  256-bit run (320-bit rounded limbs) must end pdOPT with the iteration count within 1 of it,
+  objectives within 1e-50 relative (70 printed digits) and relative gap <= 1e-60.  The 256-bit
+  oracle (GMP truncation) cannot reach 1e-60: its primal residual stalls near 1e-49 and it ends
+  pFEAS; the GPU's objectives must agree with that run's to within that run's own relative gap.
 """
 import json
 import os
@@ -41,12 +45,19 @@ def test_reference_regressions(name, sdplib_file):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("gemm", [sdpg.GEMM_PLAIN, sdpg.GEMM_OZAKI], ids=["plain", "ozaki"])
-def test_workload0_matches_oracle(gemm, workload):
+@pytest.mark.parametrize("wk", [0, 1])
+def test_workload_matches_oracle(wk, gemm, workload):
     with open(os.path.join(GOLD, "solves.json")) as f:
-        gold = json.load(f)["w0"]
-    print(f"case w0 gemm={gemm}", flush=True)
-    r = sdpg.solve(workload(0), PARAM, 0, None, gemm)
-    assert abs(r.iterations - gold["iterations"]) <= 1, (r.iterations, gold["iterations"])
-    assert r.phase.decode().strip() == gold["phase"]
-    assert rel(r.objP.decode(), gold["objP"]) < Decimal("1e-50")
-    assert rel(r.objD.decode(), gold["objD"]) < Decimal("1e-50")
+        gold = json.load(f)[f"w{wk}"]
+    o512, o256 = gold["oracle512"], gold["oracle256"]
+    print(f"case w{wk} gemm={gemm}", flush=True)
+    r = sdpg.solve(workload(wk), PARAM, 0, None, gemm)
+    print(f"  {r.iterations} iterations, {r.phase.decode().strip()}, relgap {r.relgap.decode()}")
+    assert r.phase.decode().strip() == o512["phase"] == "pdOPT"
+    assert abs(r.iterations - o512["iterations"]) <= 1, (r.iterations, o512["iterations"])
+    assert rel(r.objP70.decode(), o512["objP"]) < Decimal("1e-50")
+    assert rel(r.objD70.decode(), o512["objD"]) < Decimal("1e-50")
+    assert Decimal(r.relgap.decode()) <= Decimal("1e-60")
+    g256 = Decimal(o256["relgap"])
+    assert rel(r.objP70.decode(), o256["objP"]) <= 10 * g256
+    assert rel(r.objD70.decode(), o256["objD"]) <= 10 * g256

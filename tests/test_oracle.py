@@ -240,3 +240,14 @@ def test_oracle_newton_step_closed_forms(workload):
             v = beta * mu * invZ[q][p] - X[q][p] - XdZZ[q][p]
             ex = (u + v) / 2
             assert abs(dX[p][q] - ex) <= Fraction(1, 10 ** 110) * (1 + abs(ex))
+
+
+def test_oracle_512_converges_on_workload0(workload):
+    """The 512-bit oracle run that the GPU's 256-bit solve is compared with (solves.json)."""
+    with open(os.path.join(GOLD, "solves.json")) as f:
+        g = json.load(f)["w0"]["oracle512"]
+    param = os.path.join(os.path.dirname(GOLD), "..", "sdpa-gmp_amd", "data", "param_256_gap1e-60.sdpa")
+    r = oracle_lib.solve(workload(0), param, 512, None)
+    assert r.phase_name.decode().strip() == g["phase"] == "pdOPT"
+    assert r.iterations == g["iterations"]
+    assert r.objP70.decode() == g["objP"] and r.objD70.decode() == g["objD"]
