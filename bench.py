@@ -123,11 +123,17 @@ def main():
     if oz and oz[0]:
         avg_s = oz[1] / 1e3 / oz[0]
         ach = oz[2] / oz[0] / avg_s
-        out["roofline"] = {"kernel": "k_gemm_oz (Ozaki INT8 slice products, v_mfma_i32_32x32x32_i8)",
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "k_gemm_oz_maxcut1000_traffic.json")
+        if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated
+            with open(tfile) as f:
+                traffic = json.load(f)["traffic_bytes_per_launch"]
+        out["roofline"] = {"kernel": "k_gemm_oz (Ozaki INT8 slice products, v_mfma_i32_16x16x64_i8)",
                            "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TFLOP/s",
-                           "frac": ach / INT8_PEAK, "traffic": None,
+                           "frac": ach / INT8_PEAK, "traffic": traffic,
                            "ops_per_launch": oz[2] / oz[0], "avg_launch_us": avg_s * 1e6,
-                           "note": "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices"}
+                           "note": "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices; traffic from "
+                                   "profiles/k_gemm_oz_maxcut1000_traffic.txt"}
     shares = {k: v[1] / a.steps / step_ms for k, v in kst.items()}
     out["step_breakdown"] = {"share_of_step": shares, "dominant": max(shares, key=shares.get)}
     for f in res.values():
