@@ -1577,13 +1577,20 @@ struct Engine : EngineBase {
             if (k + nb < n) {
                 int r = n - k - nb;
                 int64_t a21 = off + (k + nb) + (int64_t)ld * k;
-                // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N)
-                if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, Pw.pv(), 0, r, GEMM_SET, 1,
-                                    0, 0, 0, gemm_mode, gemm_slices)))
+                // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N).  The plain and
+                // Ozaki forms convert their operands before the product kernel, so P overwrites A21 in
+                // place; the element form reads its operands directly and goes through Pw.
+                if (gemm_mode == -1) {
+                    if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, Pw.pv(), 0, r, GEMM_SET,
+                                        1, 0, 0, 0, gemm_mode, gemm_slices)))
+                        return rc;
+                    k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(A, a21, ld, Pw.pv(), 0, r, r, nb);
+                } else if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, A, a21, ld, GEMM_SET,
+                                           1, 0, 0, 0, gemm_mode, gemm_slices))) {
                     return rc;
-                k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(A, a21, ld, Pw.pv(), 0, r, r, nb);
+                }
                 // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
-                if ((rc = gemm.gemm(r, r, nb, Pw.pv(), 0, r, false, Pw.pv(), 0, r, true, A,
+                if ((rc = gemm.gemm(r, r, nb, A, a21, ld, false, A, a21, ld, true, A,
                                     off + (k + nb) + (int64_t)ld * (k + nb), ld, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
                                     gemm_slices, true)))
                     return rc;

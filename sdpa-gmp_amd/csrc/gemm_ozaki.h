@@ -241,14 +241,17 @@ struct OzakiWork {
         rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
         rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
         if (rc) return -4;
+        const bool same = gemm_b_is_at(g);
         k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
-        k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
+        if (!same) k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
         int64_t sa_n = (int64_t)Mp * nch * OZ_KC, sb_n = (int64_t)Np * nch * OZ_KC;
         k_oz_slice<L, S><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, ea, as);
-        k_oz_slice<L, S><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, eb, bs);
+        if (!same)
+            k_oz_slice<L, S><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, eb, bs);
         dim3 grd(Mp / OZ_T, Np / OZ_T, g.batch);
         hipEvent_t ev = prof ? prof->begin(st) : nullptr;
-        k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, as, bs, ea, eb, nbA, nbB, Mp, Np, nch);
+        k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, as, same ? as : bs, ea, same ? ea : eb, nbA, same ? nbA : nbB, Mp,
+                                            same ? Mp : Np, nch);
         // algorithmic INT8 operations: 2 per multiply-add, S(S+1)/2 slice pairs per mp multiply-add
         if (prof) prof->end(st, ev, 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
         return hipGetLastError() == hipSuccess ? 0 : -2;

@@ -145,33 +145,47 @@ __device__ __forceinline__ void fx_convert(const mpx::mp<L> &x, int32_t emax, in
 // which = 1 -> op(B)[k, r].  contig() tells whether r (true) or k is the memory-contiguous index,
 // so that consecutive threads walk consecutive addresses.
 
-// pass 1: per-row largest exponent e_r (EZERO for an all-zero row), written once per row:
-// a thread per row when consecutive rows are consecutive in memory, else a wave per row
+// pass 1: per-row largest exponent e_r (EZERO for an all-zero row: a zero's head holds EZERO),
+// written once per row.  Only the head word is read.  Rows contiguous in memory: a block covers
+// 256 / KS rows with KS threads per row striding over k (coalesced across the rows), reduced in
+// LDS; k contiguous: a wave per row.
+constexpr int FXM_KS = 8;
 template <int L>
-__global__ void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
+__global__ __launch_bounds__(256) void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
     int b = blockIdx.y;
     PV src = which == 0 ? g.A : g.B;
     const int K = g.K;
+    const uint32_t *head = src.p + (int64_t)L * src.n;
     if (fx_rcontig(g, which)) {
-        int r = blockIdx.x * blockDim.x + threadIdx.x;
-        if (r >= R) return;
+        __shared__ int32_t red[256];
+        constexpr int RPB = 256 / FXM_KS;
+        int rl = threadIdx.x % RPB, ks = threadIdx.x / RPB;
+        int r = blockIdx.x * RPB + rl;
         int32_t e = mpx::EZERO;
-        for (int k = 0; k < K; k++) {
-            int64_t idx = fx_at(g, which, b, r, k);
-            if (src.p[(int64_t)(L - 1) * src.n + idx] == 0) continue;
-            int32_t x = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
-            e = x > e ? x : e;
+        if (r < R) {
+#pragma unroll 4
+            for (int k = ks; k < K; k += FXM_KS) {
+                int32_t x = ((int32_t)head[fx_at(g, which, b, r, k)]) >> 1;
+                e = x > e ? x : e;
+            }
         }
-        eo[(int64_t)b * R + r] = e;
+        red[threadIdx.x] = e;
+        __syncthreads();
+        if (ks == 0 && r < R) {
+#pragma unroll
+            for (int q = 1; q < FXM_KS; q++) {
+                int32_t x = red[rl + q * RPB];
+                e = x > e ? x : e;
+            }
+            eo[(int64_t)b * R + r] = e;
+        }
     } else {
         int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
         int lane = threadIdx.x & 63;
         int32_t e = mpx::EZERO;
         if (r < R)
             for (int k = lane; k < K; k += 64) {
-                int64_t idx = fx_at(g, which, b, r, k);
-                if (src.p[(int64_t)(L - 1) * src.n + idx] == 0) continue;
-                int32_t x = ((int32_t)src.p[(int64_t)L * src.n + idx]) >> 1;
+                int32_t x = ((int32_t)head[fx_at(g, which, b, r, k)]) >> 1;
                 e = x > e ? x : e;
             }
         for (int off = 32; off >= 1; off >>= 1) {
@@ -185,7 +199,14 @@ __global__ void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
 // grid of k_fix_max for R rows
 inline dim3 fix_max_grid(const GemmArgs &g, int which, int R, int nb) {
     bool rc = which == 0 ? !g.transA : g.transB;
-    return rc ? dim3((unsigned)((R + 255) / 256), nb) : dim3((unsigned)((R + 3) / 4), nb);
+    return rc ? dim3((unsigned)((R + 256 / FXM_KS - 1) / (256 / FXM_KS)), nb) : dim3((unsigned)((R + 3) / 4), nb);
+}
+
+// op(B) is op(A)^T on the same data (the Cholesky trailing update P P^T): its columns are
+// op(A)'s rows, so their exponents and fixed-point / slice forms are the same
+inline bool gemm_b_is_at(const GemmArgs &g) {
+    return g.B.p == g.A.p && g.offB == g.offA && g.ldb == g.lda && g.transB != g.transA && g.N == g.M &&
+           g.sA == g.sB;
 }
 
 // pass 2: a' = round(x * 2^(W - e_r)) + 2^(32L-1), W = 32L - 2; out[((q * K) + k) * R + r] per batch
@@ -361,13 +382,14 @@ struct PlainWork {
         if (rc) return -4;
         int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
         dim3 ga((unsigned)((ta + 255) / 256), nbA), gb((unsigned)((tb + 255) / 256), nbB);
+        const bool same = gemm_b_is_at(g);
         k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
-        k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
+        if (!same) k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
         k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea, af);
-        k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb, bf);
+        if (!same) k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb, bf);
         dim3 grd((g.M + PT - 1) / PT, (g.N + PT - 1) / PT, g.batch);
         hipEvent_t ev = prof ? prof->begin(st) : nullptr;
-        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, bf, ea, eb, nbA, nbB);
+        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, same ? af : bf, ea, same ? ea : eb, nbA, same ? nbA : nbB);
         if (prof) prof->end(st, ev, 1, gemm_outputs(g) * (double)g.K);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
