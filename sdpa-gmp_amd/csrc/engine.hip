@@ -1,2037 +1,10 @@
-// engine.hip — MI355X Newton engine for SDPA-GMP: kernels, Engine<L>, C-ABI.
-//
-// Hot path of one IPM iteration (reference Newton::Mehrotra,
-// sdpa_newton.cpp:1375-1414) on the GPU in limb-planar fixed-limb
-// multiple precision: right-hand side (compute_rMat + Make_gVec), Schur
-// complement (Make_bMat: F1/F2/F3 + LP), Cholesky (Rpotrf), triangular solves
-// (Rtrsv x2), dZ (multi_plusToA) and dX (triple product + getSymmetrize), plus
-// the residuals of Residuals::compute.  Dense products go through the GEMM
-// engine of gemm.hip (plain limb MAD or Ozaki INT8 MFMA).
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <string>
-#include <vector>
-
-#include "engine_common.h"
-#include "gemm.h"
-#include "host_la.h"
-#include "limbs.h"
-#include "mpx.h"
-#include "sdpg.h"
-
-using namespace mpx;
+// engine.hip — C-ABI of the MI355X Newton engine (include/sdpg.h).  The engine itself
+// (kernels, Engine<L>) is in engine_impl.h, instantiated per limb count in engine_inst.hip.
+#include "engine_impl.h"
 
 namespace sdpg {
-
 static thread_local std::string g_err;
-static void set_err(const std::string &s) { g_err = s; }
-
-#define HIPCHK(x)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess) {                                                                \
-            set_err(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);            \
-            return SDPG_EHIP;                                                                  \
-        }                                                                                      \
-    } while (0)
-
-static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
-
-// ------------------------------------------------------------------ kernels
-
-template <int L>
-__global__ void k_unpack(const uint32_t *__restrict__ src, PV dst, int64_t off, int64_t cnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    const uint32_t *s = src + i * (L + 2);
-    mp<L> x;
-    x.e = (int32_t)s[0];
-    x.s = s[1] & 1u;
-#pragma unroll
-    for (int k = 0; k < L; k++) x.m[k] = s[2 + k];
-    if (x.m[L - 1] == 0) x = zero<L>();
-    stm<L>(dst, off + i, x);
-}
-
-template <int L>
-__global__ void k_pack(PV src, int64_t off, int64_t cnt, uint32_t *__restrict__ dst) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    mp<L> x = ldm<L>(src, off + i);
-    uint32_t *d = dst + i * (L + 2);
-    d[0] = (uint32_t)x.e;
-    d[1] = x.s;
-#pragma unroll
-    for (int k = 0; k < L; k++) d[2 + k] = x.m[k];
-}
-
-template <int L>
-__global__ void k_zero(PV dst, int64_t off, int64_t cnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    stm<L>(dst, off + i, zero<L>());
-}
-
-// out = a*X + b*Y  (a, b scalars; Y may be null => b ignored)
-template <int L>
-__global__ void k_lincomb(PV out, PV X, PV Y, mp<L> a, mp<L> b, int useY, int64_t cnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    mp<L> r = mul<L>(a, ldm<L>(X, i));
-    if (useY) r = add<L>(r, mul<L>(b, ldm<L>(Y, i)));
-    stm<L>(out, i, r);
-}
-
-// out = X + s*Y    (Raxpy form: y += s*x as in mplapack/Raxpy_omp.cpp)
-template <int L>
-__global__ void k_axpy(PV out, PV X, PV Y, mp<L> s, int64_t cnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    stm<L>(out, i, add<L>(ldm<L>(X, i), mul<L>(s, ldm<L>(Y, i))));
-}
-
-// Lal::getSymmetrize on one n x n block at offset off (sdpa_linear.cpp:457-482):
-// lower = (lower + upper) / 2, upper = lower
-template <int L>
-__global__ void k_symmetrize(PV A, int64_t off, int n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n * n) return;
-    int i = (int)(t % n), j = (int)(t / n);
-    if (i <= j) return;  // thread owns the strictly-lower (i, j)
-    const mp<L> half = ldexp2<L>(from_int<L>(1), -1);
-    mp<L> lo = ldm<L>(A, off + i + (int64_t)n * j);
-    mp<L> up = ldm<L>(A, off + j + (int64_t)n * i);
-    mp<L> v = mul<L>(half, add<L>(lo, up));
-    stm<L>(A, off + i + (int64_t)n * j, v);
-    stm<L>(A, off + j + (int64_t)n * i, v);
-}
-
-// Constraint descriptor of block (i, l)
-struct Desc {
-    int32_t kind;  // 0 absent, 1 sparse, 2 dense
-    int32_t n;
-    int64_t off;  // sparse: first entry index; dense: first value index
-    int32_t nnz;  // sparse stored entries
-    int32_t effect;
-    int64_t blkoff;  // offset of block l inside a dense linear space
-};
-
-// block-tree sum of one mp value per thread (blockDim.x a power of two, LDS planes of blockDim.x)
-template <int L>
-__device__ mp<L> block_sum(mp<L> v, uint32_t *sh) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    lds_st<L>(sh, nt, t, v);
-    __syncthreads();
-    for (int h = nt / 2; h >= 1; h >>= 1) {
-        if (t < h) lds_st<L>(sh, nt, t, add<L>(lds_ld<L>(sh, nt, t), lds_ld<L>(sh, nt, t + h)));
-        __syncthreads();
-    }
-    return lds_ld<L>(sh, nt, 0);
-}
-
-// one wave-wide sum (64 lanes, LDS planes of the block)
-template <int L>
-__device__ mp<L> wave_sum64(mp<L> v, uint32_t *sh) {
-    const int t = threadIdx.x, lane = t & 63;
-    lds_st<L>(sh, blockDim.x, t, v);
-    __syncthreads();
-    for (int h = 32; h >= 1; h >>= 1) {
-        if (lane < h) lds_st<L>(sh, blockDim.x, t, add<L>(lds_ld<L>(sh, blockDim.x, t), lds_ld<L>(sh, blockDim.x, t + h)));
-        __syncthreads();
-    }
-    return lds_ld<L>(sh, blockDim.x, t - lane);
-}
-
-// For each constraint c: ip_c = A_c . D (SDP blocks + LP), one wave per constraint (lanes stride
-// over the stored entries, then a tree sum), then
-//       mode 0: out_c = -ip + R_c   (Make_gVec: gVec = -(A.DLS1) + primalVec)
-//       mode 1: out_c =  b_c - ip   (Residuals: primalVec = b - A.X)
-//       mode 2: out_c =  ip          (objective C.X)
-// Sparse blocks follow the _OPENMP branch of sdpa_linear.cpp:235-254 per entry (v D_ii, or
-// v (D_ij + D_ji)), dense blocks Rdot over n^2 values; the summation order is the tree's.
-template <int L>
-__global__ __launch_bounds__(256) void k_con_inner(int ncon, int nsdp, const Desc *desc, const int32_t *ri,
-                                                   const int32_t *ci, PV sv, PV dv, const int64_t *lp_ptr,
-                                                   const int32_t *lp_pos, PV lp_val, int64_t lpbase, PV D, PV R,
-                                                   PV out, int mode, int whole_block) {
-    __shared__ uint32_t sh[(L + 1) * 256];
-    // whole_block: the workgroup works on constraint blockIdx.x; otherwise one wave per constraint
-    const int lane = whole_block ? threadIdx.x : (threadIdx.x & 63);
-    const int G = whole_block ? blockDim.x : 64;
-    const int c = whole_block ? blockIdx.x : blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    mp<L> acc = zero<L>();
-    if (c < ncon) {
-        for (int l = 0; l < nsdp; l++) {
-            Desc d = desc[(int64_t)c * nsdp + l];
-            if (d.kind == 1) {
-                for (int t = lane; t < d.nnz; t += G) {
-                    int64_t e = d.off + t;
-                    int i = ri[e], j = ci[e];
-                    mp<L> v = ldm<L>(sv, e);
-                    mp<L> temp;
-                    if (i == j) {
-                        temp = mul<L>(v, ldm<L>(D, d.blkoff + i + (int64_t)d.n * j));
-                    } else {
-                        temp = add<L>(ldm<L>(D, d.blkoff + i + (int64_t)d.n * j),
-                                      ldm<L>(D, d.blkoff + j + (int64_t)d.n * i));
-                        temp = mul<L>(temp, v);
-                    }
-                    acc = add<L>(acc, temp);
-                }
-            } else if (d.kind == 2) {
-                int64_t nn = (int64_t)d.n * d.n;
-                for (int64_t k = lane; k < nn; k += G)
-                    acc = fma_acc<L>(acc, ldm<L>(dv, d.off + k), ldm<L>(D, d.blkoff + k));
-            }
-        }
-        for (int64_t t = lp_ptr[c] + lane; t < lp_ptr[c + 1]; t += G)
-            acc = fma_acc<L>(acc, ldm<L>(lp_val, t), ldm<L>(D, lpbase + lp_pos[t]));
-    }
-    mp<L> ret = whole_block ? block_sum<L>(acc, sh) : wave_sum64<L>(acc, sh);
-    if (c >= ncon || lane != 0) return;
-    mp<L> o;
-    if (mode == 0) {
-        o = add<L>(neg<L>(ret), ldm<L>(R, c));
-    } else if (mode == 1) {
-        o = sub<L>(ldm<L>(R, c), ret);
-    } else {
-        o = ret;
-    }
-    stm<L>(out, c, o);
-}
-
-// Scatter sum_i s_i A_i into the dense linear space (multi_plusToA, sdpa_dataset.cpp:488-493).
-// pos_ptr/pos_con/pos_ent: per position, the sparse entries touching it (ordered by constraint);
-// dense constraints of the position's block: dcon_ptr/dcon (per block), value offset dcon_off.
-// mode 0 (dZ):  out = -acc + (useR ? R : 0)        (compute_DzMat)
-// mode 1 (Rd):  out = -acc + Cfull - Z             (Residuals::compute)
-template <int L>
-__global__ void k_scatter(int64_t npos, const int64_t *pos_ptr, const int32_t *pos_con, const int64_t *pos_ent,
-                          PV sv, PV svals_dense, const int32_t *pos_blk, const int64_t *pos_in_blk,
-                          const int64_t *dcon_ptr, const int32_t *dcon, const int64_t *dcon_off, PV s, PV R, int useR,
-                          PV Cfull, PV Z, PV out, int mode) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npos) return;
-    mp<L> acc = zero<L>();
-    for (int64_t q = pos_ptr[t]; q < pos_ptr[t + 1]; q++) {
-        mp<L> v = mul<L>(ldm<L>(sv, pos_ent[q]), ldm<L>(s, pos_con[q]));
-        acc = add<L>(acc, v);
-    }
-    int b = pos_blk[t];
-    if (b >= 0) {
-        int64_t k = pos_in_blk[t];
-        for (int64_t q = dcon_ptr[b]; q < dcon_ptr[b + 1]; q++) {
-            acc = add<L>(acc, mul<L>(ldm<L>(s, dcon[q]), ldm<L>(svals_dense, dcon_off[q] + k)));
-        }
-    }
-    mp<L> o;
-    if (mode == 0) {
-        o = neg<L>(acc);
-        if (useR) o = add<L>(o, ldm<L>(R, t));
-    } else {
-        o = sub<L>(sub<L>(ldm<L>(Cfull, t), ldm<L>(Z, t)), acc);
-    }
-    stm<L>(out, t, o);
-}
-
-// max |x_i| over cnt elements: per-block partial maxima (one per workgroup)
-template <int L>
-__global__ void k_maxabs(PV X, int64_t cnt, PV part) {
-    __shared__ uint32_t sbuf[256 * (L + 2)];
-    mp<L> best = zero<L>();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
-        mp<L> v = absv<L>(ldm<L>(X, i));
-        if (cmp_abs<L>(v, best) > 0) best = v;
-    }
-    uint32_t *me = sbuf + threadIdx.x * (L + 2);
-    me[0] = (uint32_t)best.e;
-    me[1] = best.s;
-#pragma unroll
-    for (int k = 0; k < L; k++) me[2 + k] = best.m[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        mp<L> b = zero<L>();
-        for (int t = 0; t < (int)blockDim.x; t++) {
-            uint32_t *o = sbuf + t * (L + 2);
-            mp<L> v;
-            v.e = (int32_t)o[0];
-            v.s = o[1];
-#pragma unroll
-            for (int k = 0; k < L; k++) v.m[k] = o[2 + k];
-            if (cmp_abs<L>(v, b) > 0) b = v;
-        }
-        stm<L>(part, blockIdx.x, b);
-    }
-}
-
-// ---- Schur complement ----
-
-// F_c = A_c * invZ for sparse A_c by the symmetric row expansion (Lal::multiply(D,S,D),
-// sdpa_linear.cpp:877-955): row r of F = sum over expansion entries (r, q, v) of v * invZ[q, :]
-// grid: x over n*n outputs, y over the constraint list
-template <int L>
-__global__ void k_spmm_rows(int n, const int64_t *exp_rowptr, const int32_t *exp_col, const int64_t *exp_ent, PV sv,
-                            PV invZ, int64_t zoff, PV F, int64_t fstride, const int32_t *sp_list) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int k = blockIdx.y;
-    if (t >= (int64_t)n * n) return;
-    int r = (int)(t % n), c = (int)(t / n);
-    int64_t base = (int64_t)k * (n + 1);
-    mp<L> acc = zero<L>();
-    for (int64_t q = exp_rowptr[base + r]; q < exp_rowptr[base + r + 1]; q++) {
-        acc = fma_acc<L>(acc, ldm<L>(sv, exp_ent[q]), ldm<L>(invZ, zoff + exp_col[q] + (int64_t)n * c));
-    }
-    stm<L>(F, (int64_t)sp_list[k] * fstride + r + (int64_t)n * c, acc);
-}
-
-struct Pair {
-    int32_t i, j;    // owner i (formula of i), partner j
-    int32_t slot;    // workspace slot of F_i / G_i
-    int32_t form;    // 1, 2, 3
-};
-
-// B[i,j] += value, B[j,i] += value (i != j) — one thread per pair of one block
-template <int L>
-__device__ __forceinline__ void bacc(PV B, int m, int i, int j, const mp<L> &v) {
-    int64_t a = i + (int64_t)m * j;
-    stm<L>(B, a, add<L>(ldm<L>(B, a), v));
-    if (i != j) {
-        int64_t b = j + (int64_t)m * i;
-        stm<L>(B, b, add<L>(ldm<L>(B, b), v));
-    }
-}
-
-// A (block) . D by one thread (sparse: the _OPENMP branch of sdpa_linear.cpp:235-254; dense: Rdot)
-template <int L>
-__device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, PV sv, PV dv, PV D, int64_t doff) {
-    mp<L> ret = zero<L>();
-    if (d.kind == 1) {
-        for (int t = 0; t < d.nnz; t++) {
-            int64_t e = d.off + t;
-            int i = ri[e], j = ci[e];
-            mp<L> v = ldm<L>(sv, e);
-            mp<L> temp;
-            if (i == j) {
-                temp = mul<L>(v, ldm<L>(D, doff + i + (int64_t)d.n * j));
-            } else {
-                temp = add<L>(ldm<L>(D, doff + i + (int64_t)d.n * j), ldm<L>(D, doff + j + (int64_t)d.n * i));
-                temp = mul<L>(temp, v);
-            }
-            ret = add<L>(ret, temp);
-        }
-    } else if (d.kind == 2) {
-        int64_t nn = (int64_t)d.n * d.n;
-        for (int64_t k = 0; k < nn; k++) ret = fma_acc<L>(ret, ldm<L>(dv, d.off + k), ldm<L>(D, doff + k));
-    }
-    return ret;
-}
-
-// F1: value = A_j . G_i (Newton::calF1, sdpa_newton.cpp:897)
-// F2: sparse A_j: sum v * Rdot(n, X[alpha,:], F_i[:,beta]) (+mirror)  (Newton::calF2, sdpa_newton.cpp:899-937)
-//     dense A_j : A_j . G_i with G_i = X F_i
-// F3: Newton::calF3 (sdpa_newton.cpp:939-1005)
-template <int L>
-__global__ void k_pairs(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc, const int32_t *ri,
-                        const int32_t *ci, PV sv, PV dv, PV W, int64_t wstride, PV Fw, PV X, PV invZ, int64_t boff,
-                        PV B, int m) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
-    Pair p = pairs[t];
-    Desc dj = desc[(int64_t)p.j * nsdp + l];
-    int n = dj.n;
-    mp<L> value = zero<L>();
-    if (p.form == 1 || (p.form == 2 && dj.kind == 2)) {
-        value = blk_inner<L>(dj, ri, ci, sv, dv, W, (int64_t)p.slot * wstride);
-    } else if (p.form == 2) {
-        int64_t foff = (int64_t)p.slot * wstride;
-        for (int e = 0; e < dj.nnz; e++) {
-            int64_t ee = dj.off + e;
-            int alpha = ri[ee], beta = ci[ee];
-            mp<L> v1 = ldm<L>(sv, ee);
-            mp<L> v2 = zero<L>();
-            for (int k = 0; k < n; k++)
-                v2 = fma_acc<L>(v2, ldm<L>(X, boff + alpha + (int64_t)n * k), ldm<L>(Fw, foff + k + (int64_t)n * beta));
-            value = add<L>(value, mul<L>(v1, v2));
-            if (alpha != beta) {
-                v2 = zero<L>();
-                for (int k = 0; k < n; k++)
-                    v2 = fma_acc<L>(v2, ldm<L>(X, boff + beta + (int64_t)n * k),
-                                    ldm<L>(Fw, foff + k + (int64_t)n * alpha));
-                value = add<L>(value, mul<L>(v1, v2));
-            }
-        }
-    } else {
-        Desc di = desc[(int64_t)p.i * nsdp + l];
-        mp<L> local = zero<L>();
-        for (int i1 = 0; i1 < dj.nnz; i1++) {
-            int64_t e1 = dj.off + i1;
-            int alpha = ri[e1], beta = ci[e1];
-            mp<L> v1 = ldm<L>(sv, e1);
-            mp<L> sum = zero<L>();
-            for (int i2 = 0; i2 < di.nnz; i2++) {
-                int64_t e2 = di.off + i2;
-                int gamma = ri[e2], delta = ci[e2];
-                mp<L> v2 = ldm<L>(sv, e2);
-                mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * beta)),
-                                   ldm<L>(X, boff + alpha + (int64_t)n * gamma));
-                sum = add<L>(sum, plu);
-                if (gamma != delta) {
-                    plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * beta)),
-                                 ldm<L>(X, boff + alpha + (int64_t)n * delta));
-                    sum = add<L>(sum, plu);
-                }
-            }
-            local = add<L>(local, mul<L>(v1, sum));
-            if (alpha == beta) continue;
-            sum = zero<L>();
-            for (int i2 = 0; i2 < di.nnz; i2++) {
-                int64_t e2 = di.off + i2;
-                int gamma = ri[e2], delta = ci[e2];
-                mp<L> v2 = ldm<L>(sv, e2);
-                mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * alpha)),
-                                   ldm<L>(X, boff + beta + (int64_t)n * gamma));
-                sum = add<L>(sum, plu);
-                if (gamma != delta) {
-                    plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * alpha)),
-                                 ldm<L>(X, boff + beta + (int64_t)n * delta));
-                    sum = add<L>(sum, plu);
-                }
-            }
-            local = add<L>(local, mul<L>(v1, sum));
-        }
-        value = local;
-    }
-    bacc<L>(B, m, p.i, p.j, value);
-}
-
-// LP part (compute_bMat_dense_LP, sdpa_newton.cpp:1190-1225): pairs within one LP variable
-struct LPPair {
-    int32_t i, j;
-    int32_t lp;        // LP index
-    int32_t pad;
-    int64_t ei, ej;    // value indices
-};
-template <int L>
-__global__ void k_pairs_lp(int npairs, const LPPair *pairs, PV lpv, PV X, PV invZ, int64_t lpbase, PV B, int m) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
-    LPPair p = pairs[t];
-    mp<L> v = mul<L>(ldm<L>(X, lpbase + p.lp), ldm<L>(invZ, lpbase + p.lp));
-    v = mul<L>(v, ldm<L>(lpv, p.ei));
-    v = mul<L>(v, ldm<L>(lpv, p.ej));
-    bacc<L>(B, m, p.i, p.j, v);
-}
-
-// ---- F1 inner products in block fixed point ----
-// Constraint values of the sparse blocks, fixed once per problem: one exponent per (i, l)
-// block (its largest entry), offset-binary L-limb integers (W = 32L - 2) per entry, and the
-// offset sum RA = sum a' (L+1 limbs).  One thread per descriptor.
-template <int L>
-__global__ void k_fix_entries(int ndesc, const Desc *desc, PV sv, uint32_t *afx, int64_t sn, int32_t *ea,
-                              uint32_t *ra) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ndesc) return;
-    Desc d = desc[t];
-    int32_t emax = mpx::EZERO;
-    if (d.kind == 1)
-        for (int e = 0; e < d.nnz; e++) {
-            mp<L> x = ldm<L>(sv, d.off + e);
-            if (!is_zero<L>(x) && x.e > emax) emax = x.e;
-        }
-    if (emax == mpx::EZERO) emax = 0;
-    uint32_t rs[L + 1];
-#pragma unroll
-    for (int q = 0; q <= L; q++) rs[q] = 0;
-    if (d.kind == 1)
-        for (int e = 0; e < d.nnz; e++) {
-            uint32_t a[L];
-            fx_convert<L>(ldm<L>(sv, d.off + e), emax, 2, a);
-            uint32_t c = 0;
-#pragma unroll
-            for (int q = 0; q < L; q++) {
-                uint64_t s = (uint64_t)rs[q] + a[q] + c;
-                rs[q] = (uint32_t)s;
-                c = (uint32_t)(s >> 32);
-                afx[(int64_t)q * sn + d.off + e] = a[q];
-            }
-            rs[L] += c;
-        }
-    ea[t] = emax;
-#pragma unroll
-    for (int q = 0; q <= L; q++) ra[(int64_t)q * ndesc + t] = rs[q];
-}
-
-// G_s -> symmetric fixed form Gs[p,q] = G[p,q] + G[q,p] (p != q), G[p,p] on the diagonal,
-// exact integer sums of W = 32L - 3 fixed values (one exponent per slot), offset-binary.
-// One workgroup per slot (slot = blockIdx.x + s0).
-template <int L>
-__global__ __launch_bounds__(256) void k_fix_G(int n, int s0, PV G, int64_t wstride, uint32_t *gfx, int64_t gplane,
-                                               int32_t *eg) {
-    __shared__ int32_t smax[256];
-    int s = s0 + blockIdx.x;
-    int64_t base = (int64_t)s * wstride;
-    int64_t nn = (int64_t)n * n;
-    int32_t em = mpx::EZERO;
-    for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
-        mp<L> x = ldm<L>(G, base + t);
-        if (!is_zero<L>(x) && x.e > em) em = x.e;
-    }
-    smax[threadIdx.x] = em;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w && smax[threadIdx.x + w] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + w];
-        __syncthreads();
-    }
-    int32_t emax = smax[0] == mpx::EZERO ? 0 : smax[0];
-    for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
-        int p = (int)(t % n), q = (int)(t / n);
-        if (p > q) continue;
-        uint32_t a[L], b[L];
-        fx_convert<L>(ldm<L>(G, base + p + (int64_t)n * q), emax, 3, a, false);
-        if (p != q) {
-            fx_convert<L>(ldm<L>(G, base + q + (int64_t)n * p), emax, 3, b, false);
-            uint32_t c = 0;
-#pragma unroll
-            for (int k = 0; k < L; k++) {
-                uint64_t sum = (uint64_t)a[k] + b[k] + c;
-                a[k] = (uint32_t)sum;
-                c = (uint32_t)(sum >> 32);
-            }
-        }
-        a[L - 1] += 0x80000000u;
-#pragma unroll
-        for (int k = 0; k < L; k++) {
-            gfx[(int64_t)k * gplane + base + p + (int64_t)n * q] = a[k];
-            gfx[(int64_t)k * gplane + base + q + (int64_t)n * p] = a[k];
-        }
-    }
-    if (threadIdx.x == 0) eg[s] = emax;
-}
-
-// value = A_j . G_i = sum over the stored entries (p, q, v) of A_j of v * Gs_i[p, q]
-// (Newton::calF1 -> getInnerProduct(Sparse, Dense), sdpa_linear.cpp:235-254), exact in
-// offset-binary fixed point: multi-limb MADs into 96-bit columns, one normalisation per pair.
-template <int L>
-__global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc,
-                                                  const int32_t *ri, const int32_t *ci, const uint32_t *afx,
-                                                  int64_t sn, const int32_t *ea, const uint32_t *ra, int ndesc,
-                                                  const uint32_t *gfx, int64_t gplane, int64_t wstride,
-                                                  const int32_t *eg, PV B, int m) {
-    constexpr int S0 = (L >= 2) ? L - 2 : 0;
-    constexpr int NC = 2 * L - 1 - S0;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
-    Pair p = pairs[t];
-    int di = p.j * nsdp + l;
-    Desc dj = desc[di];
-    int n = dj.n;
-    int64_t gb = (int64_t)p.slot * wstride;
-    uint64_t lo[NC];
-    uint32_t hi[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        lo[c] = 0;
-        hi[c] = 0;
-    }
-    uint32_t rg[L + 1];
-#pragma unroll
-    for (int q = 0; q <= L; q++) rg[q] = 0;
-    for (int e = 0; e < dj.nnz; e++) {
-        int64_t ee = dj.off + e;
-        int64_t gi = gb + ri[ee] + (int64_t)n * ci[ee];
-        uint32_t a[L], g[L];
-#pragma unroll
-        for (int q = 0; q < L; q++) {
-            a[q] = afx[(int64_t)q * sn + ee];
-            g[q] = gfx[(int64_t)q * gplane + gi];
-        }
-        uint32_t c = 0;
-#pragma unroll
-        for (int q = 0; q < L; q++) {
-            uint64_t s = (uint64_t)rg[q] + g[q] + c;
-            rg[q] = (uint32_t)s;
-            c = (uint32_t)(s >> 32);
-        }
-        rg[L] += c;
-#pragma unroll
-        for (int s = S0; s <= 2 * L - 2; s++) {
-#pragma unroll
-            for (int q = 0; q < L; q++) {
-                int r = s - q;
-                if (r >= 0 && r < L) mpx::mac(lo[s - S0], hi[s - S0], a[q], g[r]);
-            }
-        }
-    }
-    uint32_t ras[L + 1];
-#pragma unroll
-    for (int q = 0; q <= L; q++) ras[q] = ra[(int64_t)q * ndesc + di];
-    int32_t E = ea[di] + eg[p.slot] - (32 * L - 2) - (32 * L - 3);
-    mp<L> v = fx_finish<L>(lo, hi, dj.nnz, ras, rg, E);
-    bacc<L>(B, m, p.i, p.j, v);
-}
-
-// mirror: B[j,i] = B[i,j] is maintained by bacc; nothing else needed.
-
-// ---- Cholesky (Rpotrf lower), blocked right-looking ----
-// k_potrf_diag: one workgroup of 1024 threads factors the nb x nb diagonal block at (k, k)
-// (nb <= 32) and forms its inverse Linv (nb x nb, leading dimension nb) by forward substitution
-// on the identity.  Pivot j: r_j = 1/sqrt(d_j) from one fixed-point Newton iteration
-// (mpx::rsqrt_fx); L_jj = d_j r_j and the column below is scaled by r_j (the reference divides
-// by L_jj, mplapack/Rtrsm.cpp:264-270).  Look-ahead: while waves 1-15 apply column j to the
-// trailing block and to the inverse, wave 0 forms the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2
-// and its r_{j+1}, so the serial chain per column is one multiply-subtract plus one rsqrt.
-// info receives the first failing global pivot (1-based, as Rpotrf2's info,
-// mplapack/Rpotrf2.cpp:86-97); once info is set, later launches return at once.
-// The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
-// serial pivot chain runs; only the finished factor, Linv and the pivots go back to HBM.
-// The matrix is A[off + i + ld*j]; the block starts at (k, k); rdiag[roff + k + j] receives the
-// reciprocal pivots, Linv[loff + i + nb*j] the block inverse.
-template <int L>
-__device__ __forceinline__ void sr_put(uint32_t *sr, const mp<L> &r) {
-    sr[0] = (uint32_t)r.e;
-    sr[1] = r.s;
-#pragma unroll
-    for (int q = 0; q < L; q++) sr[2 + q] = r.m[q];
-}
-template <int L>
-__device__ __forceinline__ mp<L> sr_get(const uint32_t *sr) {
-    mp<L> r;
-    r.e = (int32_t)sr[0];
-    r.s = sr[1];
-#pragma unroll
-    for (int q = 0; q < L; q++) r.m[q] = sr[2 + q];
-    return r;
-}
-
-template <int L>
-__global__ __launch_bounds__(1024) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
-                                                     PV Linv, int64_t loff, int *info) {
-    extern __shared__ uint32_t shm[];
-    __shared__ uint32_t sr[L + 2];
-    __shared__ int sfail;
-    if (*info) return;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int P = nb * nb;
-    uint32_t *sA = shm, *sX = shm + (L + 1) * P;
-    auto at = [&](int i, int j) -> int64_t { return off + (int64_t)(k + i) + (int64_t)ld * (k + j); };
-    for (int t = tid; t < P; t += nt) {
-        int i = t % nb, j = t / nb;
-        lds_st<L>(sA, P, t, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
-        lds_st<L>(sX, P, t, i == j ? from_int<L>(1) : zero<L>());
-    }
-    __syncthreads();
-    if (tid == 0) {  // first pivot
-        mp<L> d = lds_ld<L>(sA, P, 0);
-        sfail = sgn<L>(d) <= 0 ? k + 1 : 0;
-        if (!sfail) sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        if (sfail) {
-            if (tid == 0) *info = sfail;
-            return;
-        }
-        const int nc = nb - (j + 1);
-        // phase 1: scale column j and inverse row j by r_j; L_jj = d_j r_j
-        if (tid < nc || (tid >= 32 && tid <= 32 + j) || tid == 64) {
-            mp<L> r = sr_get<L>(sr);
-            if (tid == 64) {
-                lds_st<L>(sA, P, j + nb * j, mul<L>(lds_ld<L>(sA, P, j + nb * j), r));
-                stm<L>(rdiag, roff + k + j, r);
-            } else {
-                int t = tid < nc ? (j + 1 + tid) + nb * j : j + nb * (tid - 32);
-                uint32_t *base = tid < nc ? sA : sX;
-                lds_st<L>(base, P, t, mul<L>(r, lds_ld<L>(base, P, t)));
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // wave 0: the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2 and its r_{j+1}
-            if (nc > 0) {
-                mp<L> l = lds_ld<L>(sA, P, (j + 1) + nb * j);
-                mp<L> d = sub<L>(lds_ld<L>(sA, P, (j + 1) + nb * (j + 1)), mul<L>(l, l));
-                lds_st<L>(sA, P, (j + 1) + nb * (j + 1), d);
-                if (sgn<L>(d) <= 0)
-                    sfail = k + j + 2;
-                else
-                    sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
-            }
-        } else if (tid >= 64) {
-            // waves 1-15: trailing update A[i, c] -= L[i, j] L[c, j] (j < c <= i, except the next
-            // pivot) and the inverse X[i, c] -= L[i, j] X[j, c] (i > j, c <= j)
-            const int items = nc * nc + nc * (j + 1);
-            for (int t = tid - 64; t < items; t += nt - 64) {
-                if (t < nc * nc) {
-                    int c = j + 1 + t % nc, i = j + 1 + t / nc;
-                    if (i < c || (i == j + 1 && c == j + 1)) continue;
-                    mp<L> v = sub<L>(lds_ld<L>(sA, P, i + nb * c),
-                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sA, P, c + nb * j)));
-                    lds_st<L>(sA, P, i + nb * c, v);
-                } else {
-                    int u = t - nc * nc;
-                    int i = j + 1 + u % nc, c = u / nc;
-                    mp<L> v = sub<L>(lds_ld<L>(sX, P, i + nb * c),
-                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sX, P, j + nb * c)));
-                    lds_st<L>(sX, P, i + nb * c, v);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < P; t += nt) {
-        int i = t % nb, j = t / nb;
-        if (i >= j) stm<L>(A, at(i, j), lds_ld<L>(sA, P, t));
-        stm<L>(Linv, loff + t, lds_ld<L>(sX, P, t));
-    }
-}
-
-// dst[i + ldd*j] = src[i + lds*j] for an r x c sub-matrix
-template <int L>
-__global__ void k_copy2d(PV dst, int64_t doff, int ldd, PV src, int64_t soff, int lds, int r, int c) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)r * c) return;
-    int i = (int)(t % r), j = (int)(t / r);
-    stm<L>(dst, doff + i + (int64_t)ldd * j, ldm<L>(src, soff + i + (int64_t)lds * j));
-}
-
-// Lal::solveSystems (sdpa_linear.cpp:683-695): L y = g then L^T x = y, blocked by the
-// factorisation's diagonal blocks.  Diagonal step: x_k = inv(L_kk) x_k (trans: inv(L_kk)^T x_k)
-// with the stored block inverse; off-diagonal step: the remaining entries of x lose the block's
-// contribution.  One workgroup per diagonal step; 8 threads share each output's dot product.
-template <int L>
-__global__ __launch_bounds__(256) void k_trsv_diag(PV linv, int64_t loff, int nb, PV x, int64_t xo, int trans) {
-    __shared__ uint32_t sx[(L + 1) * 32];
-    __shared__ uint32_t sp[(L + 1) * 256];
-    const int tid = threadIdx.x, i = tid >> 3, q = tid & 7;
-    if (tid < nb) lds_st<L>(sx, 32, tid, ldm<L>(x, xo + tid));
-    __syncthreads();
-    mp<L> acc = zero<L>();
-    if (i < nb)
-        for (int j = q; j < nb; j += 8) {
-            if (trans ? (j < i) : (j > i)) continue;
-            mp<L> a = ldm<L>(linv, loff + (trans ? (j + (int64_t)nb * i) : (i + (int64_t)nb * j)));
-            acc = add<L>(acc, mul<L>(a, lds_ld<L>(sx, 32, j)));
-        }
-    lds_st<L>(sp, 256, tid, acc);
-    __syncthreads();
-    for (int h = 4; h >= 1; h >>= 1) {
-        if (q < h) lds_st<L>(sp, 256, tid, add<L>(lds_ld<L>(sp, 256, tid), lds_ld<L>(sp, 256, tid + h)));
-        __syncthreads();
-    }
-    if (q == 0 && i < nb) stm<L>(x, xo + i, lds_ld<L>(sp, 256, tid));
-}
-
-// forward (trans 0): x[r] -= sum_t A[r, k0+t] x[k0+t] for r >= k0+nb;
-// backward (trans 1): x[r] -= sum_t A[k0+t, r] x[k0+t] for r < k0
-template <int L>
-__global__ __launch_bounds__(256) void k_trsv_update(PV A, int m, int k0, int nb, PV x, int trans) {
-    __shared__ uint32_t sx[(L + 1) * 32];
-    __shared__ uint32_t sp[(L + 1) * 256];
-    const int tid = threadIdx.x, q = tid & 7;
-    const int r0 = trans ? 0 : k0 + nb;
-    const int nrows = trans ? k0 : m - k0 - nb;
-    const int r = blockIdx.x * 32 + (tid >> 3);
-    if (tid < nb) lds_st<L>(sx, 32, tid, ldm<L>(x, k0 + tid));
-    __syncthreads();
-    mp<L> acc = zero<L>();
-    if (r < nrows) {
-        int row = r0 + r;
-        for (int t = q; t < nb; t += 8) {
-            int64_t idx = trans ? ((int64_t)(k0 + t) + (int64_t)m * row) : ((int64_t)row + (int64_t)m * (k0 + t));
-            acc = add<L>(acc, mul<L>(ldm<L>(A, idx), lds_ld<L>(sx, 32, t)));
-        }
-    }
-    lds_st<L>(sp, 256, tid, acc);
-    __syncthreads();
-    for (int h = 4; h >= 1; h >>= 1) {
-        if (q < h) lds_st<L>(sp, 256, tid, add<L>(lds_ld<L>(sp, 256, tid), lds_ld<L>(sp, 256, tid + h)));
-        __syncthreads();
-    }
-    if (q == 0 && r < nrows) {
-        int row = r0 + r;
-        stm<L>(x, row, sub<L>(ldm<L>(x, row), lds_ld<L>(sp, 256, tid)));
-    }
-}
-
-// LP part of dX / r_zinv / x_rd_zinv : out = a*x*y*z style triple on diagonal entries
-template <int L>
-__global__ void k_lp_triple(PV out, PV A, PV Bm, PV C, int64_t base, int nlp) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlp) return;
-    mp<L> v = mul<L>(ldm<L>(A, base + t), ldm<L>(Bm, base + t));
-    v = mul<L>(v, ldm<L>(C, base + t));
-    stm<L>(out, base + t, v);
-}
-
-// ---- reductions, matrix-vector products and point factorisation (Solutions / Jal / Lal) ----
-
-// part[blockIdx.x] = sum over this block's share of x_i * y_i (y.p == nullptr: of x_i); when idx is
-// given, element t is x[xo + idx[t]]
-template <int L>
-__global__ __launch_bounds__(256) void k_dot_part(PV x, int64_t xo, PV y, int64_t yo, const int64_t *idx, int64_t n,
-                                                  PV part) {
-    __shared__ uint32_t sh[(L + 1) * 256];
-    mp<L> acc = zero<L>();
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t ix = idx ? idx[i] : i;
-        mp<L> a = ldm<L>(x, xo + ix);
-        acc = add<L>(acc, y.p ? mul<L>(a, ldm<L>(y, yo + i)) : a);
-    }
-    mp<L> r = block_sum<L>(acc, sh);
-    if (threadIdx.x == 0) stm<L>(part, blockIdx.x, r);
-}
-
-// y[yo + j] = sum_i A[ao + i + n j] x[xo + i] over i in [0, n) (tri 0), [j, n) (tri 1) or [0, j] (tri 2);
-// one wave per output
-template <int L>
-__global__ __launch_bounds__(256) void k_gemv_t(PV A, int64_t ao, int n, int tri, PV x, int64_t xo, PV y, int64_t yo) {
-    __shared__ uint32_t sh[(L + 1) * 256];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int j = blockIdx.x * 4 + w;
-    mp<L> acc = zero<L>();
-    if (j < n) {
-        int i0 = tri == 1 ? j : 0, i1 = tri == 2 ? j + 1 : n;
-        for (int i = i0 + lane; i < i1; i += 64)
-            acc = add<L>(acc, mul<L>(ldm<L>(A, ao + i + (int64_t)n * j), ldm<L>(x, xo + i)));
-    }
-    uint32_t *s = sh;  // per-wave tree inside the block planes
-    lds_st<L>(s, 256, threadIdx.x, acc);
-    __syncthreads();
-    for (int h = 32; h >= 1; h >>= 1) {
-        if (lane < h)
-            lds_st<L>(s, 256, threadIdx.x, add<L>(lds_ld<L>(s, 256, threadIdx.x), lds_ld<L>(s, 256, threadIdx.x + h)));
-        __syncthreads();
-    }
-    if (lane == 0 && j < n) stm<L>(y, yo + j, lds_ld<L>(s, 256, threadIdx.x));
-}
-
-// dst[do + j + n i] = src[so + i + n j]
-template <int L>
-__global__ void k_transpose(PV src, int64_t so, PV dst, int64_t dof, int n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n * n) return;
-    int i = (int)(t % n), j = (int)(t / n);
-    stm<L>(dst, dof + j + (int64_t)n * i, ldm<L>(src, so + t));
-}
-
-// LP part of Jal::getInvChol / getInvCholAndInv (sdpa_jordan.cpp:140-171): 1/sqrt(x), 1/sqrt(z), 1/z
-template <int L>
-__global__ void k_lp_inverse(PV X, PV Z, PV icx, PV icz, PV iz, int64_t base, int nlp, int *info) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlp) return;
-    mp<L> x = ldm<L>(X, base + t), z = ldm<L>(Z, base + t);
-    if (sgn<L>(x) < 0 || sgn<L>(z) < 0) {
-        atomicMax(info, 1);
-        return;
-    }
-    mp<L> rx = mpx::rsqrt<L>(x), rz = mpx::rsqrt<L>(z);
-    stm<L>(icx, base + t, rx);
-    stm<L>(icz, base + t, rz);
-    stm<L>(iz, base + t, mul<L>(rz, rz));
-}
-
-// LP part of Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:80-86): x * l * l per entry
-template <int L>
-__global__ void k_lp_triple_val(PV Xm, PV Lm, int64_t base, int nlp, PV out) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nlp) return;
-    mp<L> l = ldm<L>(Lm, base + t);
-    stm<L>(out, t, mul<L>(mul<L>(ldm<L>(Xm, base + t), l), l));
-}
-
-// ------------------------------------------------------------------ Engine
-
-struct EngineBase {
-    virtual ~EngineBase() {}
-    int nlimb = 0, prec = 0, m = 0, nsdp = 0, nlp = 0, device = 0;
-    std::vector<int> sizes;
-    std::vector<int64_t> blkoff;
-    int64_t Nd = 0;  // dense linear space elements
-    bool committed = false;
-    int gemm_mode = SDPG_GEMM_PLAIN, gemm_slices = 0;
-    double timers[8] = {0};
-    // staged input
-    struct Blk {
-        bool set = false, dense = false;
-        std::vector<int> r, c;
-        std::vector<uint32_t> v;  // packed mp
-    };
-    std::vector<Blk> blk;  // (m+1)*nsdp, index (con+1)*nsdp + l
-    std::vector<std::vector<int>> lp_pos;
-    std::vector<std::vector<uint32_t>> lp_val;  // per con (m+1)
-    std::vector<uint32_t> bvec;
-    std::vector<int> formula;
-
-    virtual int commit() = 0;
-    virtual int set_point(const uint32_t *X, const uint32_t *Z, const uint32_t *invZ, const uint32_t *y) = 0;
-    virtual int residuals(uint32_t *nrp, uint32_t *nrd, uint32_t *objP, uint32_t *objD) = 0;
-    virtual int gvec(int corrector, const uint32_t *mu, const uint32_t *beta, int phase) = 0;
-    virtual int schur() = 0;
-    virtual int potrf(int *info) = 0;
-    virtual int potrs() = 0;
-    virtual int dz(int phase) = 0;
-    virtual int dx() = 0;
-    virtual int factor_point(int *info) = 0;
-    virtual int update(const uint32_t *alphaP, const uint32_t *alphaD) = 0;
-    virtual int inner(int what, uint32_t *out) = 0;
-    virtual int min_eigen(int which, uint32_t *out) = 0;
-    virtual int get(int what, uint32_t *out) = 0;
-    virtual int put(int what, const uint32_t *in) = 0;
-    virtual int sync() = 0;
-    virtual bool slices_ok(int s) const = 0;
-    virtual int profile(int on, double *out, int n) = 0;
-};
-
-template <int L>
-struct Engine : EngineBase {
-    hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // device planar arrays
-    struct Arr {
-        uint32_t *p = nullptr;
-        int64_t n = 0;
-        PV pv() const { return PV{p, n}; }
-    };
-    Arr X, Z, invZ, Rd, rzinv, xrd, dX, dZ, W1, W2, DLS1, Cfull;
-    Arr y, Rp, g, dy, b, rdiag, scal;
-    Arr Bm;
-    Arr sval, dval, lpval;  // constraint data
-    Arr Fw, Gw;             // Schur workspaces
-    uint32_t *afx = nullptr, *rax = nullptr, *gfx = nullptr;  // fixed-point constraint values, G slots
-    int32_t *eax = nullptr, *egx = nullptr;
-    int64_t sn_total = 0, gplane = 0;
-    int64_t wstride = 0, wslots = 0;
-    // index arrays
-    Desc *d_desc = nullptr, *d_cdesc = nullptr;
-    int32_t *d_ri = nullptr, *d_ci = nullptr;
-    int64_t *d_lp_ptr = nullptr, *d_clp_ptr = nullptr;
-    int32_t *d_lp_pos = nullptr;
-    int64_t lp_c_base = 0;  // C's LP values start
-    // scatter index
-    int64_t *d_pos_ptr = nullptr, *d_pos_ent = nullptr, *d_pos_in_blk = nullptr, *d_dcon_ptr = nullptr,
-            *d_dcon_off = nullptr;
-    int32_t *d_pos_con = nullptr, *d_pos_blk = nullptr, *d_dcon = nullptr;
-    // Schur plan per block
-    struct BlockPlan {
-        std::vector<int> f1_sparse, f1_dense, f2_sparse, f2_dense;  // constraints needing F (and G)
-        std::vector<int> slot_of;  // constraint -> slot (or -1)
-        int nslots = 0;
-        int32_t *d_slot_cons = nullptr;  // per slot: sparse expansion slot index (for spmm)
-        int n_sp_slots = 0;
-        int32_t *d_sp_list = nullptr;    // slots of sparse constraints, in order
-        std::vector<int> sp_slots, dn_slots, g_slots;  // slots by kind; g_slots: need G
-        std::vector<int> dn_cons;
-        Pair *d_pairs = nullptr, *d_pairs_fx = nullptr;
-        int npairs = 0, npairs_fx = 0;
-        int64_t *d_exp_rowptr = nullptr, *d_exp_ent = nullptr;
-        int32_t *d_exp_col = nullptr, *d_exp_slot = nullptr;
-    };
-    std::vector<BlockPlan> plan;
-    LPPair *d_lppairs = nullptr;
-    int nlppairs = 0;
-    int *d_info = nullptr;
-    std::vector<void *> allocs;
-    KProf kprof;
-    GemmEngine<L> gemm;
-
-    ~Engine() override {
-        if (st) hipStreamSynchronize(st);
-        for (void *p : allocs) hipFree(p);
-        if (stage) hipFree(stage);
-        if (ev0) hipEventDestroy(ev0);
-        if (ev1) hipEventDestroy(ev1);
-        if (st) hipStreamDestroy(st);
-    }
-
-    template <class T>
-    int dalloc(T **p, int64_t count) {
-        *p = nullptr;
-        if (count <= 0) count = 1;
-        hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
-        if (e != hipSuccess) {
-            set_err(std::string("hipMalloc failed: ") + hipGetErrorString(e));
-            return SDPG_ENOMEM;
-        }
-        allocs.push_back(*p);
-        return 0;
-    }
-    int aalloc(Arr &a, int64_t n) {
-        a.n = n > 0 ? n : 1;
-        return dalloc(&a.p, (int64_t)(L + 1) * a.n);
-    }
-    template <class T>
-    int upload(T **p, const std::vector<T> &v) {
-        int rc = dalloc(p, (int64_t)v.size());
-        if (rc) return rc;
-        if (!v.empty()) {
-            HIPCHK(hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
-            HIPCHK(hipStreamSynchronize(st));  // the host vector may be released by the caller
-        }
-        return 0;
-    }
-    // staging buffer for packed transfers (grown on demand, reused afterwards)
-    uint32_t *stage = nullptr;
-    size_t stage_cap = 0;
-    int ensure_stage(size_t words) {
-        if (words <= stage_cap) return 0;
-        HIPCHK(hipStreamSynchronize(st));
-        if (stage) HIPCHK(hipFree(stage));
-        stage = nullptr;
-        size_t n = words + words / 4 + 1024;
-        HIPCHK(hipMalloc((void **)&stage, sizeof(uint32_t) * n));
-        stage_cap = n;
-        return 0;
-    }
-    // packed host -> planar device (synchronous with respect to the host buffer)
-    int put_packed(Arr &a, int64_t off, const uint32_t *src, int64_t cnt) {
-        if (cnt <= 0) return 0;
-        int rc = ensure_stage((size_t)(L + 2) * cnt);
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(st));
-        HIPCHK(hipMemcpyAsync(stage, src, sizeof(uint32_t) * (L + 2) * cnt, hipMemcpyHostToDevice, st));
-        k_unpack<L><<<cdiv(cnt, 256), 256, 0, st>>>(stage, a.pv(), off, cnt);
-        HIPCHK(hipStreamSynchronize(st));
-        return 0;
-    }
-    int get_packed(const Arr &a, int64_t off, uint32_t *dst, int64_t cnt) {
-        if (cnt <= 0) return 0;
-        int rc = ensure_stage((size_t)(L + 2) * cnt);
-        if (rc) return rc;
-        k_pack<L><<<cdiv(cnt, 256), 256, 0, st>>>(a.pv(), off, cnt, stage);
-        HIPCHK(hipMemcpyAsync(dst, stage, sizeof(uint32_t) * (L + 2) * cnt, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        return 0;
-    }
-    void zero_arr(Arr &a, int64_t off, int64_t cnt) {
-        if (cnt > 0) k_zero<L><<<cdiv(cnt, 256), 256, 0, st>>>(a.pv(), off, cnt);
-    }
-    static mp<L> unpack1(const uint32_t *w) {
-        mp<L> x;
-        x.e = (int32_t)w[0];
-        x.s = w[1] & 1u;
-        for (int k = 0; k < L; k++) x.m[k] = w[2 + k];
-        if (x.m[L - 1] == 0) x = zero<L>();
-        return x;
-    }
-    static void pack1(const mp<L> &x, uint32_t *w) {
-        w[0] = (uint32_t)x.e;
-        w[1] = x.s;
-        for (int k = 0; k < L; k++) w[2 + k] = x.m[k];
-    }
-    void tstart() { hipEventRecord(ev0, st); }
-    int tstop(int slot) {
-        hipEventRecord(ev1, st);
-        HIPCHK(hipEventSynchronize(ev1));
-        float ms = 0;
-        hipEventElapsedTime(&ms, ev0, ev1);
-        timers[slot] += ms;
-        return 0;
-    }
-
-    int init() {
-        HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIPCHK(hipEventCreate(&ev0));
-        HIPCHK(hipEventCreate(&ev1));
-        {
-            int v = 0;
-            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
-                lds_max = (size_t)v;
-        }
-        blkoff.resize(nsdp + 1);
-        Nd = 0;
-        for (int l = 0; l < nsdp; l++) {
-            blkoff[l] = Nd;
-            Nd += (int64_t)sizes[l] * sizes[l];
-        }
-        blkoff[nsdp] = Nd;
-        Nd += nlp;
-        blk.assign((size_t)(m + 1) * nsdp, Blk());
-        lp_pos.assign(m + 1, {});
-        lp_val.assign(m + 1, {});
-        return gemm.init(st, &kprof);
-    }
-
-    int commit() override {
-        const int W = L + 2;
-        // ---- constraint data: sparse entries, dense values, descriptors ----
-        std::vector<int32_t> ri, ci;
-        std::vector<uint32_t> sv, dvv;
-        std::vector<Desc> desc((size_t)(m + 1) * nsdp);
-        int64_t sn = 0, dn = 0;
-        for (int c = -1; c < m; c++)
-            for (int l = 0; l < nsdp; l++) {
-                Blk &B = blk[(size_t)(c + 1) * nsdp + l];
-                Desc d{};
-                d.n = sizes[l];
-                d.blkoff = blkoff[l];
-                if (!B.set) {
-                    d.kind = 0;
-                } else if (B.dense) {
-                    d.kind = 2;
-                    d.off = dn;
-                    d.nnz = sizes[l] * sizes[l];
-                    d.effect = d.nnz;
-                    dvv.insert(dvv.end(), B.v.begin(), B.v.end());
-                    dn += (int64_t)sizes[l] * sizes[l];
-                } else {
-                    d.kind = 1;
-                    d.off = sn;
-                    d.nnz = (int)B.r.size();
-                    int eff = 0;
-                    for (size_t t = 0; t < B.r.size(); t++) {
-                        ri.push_back(B.r[t]);
-                        ci.push_back(B.c[t]);
-                        eff += (B.r[t] == B.c[t]) ? 1 : 2;
-                    }
-                    d.effect = eff;
-                    sv.insert(sv.end(), B.v.begin(), B.v.end());
-                    sn += (int64_t)B.r.size();
-                }
-                desc[(size_t)(c + 1) * nsdp + l] = d;
-            }
-        int rc;
-        if ((rc = aalloc(sval, sn))) return rc;
-        if ((rc = put_packed(sval, 0, sv.data(), sn))) return rc;
-        if ((rc = aalloc(dval, dn))) return rc;
-        if ((rc = put_packed(dval, 0, dvv.data(), dn))) return rc;
-        if ((rc = upload(&d_ri, ri))) return rc;
-        if ((rc = upload(&d_ci, ci))) return rc;
-        std::vector<Desc> cdesc(desc.begin(), desc.begin() + nsdp), adesc(desc.begin() + nsdp, desc.end());
-        if ((rc = upload(&d_cdesc, cdesc))) return rc;
-        if ((rc = upload(&d_desc, adesc))) return rc;
-        hdesc = adesc;
-        // ---- LP data: constraints 0..m-1 then C ----
-        std::vector<int64_t> lptr(m + 1, 0), clptr(2, 0);
-        std::vector<int32_t> lpos;
-        std::vector<uint32_t> lv;
-        for (int c = 0; c < m; c++) {
-            lptr[c] = (int64_t)lpos.size();
-            for (size_t t = 0; t < lp_pos[c + 1].size(); t++) lpos.push_back(lp_pos[c + 1][t]);
-            lv.insert(lv.end(), lp_val[c + 1].begin(), lp_val[c + 1].end());
-        }
-        lptr[m] = (int64_t)lpos.size();
-        lp_c_base = (int64_t)lpos.size();
-        clptr[0] = lp_c_base;
-        for (size_t t = 0; t < lp_pos[0].size(); t++) lpos.push_back(lp_pos[0][t]);
-        lv.insert(lv.end(), lp_val[0].begin(), lp_val[0].end());
-        clptr[1] = (int64_t)lpos.size();
-        if ((rc = upload(&d_lp_ptr, lptr))) return rc;
-        if ((rc = upload(&d_clp_ptr, clptr))) return rc;
-        if ((rc = upload(&d_lp_pos, lpos))) return rc;
-        if ((rc = aalloc(lpval, (int64_t)lpos.size()))) return rc;
-        if ((rc = put_packed(lpval, 0, lv.data(), (int64_t)lpos.size()))) return rc;
-        // ---- scatter index over the dense linear space (A_i only, ordered by constraint) ----
-        {
-            std::vector<std::vector<std::pair<int, int64_t>>> lists(Nd);
-            std::vector<int64_t> dcon_ptr(nsdp + 1, 0), dcon_off;
-            std::vector<int32_t> dcon;
-            for (int c = 0; c < m; c++)
-                for (int l = 0; l < nsdp; l++) {
-                    const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
-                    if (d.kind == 1) {
-                        for (int t = 0; t < d.nnz; t++) {
-                            int64_t e = d.off + t;
-                            int i = ri[e], j = ci[e];
-                            lists[blkoff[l] + i + (int64_t)sizes[l] * j].push_back({c, e});
-                            if (i != j) lists[blkoff[l] + j + (int64_t)sizes[l] * i].push_back({c, e});
-                        }
-                    }
-                }
-            for (int l = 0; l < nsdp; l++) {
-                dcon_ptr[l] = (int64_t)dcon.size();
-                for (int c = 0; c < m; c++) {
-                    const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
-                    if (d.kind == 2) {
-                        dcon.push_back(c);
-                        dcon_off.push_back(d.off);
-                    }
-                }
-            }
-            dcon_ptr[nsdp] = (int64_t)dcon.size();
-            // LP positions
-            for (int c = 0; c < m; c++)
-                for (int64_t t = lptr[c]; t < lptr[c + 1]; t++) lists[blkoff[nsdp] + lpos[t]].push_back({c, -1 - t});
-            std::vector<int64_t> pptr(Nd + 1, 0), pent, pinb(Nd, 0);
-            std::vector<int32_t> pcon, pblk(Nd, -1);
-            for (int64_t q = 0; q < Nd; q++) {
-                pptr[q] = (int64_t)pcon.size();
-                for (auto &pr : lists[q]) {
-                    pcon.push_back(pr.first);
-                    pent.push_back(pr.second);
-                }
-            }
-            pptr[Nd] = (int64_t)pcon.size();
-            for (int l = 0; l < nsdp; l++)
-                for (int64_t k = 0; k < (int64_t)sizes[l] * sizes[l]; k++) {
-                    pblk[blkoff[l] + k] = l;
-                    pinb[blkoff[l] + k] = k;
-                }
-            // LP entries reference the lp value array: encode with a separate value view (offset by sn)
-            // by copying LP values after the sparse values into a combined scatter-value array.
-            bool has_lp_entries = false;
-            for (auto &e : pent)
-                if (e < 0) has_lp_entries = true;
-            if (has_lp_entries) {
-                // combined array = sval ++ lpval(A part)
-                Arr comb;
-                if ((rc = aalloc(comb, sn + lptr[m]))) return rc;
-                std::vector<uint32_t> allv(sv);
-                allv.insert(allv.end(), lv.begin(), lv.begin() + lptr[m] * W);
-                if ((rc = put_packed(comb, 0, allv.data(), sn + lptr[m]))) return rc;
-                for (auto &e : pent)
-                    if (e < 0) e = sn + (-1 - e);
-                scat_vals = comb;
-            } else {
-                scat_vals = sval;
-            }
-            if ((rc = upload(&d_pos_ptr, pptr))) return rc;
-            if ((rc = upload(&d_pos_con, pcon))) return rc;
-            if ((rc = upload(&d_pos_ent, pent))) return rc;
-            if ((rc = upload(&d_pos_blk, pblk))) return rc;
-            if ((rc = upload(&d_pos_in_blk, pinb))) return rc;
-            if ((rc = upload(&d_dcon_ptr, dcon_ptr))) return rc;
-            if ((rc = upload(&d_dcon, dcon))) return rc;
-            if ((rc = upload(&d_dcon_off, dcon_off))) return rc;
-        }
-        // ---- full C as a dense linear space ----
-        {
-            std::vector<mp<L>> cf(Nd, zero<L>());
-            for (int l = 0; l < nsdp; l++) {
-                const Blk &B = blk[l];
-                if (!B.set) continue;
-                int n = sizes[l];
-                if (B.dense) {
-                    for (int64_t k = 0; k < (int64_t)n * n; k++) cf[blkoff[l] + k] = unpack1(&B.v[k * W]);
-                } else {
-                    for (size_t t = 0; t < B.r.size(); t++) {
-                        mp<L> v = unpack1(&B.v[t * W]);
-                        int i = B.r[t], j = B.c[t];
-                        cf[blkoff[l] + i + (int64_t)n * j] = v;
-                        cf[blkoff[l] + j + (int64_t)n * i] = v;
-                    }
-                }
-            }
-            for (size_t t = 0; t < lp_pos[0].size(); t++) cf[blkoff[nsdp] + lp_pos[0][t]] = unpack1(&lp_val[0][t * W]);
-            std::vector<uint32_t> pk(Nd * W);
-            for (int64_t q = 0; q < Nd; q++) pack1(cf[q], &pk[q * W]);
-            if ((rc = aalloc(Cfull, Nd))) return rc;
-            if ((rc = put_packed(Cfull, 0, pk.data(), Nd))) return rc;
-        }
-        // ---- work arrays ----
-        Arr *dls[] = {&X, &Z, &invZ, &Rd, &rzinv, &xrd, &dX, &dZ, &W1, &W2, &DLS1};
-        for (Arr *a : dls)
-            if ((rc = aalloc(*a, Nd))) return rc;
-        Arr *vec[] = {&y, &Rp, &g, &dy, &b, &rdiag};
-        for (Arr *a : vec)
-            if ((rc = aalloc(*a, m))) return rc;
-        if ((rc = aalloc(scal, 4096))) return rc;
-        if ((rc = aalloc(Bm, (int64_t)m * m))) return rc;
-        if ((rc = dalloc(&d_info, 1))) return rc;
-        for (Arr *a : dls) zero_arr(*a, 0, Nd);
-        if ((rc = alloc_factor_space())) return rc;
-        if (!bvec.empty() && (rc = put_packed(b, 0, bvec.data(), m))) return rc;
-        // ---- Schur plan: pair rule (sdpa_newton.cpp:1052-1054) and workspace slots ----
-        plan.assign(nsdp, BlockPlan());
-        int64_t maxslots = 0, maxn2 = 1;
-        for (int l = 0; l < nsdp; l++) {
-            BlockPlan &P = plan[l];
-            int n = sizes[l];
-            maxn2 = std::max<int64_t>(maxn2, (int64_t)n * n);
-            std::vector<int> cons;
-            for (int c = 0; c < m; c++)
-                if (desc[(size_t)(c + 1) * nsdp + l].kind) cons.push_back(c);
-            P.slot_of.assign(m, -1);
-            std::vector<Pair> pairs, pairs_fx;
-            std::vector<int> slot_con;
-            for (int i : cons) {
-                const Desc &di = desc[(size_t)(i + 1) * nsdp + l];
-                int f = formula.empty() ? 1 : formula[(size_t)i * nsdp + l];
-                if (f < 1 || f > 3) f = 1;
-                // F2 (calF2, sdpa_newton.cpp:899-937) sums v * X[alpha,:] F[:,beta] per entry of A_j; that is
-                // A_j . (X F_i), so F2 owners also get G_i = X F_i from the batched GEMM and share the F1
-                // inner-product kernels (n^3 extra multiply-adds per F2 constraint instead of nnz_j * n per pair).
-                bool needG = (f == 1 || f == 2);
-                if (f == 1 || f == 2) {
-                    P.slot_of[i] = P.nslots++;
-                    slot_con.push_back(i);
-                    if (di.kind == 1)
-                        P.sp_slots.push_back(P.slot_of[i]);
-                    else
-                        P.dn_slots.push_back(P.slot_of[i]);
-                    if (needG) P.g_slots.push_back(P.slot_of[i]);
-                }
-                for (int j : cons) {
-                    const Desc &dj = desc[(size_t)(j + 1) * nsdp + l];
-                    if (di.effect < dj.effect || (di.effect == dj.effect && i < j)) continue;
-                    if ((f == 1 || f == 2) && dj.kind == 1)
-                        pairs_fx.push_back(Pair{i, j, P.slot_of[i], f});
-                    else
-                        pairs.push_back(Pair{i, j, P.slot_of[i], f});
-                }
-            }
-            P.npairs = (int)pairs.size();
-            if ((rc = upload(&P.d_pairs, pairs))) return rc;
-            P.npairs_fx = (int)pairs_fx.size();
-            if ((rc = upload(&P.d_pairs_fx, pairs_fx))) return rc;
-            P.dn_cons.clear();
-            for (int s : P.dn_slots) P.dn_cons.push_back(slot_con[s]);
-            // symmetric row expansion of sparse slots (for F = A invZ)
-            std::vector<int64_t> rowptr;
-            std::vector<int32_t> col, slotlist;
-            std::vector<int64_t> ent;
-            int k = 0;
-            for (int s : P.sp_slots) {
-                int c = slot_con[s];
-                const Desc &d = desc[(size_t)(c + 1) * nsdp + l];
-                std::vector<std::vector<std::pair<int, int64_t>>> rows(n);
-                for (int t = 0; t < d.nnz; t++) {
-                    int64_t e = d.off + t;
-                    int i = ri[e], j = ci[e];
-                    rows[i].push_back({j, e});
-                    if (i != j) rows[j].push_back({i, e});
-                }
-                for (int r = 0; r < n; r++) {
-                    rowptr.push_back((int64_t)col.size());
-                    for (auto &pr : rows[r]) {
-                        col.push_back(pr.first);
-                        ent.push_back(pr.second);
-                    }
-                }
-                rowptr.push_back((int64_t)col.size());  // sentinel of this slot (n+1 entries per slot)
-                slotlist.push_back(k++);
-            }
-            P.n_sp_slots = k;
-            if ((rc = upload(&P.d_exp_rowptr, rowptr))) return rc;
-            if ((rc = upload(&P.d_exp_col, col))) return rc;
-            if ((rc = upload(&P.d_exp_ent, ent))) return rc;
-            if ((rc = upload(&P.d_exp_slot, slotlist))) return rc;
-            std::vector<int32_t> spl(P.sp_slots.begin(), P.sp_slots.end());
-            if ((rc = upload(&P.d_sp_list, spl))) return rc;
-            maxslots = std::max<int64_t>(maxslots, P.nslots);
-        }
-        wstride = maxn2;
-        wslots = std::max<int64_t>(maxslots, 1);
-        if ((rc = aalloc(Fw, wstride * wslots))) return rc;
-        if ((rc = aalloc(Gw, wstride * wslots))) return rc;
-        gplane = wstride * wslots;
-        if ((rc = dalloc(&gfx, (int64_t)L * gplane))) return rc;
-        if ((rc = dalloc(&egx, wslots))) return rc;
-        {  // fixed-point constraint values for the F1 inner products
-            int ndesc = m * nsdp;
-            sn_total = std::max<int64_t>(sn, 1);
-            if ((rc = dalloc(&afx, (int64_t)L * sn_total))) return rc;
-            if ((rc = dalloc(&eax, std::max(ndesc, 1)))) return rc;
-            if ((rc = dalloc(&rax, (int64_t)(L + 1) * std::max(ndesc, 1)))) return rc;
-            if (ndesc)
-                k_fix_entries<L><<<cdiv(ndesc, 64), 64, 0, st>>>(ndesc, d_desc, sval.pv(), afx, sn_total, eax, rax);
-            HIPCHK(hipGetLastError());
-        }
-        // LP pairs
-        {
-            std::vector<LPPair> lpp;
-            for (int q = 0; q < nlp; q++) {
-                std::vector<std::pair<int, int64_t>> cl;
-                for (int c = 0; c < m; c++)
-                    for (int64_t t = lptr[c]; t < lptr[c + 1]; t++)
-                        if (lpos[t] == q) cl.push_back({c, t});
-                for (size_t a = 0; a < cl.size(); a++)
-                    for (size_t bb = a; bb < cl.size(); bb++)
-                        lpp.push_back(LPPair{cl[a].first, cl[bb].first, q, 0, cl[a].second, cl[bb].second});
-            }
-            nlppairs = (int)lpp.size();
-            if ((rc = upload(&d_lppairs, lpp))) return rc;
-        }
-        HIPCHK(hipStreamSynchronize(st));
-        // release staging memory
-        blk.clear();
-        blk.shrink_to_fit();
-        committed = true;
-        return 0;
-    }
-    Arr scat_vals;
-
-    int set_point(const uint32_t *Xp, const uint32_t *Zp, const uint32_t *iZp, const uint32_t *yp) override {
-        int rc;
-        if (Xp && (rc = put_packed(X, 0, Xp, Nd))) return rc;
-        if (Zp && (rc = put_packed(Z, 0, Zp, Nd))) return rc;
-        if (iZp && (rc = put_packed(invZ, 0, iZp, Nd))) return rc;
-        if (yp && (rc = put_packed(y, 0, yp, m))) return rc;
-        HIPCHK(hipStreamSynchronize(st));
-        return 0;
-    }
-
-    // per-block dense products via the GEMM engine
-    int block_gemm(Arr &C, const Arr &A, const Arr &B, int l) {
-        int n = sizes[l];
-        int64_t o = blkoff[l];
-        return gemm.gemm(n, n, n, A.pv(), o, n, false, B.pv(), o, n, false, C.pv(), o, n, GEMM_SET, 1, 0, 0, 0,
-                         gemm_mode, gemm_slices);
-    }
-    // out = A*B*C per block (ns_jordan_triple_product); tmp receives A*B
-    int triple(Arr &out, const Arr &A, const Arr &B, const Arr &C, Arr &tmp) {
-        int rc;
-        for (int l = 0; l < nsdp; l++) {
-            if ((rc = block_gemm(tmp, A, B, l))) return rc;
-            if ((rc = block_gemm(out, tmp, C, l))) return rc;
-        }
-        if (nlp) k_lp_triple<L><<<cdiv(nlp, 128), 128, 0, st>>>(out.pv(), A.pv(), B.pv(), C.pv(), blkoff[nsdp], nlp);
-        return 0;
-    }
-
-    int maxabs(const Arr &A, int64_t cnt, mp<L> &out) {
-        int nb = (int)std::min<int64_t>(64, std::max<int64_t>(1, cdiv(cnt, 256)));
-        k_maxabs<L><<<nb, 256, 0, st>>>(A.pv(), cnt, scal.pv());
-        std::vector<uint32_t> h((size_t)nb * (L + 2));
-        int rc = get_packed(scal, 0, h.data(), nb);
-        if (rc) return rc;
-        out = zero<L>();
-        for (int i = 0; i < nb; i++) {
-            mp<L> v = unpack1(&h[(size_t)i * (L + 2)]);
-            if (cmp_abs<L>(v, out) > 0) out = v;
-        }
-        return 0;
-    }
-
-    int residuals(uint32_t *nrp, uint32_t *nrd, uint32_t *objP, uint32_t *objD) override {
-        tstart();
-        // Rp = b - A.X
-        k_con_inner<L><<<cdiv(m, 4), 256, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
-                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), Rp.pv(), 1, 0);
-        // Rd = C - Z - sum y_i A_i
-        k_scatter<L><<<cdiv(Nd, 128), 128, 0, st>>>(Nd, d_pos_ptr, d_pos_con, d_pos_ent, scat_vals.pv(), dval.pv(),
-                                                     d_pos_blk, d_pos_in_blk, d_dcon_ptr, d_dcon, d_dcon_off, y.pv(),
-                                                     Rd.pv(), 0, Cfull.pv(), Z.pv(), Rd.pv(), 1);
-        int rc = tstop(5);
-        if (rc) return rc;
-        mp<L> v;
-        if (nrp) {
-            if ((rc = maxabs(Rp, m, v))) return rc;
-            pack1(v, nrp);
-        }
-        if (nrd) {
-            if ((rc = maxabs(Rd, Nd, v))) return rc;
-            pack1(v, nrd);
-        }
-        if (objP) {  // C . X
-            k_con_inner<L><<<1, 256, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr,
-                                              d_lp_pos, lpval.pv(), blkoff[nsdp], X.pv(), b.pv(), PV{scal.p, scal.n}, 2,
-                                              1);
-            std::vector<uint32_t> h(L + 2);
-            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
-            memcpy(objP, h.data(), sizeof(uint32_t) * (L + 2));
-        }
-        if (objD) {
-            if ((rc = dot(b, 0, &y, 0, nullptr, m, v))) return rc;
-            pack1(v, objD);
-        }
-        return 0;
-    }
-
-    int gvec(int corrector, const uint32_t *mu, const uint32_t *beta, int phase) override {
-        tstart();
-        mp<L> target = mul<L>(unpack1(beta), unpack1(mu));
-        mp<L> mone = from_int<L>(-1);
-        int rc;
-        // compute_rMat: r_zinv = (beta mu) invZ - X  [- dX dZ invZ]
-        k_lincomb<L><<<cdiv(Nd, 256), 256, 0, st>>>(rzinv.pv(), invZ.pv(), X.pv(), target, mone, 1, Nd);
-        if (corrector) {
-            if ((rc = triple(DLS1, dX, dZ, invZ, W2))) return rc;
-            k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(rzinv.pv(), rzinv.pv(), DLS1.pv(), mone, Nd);
-        }
-        // Make_gVec: DLS1 = r_zinv - x_rd_zinv (infeasible phases), gVec = -(A . DLS1) + Rp
-        if (phase == SDPG_PFEAS || phase == SDPG_NOINFO) {
-            if (!corrector)
-                if ((rc = triple(xrd, X, Rd, invZ, W2))) return rc;
-            k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(DLS1.pv(), rzinv.pv(), xrd.pv(), mone, Nd);
-        } else {
-            HIPCHK(hipMemcpyAsync(DLS1.p, rzinv.p, sizeof(uint32_t) * (L + 1) * Nd, hipMemcpyDeviceToDevice, st));
-        }
-        k_con_inner<L><<<cdiv(m, 4), 256, 0, st>>>(m, nsdp, d_desc, d_ri, d_ci, sval.pv(), dval.pv(), d_lp_ptr,
-                                                   d_lp_pos, lpval.pv(), blkoff[nsdp], DLS1.pv(), Rp.pv(), g.pv(), 0, 0);
-        return tstop(3);
-    }
-
-    int schur() override {
-        tstart();
-        zero_arr(Bm, 0, (int64_t)m * m);
-        int rc;
-        for (int l = 0; l < nsdp; l++) {
-            BlockPlan &P = plan[l];
-            int n = sizes[l];
-            int64_t o = blkoff[l];
-            // F_i = A_i invZ  (sparse: row expansion; dense: GEMM)
-            if (P.n_sp_slots) {
-                dim3 grid(cdiv((int64_t)n * n, 128), P.n_sp_slots);
-                k_spmm_rows<L><<<grid, 128, 0, st>>>(n, P.d_exp_rowptr, P.d_exp_col, P.d_exp_ent, sval.pv(),
-                                                     invZ.pv(), o, Fw.pv(), wstride, P.d_sp_list);
-            }
-            if (!P.dn_slots.empty()) {
-                // dense constraints: batched F_s = A_c invZ; their values are contiguous only if the
-                // constraints are, so launch per constraint run of equal stride
-                for (size_t t = 0; t < P.dn_slots.size(); t++) {
-                    int s = P.dn_slots[t], c = P.dn_cons[t];
-                    const Desc dd = hdesc[(size_t)c * nsdp + l];
-                    if ((rc = gemm.gemm(n, n, n, dval.pv(), dd.off, n, false, invZ.pv(), o, n, false, Fw.pv(),
-                                        (int64_t)s * wstride, n, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
-                        return rc;
-                }
-            }
-            // G_i = X F_i for slots needing it: one batched GEMM with X shared
-            if (!P.g_slots.empty()) {
-                // slots needing G are contiguous ranges in practice; batch over maximal runs
-                size_t a = 0;
-                while (a < P.g_slots.size()) {
-                    size_t e = a + 1;
-                    while (e < P.g_slots.size() && P.g_slots[e] == P.g_slots[e - 1] + 1) e++;
-                    int s0 = P.g_slots[a], cnt = (int)(e - a);
-                    if ((rc = gemm.gemm(n, n, n, X.pv(), o, n, false, Fw.pv(), (int64_t)s0 * wstride, n, false,
-                                        Gw.pv(), (int64_t)s0 * wstride, n, GEMM_SET, cnt, 0, wstride, wstride,
-                                        gemm_mode, gemm_slices)))
-                        return rc;
-                    a = e;
-                }
-            }
-            if (P.npairs_fx) {
-                // fixed symmetric G for the F1 slots (runs of consecutive slots)
-                size_t a = 0;
-                while (a < P.g_slots.size()) {
-                    size_t e = a + 1;
-                    while (e < P.g_slots.size() && P.g_slots[e] == P.g_slots[e - 1] + 1) e++;
-                    k_fix_G<L><<<(int)(e - a), 256, 0, st>>>(n, P.g_slots[a], Gw.pv(), wstride, gfx, gplane, egx);
-                    a = e;
-                }
-                k_pairs_fx<L><<<cdiv(P.npairs_fx, 128), 128, 0, st>>>(
-                    P.npairs_fx, P.d_pairs_fx, l, nsdp, d_desc, d_ri, d_ci, afx, sn_total, eax, rax, m * nsdp, gfx,
-                    gplane, wstride, egx, Bm.pv(), m);
-            }
-            if (P.npairs)
-                k_pairs<L><<<cdiv(P.npairs, 64), 64, 0, st>>>(P.npairs, P.d_pairs, l, nsdp, d_desc, d_ri, d_ci,
-                                                               sval.pv(), dval.pv(), Gw.pv(), wstride, Fw.pv(), X.pv(),
-                                                               invZ.pv(), o, Bm.pv(), m);
-        }
-        if (nlppairs)
-            k_pairs_lp<L><<<cdiv(nlppairs, 64), 64, 0, st>>>(nlppairs, d_lppairs, lpval.pv(), X.pv(), invZ.pv(),
-                                                             blkoff[nsdp], Bm.pv(), m);
-        HIPCHK(hipGetLastError());
-        return tstop(0);
-    }
-    std::vector<Desc> hdesc;  // host copy of the A descriptors (index c*nsdp + l)
-
-    // Cholesky workspaces: inverses of every diagonal block (LinvB for the Schur matrix, LinvW for
-    // the blocks of X and Z), the panel below the current block, pivots of X / Z blocks
-    Arr LinvB, LinvW, Pw, rdW;
-    Arr icX, icZ, Wc, Wt;        // inverse Cholesky factors of X and Z, factor / inverse workspaces
-    Arr lq, lqold, lr, lw, ltmp;  // Lanczos vectors
-    int64_t *d_diag_idx = nullptr;
-    int64_t ndiag = 0, maxn = 1;
-    size_t lds_max = 65536;
-    int NBK = 32;
-    size_t potrf_shbytes = 0;
-
-    int alloc_factor_space() {
-        NBK = 32;
-        while (NBK > 8 && (size_t)2 * (L + 1) * NBK * NBK * 4 > lds_max) NBK /= 2;
-        potrf_shbytes = (size_t)2 * (L + 1) * NBK * NBK * 4;
-        HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)potrf_shbytes));
-        maxn = 1;
-        for (int l = 0; l < nsdp; l++) maxn = std::max<int64_t>(maxn, sizes[l]);
-        int rc;
-        int64_t nbl_m = (m + NBK - 1) / NBK, nbl_n = (maxn + NBK - 1) / NBK;
-        if ((rc = aalloc(LinvB, nbl_m * NBK * NBK))) return rc;
-        if ((rc = aalloc(LinvW, nbl_n * NBK * NBK))) return rc;
-        if ((rc = aalloc(Pw, std::max<int64_t>(m, maxn) * NBK))) return rc;
-        if ((rc = aalloc(rdW, maxn))) return rc;
-        if ((rc = aalloc(icX, Nd)) || (rc = aalloc(icZ, Nd))) return rc;
-        if ((rc = aalloc(Wc, maxn * maxn)) || (rc = aalloc(Wt, maxn * maxn))) return rc;
-        Arr *lv[] = {&lq, &lqold, &lr, &lw, &ltmp};
-        for (Arr *v : lv)
-            if ((rc = aalloc(*v, maxn))) return rc;
-        std::vector<int64_t> di;
-        for (int l = 0; l < nsdp; l++)
-            for (int i = 0; i < sizes[l]; i++) di.push_back(blkoff[l] + i + (int64_t)sizes[l] * i);
-        for (int q = 0; q < nlp; q++) di.push_back(blkoff[nsdp] + q);
-        ndiag = (int64_t)di.size();
-        if ((rc = upload(&d_diag_idx, di))) return rc;
-        return 0;
-    }
-
-    // Rpotrf("Lower") of the n x n matrix A[off + i + ld j], blocked right-looking: per block of NBK
-    // columns a one-workgroup factorisation of the diagonal block (which also forms its inverse,
-    // kept in linv at block index k / NBK), the panel below it as a GEMM with that inverse, and the
-    // lower-triangular trailing update as a GEMM.  Pivot failures set d_info (checked by the caller).
-    int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
-        int rc;
-        for (int k = 0; k < n; k += NBK) {
-            int nb = std::min(NBK, n - k);
-            int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
-            hipEvent_t ev = kprof.begin(st);
-            k_potrf_diag<L><<<1, 1024, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
-            kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0);
-            if (k + nb < n) {
-                int r = n - k - nb;
-                int64_t a21 = off + (k + nb) + (int64_t)ld * k;
-                // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N).  The plain and
-                // Ozaki forms convert their operands before the product kernel, so P overwrites A21 in
-                // place; the element form reads its operands directly and goes through Pw.
-                if (gemm_mode == -1) {
-                    if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, Pw.pv(), 0, r, GEMM_SET,
-                                        1, 0, 0, 0, gemm_mode, gemm_slices)))
-                        return rc;
-                    k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(A, a21, ld, Pw.pv(), 0, r, r, nb);
-                } else if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, A, a21, ld, GEMM_SET,
-                                           1, 0, 0, 0, gemm_mode, gemm_slices))) {
-                    return rc;
-                }
-                // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
-                if ((rc = gemm.gemm(r, r, nb, A, a21, ld, false, A, a21, ld, true, A,
-                                    off + (k + nb) + (int64_t)ld * (k + nb), ld, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
-                                    gemm_slices, true)))
-                    return rc;
-            }
-        }
-        HIPCHK(hipGetLastError());
-        return 0;
-    }
-    int reset_info() {
-        k_fill_i32<<<1, 1, 0, st>>>(d_info, 1, 0);
-        HIPCHK(hipGetLastError());
-        return 0;
-    }
-    int read_info(int *info) {
-        int hinfo = 0;
-        HIPCHK(hipMemcpyAsync(&hinfo, d_info, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        *info = hinfo;
-        return 0;
-    }
-
-    int potrf(int *info) override {
-        tstart();
-        int rc;
-        if ((rc = reset_info())) return rc;
-        if ((rc = chol(Bm.pv(), 0, m, m, rdiag.pv(), 0, LinvB))) return rc;
-        if ((rc = read_info(info))) return rc;
-        return tstop(1);
-    }
-
-    // Lal::getInvLowTriangularMatrix: out[oo + i + ldo j] = inv(L) for the Cholesky factor in
-    // Lm[lo + i + ld j] whose diagonal-block inverses are in linv (block index s / NBK).  Recursive
-    // on halves: inv([A 0; C B]) = [inv(A) 0; -inv(B) C inv(A)  inv(B)].  The strictly upper part of
-    // out must be zero on entry.
-    int trtri(PV Lm, int64_t lo, int ld, int s, int len, Arr &linv, PV out, int64_t oo, int ldo) {
-        int rc;
-        if (len <= NBK) {
-            k_copy2d<L><<<cdiv((int64_t)len * len, 256), 256, 0, st>>>(out, oo + s + (int64_t)ldo * s, ldo, linv.pv(),
-                                                                     (int64_t)(s / NBK) * NBK * NBK, len, len, len);
-            return 0;
-        }
-        int nblk = (len + NBK - 1) / NBK;
-        int n1 = ((nblk + 1) / 2) * NBK, n2 = len - n1;
-        if ((rc = trtri(Lm, lo, ld, s, n1, linv, out, oo, ldo))) return rc;
-        if ((rc = trtri(Lm, lo, ld, s + n1, n2, linv, out, oo, ldo))) return rc;
-        // Wt = C inv(A)   (n2 x n1)
-        if ((rc = gemm.gemm(n2, n1, n1, Lm, lo + (s + n1) + (int64_t)ld * s, ld, false, out, oo + s + (int64_t)ldo * s,
-                            ldo, false, Wt.pv(), 0, n2, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
-            return rc;
-        // out21 = 0 - inv(B) Wt
-        return gemm.gemm(n2, n1, n2, out, oo + (s + n1) + (int64_t)ldo * (s + n1), ldo, false, Wt.pv(), 0, n2, false,
-                         out, oo + (s + n1) + (int64_t)ldo * s, ldo, GEMM_SUB, 1, 0, 0, 0, gemm_mode, gemm_slices);
-    }
-
-    // Solutions::computeInverse (sdpa_dataset.cpp:96-118): invCholX = inv(chol(X)),
-    // invCholZ = inv(chol(Z)), invZ = invCholZ^T invCholZ (Jal::getInvChol / getInvCholAndInv,
-    // sdpa_jordan.cpp:123-171); *info = 1 when X or Z is not positive definite.
-    int factor_point(int *info) override {
-        tstart();
-        int rc;
-        if ((rc = reset_info())) return rc;
-        for (int l = 0; l < nsdp; l++) {
-            int n = sizes[l];
-            int64_t o = blkoff[l];
-            for (int which = 0; which < 2; which++) {
-                Arr &src = which ? Z : X;
-                Arr &ic = which ? icZ : icX;
-                k_copy2d<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(Wc.pv(), 0, n, src.pv(), o, n, n, n);
-                if ((rc = chol(Wc.pv(), 0, n, n, rdW.pv(), 0, LinvW))) return rc;
-                zero_arr(ic, o, (int64_t)n * n);
-                if ((rc = trtri(Wc.pv(), 0, n, 0, n, LinvW, ic.pv(), o, n))) return rc;
-            }
-            if ((rc = gemm.gemm(n, n, n, icZ.pv(), o, n, true, icZ.pv(), o, n, false, invZ.pv(), o, n, GEMM_SET, 1, 0,
-                                0, 0, gemm_mode, gemm_slices)))
-                return rc;
-        }
-        if (nlp)
-            k_lp_inverse<L><<<cdiv(nlp, 128), 128, 0, st>>>(X.pv(), Z.pv(), icX.pv(), icZ.pv(), invZ.pv(),
-                                                            blkoff[nsdp], nlp, d_info);
-        HIPCHK(hipGetLastError());
-        int hinfo = 0;
-        if ((rc = read_info(&hinfo))) return rc;
-        *info = hinfo ? 1 : 0;
-        return tstop(6);
-    }
-
-    // Solutions::update (sdpa_dataset.cpp:120-133): X += alphaP dX, y += alphaD dy, Z += alphaD dZ
-    int update(const uint32_t *alphaP, const uint32_t *alphaD) override {
-        mp<L> aP = unpack1(alphaP), aD = unpack1(alphaD);
-        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(X.pv(), X.pv(), dX.pv(), aP, Nd);
-        k_axpy<L><<<cdiv(m, 256), 256, 0, st>>>(y.pv(), y.pv(), dy.pv(), aD, m);
-        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(Z.pv(), Z.pv(), dZ.pv(), aD, Nd);
-        HIPCHK(hipGetLastError());
-        return 0;
-    }
-
-    // sum of x_i y_i (or of x[idx[i]]) with a two-level reduction; the partial sums are added on the host
-    int dot(const Arr &xa, int64_t xo, const Arr *ya, int64_t yo, const int64_t *idx, int64_t n, mp<L> &out) {
-        int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, cdiv(n, 1024)));
-        k_dot_part<L><<<nb, 256, 0, st>>>(xa.pv(), xo, ya ? ya->pv() : PV{nullptr, 0}, yo, idx, n, scal.pv());
-        HIPCHK(hipGetLastError());
-        std::vector<uint32_t> h((size_t)nb * (L + 2));
-        int rc = get_packed(scal, 0, h.data(), nb);
-        if (rc) return rc;
-        out = zero<L>();
-        for (int i = 0; i < nb; i++) out = add<L>(out, unpack1(&h[(size_t)i * (L + 2)]));
-        return 0;
-    }
-
-    // scalar inner products of the step-length / SolveInfo logic (sdpa_parts.cpp:362-860)
-    int inner(int what, uint32_t *out) override {
-        mp<L> v = zero<L>();
-        int rc = 0;
-        switch (what) {
-        case SDPG_IP_XZ: rc = dot(X, 0, &Z, 0, nullptr, Nd, v); break;
-        case SDPG_IP_XDZ: rc = dot(X, 0, &dZ, 0, nullptr, Nd, v); break;
-        case SDPG_IP_DXZ: rc = dot(dX, 0, &Z, 0, nullptr, Nd, v); break;
-        case SDPG_IP_DXDZ: rc = dot(dX, 0, &dZ, 0, nullptr, Nd, v); break;
-        case SDPG_IP_BY: rc = dot(b, 0, &y, 0, nullptr, m, v); break;
-        case SDPG_IP_BDY: rc = dot(b, 0, &dy, 0, nullptr, m, v); break;
-        case SDPG_IP_TRX: rc = dot(X, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
-        case SDPG_IP_TRZ: rc = dot(Z, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
-        case SDPG_IP_CX:
-        case SDPG_IP_CDX: {
-            const Arr &M_ = what == SDPG_IP_CX ? X : dX;
-            k_con_inner<L><<<1, 256, 0, st>>>(1, nsdp, d_cdesc, d_ri, d_ci, sval.pv(), dval.pv(), d_clp_ptr, d_lp_pos,
-                                              lpval.pv(), blkoff[nsdp], M_.pv(), b.pv(), PV{scal.p, scal.n}, 2, 1);
-            std::vector<uint32_t> h(L + 2);
-            if ((rc = get_packed(scal, 0, h.data(), 1))) return rc;
-            v = unpack1(h.data());
-            break;
-        }
-        default: set_err("sdpg_inner: unknown selector"); return SDPG_EARG;
-        }
-        if (rc) return rc;
-        pack1(v, out);
-        return 0;
-    }
-
-    // Lal::getMinEigen (sdpa_linear.cpp:27-124) for one block: Lanczos on lMat xMat lMat^T with the
-    // matrix-vector products on the device; the tridiagonal eigenproblem (Rsteqr there) on the host
-    int lanczos(int n, int64_t o, const Arr &Lm, const Arr &Xm, mp<L> &res) {
-        using hla::operator+;
-        using hla::operator-;
-        using hla::operator*;
-        using hla::operator/;
-        using hla::operator<;
-        using hla::operator>;
-        using hla::operator>=;
-        using MM = hla::M<L>;
-        int rc;
-        // U = lMat^T so that lMat x is a column-contiguous product
-        k_transpose<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(Lm.pv(), o, Wt.pv(), 0, n);
-        mp<L> mn = MM::d(1.0e+51), mn_old = MM::d(1.0e+52), mn_min = MM::d(1.0e+50);
-        mp<L> error = MM::d(1.0e+10), one = MM::i(1);
-        std::vector<mp<L>> diagVec(n, mn_min), diagVec2(n, zero<L>()), out(n), bb(n);
-        // the loop ends by k = sqrt(n) + 10, so the tridiagonal eigenvectors need that many rows
-        const int ldq = std::min(n, (int)std::sqrt((double)n) + 12);
-        std::vector<mp<L>> Q((size_t)ldq * ldq);
-        mp<L> nD = MM::i(n);
-        mp<L> beta = hla::sqrt(nD);
-        const mp<L> sqrtn10 = hla::sqrt(nD) + MM::i(10), sqrtn9 = hla::sqrt(MM::i(n + 9));
-        const mp<L> c16 = MM::d(1.0e-16), c5 = MM::d(1.0e-5), c8 = MM::d(1.0e-8), c2 = MM::d(1.0e-2),
-                    c4 = MM::d(1.0e-4);
-        zero_arr(lq, 0, n);
-        {
-            std::vector<uint32_t> ones((size_t)n * (L + 2));
-            for (int i = 0; i < n; i++) pack1(one, &ones[(size_t)i * (L + 2)]);
-            if ((rc = put_packed(lr, 0, ones.data(), n))) return rc;
-        }
-        int k = 0, kk = 0;
-        const mp<L> zr = zero<L>();
-        for (;;) {
-            mp<L> kf = MM::i(k);
-            if (!(k < n) || !(kf < sqrtn10) || !(beta > c16)) break;
-            bool condA = hla::abs(mn - mn_old) > c5 * hla::abs(mn) + c8;
-            bool condB = hla::abs(error * beta) > c2 * hla::abs(mn) + c4;
-            if (!(condA || condB)) break;
-            HIPCHK(hipMemcpyAsync(lqold.p, lq.p, sizeof(uint32_t) * (L + 1) * lq.n, hipMemcpyDeviceToDevice, st));
-            mp<L> value = one / beta;
-            k_lincomb<L><<<cdiv(n, 256), 256, 0, st>>>(lq.pv(), lr.pv(), lr.pv(), value, zr, 0, n);
-            // w = lMat^T q ; tmp = xMat w ; w = lMat tmp
-            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Lm.pv(), o, n, 1, lq.pv(), 0, lw.pv(), 0);
-            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Xm.pv(), o, n, 0, lw.pv(), 0, ltmp.pv(), 0);
-            k_gemv_t<L><<<cdiv(n, 4), 256, 0, st>>>(Wt.pv(), 0, n, 2, ltmp.pv(), 0, lw.pv(), 0);
-            mp<L> alpha;
-            if ((rc = dot(lq, 0, &lw, 0, nullptr, n, alpha))) return rc;
-            diagVec[k] = alpha;
-            // r = w - alpha q - beta qold
-            k_lincomb<L><<<cdiv(n, 256), 256, 0, st>>>(lr.pv(), lw.pv(), lq.pv(), one, mpx::neg<L>(alpha), 1, n);
-            k_axpy<L><<<cdiv(n, 256), 256, 0, st>>>(lr.pv(), lr.pv(), lqold.pv(), mpx::neg<L>(beta), n);
-            if (MM::i(kk) >= hla::sqrt(MM::i(k)) || k == n - 1 || kf > sqrtn9) {
-                kk = 0;
-                out = diagVec;
-                bb = diagVec2;
-                out[n - 1] = diagVec[k];
-                bb[n - 1] = zr;
-                int info = hla::tridiag_ql<L>(k + 1, out.data(), bb.data(), Q.data(), ldq);
-                if (info > 0) break;
-                mn_old = mn;
-                mn = out[0];
-                error = Q[k];
-            }
-            mp<L> rr;
-            if ((rc = dot(lr, 0, &lr, 0, nullptr, n, rr))) return rc;
-            beta = hla::sqrt(rr);
-            diagVec2[k] = beta;
-            ++k;
-            ++kk;
-        }
-        res = mn - hla::abs(error * beta);
-        return 0;
-    }
-
-    // Jal::getMinEigen(lMat, xMat) (sdpa_jordan.cpp:55-88): which 0 = (invCholX, dX), 1 = (invCholZ, dZ)
-    int min_eigen(int which, uint32_t *out) override {
-        tstart();
-        using hla::operator<;
-        const Arr &Lm = which ? icZ : icX;
-        const Arr &Xm = which ? dZ : dX;
-        mp<L> mn = mpx::from_double<L>(1.0E50);
-        int rc;
-        for (int l = 0; l < nsdp; l++) {
-            int n = sizes[l];
-            int64_t o = blkoff[l];
-            mp<L> v;
-            if (n > 20) {
-                if ((rc = lanczos(n, o, Lm, Xm, v))) return rc;
-            } else {
-                // Lal::let(DLS2 = xMat lMat^T), DLS1 = lMat DLS2, Rsyev on DLS1 (host, cyclic Jacobi)
-                if ((rc = gemm.gemm(n, n, n, Xm.pv(), o, n, false, Lm.pv(), o, n, true, Wt.pv(), 0, n, GEMM_SET, 1, 0, 0,
-                                    0, gemm_mode, gemm_slices)))
-                    return rc;
-                if ((rc = gemm.gemm(n, n, n, Lm.pv(), o, n, false, Wt.pv(), 0, n, false, Wc.pv(), 0, n, GEMM_SET, 1, 0,
-                                    0, 0, gemm_mode, gemm_slices)))
-                    return rc;
-                std::vector<uint32_t> h((size_t)n * n * (L + 2));
-                if ((rc = get_packed(Wc, 0, h.data(), (int64_t)n * n))) return rc;
-                std::vector<mp<L>> w((size_t)n * n);
-                for (size_t q = 0; q < w.size(); q++) w[q] = unpack1(&h[q * (L + 2)]);
-                v = hla::jacobi_min_eig<L>(n, w.data());
-            }
-            if (v < mn) mn = v;
-        }
-        if (nlp) {
-            k_lp_triple_val<L><<<cdiv(nlp, 128), 128, 0, st>>>(Xm.pv(), Lm.pv(), blkoff[nsdp], nlp, Wt.pv());
-            std::vector<uint32_t> h((size_t)nlp * (L + 2));
-            if ((rc = get_packed(Wt, 0, h.data(), nlp))) return rc;
-            for (int q = 0; q < nlp; q++) {
-                mp<L> v = unpack1(&h[(size_t)q * (L + 2)]);
-                if (v < mn) mn = v;
-            }
-        }
-        pack1(mn, out);
-        return tstop(7);
-    }
-
-    int potrs() override {
-        tstart();
-        HIPCHK(hipMemcpyAsync(dy.p, g.p, sizeof(uint32_t) * (L + 1) * dy.n, hipMemcpyDeviceToDevice, st));
-        const int nblk = (m + NBK - 1) / NBK;
-        for (int kb = 0; kb < nblk; kb++) {  // L y = g
-            int k0 = kb * NBK, nb = std::min(NBK, m - k0);
-            k_trsv_diag<L><<<1, 256, 0, st>>>(LinvB.pv(), (int64_t)kb * NBK * NBK, nb, dy.pv(), k0, 0);
-            if (k0 + nb < m) k_trsv_update<L><<<cdiv(m - k0 - nb, 32), 256, 0, st>>>(Bm.pv(), m, k0, nb, dy.pv(), 0);
-        }
-        for (int kb = nblk - 1; kb >= 0; kb--) {  // L^T x = y
-            int k0 = kb * NBK, nb = std::min(NBK, m - k0);
-            k_trsv_diag<L><<<1, 256, 0, st>>>(LinvB.pv(), (int64_t)kb * NBK * NBK, nb, dy.pv(), k0, 1);
-            if (k0 > 0) k_trsv_update<L><<<cdiv(k0, 32), 256, 0, st>>>(Bm.pv(), m, k0, nb, dy.pv(), 1);
-        }
-        HIPCHK(hipGetLastError());
-        return tstop(2);
-    }
-
-    int dz(int phase) override {
-        tstart();
-        int useR = (phase == SDPG_PFEAS || phase == SDPG_NOINFO) ? 1 : 0;
-        k_scatter<L><<<cdiv(Nd, 128), 128, 0, st>>>(Nd, d_pos_ptr, d_pos_con, d_pos_ent, scat_vals.pv(), dval.pv(),
-                                                     d_pos_blk, d_pos_in_blk, d_dcon_ptr, d_dcon, d_dcon_off, dy.pv(),
-                                                     Rd.pv(), useR, Cfull.pv(), Z.pv(), dZ.pv(), 0);
-        HIPCHK(hipGetLastError());
-        return tstop(4);
-    }
-
-    int dx() override {
-        tstart();
-        int rc;
-        if ((rc = triple(W1, X, dZ, invZ, W2))) return rc;
-        mp<L> mone = from_int<L>(-1);
-        k_axpy<L><<<cdiv(Nd, 256), 256, 0, st>>>(dX.pv(), rzinv.pv(), W1.pv(), mone, Nd);
-        for (int l = 0; l < nsdp; l++) {
-            int n = sizes[l];
-            k_symmetrize<L><<<cdiv((int64_t)n * n, 256), 256, 0, st>>>(dX.pv(), blkoff[l], n);
-        }
-        HIPCHK(hipGetLastError());
-        return tstop(4);
-    }
-
-    Arr *sel(int what, int64_t &cnt) {
-        switch (what) {
-        case SDPG_X: cnt = Nd; return &X;
-        case SDPG_Z: cnt = Nd; return &Z;
-        case SDPG_INVZ: cnt = Nd; return &invZ;
-        case SDPG_Y: cnt = m; return &y;
-        case SDPG_RP: cnt = m; return &Rp;
-        case SDPG_RD: cnt = Nd; return &Rd;
-        case SDPG_GVEC: cnt = m; return &g;
-        case SDPG_B: cnt = (int64_t)m * m; return &Bm;
-        case SDPG_L: cnt = (int64_t)m * m; return &Bm;
-        case SDPG_DY: cnt = m; return &dy;
-        case SDPG_DZ: cnt = Nd; return &dZ;
-        case SDPG_DX: cnt = Nd; return &dX;
-        case SDPG_RZINV: cnt = Nd; return &rzinv;
-        case SDPG_INVCHOLX: cnt = Nd; return &icX;
-        case SDPG_INVCHOLZ: cnt = Nd; return &icZ;
-        }
-        return nullptr;
-    }
-    int get(int what, uint32_t *out) override {
-        int64_t cnt = 0;
-        Arr *a = sel(what, cnt);
-        if (!a) return SDPG_EARG;
-        return get_packed(*a, 0, out, cnt);
-    }
-    int put(int what, const uint32_t *in) override {
-        int64_t cnt = 0;
-        Arr *a = sel(what, cnt);
-        if (!a) return SDPG_EARG;
-        int rc = put_packed(*a, 0, in, cnt);
-        if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(st));
-        return 0;
-    }
-    int sync() override {
-        HIPCHK(hipStreamSynchronize(st));
-        return 0;
-    }
-    bool slices_ok(int s) const override { return OzakiWork<L>::slices_ok(s) != 0; }
-    // on: 1 start (reset), 0 stop; out (n >= 3 * KProf::NK): count, ms, work per kernel kind
-    int profile(int on, double *out, int n) override {
-        HIPCHK(hipStreamSynchronize(st));
-        kprof.flush();
-        if (out)
-            for (int k = 0; k < KProf::NK && 3 * k + 2 < n; k++) {
-                out[3 * k] = (double)kprof.cnt[k];
-                out[3 * k + 1] = kprof.ms[k];
-                out[3 * k + 2] = kprof.work[k];
-            }
-        if (on) kprof.reset();
-        kprof.on = on != 0;
-        return 0;
-    }
-};
-
-// Standalone multiple-precision GEMM on host buffers (packed words), one call = one stream:
-// C = op(A) op(B), C - op(A) op(B) or C + op(A) op(B), column-major as Rgemm.
-template <int L>
-static int rgemm_L(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
-                   const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower,
-                   double *ms) {
-    const int W = L + 2;
-    int64_t na = (int64_t)lda * (transa ? m : k), nb = (int64_t)ldb * (transb ? k : n), nc = (int64_t)ldc * n;
-    if (na <= 0) na = 1;
-    if (nb <= 0) nb = 1;
-    hipStream_t st = nullptr;
-    uint32_t *dA = nullptr, *dB = nullptr, *dC = nullptr, *stg = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = SDPG_OK;
-    int64_t stg_n = std::max(na, std::max(nb, nc)) * W;
-    auto fail = [&](const char *what) {
-        set_err(std::string("sdpg_rgemm: ") + what);
-        rc = SDPG_EHIP;
-    };
-    {
-        GemmEngine<L> ge;
-        if (hipSetDevice(device) != hipSuccess) {
-            set_err("sdpg_rgemm: bad device");
-            return SDPG_EARG;
-        }
-        if (hipStreamCreate(&st) != hipSuccess) {
-            fail("stream");
-            return rc;
-        }
-        if (hipMalloc((void **)&dA, sizeof(uint32_t) * (L + 1) * na) != hipSuccess ||
-            hipMalloc((void **)&dB, sizeof(uint32_t) * (L + 1) * nb) != hipSuccess ||
-            hipMalloc((void **)&dC, sizeof(uint32_t) * (L + 1) * nc) != hipSuccess ||
-            hipMalloc((void **)&stg, sizeof(uint32_t) * stg_n) != hipSuccess) {
-            set_err("sdpg_rgemm: out of device memory");
-            rc = SDPG_ENOMEM;
-        }
-        PV pa{dA, na}, pb{dB, nb}, pc{dC, nc};
-        auto upload = [&](const uint32_t *src, int64_t cnt, PV dst) {
-            if (rc) return;
-            if (hipMemcpyAsync(stg, src, sizeof(uint32_t) * W * cnt, hipMemcpyHostToDevice, st) != hipSuccess)
-                return fail("copy");
-            k_unpack<L><<<cdiv(cnt, 256), 256, 0, st>>>(stg, dst, 0, cnt);
-            if (hipStreamSynchronize(st) != hipSuccess) fail("unpack");
-        };
-        if (k > 0) {
-            upload(A, (int64_t)lda * (transa ? m : k), pa);
-            upload(B, (int64_t)ldb * (transb ? k : n), pb);
-        }
-        upload(C, nc, pc);
-        if (!rc) {
-            ge.init(st);
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
-            (void)hipEventRecord(e0, st);
-            int g = ge.gemm(m, n, k, pa, 0, lda, transa != 0, pb, 0, ldb, transb != 0, pc, 0, ldc, op, 1, 0, 0, 0,
-                            mode, slices, lower != 0);
-            (void)hipEventRecord(e1, st);
-            if (g == -1) {
-                set_err("sdpg_rgemm: slice count not available at this precision");
-                rc = SDPG_EARG;
-            } else if (g == -5) {
-                set_err("sdpg_rgemm: the Ozaki form is built for up to 384-bit limbs");
-                rc = SDPG_EPREC;
-            } else if (g) {
-                fail("gemm launch");
-            }
-        }
-        if (!rc) {
-            k_pack<L><<<cdiv(nc, 256), 256, 0, st>>>(pc, 0, nc, stg);
-            if (hipMemcpyAsync(C, stg, sizeof(uint32_t) * W * nc, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                fail("download");
-            float t = 0;
-            if (!rc && ms && hipEventElapsedTime(&t, e0, e1) == hipSuccess) *ms = t;
-        }
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(dA);
-    (void)hipFree(dB);
-    (void)hipFree(dC);
-    (void)hipFree(stg);
-    (void)hipStreamDestroy(st);
-    return rc;
-}
-
+void set_err(const std::string &s) { g_err = s; }
 }  // namespace sdpg
 
 // ------------------------------------------------------------------ C-ABI
@@ -2050,7 +23,7 @@ int sdpg_limbs(int prec_bits) { return mpx::limbs_for_bits(prec_bits); }
 
 static EngineBase *make_engine(int L) {
 #define MK_(LL) \
-    if (L == LL) return new Engine<LL>();
+    if (L == LL) return new_engine_##LL();
     SDPG_FOR_EACH_L(MK_)
 #undef MK_
     return nullptr;
@@ -2092,7 +65,7 @@ int sdpg_ctx_create(sdpg_ctx **ctx, int prec_bits, int m, int n_sdp, const int *
         }
     int rc = 0;
 #define IN_(LL) \
-    if (L == LL) rc = static_cast<Engine<LL> *>(e)->init();
+    if (L == LL) rc = init_engine_##LL(e);
     SDPG_FOR_EACH_L(IN_)
 #undef IN_
     if (rc) {
@@ -2325,8 +298,8 @@ int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, 
     if (m == 0 || n == 0) return SDPG_OK;
     int L = mpx::limbs_for_bits(prec_bits);
 #define RG_(LL)                                                                                                    \
-    if (L == LL) return rgemm_L<LL>(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, \
-                                    ms);
+    if (L == LL) return rgemm_##LL(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, \
+                                   ms);
     SDPG_FOR_EACH_L(RG_)
 #undef RG_
     set_err("sdpg_rgemm: unsupported precision");

@@ -1,0 +1,25 @@
+// engine_inst.hip — instantiation of Engine<L> and the standalone GEMM for one limb count.
+// Compiled once per L of SDPG_FOR_EACH_L with -DSDPG_INST_L=<L>.
+#include "engine_impl.h"
+
+#ifndef SDPG_INST_L
+#error "compile with -DSDPG_INST_L=<limb count>"
+#endif
+
+#define SDPG_CAT2_(a, b) a##b
+#define SDPG_CAT_(a, b) SDPG_CAT2_(a, b)
+
+namespace sdpg {
+
+EngineBase *SDPG_CAT_(new_engine_, SDPG_INST_L)() { return new Engine<SDPG_INST_L>(); }
+
+int SDPG_CAT_(init_engine_, SDPG_INST_L)(EngineBase *e) { return static_cast<Engine<SDPG_INST_L> *>(e)->init(); }
+
+int SDPG_CAT_(rgemm_, SDPG_INST_L)(int device, int transa, int transb, int m, int n, int k, const uint32_t *A,
+                                   int lda, const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode,
+                                   int slices, int lower, double *ms) {
+    return rgemm_L<SDPG_INST_L>(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower,
+                                ms);
+}
+
+}  // namespace sdpg

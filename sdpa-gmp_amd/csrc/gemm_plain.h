@@ -229,7 +229,7 @@ __global__ void k_fix_conv(GemmArgs g, int which, int R, const int32_t *eo, uint
     for (int q = 0; q < L; q++) out[base + ((int64_t)q * K + k) * R + r] = a[q];
 }
 
-__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+static __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
 }
