@@ -584,114 +584,110 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 
 // ---- Cholesky (Rpotrf lower), blocked right-looking ----
 // k_potrf_diag: one workgroup of 1024 threads factors the nb x nb diagonal block at (k, k)
-// (nb <= 32) and forms its inverse Linv (nb x nb, leading dimension nb) by forward substitution
-// on the identity.  Pivot j: r_j = 1/sqrt(d_j) from one fixed-point Newton iteration
-// (mpx::rsqrt_fx); L_jj = d_j r_j and the column below is scaled by r_j (the reference divides
-// by L_jj, mplapack/Rtrsm.cpp:264-270).  Look-ahead: while waves 1-15 apply column j to the
-// trailing block and to the inverse, wave 0 forms the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2
-// and its r_{j+1}, so the serial chain per column is one multiply-subtract plus one rsqrt.
-// info receives the first failing global pivot (1-based, as Rpotrf2's info,
-// mplapack/Rpotrf2.cpp:86-97); once info is set, later launches return at once.
-// The block and its inverse live in LDS (dynamic shared memory: 2 (L+1) nb^2 words) while the
-// serial pivot chain runs; only the finished factor, Linv and the pivots go back to HBM.
-// The matrix is A[off + i + ld*j]; the block starts at (k, k); rdiag[roff + k + j] receives the
-// reciprocal pivots, Linv[loff + i + nb*j] the block inverse.
+// (nb <= 32) and forms its inverse Linv (nb x nb, leading dimension nb).
+//
+// Division-free elimination (the Bareiss form of LDL^T), so that the serial chain of pivots
+// carries no square root and no reciprocal.  With S^(j) the Schur complement after j pivots
+// and c_j > 0 a running scale, the block holds At^(j) = c_j S^(j) and step j does, for every
+// i > j at once,
+//     At_ic <- 2^-e_j (d_j At_ic - At_ij At_cj)     (j < c <= i)
+//     Xt_ic <- 2^-e_j (d_j Xt_ic - At_ij Xt_jc)     (c <= j, and c = i where Xt_ji = 0)
+// with d_j = At_jj and e_j its binary exponent (an exact rescaling that keeps c_j in [2^-j, 1]);
+// Xt starts as the identity and row i ends as c_i times row i of inv(unit-lower factor), and
+// Xt_ii = c_i.  The true pivot is S_jj = d_j / c_j, so with r_j = 1/sqrt(c_j d_j), formed for
+// all j in parallel after the chain,
+//     L_ic = At_ic r_c,   inv(L)_ic = Xt_ic r_i,   1 / L_jj = c_j r_j.
+// Each step is one multiply-multiply-subtract per entry and one barrier; r_j is one fixed-point
+// Newton reciprocal square root (mpx::rsqrt_fx).  The reference divides each column by L_jj
+// after a square root per pivot (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:264-270); the
+// factor is the same up to rounding.  Since c_j > 0, d_j <= 0 exactly when the pivot S_jj <= 0:
+// info receives the first failing global pivot (1-based, as Rpotrf2's info); once info is set,
+// later launches return at once.
+// LDS (dynamic): At and Xt as (L+1) planes of nb (nb+1) words each, then r_j as (L+1) planes of nb.
 template <int L>
-__device__ __forceinline__ void sr_put(uint32_t *sr, const mp<L> &r) {
-    sr[0] = (uint32_t)r.e;
-    sr[1] = r.s;
-#pragma unroll
-    for (int q = 0; q < L; q++) sr[2 + q] = r.m[q];
-}
-template <int L>
-__device__ __forceinline__ mp<L> sr_get(const uint32_t *sr) {
-    mp<L> r;
-    r.e = (int32_t)sr[0];
-    r.s = sr[1];
-#pragma unroll
-    for (int q = 0; q < L; q++) r.m[q] = sr[2 + q];
-    return r;
+__device__ __forceinline__ mp<L> fms_scaled(const mp<L> &a, const mp<L> &b, const mp<L> &c, const mp<L> &d,
+                                            int32_t e) {
+    mp<L> v = sub<L>(mul<L>(a, b), mul<L>(c, d));
+    if (!is_zero<L>(v)) v.e -= e;
+    return v;
 }
 
 template <int L>
 __global__ __launch_bounds__(1024) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
                                                      PV Linv, int64_t loff, int *info) {
     extern __shared__ uint32_t shm[];
-    __shared__ uint32_t sr[L + 2];
-    __shared__ int sfail;
     if (*info) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int P = nb * nb;
-    uint32_t *sA = shm, *sX = shm + (L + 1) * P;
+    // LDS leading dimension nb + 1 (odd): a wave touching consecutive columns of one row, or
+    // consecutive rows of one column, hits distinct banks
+    const int lq = nb + 1, P = nb * lq;
+    uint32_t *sA = shm, *sX = shm + (L + 1) * P, *sR = shm + 2 * (L + 1) * P;
     auto at = [&](int i, int j) -> int64_t { return off + (int64_t)(k + i) + (int64_t)ld * (k + j); };
-    for (int t = tid; t < P; t += nt) {
+    for (int t = tid; t < nb * nb; t += nt) {
         int i = t % nb, j = t / nb;
-        lds_st<L>(sA, P, t, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
-        lds_st<L>(sX, P, t, i == j ? from_int<L>(1) : zero<L>());
-    }
-    __syncthreads();
-    if (tid == 0) {  // first pivot
-        mp<L> d = lds_ld<L>(sA, P, 0);
-        sfail = sgn<L>(d) <= 0 ? k + 1 : 0;
-        if (!sfail) sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
+        lds_st<L>(sA, P, i + lq * j, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
+        lds_st<L>(sX, P, i + lq * j, i == j ? from_int<L>(1) : zero<L>());
     }
     __syncthreads();
     for (int j = 0; j < nb; j++) {
-        if (sfail) {
-            if (tid == 0) *info = sfail;
+        // every thread reads the same pivot, so the failure exit is uniform
+        const mp<L> d = lds_ld<L>(sA, P, j + lq * j);
+        if (sgn<L>(d) <= 0) {
+            if (tid == 0) *info = k + j + 1;
             return;
         }
-        const int nc = nb - (j + 1);
-        // phase 1: scale column j and inverse row j by r_j; L_jj = d_j r_j
-        if (tid < nc || (tid >= 32 && tid <= 32 + j) || tid == 64) {
-            mp<L> r = sr_get<L>(sr);
-            if (tid == 64) {
-                lds_st<L>(sA, P, j + nb * j, mul<L>(lds_ld<L>(sA, P, j + nb * j), r));
-                stm<L>(rdiag, roff + k + j, r);
+        const int n1 = nb - 1 - j;
+        const int nA = n1 * (n1 + 1) / 2, nX = n1 * (j + 1);
+        const int t = tid;
+        if (t < nA + nX + n1) {
+            // one code path for both kinds of entries (no divergence inside a wave): the updated
+            // value U, At_ij and the third operand T, with U <- 2^-e (d U - At_ij T)
+            int i, c;
+            uint32_t *pU;
+            int iT;
+            if (t < nA) {
+                // lower triangle of the trailing block, row-major: t = ip (ip + 1) / 2 + cp, cp <= ip
+                int ip = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+                if ((ip + 1) * (ip + 2) / 2 <= t) ip++;
+                if (ip * (ip + 1) / 2 > t) ip--;
+                i = j + 1 + ip;
+                c = j + 1 + (t - ip * (ip + 1) / 2);
+                pU = sA;
+                iT = c + lq * j;  // At_cj
             } else {
-                int t = tid < nc ? (j + 1 + tid) + nb * j : j + nb * (tid - 32);
-                uint32_t *base = tid < nc ? sA : sX;
-                lds_st<L>(base, P, t, mul<L>(r, lds_ld<L>(base, P, t)));
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // wave 0: the next pivot d_{j+1} = A_{j+1,j+1} - L_{j+1,j}^2 and its r_{j+1}
-            if (nc > 0) {
-                mp<L> l = lds_ld<L>(sA, P, (j + 1) + nb * j);
-                mp<L> d = sub<L>(lds_ld<L>(sA, P, (j + 1) + nb * (j + 1)), mul<L>(l, l));
-                lds_st<L>(sA, P, (j + 1) + nb * (j + 1), d);
-                if (sgn<L>(d) <= 0)
-                    sfail = k + j + 2;
-                else
-                    sr_put<L>(sr, mpx::rsqrt_fx<L>(d));
-            }
-        } else if (tid >= 64) {
-            // waves 1-15: trailing update A[i, c] -= L[i, j] L[c, j] (j < c <= i, except the next
-            // pivot) and the inverse X[i, c] -= L[i, j] X[j, c] (i > j, c <= j)
-            const int items = nc * nc + nc * (j + 1);
-            for (int t = tid - 64; t < items; t += nt - 64) {
-                if (t < nc * nc) {
-                    int c = j + 1 + t % nc, i = j + 1 + t / nc;
-                    if (i < c || (i == j + 1 && c == j + 1)) continue;
-                    mp<L> v = sub<L>(lds_ld<L>(sA, P, i + nb * c),
-                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sA, P, c + nb * j)));
-                    lds_st<L>(sA, P, i + nb * c, v);
+                // Xt_ic for c <= j, then the diagonal Xt_ii, whose T = Xt_ji is an upper entry (zero)
+                const int u = t - nA;
+                if (u < nX) {
+                    i = j + 1 + u / (j + 1);
+                    c = u % (j + 1);
                 } else {
-                    int u = t - nc * nc;
-                    int i = j + 1 + u % nc, c = u / nc;
-                    mp<L> v = sub<L>(lds_ld<L>(sX, P, i + nb * c),
-                                     mul<L>(lds_ld<L>(sA, P, i + nb * j), lds_ld<L>(sX, P, j + nb * c)));
-                    lds_st<L>(sX, P, i + nb * c, v);
+                    i = j + 1 + (u - nX);
+                    c = i;
                 }
+                pU = sX;
+                iT = j + lq * c + (L + 1) * P;  // Xt_jc: offset from sA to sX folded in
             }
+            const int iU = i + lq * c;
+            mp<L> v = fms_scaled<L>(d, lds_ld<L>(pU, P, iU), lds_ld<L>(sA, P, i + lq * j), lds_ld<L>(sA, P, iT), d.e);
+            lds_st<L>(pU, P, iU, v);
         }
         __syncthreads();
     }
-    for (int t = tid; t < P; t += nt) {
-        int i = t % nb, j = t / nb;
-        if (i >= j) stm<L>(A, at(i, j), lds_ld<L>(sA, P, t));
-        stm<L>(Linv, loff + t, lds_ld<L>(sX, P, t));
+    if (tid < nb) {
+        const mp<L> cj = lds_ld<L>(sX, P, tid + lq * tid), dj = lds_ld<L>(sA, P, tid + lq * tid);
+        const mp<L> r = mpx::rsqrt_fx<L>(mul<L>(cj, dj));
+        lds_st<L>(sR, nb, tid, r);
+        stm<L>(rdiag, roff + k + tid, mul<L>(cj, r));
+    }
+    __syncthreads();
+    for (int t = tid; t < nb * nb; t += nt) {
+        int i = t % nb, c = t / nb;
+        if (i >= c) {
+            stm<L>(A, at(i, c), mul<L>(lds_ld<L>(sA, P, i + lq * c), lds_ld<L>(sR, nb, c)));
+            stm<L>(Linv, loff + t, mul<L>(lds_ld<L>(sX, P, i + lq * c), lds_ld<L>(sR, nb, i)));
+        } else {
+            stm<L>(Linv, loff + t, zero<L>());
+        }
     }
 }
 
@@ -1541,8 +1537,8 @@ struct Engine : EngineBase {
 
     int alloc_factor_space() {
         NBK = 32;
-        while (NBK > 8 && (size_t)2 * (L + 1) * NBK * NBK * 4 > lds_max) NBK /= 2;
-        potrf_shbytes = (size_t)2 * (L + 1) * NBK * NBK * 4;
+        while (NBK > 8 && ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4 > lds_max) NBK /= 2;
+        potrf_shbytes = ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4;
         HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)potrf_shbytes));
         maxn = 1;

@@ -60,18 +60,59 @@ MPX_HD uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t r) {
 #endif
 }
 
-// lo64 += a*b with carry into hi32 (one v_mad_u64_u32 + one v_addc on the GPU)
+// lo64 += a*b with carry into hi32 (one v_mad_u64_u32 + one v_addc on the GPU).  Not volatile:
+// the compiler may interleave independent accumulator chains, which is where the speed of the
+// products below comes from (a dependent chain of these pairs costs about 28 cycles per pair in
+// one wave, independent chains issue back to back).
 MPX_HD void mac(uint64_t &lo, uint32_t &hi, uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32 %1, vcc, 0, %1, vcc"
-                 : "+v"(lo), "+v"(hi)
-                 : "v"(a), "v"(b)
-                 : "vcc");
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32 %1, vcc, 0, %1, vcc"
+        : "+v"(lo), "+v"(hi)
+        : "v"(a), "v"(b)
+        : "vcc");
 #else
     uint64_t p = (uint64_t)a * b;
     lo += p;
     hi += (lo < p);
 #endif
+}
+
+// Two independent multiply-adds (lo0 += a0*b0, lo1 += a1*b1, carries into hi0 / hi1) with their
+// own carry registers, so the second multiply issues before the first carry is consumed.
+MPX_HD void mac2(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_t &lo1, uint32_t &hi1, uint32_t a1,
+                 uint32_t b1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %1, %5, %8, %9, %1\n\t"
+        "v_addc_co_u32 %2, %4, 0, %2, %4\n\t"
+        "v_addc_co_u32 %3, %5, 0, %3, %5"
+        : "+v"(lo0), "+v"(lo1), "+v"(hi0), "+v"(hi1), "=&s"(c0), "=&s"(c1)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1));
+#else
+    mac(lo0, hi0, a0, b0);
+    mac(lo1, hi1, a1, b1);
+#endif
+}
+
+// Column sums of a[0..N) x b[0..N) for the columns S0..2N-2 into acc[s - S0]: pairs of
+// multiply-adds on adjacent columns (p, q) and (p, q + 1) go through mac2.
+template <int N, int S0>
+MPX_HD void col_products(const uint32_t (&a)[N], const uint32_t (&b)[N], uint64_t (&lo)[2 * N - 1 - S0],
+                         uint32_t (&hi)[2 * N - 1 - S0]) {
+#pragma unroll
+    for (int p = 0; p < N; p++) {
+#pragma unroll
+        for (int q = 0; q < N; q += 2) {
+            const int s0 = p + q, s1 = p + q + 1;
+            if (q + 1 < N && s0 >= S0) {
+                mac2(lo[s0 - S0], hi[s0 - S0], a[p], b[q], lo[s1 - S0], hi[s1 - S0], a[p], b[q + 1]);
+            } else {
+                if (s0 >= S0) mac(lo[s0 - S0], hi[s0 - S0], a[p], b[q]);
+                if (q + 1 < N && s1 >= S0) mac(lo[s1 - S0], hi[s1 - S0], a[p], b[q + 1]);
+            }
+        }
+    }
 }
 
 template <int L>
@@ -272,26 +313,38 @@ MPX_HD mp<L> sub(const mp<L> &a, const mp<L> &b) {
     return add<L>(a, neg<L>(b));
 }
 
-// Product of mantissas: P[0..2L) top columns (columns below L-3 are skipped).
+// Carry-propagate NC column sums (column c = lo[c] + hi[c] 2^64) into NC + 1 words P[0..NC].
+template <int NC>
+MPX_HD void col_carry(const uint64_t (&lo)[NC], const uint32_t (&hi)[NC], uint32_t *P) {
+    uint64_t cl = 0;  // carry into the next column, < 2^64
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        uint64_t t = lo[c] + cl;
+        uint32_t cy = t < cl ? 1u : 0u;
+        P[c] = (uint32_t)t;
+        cl = (t >> 32) | ((uint64_t)(hi[c] + cy) << 32);
+    }
+    P[NC] = (uint32_t)cl;
+}
+
+// Product of mantissas: P[0..2L) top columns (columns below L-3 are skipped and read as zero).
+// Every column has its own 96-bit accumulator, so consecutive multiply-adds are independent
+// (issued in pairs by mac2); the carries run once at the end.
 template <int L>
 MPX_HD void mant_mul(const uint32_t (&a)[L], const uint32_t (&b)[L], uint32_t (&P)[2 * L]) {
     constexpr int S0 = (L >= 3) ? L - 3 : 0;
-    uint64_t lo = 0;
-    uint32_t hi = 0;
+    constexpr int NC = 2 * L - 1 - S0;
+    uint64_t lo[NC];
+    uint32_t hi[NC];
 #pragma unroll
-    for (int s = 0; s < 2 * L - 1; s++) {
-        if (s >= S0) {
-#pragma unroll
-            for (int p = 0; p < L; p++) {
-                int q = s - p;
-                if (q >= 0 && q < L) mac(lo, hi, a[p], b[q]);
-            }
-        }
-        P[s] = (uint32_t)lo;
-        lo = (lo >> 32) | ((uint64_t)hi << 32);
-        hi = 0;
+    for (int c = 0; c < NC; c++) {
+        lo[c] = 0;
+        hi[c] = 0;
     }
-    P[2 * L - 1] = (uint32_t)lo;
+    col_products<L, S0>(a, b, lo, hi);
+#pragma unroll
+    for (int s = 0; s < S0; s++) P[s] = 0;
+    col_carry<NC>(lo, hi, P + S0);
 }
 
 template <int L>
@@ -470,28 +523,25 @@ MPX_HD mp<L> rsqrt(const mp<L> &a) {
 // ---- fixed-point reciprocal square root for the Cholesky pivots ----
 // Fixed point with 2 integer bits: an N-word unsigned X stands for X / 2^(32N-2).
 
-// r = (a * b) >> (32N - 2), product columns below N-2 skipped (error below 2^-(32N-36))
+// r = (a * b) >> (32N - 2), product columns below N-2 skipped (error below 2^-(32N-36));
+// column-parallel accumulators as in mant_mul
 template <int N>
 MPX_HD void fx2_mulhi(const uint32_t (&a)[N], const uint32_t (&b)[N], uint32_t (&r)[N]) {
     constexpr int S0 = N >= 2 ? N - 2 : 0;
-    uint32_t P[2 * N + 1];
-    uint64_t lo = 0;
-    uint32_t hi = 0;
+    constexpr int NC = 2 * N - 1 - S0;
+    uint64_t lo[NC];
+    uint32_t hi[NC];
 #pragma unroll
-    for (int s = 0; s < 2 * N - 1; s++) {
-        if (s >= S0) {
-#pragma unroll
-            for (int p = 0; p < N; p++) {
-                int q = s - p;
-                if (q >= 0 && q < N) mac(lo, hi, a[p], b[q]);
-            }
-        }
-        P[s] = (uint32_t)lo;
-        lo = (lo >> 32) | ((uint64_t)hi << 32);
-        hi = 0;
+    for (int c = 0; c < NC; c++) {
+        lo[c] = 0;
+        hi[c] = 0;
     }
-    P[2 * N - 1] = (uint32_t)lo;
-    P[2 * N] = (uint32_t)(lo >> 32);
+    col_products<N, S0>(a, b, lo, hi);
+    uint32_t P[2 * N + 1];
+#pragma unroll
+    for (int s = 0; s < S0; s++) P[s] = 0;
+    col_carry<NC>(lo, hi, P + S0);
+    P[2 * N] = 0;
 #pragma unroll
     for (int k = 0; k < N; k++) r[k] = (P[N + k] << 2) | (P[N + k - 1] >> 30);
 }
