@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -300,6 +301,13 @@ struct Pair {
     int32_t form;    // 1, 2, 3
 };
 
+// F1/F2 pairs whose partner A_j is dense, served from the Schur inner-product GEMM:
+// value = Cs[idx] = A_j . G_i (sdpa_newton.cpp:897 with the dense branch of getInnerProduct)
+struct GvPair {
+    int32_t i, j;
+    int64_t idx;  // slot_i + M * rank_j in the GEMM output
+};
+
 // B[i,j] += value, B[j,i] += value (i != j) — one thread per pair of one block
 template <int L>
 __device__ __forceinline__ void bacc(PV B, int m, int i, int j, const mp<L> &v) {
@@ -412,6 +420,82 @@ __global__ void k_pairs(int npairs, const Pair *pairs, int l, int nsdp, const De
         value = local;
     }
     bacc<L>(B, m, p.i, p.j, value);
+}
+
+// Packed lower triangle of count n x n matrices (column-major packed index
+// t = b n - b (b - 1) / 2 + (a - b) for a >= b): dst[t] = src[a, b] (sum = 0), or
+// src[a, a] on the diagonal and src[a, b] + src[b, a] below it (sum = 1).  For symmetric A,
+// A . G = sum_t packA[t] packG[t] with the second form for G, which halves the inner-product length.
+// One workgroup per 16 x 16 tile (a-block >= b-block) of one matrix: the tile and its mirror are
+// read with coalesced loads into LDS, so the transposed operand costs no strided HBM reads.
+constexpr int SYMP_T = 16;
+template <int L>
+__global__ __launch_bounds__(256) void k_sym_pack(PV src, int64_t soff, int64_t sstride, PV dst, int64_t doff,
+                                                  int64_t dstride, int n, int sum) {
+    __shared__ uint32_t t1[L + 1][SYMP_T][SYMP_T + 1], t2[L + 1][SYMP_T][SYMP_T + 1];
+    // lower tile (ia, ib), ia >= ib, from the linear tile index
+    int r = blockIdx.x, ia = 0;
+    while (r > ia) {
+        r -= ia + 1;
+        ia++;
+    }
+    const int ib = r;
+    const int c = blockIdx.y, tx = threadIdx.x % SYMP_T, ty = threadIdx.x / SYMP_T;
+    const int64_t so = soff + (int64_t)c * sstride;
+    const int a0 = ia * SYMP_T, b0 = ib * SYMP_T;
+    {
+        int a = a0 + tx, b = b0 + ty;  // t1[ty][tx] = src[a, b]
+        if (a < n && b < n) {
+            mp<L> v = ldm<L>(src, so + a + (int64_t)n * b);
+#pragma unroll
+            for (int q = 0; q < L; q++) t1[q][ty][tx] = v.m[q];
+            t1[L][ty][tx] = (uint32_t)((v.e << 1) | (int32_t)(v.s & 1u));
+        }
+        int bb = b0 + tx, aa = a0 + ty;  // t2[ty][tx] = src[b0 + tx, a0 + ty]
+        if (sum && aa < n && bb < n) {
+            mp<L> v = ldm<L>(src, so + bb + (int64_t)n * aa);
+#pragma unroll
+            for (int q = 0; q < L; q++) t2[q][ty][tx] = v.m[q];
+            t2[L][ty][tx] = (uint32_t)((v.e << 1) | (int32_t)(v.s & 1u));
+        }
+    }
+    __syncthreads();
+    const int a = a0 + tx, b = b0 + ty;
+    if (a >= n || b >= n || a < b) return;
+    mp<L> v;
+#pragma unroll
+    for (int q = 0; q < L; q++) v.m[q] = t1[q][ty][tx];
+    {
+        int32_t h = (int32_t)t1[L][ty][tx];
+        v.e = h >> 1;
+        v.s = (uint32_t)h & 1u;
+    }
+    if (sum && a != b) {  // src[b, a] = t2[tx][ty]
+        mp<L> w;
+#pragma unroll
+        for (int q = 0; q < L; q++) w.m[q] = t2[q][tx][ty];
+        int32_t h = (int32_t)t2[L][tx][ty];
+        w.e = h >> 1;
+        w.s = (uint32_t)h & 1u;
+        v = add<L>(v, w);
+    }
+    int64_t pt = (int64_t)b * n - (int64_t)b * (b - 1) / 2 + (a - b);
+    stm<L>(dst, doff + (int64_t)c * dstride + pt, v);
+}
+
+// launch shape of k_sym_pack for n x n matrices: lower 16 x 16 tiles
+inline unsigned sym_pack_tiles(int n) {
+    unsigned nb = (unsigned)((n + SYMP_T - 1) / SYMP_T);
+    return nb * (nb + 1) / 2;
+}
+
+// dense-partner F1/F2 pairs: B[i,j] += Cs[idx] (and the mirror)
+template <int L>
+__global__ void k_pairs_gv(int npairs, const GvPair *pairs, PV Cs, PV B, int m) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    GvPair p = pairs[t];
+    bacc<L>(B, m, p.i, p.j, ldm<L>(Cs, p.idx));
 }
 
 // LP part (compute_bMat_dense_LP, sdpa_newton.cpp:1190-1225): pairs within one LP variable
@@ -903,6 +987,8 @@ struct Engine : EngineBase {
     Arr Bm;
     Arr sval, dval, lpval;  // constraint data
     Arr Fw, Gw;             // Schur workspaces
+    Arr Csw;                // Schur inner products with dense partners (slots x dense constraints)
+    Arr dpk;                // packed lower triangles of those dense constraints
     uint32_t *afx = nullptr, *rax = nullptr, *gfx = nullptr;  // fixed-point constraint values, G slots
     int32_t *eax = nullptr, *egx = nullptr;
     int64_t sn_total = 0, gplane = 0;
@@ -929,6 +1015,14 @@ struct Engine : EngineBase {
         std::vector<int> dn_cons;
         Pair *d_pairs = nullptr, *d_pairs_fx = nullptr;
         int npairs = 0, npairs_fx = 0;
+        // dense partners: Cs = Gw^T [vec A_j] over the dense constraints of the block (GEMM, K = n^2)
+        GvPair *d_pairs_gv = nullptr;
+        int npairs_gv = 0, gv_M = 0, gv_N = 0;
+        bool gv_lower = false;
+        int64_t gv_boff = 0;  // dval offset of the first dense constraint of the block
+        int64_t gv_pkoff = 0;  // offset of the block's packed dense constraints in dpk
+        // runs of dense slots (first slot, dval offset, count) with consecutive slots and values
+        std::vector<std::array<int64_t, 3>> dn_runs;
         int64_t *d_exp_rowptr = nullptr, *d_exp_ent = nullptr;
         int32_t *d_exp_col = nullptr, *d_exp_slot = nullptr;
     };
@@ -1065,8 +1159,10 @@ struct Engine : EngineBase {
         std::vector<uint32_t> sv, dvv;
         std::vector<Desc> desc((size_t)(m + 1) * nsdp);
         int64_t sn = 0, dn = 0;
-        for (int c = -1; c < m; c++)
-            for (int l = 0; l < nsdp; l++) {
+        // block-major: the dense values of one block are contiguous in constraint order (C first), so
+        // the dense constraints of a block form one strided operand of the batched and Schur GEMMs
+        for (int l = 0; l < nsdp; l++)
+            for (int c = -1; c < m; c++) {
                 Blk &B = blk[(size_t)(c + 1) * nsdp + l];
                 Desc d{};
                 d.n = sizes[l];
@@ -1239,7 +1335,7 @@ struct Engine : EngineBase {
         if (!bvec.empty() && (rc = put_packed(b, 0, bvec.data(), m))) return rc;
         // ---- Schur plan: pair rule (sdpa_newton.cpp:1052-1054) and workspace slots ----
         plan.assign(nsdp, BlockPlan());
-        int64_t maxslots = 0, maxn2 = 1;
+        int64_t maxslots = 0, maxn2 = 1, maxcs = 0;
         for (int l = 0; l < nsdp; l++) {
             BlockPlan &P = plan[l];
             int n = sizes[l];
@@ -1249,7 +1345,14 @@ struct Engine : EngineBase {
                 if (desc[(size_t)(c + 1) * nsdp + l].kind) cons.push_back(c);
             P.slot_of.assign(m, -1);
             std::vector<Pair> pairs, pairs_fx;
+            std::vector<GvPair> pairs_gv;
+            std::vector<std::array<int, 4>> gvt;  // (i, j, slot_i, rank_j) of the dense-partner pairs
             std::vector<int> slot_con;
+            // rank of each dense constraint of the block (its column in the Schur inner-product GEMM)
+            std::vector<int> drank(m, -1);
+            int ndense = 0;
+            for (int c : cons)
+                if (desc[(size_t)(c + 1) * nsdp + l].kind == 2) drank[c] = ndense++;
             for (int i : cons) {
                 const Desc &di = desc[(size_t)(i + 1) * nsdp + l];
                 int f = formula.empty() ? 1 : formula[(size_t)i * nsdp + l];
@@ -1272,6 +1375,8 @@ struct Engine : EngineBase {
                     if (di.effect < dj.effect || (di.effect == dj.effect && i < j)) continue;
                     if ((f == 1 || f == 2) && dj.kind == 1)
                         pairs_fx.push_back(Pair{i, j, P.slot_of[i], f});
+                    else if ((f == 1 || f == 2) && dj.kind == 2)
+                        gvt.push_back({i, j, P.slot_of[i], drank[j]});
                     else
                         pairs.push_back(Pair{i, j, P.slot_of[i], f});
                 }
@@ -1280,8 +1385,44 @@ struct Engine : EngineBase {
             if ((rc = upload(&P.d_pairs, pairs))) return rc;
             P.npairs_fx = (int)pairs_fx.size();
             if ((rc = upload(&P.d_pairs_fx, pairs_fx))) return rc;
+            P.npairs_gv = (int)gvt.size();
+            if (P.npairs_gv) {
+                // GEMM rows: the slots (G_s), columns: the dense constraints in rank order, whose values are
+                // consecutive n^2 runs from gv_boff
+                P.gv_M = P.nslots;
+                P.gv_N = ndense;
+                for (const auto &q : gvt) pairs_gv.push_back(GvPair{q[0], q[1], (int64_t)q[2] + (int64_t)q[3] * P.gv_M});
+                P.gv_boff = -1;
+                for (int c : cons) {
+                    const Desc &dc = desc[(size_t)(c + 1) * nsdp + l];
+                    if (dc.kind != 2) continue;
+                    if (P.gv_boff < 0) P.gv_boff = dc.off;
+                    if (dc.off != P.gv_boff + (int64_t)drank[c] * n * n) {
+                        set_err("commit: dense constraint values are not contiguous per block");
+                        return SDPG_ESTATE;
+                    }
+                }
+                // the lower triangle suffices when every pair has slot_i >= rank_j
+                P.gv_lower = true;
+                for (const GvPair &q : pairs_gv)
+                    if (q.idx % P.gv_M < q.idx / P.gv_M) P.gv_lower = false;
+                maxcs = std::max<int64_t>(maxcs, (int64_t)P.gv_M * P.gv_N);
+            }
+            if ((rc = upload(&P.d_pairs_gv, pairs_gv))) return rc;
             P.dn_cons.clear();
             for (int s : P.dn_slots) P.dn_cons.push_back(slot_con[s]);
+            P.dn_runs.clear();
+            for (size_t t = 0; t < P.dn_slots.size(); t++) {
+                int64_t off = desc[(size_t)(P.dn_cons[t] + 1) * nsdp + l].off;
+                if (!P.dn_runs.empty()) {
+                    auto &r = P.dn_runs.back();
+                    if (P.dn_slots[t] == r[0] + r[2] && off == r[1] + r[2] * (int64_t)n * n) {
+                        r[2]++;
+                        continue;
+                    }
+                }
+                P.dn_runs.push_back({(int64_t)P.dn_slots[t], off, 1});
+            }
             // symmetric row expansion of sparse slots (for F = A invZ)
             std::vector<int64_t> rowptr;
             std::vector<int32_t> col, slotlist;
@@ -1318,6 +1459,26 @@ struct Engine : EngineBase {
         }
         wstride = maxn2;
         wslots = std::max<int64_t>(maxslots, 1);
+        if (maxcs && (rc = aalloc(Csw, maxcs))) return rc;
+        {  // packed lower triangles of the dense constraints that feed the Schur inner-product GEMM
+            int64_t tot = 0;
+            for (int l = 0; l < nsdp; l++)
+                if (plan[l].npairs_gv) {
+                    plan[l].gv_pkoff = tot;
+                    tot += (int64_t)sizes[l] * (sizes[l] + 1) / 2 * plan[l].gv_N;
+                }
+            if (tot) {
+                if ((rc = aalloc(dpk, tot))) return rc;
+                for (int l = 0; l < nsdp; l++) {
+                    const BlockPlan &P = plan[l];
+                    if (!P.npairs_gv) continue;
+                    int n = sizes[l];
+                    k_sym_pack<L><<<dim3(sym_pack_tiles(n), P.gv_N), 256, 0, st>>>(
+                        dval.pv(), P.gv_boff, (int64_t)n * n, dpk.pv(), P.gv_pkoff, (int64_t)n * (n + 1) / 2, n, 0);
+                }
+                HIPCHK(hipGetLastError());
+            }
+        }
         if ((rc = aalloc(Fw, wstride * wslots))) return rc;
         if ((rc = aalloc(Gw, wstride * wslots))) return rc;
         gplane = wstride * wslots;
@@ -1472,16 +1633,13 @@ struct Engine : EngineBase {
                 k_spmm_rows<L><<<grid, 128, 0, st>>>(n, P.d_exp_rowptr, P.d_exp_col, P.d_exp_ent, sval.pv(),
                                                      invZ.pv(), o, Fw.pv(), wstride, P.d_sp_list);
             }
-            if (!P.dn_slots.empty()) {
-                // dense constraints: batched F_s = A_c invZ; their values are contiguous only if the
-                // constraints are, so launch per constraint run of equal stride
-                for (size_t t = 0; t < P.dn_slots.size(); t++) {
-                    int s = P.dn_slots[t], c = P.dn_cons[t];
-                    const Desc dd = hdesc[(size_t)c * nsdp + l];
-                    if ((rc = gemm.gemm(n, n, n, dval.pv(), dd.off, n, false, invZ.pv(), o, n, false, Fw.pv(),
-                                        (int64_t)s * wstride, n, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices)))
-                        return rc;
-                }
+            // dense constraints: F_s = A_c invZ, one batched GEMM per run of consecutive slots whose values
+            // are consecutive in dval (one run when every constraint of the block is dense)
+            for (const auto &r : P.dn_runs) {
+                if ((rc = gemm.gemm(n, n, n, dval.pv(), r[1], n, false, invZ.pv(), o, n, false, Fw.pv(),
+                                    r[0] * wstride, n, GEMM_SET, (int)r[2], (int64_t)n * n, 0, wstride, gemm_mode,
+                                    gemm_slices, false, GEMM_KEEP_A)))
+                    return rc;
             }
             // G_i = X F_i for slots needing it: one batched GEMM with X shared
             if (!P.g_slots.empty()) {
@@ -1515,6 +1673,18 @@ struct Engine : EngineBase {
                 k_pairs<L><<<cdiv(P.npairs, 64), 64, 0, st>>>(P.npairs, P.d_pairs, l, nsdp, d_desc, d_ri, d_ci,
                                                                sval.pv(), dval.pv(), Gw.pv(), wstride, Fw.pv(), X.pv(),
                                                                invZ.pv(), o, Bm.pv(), m);
+            if (P.npairs_gv) {
+                // inner products with the dense partners as one GEMM over packed lower triangles (F is no
+                // longer needed, its slots take the packed G):  Cs[s, r] = packG_s . packA_r
+                const int np = n * (n + 1) / 2;
+                k_sym_pack<L><<<dim3(sym_pack_tiles(n), P.gv_M), 256, 0, st>>>(Gw.pv(), 0, wstride, Fw.pv(), 0,
+                                                                              wstride, n, 1);
+                if ((rc = gemm.gemm(P.gv_M, P.gv_N, np, Fw.pv(), 0, (int)wstride, true, dpk.pv(), P.gv_pkoff, np, false,
+                                    Csw.pv(), 0, P.gv_M, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices, P.gv_lower,
+                                    GEMM_KEEP_B)))
+                    return rc;
+                k_pairs_gv<L><<<cdiv(P.npairs_gv, 64), 64, 0, st>>>(P.npairs_gv, P.d_pairs_gv, Csw.pv(), Bm.pv(), m);
+            }
         }
         if (nlppairs)
             k_pairs_lp<L><<<cdiv(nlppairs, 64), 64, 0, st>>>(nlppairs, d_lppairs, lpval.pv(), X.pv(), invZ.pv(),

@@ -21,12 +21,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "engine_common.h"
 #include "mpx.h"
 
 namespace sdpg {
 
 enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
+enum { GEMM_KEEP_A = 1, GEMM_KEEP_B = 2 };
 
 struct GemmArgs {
     int M, N, K;
@@ -76,6 +79,55 @@ __global__ void k_gemm_elem(GemmArgs g) {
     stm<L>(g.C, c, acc);
 }
 
+// Operands whose data never change while a context lives (the constraint matrices A_i) can keep
+// their converted form (block fixed point or INT8 slices, with the row exponents) between calls.
+// OpKey identifies such an operand and its conversion; OpCache owns the converted copies.
+struct OpKey {
+    const void *p;
+    int64_t off, stride;
+    int ld, trans, R, K, nb, form;
+    bool operator==(const OpKey &o) const {
+        return p == o.p && off == o.off && stride == o.stride && ld == o.ld && trans == o.trans && R == o.R &&
+               K == o.K && nb == o.nb && form == o.form;
+    }
+};
+template <class T>
+struct OpCache {
+    struct Ent {
+        OpKey key;
+        T *data;
+        int32_t *exps;
+    };
+    std::vector<Ent> ents;
+    ~OpCache() {
+        for (Ent &e : ents) {
+            (void)hipFree(e.data);
+            (void)hipFree(e.exps);
+        }
+    }
+    // the entry for key (fresh = true when it was just allocated and still has to be filled)
+    Ent *get(const OpKey &key, size_t n_data, size_t n_exps, bool &fresh) {
+        for (Ent &e : ents)
+            if (e.key == key) {
+                fresh = false;
+                return &e;
+            }
+        Ent e{key, nullptr, nullptr};
+        if (hipMalloc((void **)&e.data, sizeof(T) * (n_data ? n_data : 1)) != hipSuccess) return nullptr;
+        if (hipMalloc((void **)&e.exps, sizeof(int32_t) * (n_exps ? n_exps : 1)) != hipSuccess) {
+            (void)hipFree(e.data);
+            return nullptr;
+        }
+        ents.push_back(e);
+        fresh = true;
+        return &ents.back();
+    }
+};
+inline OpKey op_key(const GemmArgs &g, int which, int form) {
+    return which == 0 ? OpKey{g.A.p, g.offA, g.sA, g.lda, g.transA, g.M, g.K, g.sA ? g.batch : 1, form}
+                      : OpKey{g.B.p, g.offB, g.sB, g.ldb, g.transB, g.N, g.K, g.sB ? g.batch : 1, form};
+}
+
 // element (r, k) of an operand seen as R rows over K (which = 0: op(A)[r, k]; 1: op(B)[k, r])
 __device__ __forceinline__ int64_t fx_at(const GemmArgs &g, int which, int b, int r, int k) {
     return which == 0 ? idxA(g, r, k, b) : idxB(g, k, r, b);
@@ -103,10 +155,11 @@ struct GemmEngine {
         ozaki.prof = prof;
         return 0;
     }
-    // mode: 0 plain, 1 ozaki, -1 element (reference) form
+    // mode: 0 plain, 1 ozaki, -1 element (reference) form; keep: GEMM_KEEP_A / GEMM_KEEP_B mark an
+    // operand whose data never change, whose converted form is then kept and reused
     int gemm(int M, int N, int K, PV A, int64_t offA, int lda, bool transA, PV B, int64_t offB, int ldb, bool transB,
              PV C, int64_t offC, int ldc, int op, int batch, int64_t sA, int64_t sB, int64_t sC, int mode, int slices,
-             bool lower = false) {
+             bool lower = false, int keep = 0) {
         if (M <= 0 || N <= 0 || batch <= 0) return 0;
         GemmArgs g{M, N, K, A, offA, lda, transA ? 1 : 0, B, offB, ldb, transB ? 1 : 0, C, offC, ldc, op,
                    batch, sA, sB, sC, lower ? 1 : 0};
@@ -118,8 +171,8 @@ struct GemmEngine {
             k_gemm_elem<L><<<grd, blk, 0, st>>>(g);
             return hipGetLastError() == hipSuccess ? 0 : -2;
         }
-        if (mode == 1) return ozaki.run(g, st, slices);
-        return plain.run(g, st);
+        if (mode == 1) return ozaki.run(g, st, slices, keep);
+        return plain.run(g, st, keep);
     }
 };
 

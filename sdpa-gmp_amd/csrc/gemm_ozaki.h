@@ -5,13 +5,17 @@
 //    |Y| < 2^(8S-2); the same for the columns of op(B).
 // 2. Slicing (k_oz_slice): Y = sum_{p<S} y_p 256^(S-1-p) with balanced digits
 //    y_p in [-128, 127] (y_0 in [-64, 64]).  The digits are stored as int8 planes
-//    s[chunk][p][row][16]: 16 consecutive k of one row for one slice.
-// 3. Products (k_gemm_oz): for the output digit position d = p + q the slice products
-//    T_d = sum_{p+q=d} sum_k a_p[i,k] b_q[k,j] are exact INT32 sums; every
-//    v_mfma_i32_16x16x64_i8 issues four (p, q) pairs at once (lane group g = lane / 16 carries
-//    pair 4u + g, each over the same 16 k of a K chunk).  Pairs with
+//    s[chunk][p][row][KC]: KC consecutive k of one row for one slice.
+// 3. Products: for the output digit position d = p + q the slice products
+//    T_d = sum_{p+q=d} sum_k a_p[i,k] b_q[k,j] are exact INT32 sums.  Pairs with
 //    p + q >= S lie below 2^-(8S-8) of the row/column scale and are not formed
-//    (S(S+1)/2 slice products instead of S^2).
+//    (S(S+1)/2 slice products instead of S^2).  Two kernels:
+//    - k_gemm_oz (K <= 64, the Cholesky panels and trailing updates): KC = 16; every
+//      v_mfma_i32_16x16x64_i8 issues four (p, q) pairs of one d at once (lane group g = lane / 16
+//      carries pair 4u + g, each over the same 16 k), and each chunk's T_d is carried at once;
+//    - k_gemm_ozk (long K: the F1 products and Schur inner products): KC = 64; one MFMA per
+//      (p, q) over 64 k into a per-digit INT32 accumulator that lives across chunks and is
+//      carried every 32 chunks.
 // 4. Recombination: sum_d T_d 256^-d is carried digit by digit from d = S-1 up to 0
 //    into an exact fixed-point accumulator (S/4 fraction words + a 64-bit integer part)
 //    held in registers across K chunks, then rounded once to a limb float with
@@ -19,6 +23,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "engine_common.h"
 #include "mpx.h"
@@ -30,22 +36,24 @@ typedef int oz_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int OZ_T = 32;   // output tile edge of one workgroup (2 x 2 waves of 16 x 16)
 constexpr int OZ_KC = 16;  // k per chunk (one 16-byte operand per lane and slice)
+constexpr int OZ_LONGK_MIN = 64;  // K above which the long-K kernel (k_gemm_ozk) is used
 
-// slice planes: ((((b * nch + ch) * S + p) * Rp + r) * 16 + kk), Rp = R rounded up to OZ_T
-template <int L, int S>
+// slice planes: ((((b * nch + ch) * S + p) * Rp + r) * KC + kk), Rp = R rounded up to OZ_T; KC = k per
+// chunk (16 for the short-K kernel, 64 for the long-K kernel)
+template <int L, int S, int KC>
 __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const int32_t *eo, int8_t *out) {
     static_assert(S % 4 == 0 && S <= 4 * L, "slice count");
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = blockIdx.y;
-    const int Kp = nch * OZ_KC;
+    const int Kp = nch * KC;
     if (t >= (int64_t)Rp * Kp) return;
-    int kk = (int)(t % OZ_KC);
-    int64_t rest = t / OZ_KC;
+    int kk = (int)(t % KC);
+    int64_t rest = t / KC;
     int r = (int)(rest % Rp);
     int ch = (int)(rest / Rp);
-    int k = ch * OZ_KC + kk;
-    int8_t *o = out + ((((int64_t)b * nch + ch) * S) * Rp + r) * OZ_KC + kk;
-    const int64_t pstride = (int64_t)Rp * OZ_KC;
+    int k = ch * KC + kk;
+    int8_t *o = out + ((((int64_t)b * nch + ch) * S) * Rp + r) * KC + kk;
+    const int64_t pstride = (int64_t)Rp * KC;
     if (r >= R || k >= g.K) {
 #pragma unroll
         for (int p = 0; p < S; p++) o[p * pstride] = 0;
@@ -105,6 +113,52 @@ struct OzDigit<S, -1> {
                                                int64_t (&)[4]) {}
 };
 
+// Epilogue of one output: the exact fixed-point sum (RI * 2^(32 NW) + sum_w R[w] 2^(32 (NW-1-w)),
+// two's complement over NW + 2 words) is rounded once to a limb float with exponent
+// e_i + e_j - 12 - 8 (S-1) and applied to C[i, j] (set / subtract / add) in full width.
+template <int L, int S>
+__device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[S / 4], int64_t RI, int i, int j, int b,
+                                         int bA, int bB, const int32_t *__restrict__ ea,
+                                         const int32_t *__restrict__ eb) {
+    constexpr int NW = S / 4;
+    if (i >= g.M || j >= g.N) return;
+    if (g.lower && i < j) return;
+    constexpr int NX = NW + 2;
+    uint32_t x[NX];
+#pragma unroll
+    for (int w = 0; w < NW; w++) x[w] = R[NW - 1 - w];
+    x[NW] = (uint32_t)RI;
+    x[NW + 1] = (uint32_t)((uint64_t)RI >> 32);
+    uint32_t neg = x[NX - 1] >> 31;
+    if (neg) {
+        uint32_t br = 1;
+#pragma unroll
+        for (int q = 0; q < NX; q++) {
+            uint32_t v = ~x[q] + br;
+            br = (br && v == 0) ? 1u : 0u;
+            x[q] = v;
+        }
+    }
+    mpx::mp<L> res = mpx::zero<L>();
+    uint32_t lz = mpx::lead_zeros<NX>(x);
+    if (lz < 32u * NX) {
+        mpx::shl<NX>(x, lz);
+        uint32_t y[L + 1];
+#pragma unroll
+        for (int q = 0; q <= L; q++) y[q] = x[NX - 1 - L + q];
+        int32_t E = ea[(int64_t)bA * g.M + i] + eb[(int64_t)bB * g.N + j];
+        E = (E < -(1 << 28) ? 0 : E) - 12 - 8 * (S - 1);
+        int32_t ex = 32 * NX - (int32_t)lz + E;
+        mpx::round_store<L>(res, y, ex, neg);
+    }
+    int64_t cidx = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
+    if (g.op == 1)
+        res = mpx::sub<L>(ldm<L>(g.C, cidx), res);
+    else if (g.op == 2)
+        res = mpx::add<L>(ldm<L>(g.C, cidx), res);
+    stm<L>(g.C, cidx, res);
+}
+
 template <int L, int S>
 __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__restrict__ as,
                                                  const int8_t *__restrict__ bs, const int32_t *__restrict__ ea,
@@ -146,48 +200,146 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
         for (int e = 0; e < 4; e++) RI[e] += c[e];
         __syncthreads();
     }
-    int32_t Ebase = -12;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        oz_store<L, S>(g, R[e], RI[e], i0 + wr * 16 + 4 * grp + e, j0 + wc * 16 + (lane & 15), b, bA, bB, ea, eb);
+}
+
+// ---- long-K form ----
+// For K beyond one short chunk the per-chunk digit carries of k_gemm_oz (a VALU pass per 16 k)
+// dominate.  k_gemm_ozk keeps one INT32 accumulator per output digit position d across chunks of 64 k
+// (each v_mfma_i32_16x16x64_i8 forms one slice pair (p, q) over 64 k and adds into digit p + q), and
+// carries the digit sums into the fixed-point accumulator only every OZK_CARRY chunks: a digit sum
+// over one chunk is bounded by S * 64 * 2^14 < 2^25.2, so 32 chunks stay below 2^31.
+// One wave per 16 x 16 output tile, one wave per SIMD (the S digit accumulators and the S column
+// fragments of the chunk fill the register file), so the wave hides its own load latency:
+//  - the S column fragments of a chunk are requested together at its start, and batches run by
+//    descending row slice p (batch p forms the pairs (p, q <= S-1-p)), so the first, short batches
+//    need only the first column fragments while the others are in flight;
+//  - the row fragments stream through a ring of OZK_PD registers, each reloaded with the fragment
+//    OZK_PD batches ahead as soon as it is read.
+// The fixed-point accumulators (touched only at carries and in the epilogue) live in LDS.
+// Tile mapping: the tile grid of every batch entry is cut into units of BR x BC tiles (16 x 8 when the
+// grid is large); unit u runs on XCD u mod 8 (workgroups are dispatched to XCDs by id mod 8), whose
+// waves of one unit run together and share its BR row and BC column fragments in that XCD's L2
+// (without this, each wave streams its row fragments from HBM: about 8 TB/s at the MFMA rate).
+constexpr int OZK_KC = 64;
+constexpr int OZK_CARRY = 32;
+constexpr int OZK_SMAX = 40;  // register budget: S accumulators + S column fragments per lane
+constexpr int OZK_PD = 8;     // row-fragment prefetch distance (batches)
+
+// workgroups of k_gemm_ozk for a tm x tn tile grid per batch entry: units of BR x BC tiles, every XCD
+// (id mod 8) running whole units
+inline int64_t ozk_workgroups(int tm, int tn, int batch) {
+    const int BR = tm < 16 ? tm : 16, BC = (128 / BR) < tn ? 128 / BR : tn;
+    const int64_t units = (int64_t)((tm + BR - 1) / BR) * ((tn + BC - 1) / BC) * batch;
+    return 8 * (int64_t)BR * BC * ((units + 7) / 8);
+}
+
+template <int S>
+__device__ __forceinline__ void ozk_carry(oz_v4i (&acc)[S], uint32_t (*sR)[64], int lane) {
+    constexpr int NW = S / 4;
+    int64_t c[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int w = NW - 1; w >= 0; w--) {
+        uint32_t Rw[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) Rw[e] = sR[e * NW + w][lane];
+#pragma unroll
+        for (int D = 4 * w + 3; D >= 4 * w; D--) {
+            const int sh = 8 * (3 - D % 4);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int64_t v = (int64_t)acc[D][e] + c[e] + (int64_t)((Rw[e] >> sh) & 255u);
+                Rw[e] = (Rw[e] & ~(255u << sh)) | (((uint32_t)v & 255u) << sh);
+                c[e] = v >> 8;
+            }
+            acc[D] = oz_v4i{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) sR[e * NW + w][lane] = Rw[e];
+    }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        int i = i0 + wr * 16 + 4 * grp + e;
-        int j = j0 + wc * 16 + (lane & 15);
-        if (i >= M || j >= N) continue;
-        if (g.lower && i < j) continue;
-        // Z = RI * 2^(32 NW) + sum_w R[w] 2^(32 (NW-1-w)), two's complement over NW + 2 words
-        constexpr int NX = NW + 2;
-        uint32_t x[NX];
+        uint64_t ri = ((uint64_t)sR[4 * NW + 2 * e + 1][lane] << 32) | sR[4 * NW + 2 * e][lane];
+        ri = (uint64_t)((int64_t)ri + c[e]);
+        sR[4 * NW + 2 * e][lane] = (uint32_t)ri;
+        sR[4 * NW + 2 * e + 1][lane] = (uint32_t)(ri >> 32);
+    }
+}
+
+template <int L, int S>
+__global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__restrict__ as,
+                                                const int8_t *__restrict__ bs, const int32_t *__restrict__ ea,
+                                                const int32_t *__restrict__ eb, int nbA, int nbB, int Mp, int Np,
+                                                int nch) {
+    static_assert(S <= OZK_SMAX && S > OZK_PD, "long-K form register budget");
+    constexpr int NW = S / 4;
+    __shared__ uint32_t sR[4 * NW + 8][64];
+    // tile of this wave: see ozk_units (workgroup id -> XCD id % 8, its k-th unit, slot in the unit)
+    const int tm = Mp / 16, tn = Np / 16;
+    const int BR = tm < 16 ? tm : 16, BC = (128 / BR) < tn ? 128 / BR : tn, SU = BR * BC;
+    const int nsr = (tm + BR - 1) / BR, nsc = (tn + BC - 1) / BC;
+    const int id = blockIdx.x, local = id >> 3;
+    const int unit = 8 * (local / SU) + (id & 7), w = local % SU;
+    const int b = unit / (nsr * nsc), su = unit % (nsr * nsc);
+    if (b >= g.batch) return;
+    const int ti = (su % nsr) * BR + w % BR, tj = (su / nsr) * BC + w / BR;
+    if (ti >= tm || tj >= tn) return;
+    const int i0 = ti * 16, j0 = tj * 16;
+    if (g.lower && i0 + 15 < j0) return;
+    const int lane = threadIdx.x, grp = lane >> 4;
+    const int bA = nbA > 1 ? b : 0, bB = nbB > 1 ? b : 0;
 #pragma unroll
-        for (int w = 0; w < NW; w++) x[w] = R[e][NW - 1 - w];
-        x[NW] = (uint32_t)RI[e];
-        x[NW + 1] = (uint32_t)((uint64_t)RI[e] >> 32);
-        uint32_t neg = x[NX - 1] >> 31;
-        if (neg) {
-            uint32_t br = 1;
+    for (int w = 0; w < 4 * NW + 8; w++) sR[w][lane] = 0;
+    oz_v4i acc[S];
 #pragma unroll
-            for (int q = 0; q < NX; q++) {
-                uint32_t v = ~x[q] + br;
-                br = (br && v == 0) ? 1u : 0u;
-                x[q] = v;
-            }
+    for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
+    // a (chunk, slice) plane holds Rp rows of 64 bytes (4 fragments of 16); lane l reads row l % 16,
+    // bytes 16 (l / 16) .. +16 of the chunk.  Buffer loads: the batch base in the resource (SGPRs),
+    // the lane's byte offset in one VGPR and the (chunk, slice) plane offset in an SGPR, so a load in
+    // flight holds no address registers (the driver keeps one batch's planes below 4 GiB).
+    const int64_t pbA = (int64_t)Mp * OZK_KC, pbB = (int64_t)Np * OZK_KC;  // plane bytes
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(as + (int64_t)bA * nch * S * pbA), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(bs + (int64_t)bB * nch * S * pbB), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+    const int voA = ((i0 + (lane & 15)) * 4 + grp) * 16, voB = ((j0 + (lane & 15)) * 4 + grp) * 16;
+    const uint32_t psA = (uint32_t)pbA, psB = (uint32_t)pbB;
+    auto ldA = [&](int plane) -> oz_v4i {
+        return __builtin_amdgcn_raw_buffer_load_b128(rA, voA, (int)((uint32_t)plane * psA), 0);
+    };
+    auto ldB = [&](int plane) -> oz_v4i {
+        return __builtin_amdgcn_raw_buffer_load_b128(rB, voB, (int)((uint32_t)plane * psB), 0);
+    };
+    for (int ch = 0; ch < nch; ch++) {
+        const int c0 = ch * S;
+        // column fragments of the chunk, then a ring of row fragments: batch p (descending) forms the
+        // pairs (p, q <= S-1-p), so the first batches need only the first column fragments while the
+        // rest are still in flight, and every row fragment is requested OZK_PD batches ahead
+        oz_v4i bq[S], ap[OZK_PD];
+#pragma unroll
+        for (int q = 0; q < S; q++) bq[q] = ldB(c0 + q);
+#pragma unroll
+        for (int t = 0; t < OZK_PD; t++) ap[t] = ldA(c0 + S - 1 - t);
+#pragma unroll
+        for (int p = S - 1; p >= 0; p--) {
+            const int t = (S - 1 - p) % OZK_PD;
+            const oz_v4i a = ap[t];
+            if (p - OZK_PD >= 0) ap[t] = ldA(c0 + p - OZK_PD);
+#pragma unroll
+            for (int q = 0; q <= S - 1 - p; q++)
+                acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bq[q], acc[p + q], 0, 0, 0);
         }
-        mpx::mp<L> res = mpx::zero<L>();
-        uint32_t lz = mpx::lead_zeros<NX>(x);
-        if (lz < 32u * NX) {
-            mpx::shl<NX>(x, lz);
-            uint32_t y[L + 1];
+        if ((ch % OZK_CARRY) == OZK_CARRY - 1 || ch == nch - 1) ozk_carry<S>(acc, sR, lane);
+    }
 #pragma unroll
-            for (int q = 0; q <= L; q++) y[q] = x[NX - 1 - L + q];
-            int32_t E = ea[(int64_t)bA * M + i] + eb[(int64_t)bB * N + j];
-            E = (E < -(1 << 28) ? 0 : E) + Ebase - 8 * (S - 1);
-            int32_t ex = 32 * NX - (int32_t)lz + E;
-            mpx::round_store<L>(res, y, ex, neg);
-        }
-        int64_t cidx = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
-        if (g.op == 1)
-            res = mpx::sub<L>(ldm<L>(g.C, cidx), res);
-        else if (g.op == 2)
-            res = mpx::add<L>(ldm<L>(g.C, cidx), res);
-        stm<L>(g.C, cidx, res);
+    for (int e = 0; e < 4; e++) {
+        uint32_t R[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) R[w] = sR[e * NW + w][lane];
+        const int64_t RI = (int64_t)(((uint64_t)sR[4 * NW + 2 * e + 1][lane] << 32) | sR[4 * NW + 2 * e][lane]);
+        oz_store<L, S>(g, R, RI, i0 + 4 * grp + e, j0 + (lane & 15), b, bA, bB, ea, eb);
     }
 }
 
@@ -227,45 +379,100 @@ struct OzakiWork {
         return 0;
     }
     static int slices_ok(int s) { return L <= OZ_LMAX && (s == OzSlices<L>::DEF || s == OzSlices<L>::WIDE); }
+    OpCache<int8_t> kept;  // slices and exponents of operands that never change
     template <int S>
-    int run_s(const GemmArgs &g, hipStream_t st) {
-        int nbA = g.sA ? g.batch : 1, nbB = g.sB ? g.batch : 1;
-        int nch = (g.K + OZ_KC - 1) / OZ_KC;
+    int run_s(const GemmArgs &g, hipStream_t st, int keep) {
+        // long K: digit sums kept across 64-k chunks (k_gemm_ozk); short K: 16-k chunks, four slice pairs
+        // per MFMA and a carry per chunk (k_gemm_oz)
         int Mp = (g.M + OZ_T - 1) / OZ_T * OZ_T, Np = (g.N + OZ_T - 1) / OZ_T * OZ_T;
-        size_t need_a = (size_t)nbA * nch * S * Mp * OZ_KC, need_b = (size_t)nbB * nch * S * Np * OZ_KC;
-        if (need_a > cap_as || need_b > cap_bs || (size_t)nbA * g.M > cap_ea || (size_t)nbB * g.N > cap_eb)
-            if (hipStreamSynchronize(st) != hipSuccess) return -2;  // queued kernels may still read the old buffers
-        int rc = 0;
-        rc |= grow(&as, cap_as, need_a);
-        rc |= grow(&bs, cap_bs, need_b);
-        rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
-        rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
-        if (rc) return -4;
+        // the long-K kernel addresses one batch's slice planes with 32-bit buffer offsets
+        const int64_t nch64 = (g.K + OZK_KC - 1) / OZK_KC;
+        const bool longk = S <= OZK_SMAX && g.K > OZ_LONGK_MIN &&
+                           nch64 * S * (int64_t)std::max(Mp, Np) * OZK_KC < ((int64_t)1 << 32);
+        const int KC = longk ? OZK_KC : OZ_KC;
+        int nbA = g.sA ? g.batch : 1, nbB = g.sB ? g.batch : 1;
+        int nch = (g.K + KC - 1) / KC;
+        size_t need_a = (size_t)nbA * nch * S * Mp * KC, need_b = (size_t)nbB * nch * S * Np * KC;
         const bool same = gemm_b_is_at(g);
-        k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
-        if (!same) k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
-        int64_t sa_n = (int64_t)Mp * nch * OZ_KC, sb_n = (int64_t)Np * nch * OZ_KC;
-        k_oz_slice<L, S><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, ea, as);
-        if (!same)
-            k_oz_slice<L, S><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, eb, bs);
-        dim3 grd(Mp / OZ_T, Np / OZ_T, g.batch);
+        const bool keepA = (keep & GEMM_KEEP_A) != 0, keepB = (keep & GEMM_KEEP_B) != 0 && !same;
+        if ((!keepA && (need_a > cap_as || (size_t)nbA * g.M > cap_ea)) ||
+            (!keepB && (need_b > cap_bs || (size_t)nbB * g.N > cap_eb)))
+            if (hipStreamSynchronize(st) != hipSuccess) return -2;  // queued kernels may still read the old buffers
+        // row exponents and slices of each operand: kept copies (filled once) or the scratch buffers
+        int8_t *pa = as, *pb = bs;
+        int32_t *qa = ea, *qb = eb;
+        bool fillA = true, fillB = !same;
+        const int form = 1000 * S + KC;
+        if (keepA) {
+            auto *e = kept.get(op_key(g, 0, form), need_a, (size_t)nbA * g.M, fillA);
+            if (!e) return -4;
+            pa = e->data;
+            qa = e->exps;
+        } else {
+            int rc = grow(&as, cap_as, need_a) | grow(&ea, cap_ea, (size_t)nbA * g.M);
+            if (rc) return -4;
+            pa = as;
+            qa = ea;
+        }
+        if (keepB) {
+            auto *e = kept.get(op_key(g, 1, form), need_b, (size_t)nbB * g.N, fillB);
+            if (!e) return -4;
+            pb = e->data;
+            qb = e->exps;
+        } else if (!same) {
+            int rc = grow(&bs, cap_bs, need_b) | grow(&eb, cap_eb, (size_t)nbB * g.N);
+            if (rc) return -4;
+            pb = bs;
+            qb = eb;
+        }
+        int64_t sa_n = (int64_t)Mp * nch * KC, sb_n = (int64_t)Np * nch * KC;
+        if (fillA) {
+            k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, qa);
+            if (longk)
+                k_oz_slice<L, S, OZK_KC><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch,
+                                                                                                 qa, pa);
+            else
+                k_oz_slice<L, S, OZ_KC><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, qa,
+                                                                                                pa);
+        }
+        if (fillB) {
+            k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, qb);
+            if (longk)
+                k_oz_slice<L, S, OZK_KC><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch,
+                                                                                                 qb, pb);
+            else
+                k_oz_slice<L, S, OZ_KC><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, qb,
+                                                                                                pb);
+        }
+        if (same) {
+            pb = pa;
+            qb = qa;
+        }
         hipEvent_t ev = prof ? prof->begin(st) : nullptr;
-        k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, as, same ? as : bs, ea, same ? ea : eb, nbA, same ? nbA : nbB, Mp,
-                                            same ? Mp : Np, nch);
+        if (longk) {
+            const int64_t nwg = ozk_workgroups(Mp / 16, Np / 16, g.batch);
+            if constexpr (S <= OZK_SMAX)
+                k_gemm_ozk<L, S><<<dim3((unsigned)nwg, 1), 64, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp,
+                                                                        same ? Mp : Np, nch);
+        } else {
+            dim3 grd(Mp / OZ_T, Np / OZ_T, g.batch);
+            k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp, same ? Mp : Np, nch);
+        }
         // algorithmic INT8 operations: 2 per multiply-add, S(S+1)/2 slice pairs per mp multiply-add
         if (prof) prof->end(st, ev, 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
-    int run(const GemmArgs &g, hipStream_t st, int slices) {
+    int run(const GemmArgs &g, hipStream_t st, int slices, int keep = 0) {
         if constexpr (L > OZ_LMAX) {
             (void)g;
             (void)st;
             (void)slices;
+            (void)keep;
             return -5;
         } else {
             if (slices <= 0) slices = OzSlices<L>::DEF;
-            if (slices == OzSlices<L>::WIDE) return run_s<OzSlices<L>::WIDE>(g, st);
-            if (slices == OzSlices<L>::DEF) return run_s<OzSlices<L>::DEF>(g, st);
+            if (slices == OzSlices<L>::WIDE) return run_s<OzSlices<L>::WIDE>(g, st, keep);
+            if (slices == OzSlices<L>::DEF) return run_s<OzSlices<L>::DEF>(g, st, keep);
             return -1;
         }
     }

@@ -368,28 +368,56 @@ struct PlainWork {
         cap = n;
         return 0;
     }
-    int run(const GemmArgs &g, hipStream_t st) {
+    OpCache<uint32_t> kept;  // fixed-point forms and exponents of operands that never change
+    int run(const GemmArgs &g, hipStream_t st, int keep = 0) {
         int nbA = g.sA ? g.batch : 1, nbB = g.sB ? g.batch : 1;
         size_t KK = (size_t)(g.K > 0 ? g.K : 1);
-        int rc = 0;
-        if ((size_t)nbA * L * KK * g.M > cap_af || (size_t)nbB * L * KK * g.N > cap_bf ||
-            (size_t)nbA * g.M > cap_ea || (size_t)nbB * g.N > cap_eb)
+        const bool same = gemm_b_is_at(g);
+        const bool keepA = (keep & GEMM_KEEP_A) != 0, keepB = (keep & GEMM_KEEP_B) != 0 && !same;
+        const size_t need_a = (size_t)nbA * L * KK * g.M, need_b = (size_t)nbB * L * KK * g.N;
+        if ((!keepA && (need_a > cap_af || (size_t)nbA * g.M > cap_ea)) ||
+            (!keepB && (need_b > cap_bf || (size_t)nbB * g.N > cap_eb)))
             if (hipStreamSynchronize(st) != hipSuccess) return -2;  // queued kernels may still read the old buffers
-        rc |= grow(&af, cap_af, (size_t)nbA * L * KK * g.M);
-        rc |= grow(&bf, cap_bf, (size_t)nbB * L * KK * g.N);
-        rc |= grow(&ea, cap_ea, (size_t)nbA * g.M);
-        rc |= grow(&eb, cap_eb, (size_t)nbB * g.N);
-        if (rc) return -4;
+        uint32_t *fa = af, *fb = bf;
+        int32_t *qa = ea, *qb = eb;
+        bool fillA = true, fillB = !same;
+        if (keepA) {
+            auto *e = kept.get(op_key(g, 0, 0), need_a, (size_t)nbA * g.M, fillA);
+            if (!e) return -4;
+            fa = e->data;
+            qa = e->exps;
+        } else {
+            if (grow(&af, cap_af, need_a) | grow(&ea, cap_ea, (size_t)nbA * g.M)) return -4;
+            fa = af;
+            qa = ea;
+        }
+        if (keepB) {
+            auto *e = kept.get(op_key(g, 1, 0), need_b, (size_t)nbB * g.N, fillB);
+            if (!e) return -4;
+            fb = e->data;
+            qb = e->exps;
+        } else if (!same) {
+            if (grow(&bf, cap_bf, need_b) | grow(&eb, cap_eb, (size_t)nbB * g.N)) return -4;
+            fb = bf;
+            qb = eb;
+        }
         int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
         dim3 ga((unsigned)((ta + 255) / 256), nbA), gb((unsigned)((tb + 255) / 256), nbB);
-        const bool same = gemm_b_is_at(g);
-        k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, ea);
-        if (!same) k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, eb);
-        k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, ea, af);
-        if (!same) k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, eb, bf);
+        if (fillA) {
+            k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, qa);
+            k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, qa, fa);
+        }
+        if (fillB) {
+            k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, qb);
+            k_fix_conv<L><<<gb, 256, 0, st>>>(g, 1, g.N, qb, fb);
+        }
+        if (same) {
+            fb = fa;
+            qb = qa;
+        }
         dim3 grd((g.M + PT - 1) / PT, (g.N + PT - 1) / PT, g.batch);
         hipEvent_t ev = prof ? prof->begin(st) : nullptr;
-        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, af, same ? af : bf, ea, same ? ea : eb, nbA, same ? nbA : nbB);
+        k_gemm_plain<L><<<grd, 256, 0, st>>>(g, fa, fb, qa, qb, nbA, same ? nbA : nbB);
         if (prof) prof->end(st, ev, 1, gemm_outputs(g) * (double)g.K);
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }

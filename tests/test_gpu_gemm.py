@@ -99,8 +99,9 @@ FORM_IDS = ["plain", "element", "ozaki", "ozaki-wide"]
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("shape", [(70, 67, 37, False, True, 0), (33, 129, 16, True, False, 1),
-                                   (1, 5, 1, False, False, 2), (96, 96, 32, False, True, 1)],
-                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub"])
+                                   (1, 5, 1, False, False, 2), (96, 96, 32, False, True, 1),
+                                   (50, 40, 300, False, True, 1), (17, 20, 2100, True, False, 2)],
+                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub", "NT-sub-K300", "TN-add-K2100"])
 def test_rgemm_random(form, shape):
     mode, slices, bits_lost = form
     m, n, k, ta, tb, op = shape
@@ -117,12 +118,14 @@ def test_rgemm_random(form, shape):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
-def test_rgemm_lower_update(form):
-    """the Cholesky trailing update shape: C -= P P^T on the lower triangle only"""
+@pytest.mark.parametrize("k", [32, 200], ids=["K32", "K200"])
+def test_rgemm_lower_update(form, k):
+    """the Cholesky trailing update shape (K = 32) and the Schur inner-product shape (long K):
+    C -= P P^T on the lower triangle only"""
     mode, slices, bits_lost = form
-    rnd = random.Random(7)
-    m, k = 75, 32
-    print(f"case rgemm-lower mode={mode} slices={slices}", flush=True)
+    rnd = random.Random(7 + k)
+    m = 75
+    print(f"case rgemm-lower mode={mode} slices={slices} k={k}", flush=True)
     Pv = _rand_matrix(rnd, m, k, espread=4)
     Cv = _rand_matrix(rnd, m, m, espread=4)
     A, B, ex = _exact(Pv, Pv, Cv, False, True, 1)
@@ -133,11 +136,13 @@ def test_rgemm_lower_update(form):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
-def test_rgemm_small_integers_exact(form):
+@pytest.mark.parametrize("k", [40, 300], ids=["K40", "K300"])
+def test_rgemm_small_integers_exact(form, k):
+    """K = 40 runs the short-K Ozaki kernel, K = 300 the long-K one (digit sums kept across chunks)"""
     mode, slices, _ = form
-    rnd = random.Random(11)
-    m, n, k = 65, 70, 40
-    print(f"case rgemm-int mode={mode} slices={slices}", flush=True)
+    rnd = random.Random(11 + k)
+    m, n = 65, 70
+    print(f"case rgemm-int mode={mode} slices={slices} k={k}", flush=True)
     Av = [[Fraction(rnd.randint(-1000, 1000)) for _ in range(k)] for _ in range(m)]
     Bv = [[Fraction(rnd.randint(-1000, 1000)) for _ in range(n)] for _ in range(k)]
     _, _, ex = _exact(Av, Bv, None, False, False, 0)
