@@ -14,7 +14,12 @@ the Ozaki form.
 
 roofline: the Ozaki MFMA product kernel, INT8 operations = 2 * S(S+1)/2 per mp multiply-add of the
 reference loop (S slices), over its event-timed average launch duration in the timed steps, against
-the 5.0 POP/s dense INT8 peak.  The per-kernel breakdown of the step is reported beside it.
+the 5.0 POP/s dense INT8 peak.  The per-kernel breakdown of the step is reported beside it, with the
+dominant kernel (the Cholesky diagonal-block pivot chain, a latency bound).
+
+also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank-2 constraints, all F1),
+replayed on its first iterate for both forms (2 timed steps), with the roofline of the long-K Ozaki
+kernel that carries its F1 products and Schur inner products.
 
 cpu_baseline: the oracle (C restatement of the reference's mpf path, serial Cholesky as the
 reference's Rpotrf) on the same iterate, dumped by the GPU run: Make_bMat in full and the Cholesky of
@@ -37,6 +42,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 PARAM = os.path.join(PKG, "data", "param_256_gap1e-60.sdpa")
 INT8_PEAK = 5.0e15  # MI355X dense INT8 MFMA, ops/s
 HBM_PEAK = 8.0e12
+WORKLOADS = {
+    2: "max-cut n=m=1000, one block, A_i=e_i e_i^T, C=-L/4 of G(1000,0.01) with weights +-1",
+    3: "one block n=150, m=1326, A_i = u v^T + v u^T (dense), feasible by construction",
+    0: "one dense block n=15, m=21",
+    1: "blocks (50,25), m=351, about 8% nonzeros per A_i",
+}
 
 
 def structure(path: str) -> dict:
@@ -49,38 +60,22 @@ def structure(path: str) -> dict:
     return {"m": m, "blocks": blocks, "chol_mp_mac": m ** 3 / 6.0}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--at", type=int, default=20, help="iteration whose iterate the timed steps replay")
-    ap.add_argument("--workload", type=int, default=2)
-    ap.add_argument("--forms", default="ozaki,plain")
-    ap.add_argument("--no-solve", action="store_true", help="skip the full trajectories")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=500, help="Cholesky sample size of the CPU baseline")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding of the "
-                                   "Schur rows and a 2D block-cyclic Cholesky are design notes only)"}))
-        sys.exit(2)
-
+def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve: bool, dump: str | None):
+    """Replay [Make_bMat + Cholesky] steps of one workload for each GEMM form (after an optional full
+    solve); returns (structure, sha256, per-form results)."""
     import gen_problems
     import sdpg
 
     tmp = tempfile.mkdtemp(prefix="sdpg_bench_")
-    data = os.path.join(tmp, f"w{a.workload}.dat-s")
-    sha = gen_problems.generate(a.workload, data)
+    data = os.path.join(tmp, f"w{workload}.dat-s")
+    sha = gen_problems.generate(workload, data)
     info = structure(data)
-    L = sdpg.limbs_for_bits(256)
-    forms = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}
+    modes = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}
     res = {}
-    dump = os.path.join(tmp, "it_")
-    for idx, name in enumerate(a.forms.split(",")):
-        mode = forms[name]
+    for idx, name in enumerate(forms):
+        mode = modes[name]
         r_out = {}
-        if not a.no_solve:
+        if solve:
             t0 = time.time()
             r = sdpg.solve(data, PARAM, 0, None, mode)
             t_solve = time.time() - t0
@@ -91,21 +86,63 @@ def main():
                                           "solve": r.t_solve_ms / it, "gvec": r.t_gvec_ms / it,
                                           "dxdz": r.t_dxdz_ms / it, "inverse": r.t_inv_ms / it,
                                           "min_eigen": r.t_eig_ms / it}}
-            print(f"[bench] {name}: solve {r.iterations} iterations, {r_out['phase']}, relgap {r_out['relgap']}, "
-                  f"{t_solve:.1f} s; schur+chol {(r.t_schur_ms + r.t_chol_ms) / it:.2f} ms/iter over the trajectory",
-                  flush=True)
-        ms_s, ms_c, ms_w, kst = sdpg.bench(data, PARAM, 0, mode, 0, a.at, a.warmup, a.steps,
-                                           dump_prefix=dump if idx == 0 else None)
-        print(f"[bench] {name}: {a.steps} steps on iterate {a.at}: schur {ms_s:.3f} ms, potrf {ms_c:.3f} ms, "
+            print(f"[bench] w{workload} {name}: solve {r.iterations} iterations, {r_out['phase']}, relgap "
+                  f"{r_out['relgap']}, {t_solve:.1f} s; schur+chol {(r.t_schur_ms + r.t_chol_ms) / it:.2f} ms/iter "
+                  f"over the trajectory", flush=True)
+        ms_s, ms_c, ms_w, kst = sdpg.bench(data, PARAM, 0, mode, 0, at, warmup, steps,
+                                           dump_prefix=dump if (idx == 0 and dump) else None)
+        print(f"[bench] w{workload} {name}: {steps} steps on iterate {at}: schur {ms_s:.3f} ms, potrf {ms_c:.3f} ms, "
               f"wall {ms_w:.3f} ms per step; kernels {kst}", flush=True)
         r_out.update({"step_schur_ms": ms_s, "step_chol_ms": ms_c, "step_wall_ms": ms_w,
-                      "kernels": {k: {"launches": v[0], "ms_per_step": v[1] / a.steps,
+                      "kernels": {k: {"launches": v[0], "ms_per_step": v[1] / steps,
                                       "avg_launch_us": 1e3 * v[1] / max(v[0], 1)} for k, v in kst.items()},
                       "_kst": kst})
         res[name] = r_out
+    return info, sha, res
+
+
+def roofline(kst_entry, kernel: str, traffic=None, note: str = ""):
+    """MFMA roofline of one product kernel from its live event timing (launches, ms, INT8 ops)."""
+    n, ms, ops = kst_entry
+    if not n:
+        return None
+    avg_s = ms / 1e3 / n
+    ach = ops / n / avg_s
+    return {"kernel": kernel, "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": ach / INT8_PEAK, "traffic": traffic, "ops_per_launch": ops / n, "avg_launch_us": avg_s * 1e6,
+            "note": note}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--at", type=int, default=20, help="iteration whose iterate the timed steps replay")
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--forms", default="ozaki,plain")
+    ap.add_argument("--no-solve", action="store_true", help="skip the full trajectories")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the second measured workload (n=150, m=1326 dense rank-2 constraints)")
+    ap.add_argument("--cpu-n", type=int, default=500, help="Cholesky sample size of the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding of the "
+                                   "Schur rows and a 2D block-cyclic Cholesky are design notes only)"}))
+        sys.exit(2)
+
+    import sdpg
+
+    L = sdpg.limbs_for_bits(256)
+    forms = a.forms.split(",")
+    tmp = tempfile.mkdtemp(prefix="sdpg_dump_")
+    dump = os.path.join(tmp, "it_")
+    info, sha, res = measure(a.workload, forms, a.at, a.warmup, a.steps, not a.no_solve, dump)
 
     main_form = "ozaki" if "ozaki" in res else list(res)[0]
     step_ms = res[main_form]["step_wall_ms"]
+    wl = WORKLOADS.get(a.workload, f"workload {a.workload}")
     out = {
         "metric": "Schur build+Cholesky s/iter @256-bit, gap<=1e-60: Ozaki-MFMA vs plain limb",
         "value": step_ms / 1e3,
@@ -113,29 +150,33 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": step_ms,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": f"mp{32 * L}",
-        "data": "synthetic (seeded max-cut instance from gen_problems.py; no external data)",
-        "config": {"workload": f"max-cut n=m={info['m']}, one block, A_i=e_i e_i^T, 256-bit, "
-                               f"epsilonStar=epsilonDash=1e-60", "form": main_form, "replay_iteration": a.at,
-                   "input_sha256": sha},
+        "data": "synthetic (seeded instance from gen_problems.py; no external data)",
+        "config": {"workload": f"{wl}, 256-bit, epsilonStar=epsilonDash=1e-60", "form": main_form,
+                   "replay_iteration": a.at, "input_sha256": sha},
     }
     kst = res[main_form]["_kst"]
-    oz = kst["ozaki_mfma"] if main_form == "ozaki" else None
-    if oz and oz[0]:
-        avg_s = oz[1] / 1e3 / oz[0]
-        ach = oz[2] / oz[0] / avg_s
-        traffic = None
-        tfile = os.path.join(ROOT, "profiles", "k_gemm_oz_maxcut1000_traffic.json")
-        if os.path.exists(tfile):  # rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated
-            with open(tfile) as f:
-                traffic = json.load(f)["traffic_bytes_per_launch"]
-        out["roofline"] = {"kernel": "k_gemm_oz (Ozaki INT8 slice products, v_mfma_i32_16x16x64_i8)",
-                           "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TFLOP/s",
-                           "frac": ach / INT8_PEAK, "traffic": traffic,
-                           "ops_per_launch": oz[2] / oz[0], "avg_launch_us": avg_s * 1e6,
-                           "note": "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices; traffic from "
-                                   "profiles/k_gemm_oz_maxcut1000_traffic.txt"}
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_gemm_oz_maxcut1000_traffic.json")
+    if a.workload == 2 and os.path.exists(tfile):  # rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated
+        with open(tfile) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
+    if main_form == "ozaki":
+        rl = roofline(kst["ozaki_mfma"], "k_gemm_oz (Ozaki INT8 slice products, short K, v_mfma_i32_16x16x64_i8)",
+                      traffic, "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices; the Cholesky panel and "
+                               "trailing-update products (K = 32); traffic from "
+                               "profiles/k_gemm_oz_maxcut1000_traffic.txt")
+        if rl:
+            out["roofline"] = rl
     shares = {k: v[1] / a.steps / step_ms for k, v in kst.items()}
-    out["step_breakdown"] = {"share_of_step": shares, "dominant": max(shares, key=shares.get)}
+    dom = max(shares, key=shares.get)
+    out["step_breakdown"] = {"share_of_step": shares, "dominant": dom}
+    if dom == "chol_diag":
+        n, ms, work = kst["chol_diag"]
+        out["step_breakdown"]["dominant_kernel"] = {
+            "kernel": "k_potrf_diag (32 x 32 diagonal blocks of the Cholesky, one workgroup each)",
+            "bound": "latency: a serial chain of m pivots, one multiply-multiply-subtract in 320-bit limb "
+                     "arithmetic per pivot (one CU)",
+            "avg_launch_us": 1e3 * ms / max(n, 1), "us_per_pivot": 1e3 * ms / max(n, 1) / 32}
     for f in res.values():
         f.pop("_kst", None)
     out["forms"] = res
@@ -145,11 +186,34 @@ def main():
             from decimal import Decimal
             pa, po = Decimal(res["plain"]["objP"]), Decimal(res["ozaki"]["objP"])
             out["objective_rel_diff_ozaki_vs_plain"] = float(abs(pa - po) / max(abs(pa), Decimal(1)))
+
+    if not a.no_extra and a.workload != 3:
+        # the GEMM-heavy workload: F1 products (A_i Z^-1, X F_i: 2 m n^3) and the Schur inner products
+        # (m(m+1)/2 x n(n+1)/2) dominate, the regime where the INT8 form pays
+        info3, sha3, res3 = measure(3, forms, 1, 1, 2, False, None)
+        ex = {"workload": WORKLOADS[3] + ", 256-bit", "input_sha256": sha3, "replay_iteration": 1, "steps": 2}
+        for name, r in res3.items():
+            ex[name] = {"s_per_iter": r["step_wall_ms"] / 1e3, "schur_ms": r["step_schur_ms"],
+                        "chol_ms": r["step_chol_ms"], "kernels": r["kernels"]}
+        if "ozaki" in res3 and "plain" in res3:
+            ex["ozaki_speedup_vs_plain"] = res3["plain"]["step_wall_ms"] / max(res3["ozaki"]["step_wall_ms"], 1e-9)
+        if "ozaki" in res3:
+            rl3 = roofline(res3["ozaki"]["_kst"]["ozaki_mfma_longk"],
+                           "k_gemm_ozk (Ozaki INT8 slice products, long K, v_mfma_i32_16x16x64_i8)", None,
+                           "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36; F1 products (K = n) and Schur "
+                           "inner products (K = n(n+1)/2)")
+            if rl3:
+                ex["roofline"] = rl3
+        out["also_measured"] = ex
+
     if not a.no_cpu and os.path.exists(dump + "X.bin"):
+        import gen_problems
         import oracle_lib
 
         if not os.path.exists(oracle_lib.ORC_LIB):
             oracle_lib.build_oracle()
+        data = os.path.join(tmp, f"w{a.workload}.dat-s")
+        gen_problems.generate(a.workload, data)
         n = min(a.cpu_n, info["m"])
         tb, tc = oracle_lib.time_point(data, 256, dump + "X.bin", dump + "invZ.bin", L, n, 1)
         scale = (info["m"] / n) ** 3

@@ -42,7 +42,8 @@ __device__ __forceinline__ void stm(PV v, int64_t i, const mp<L> &x) {
 
 // Per-kernel device time over a measured region: events recorded on the launching stream around
 // each launch of the kernels of interest, summed after the region ends.
-// kinds: 0 Ozaki MFMA product kernel, 1 plain limb product kernel, 2 Cholesky diagonal block
+// kinds: 0 Ozaki MFMA product kernel (short K), 1 plain limb product kernel, 2 Cholesky diagonal
+// block, 3 Ozaki MFMA product kernel (long K)
 struct KProf {
     static constexpr int NK = 4;
     bool on = false;

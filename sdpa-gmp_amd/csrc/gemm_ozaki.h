@@ -459,7 +459,7 @@ struct OzakiWork {
             k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp, same ? Mp : Np, nch);
         }
         // algorithmic INT8 operations: 2 per multiply-add, S(S+1)/2 slice pairs per mp multiply-add
-        if (prof) prof->end(st, ev, 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
+        if (prof) prof->end(st, ev, longk ? 3 : 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
         return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     int run(const GemmArgs &g, hipStream_t st, int slices, int keep = 0) {
