@@ -213,12 +213,12 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 // over one chunk is bounded by S * 64 * 2^14 < 2^25.2, so 32 chunks stay below 2^31.
 // One wave per 16 x 16 output tile, one wave per SIMD (the S digit accumulators and the S column
 // fragments of the chunk fill the register file), so the wave hides its own load latency:
-//  - the S column fragments of a chunk are requested together at its start, and batches run by
-//    descending row slice p (batch p forms the pairs (p, q <= S-1-p)), so the first, short batches
-//    need only the first column fragments while the others are in flight;
-//  - the row fragments stream through a ring of OZK_PD registers, each reloaded with the fragment
-//    OZK_PD batches ahead as soon as it is read.
-// The fixed-point accumulators (touched only at carries and in the epilogue) live in LDS.
+//  - the next chunk's first 28 column fragments are copied global -> LDS by buffer loads (28 KB per
+//    wave) while the current chunk runs from registers; in the next chunk each moves LDS -> registers
+//    a few batches before its first use, so register pressure grows with the batches; the last
+//    column fragments are first used late in a chunk and load from global memory at its start;
+//  - batches run by descending row slice p (batch p forms the pairs (p, q <= S-1-p)); the row
+//    fragments stream through a ring of registers, each requested several batches ahead.
 // Tile mapping: the tile grid of every batch entry is cut into units of BR x BC tiles (16 x 8 when the
 // grid is large); unit u runs on XCD u mod 8 (workgroups are dispatched to XCDs by id mod 8), whose
 // waves of one unit run together and share its BR row and BC column fragments in that XCD's L2
@@ -226,7 +226,13 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 constexpr int OZK_KC = 64;
 constexpr int OZK_CARRY = 32;
 constexpr int OZK_SMAX = 40;  // register budget: S accumulators + S column fragments per lane
-constexpr int OZK_PD = 8;     // row-fragment prefetch distance (batches)
+
+// row-fragment prefetch distance (batches): smaller for the wide slice count, whose column fragments
+// take more registers
+template <int S>
+struct OzkPD {
+    static constexpr int value = S > 36 ? 4 : 8;
+};
 
 // workgroups of k_gemm_ozk for a tm x tn tile grid per batch entry: units of BR x BC tiles, every XCD
 // (id mod 8) running whole units
@@ -268,15 +274,29 @@ __device__ __forceinline__ void ozk_carry(oz_v4i (&acc)[S], uint32_t (*sR)[64], 
     }
 }
 
+// s_waitcnt lgkmcnt(0) alone (vmcnt and expcnt at their maxima), gfx9 encoding
+constexpr int OZK_WAIT_LGKM0 = 0xC07F;
+// column fragments q < OzkQL of the next chunk are prefetched into LDS (at most 28 KB per wave, so
+// that four waves fit in a CU's LDS with their accumulators); the others, first used by the last
+// batches of the descending order, are loaded at the start of their chunk
+template <int S>
+struct OzkQL {
+    static constexpr int value = (S - 8 < 28) ? S - 8 : 28;
+};
+constexpr int OZK_LA = 4;  // batches between reading a column fragment from LDS and its first use
+
 template <int L, int S>
 __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__restrict__ as,
                                                 const int8_t *__restrict__ bs, const int32_t *__restrict__ ea,
                                                 const int32_t *__restrict__ eb, int nbA, int nbB, int Mp, int Np,
                                                 int nch) {
-    static_assert(S <= OZK_SMAX && S > OZK_PD, "long-K form register budget");
-    constexpr int NW = S / 4;
+    constexpr int NW = S / 4, PD = OzkPD<S>::value, QL = OzkQL<S>::value;
+    static_assert(S <= OZK_SMAX && QL > 0, "long-K form register budget");
+    // LDS per wave (4 waves per CU): the next chunk's first QL column fragments (1 KB each) and the
+    // fixed-point accumulators (touched only at carries and in the epilogue)
+    __shared__ oz_v4i sB[QL][64];
     __shared__ uint32_t sR[4 * NW + 8][64];
-    // tile of this wave: see ozk_units (workgroup id -> XCD id % 8, its k-th unit, slot in the unit)
+    // tile of this wave: see ozk_workgroups (workgroup id -> XCD id % 8, its k-th unit, slot in the unit)
     const int tm = Mp / 16, tn = Np / 16;
     const int BR = tm < 16 ? tm : 16, BC = (128 / BR) < tn ? 128 / BR : tn, SU = BR * BC;
     const int nsr = (tm + BR - 1) / BR, nsc = (tn + BC - 1) / BC;
@@ -291,14 +311,14 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
     const int lane = threadIdx.x, grp = lane >> 4;
     const int bA = nbA > 1 ? b : 0, bB = nbB > 1 ? b : 0;
 #pragma unroll
-    for (int w = 0; w < 4 * NW + 8; w++) sR[w][lane] = 0;
+    for (int q = 0; q < 4 * NW + 8; q++) sR[q][lane] = 0;
     oz_v4i acc[S];
 #pragma unroll
     for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
     // a (chunk, slice) plane holds Rp rows of 64 bytes (4 fragments of 16); lane l reads row l % 16,
     // bytes 16 (l / 16) .. +16 of the chunk.  Buffer loads: the batch base in the resource (SGPRs),
-    // the lane's byte offset in one VGPR and the (chunk, slice) plane offset in an SGPR, so a load in
-    // flight holds no address registers (the driver keeps one batch's planes below 4 GiB).
+    // the lane's byte offset in one VGPR and the plane offset in an SGPR, so a load in flight holds
+    // no address registers (the driver keeps one batch's planes below 4 GiB).
     const int64_t pbA = (int64_t)Mp * OZK_KC, pbB = (int64_t)Np * OZK_KC;  // plane bytes
     const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(as + (int64_t)bA * nch * S * pbA), (short)0, (int)0xFFFFFFFFu, 0x00020000);
@@ -312,21 +332,43 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
     auto ldB = [&](int plane) -> oz_v4i {
         return __builtin_amdgcn_raw_buffer_load_b128(rB, voB, (int)((uint32_t)plane * psB), 0);
     };
+    auto ldB_lds = [&](int plane, int q) {
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, (__attribute__((address_space(3))) void *)&sB[q][0], 16, voB,
+                                                 (int)((uint32_t)plane * psB), 0, 0);
+    };
+#pragma unroll
+    for (int q = 0; q < QL; q++) ldB_lds(q, q);
     for (int ch = 0; ch < nch; ch++) {
         const int c0 = ch * S;
-        // column fragments of the chunk, then a ring of row fragments: batch p (descending) forms the
-        // pairs (p, q <= S-1-p), so the first batches need only the first column fragments while the
-        // rest are still in flight, and every row fragment is requested OZK_PD batches ahead
-        oz_v4i bq[S], ap[OZK_PD];
+        oz_v4i bq[S], ap[PD];
+        // the early column fragments were prefetched into LDS during the previous chunk (wait for those
+        // loads before this chunk issues its own, so that the wait covers nothing else)
+        __builtin_amdgcn_s_waitcnt(0);
+        // the late column fragments and the first row fragments from global memory
 #pragma unroll
-        for (int q = 0; q < S; q++) bq[q] = ldB(c0 + q);
+        for (int q = QL; q < S; q++) bq[q] = ldB(c0 + q);
 #pragma unroll
-        for (int t = 0; t < OZK_PD; t++) ap[t] = ldA(c0 + S - 1 - t);
+        for (int t = 0; t < PD; t++) ap[t] = ldA(c0 + S - 1 - t);
+#pragma unroll
+        for (int q = 0; q < OZK_LA && q < QL; q++) bq[q] = sB[q][lane];
+        // batches by descending row slice p (batch p forms the pairs (p, q <= S-1-p), first using column
+        // fragment S-1-p); column fragments come from LDS OZK_LA batches before their first use, and
+        // once the last one is read the next chunk's prefetch into LDS starts; the row fragments
+        // stream through a ring of PD registers, each reloaded PD batches ahead
 #pragma unroll
         for (int p = S - 1; p >= 0; p--) {
-            const int t = (S - 1 - p) % OZK_PD;
+            {
+                const int qn = S - 1 - p + OZK_LA;  // column fragment read from LDS at this batch
+                if (qn < QL) bq[qn] = sB[qn][lane];
+                if (qn == QL && ch + 1 < nch) {
+                    __builtin_amdgcn_s_waitcnt(OZK_WAIT_LGKM0);
+#pragma unroll
+                    for (int q = 0; q < QL; q++) ldB_lds(c0 + S + q, q);
+                }
+            }
+            const int t = (S - 1 - p) % PD;
             const oz_v4i a = ap[t];
-            if (p - OZK_PD >= 0) ap[t] = ldA(c0 + p - OZK_PD);
+            if (p - PD >= 0) ap[t] = ldA(c0 + p - PD);
 #pragma unroll
             for (int q = 0; q <= S - 1 - p; q++)
                 acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bq[q], acc[p + q], 0, 0, 0);
@@ -337,7 +379,7 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
     for (int e = 0; e < 4; e++) {
         uint32_t R[NW];
 #pragma unroll
-        for (int w = 0; w < NW; w++) R[w] = sR[e * NW + w][lane];
+        for (int q = 0; q < NW; q++) R[q] = sR[e * NW + q][lane];
         const int64_t RI = (int64_t)(((uint64_t)sR[4 * NW + 2 * e + 1][lane] << 32) | sR[4 * NW + 2 * e][lane]);
         oz_store<L, S>(g, R, RI, i0 + 4 * grp + e, j0 + (lane & 15), b, bA, bB, ea, eb);
     }
