@@ -95,7 +95,7 @@ struct OzDigit {
                 av = oz_v4i{0, 0, 0, 0};
                 bv = oz_v4i{0, 0, 0, 0};
             }
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv, av, acc, 0, 0, 0);
         }
         constexpr int w = D / 4, sh = 8 * (3 - D % 4);
 #pragma unroll
@@ -113,6 +113,10 @@ struct OzDigit<S, -1> {
                                                int64_t (&)[4]) {}
 };
 
+// Both kernels form the transposed tile, MFMA(B fragment, A fragment) = (A B)^T, so that lane l holds
+// C[i0 + l % 16, j0 + 4 (l / 16) + e]: the 16 lanes of a group touch 16 consecutive rows of a
+// column-major C (coalesced 64-byte accesses per limb plane in the read-modify-write of C).
+//
 // Epilogue of one output: the exact fixed-point sum (RI * 2^(32 NW) + sum_w R[w] 2^(32 (NW-1-w)),
 // two's complement over NW + 2 words) is rounded once to a limb float with exponent
 // e_i + e_j - 12 - 8 (S-1) and applied to C[i, j] (set / subtract / add) in full width.
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
     }
 #pragma unroll
     for (int e = 0; e < 4; e++)
-        oz_store<L, S>(g, R[e], RI[e], i0 + wr * 16 + 4 * grp + e, j0 + wc * 16 + (lane & 15), b, bA, bB, ea, eb);
+        oz_store<L, S>(g, R[e], RI[e], i0 + wr * 16 + (lane & 15), j0 + wc * 16 + 4 * grp + e, b, bA, bB, ea, eb);
 }
 
 // ---- long-K form ----
@@ -371,7 +375,7 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
             if (p - PD >= 0) ap[t] = ldA(c0 + p - PD);
 #pragma unroll
             for (int q = 0; q <= S - 1 - p; q++)
-                acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bq[q], acc[p + q], 0, 0, 0);
+                acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bq[q], a, acc[p + q], 0, 0, 0);
         }
         if ((ch % OZK_CARRY) == OZK_CARRY - 1 || ch == nch - 1) ozk_carry<S>(acc, sR, lane);
     }
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
 #pragma unroll
         for (int q = 0; q < NW; q++) R[q] = sR[e * NW + q][lane];
         const int64_t RI = (int64_t)(((uint64_t)sR[4 * NW + 2 * e + 1][lane] << 32) | sR[4 * NW + 2 * e][lane]);
-        oz_store<L, S>(g, R, RI, i0 + 4 * grp + e, j0 + (lane & 15), b, bA, bB, ea, eb);
+        oz_store<L, S>(g, R, RI, i0 + (lane & 15), j0 + 4 * grp + e, b, bA, bB, ea, eb);
     }
 }
 

@@ -7,8 +7,8 @@
 //    The row sums RA_r = sum_k a'_rk are kept (L+1 limbs).
 // 2. MAC kernel (k_gemm_plain): 16x16 outputs per workgroup, K staged through LDS
 //    in chunks of 16.  Each thread accumulates sum_k a'_ik b'_kj into 96-bit column
-//    accumulators (v_mad_u64_u32 + v_addc per limb product), skipping partial-product
-//    columns below L-2.
+//    accumulators (v_mad_u64_u32 + v_addc per limb product, independent columns issued in
+//    pairs), skipping partial-product columns below L-2.
 // 3. Epilogue: S = sum a'b' - O (RA_i + RB_j) + K O^2 = sum_k a_ik b_kj exactly (up to the
 //    skipped columns), renormalised to one limb-float with exponent e_i + e_j - 2W,
 //    then C = S, C - S or C + S.
@@ -298,14 +298,9 @@ __global__ __launch_bounds__(256) void k_gemm_plain(GemmArgs g, const uint32_t *
                 a[q] = As[q][kk][tx];
                 bq[q] = Bs[q][kk][ty];
             }
-#pragma unroll
-            for (int s = S0; s <= 2 * L - 2; s++) {
-#pragma unroll
-                for (int p = 0; p < L; p++) {
-                    int q = s - p;
-                    if (q >= 0 && q < L) mpx::mac(lo[s - S0], hi[s - S0], a[p], bq[q]);
-                }
-            }
+            // column accumulators are independent: issue the multiply-adds row by row in pairs on
+            // adjacent columns (mpx::col_products), not column by column as a dependent chain
+            mpx::col_products<L, S0>(a, bq, lo, hi);
         }
         __syncthreads();
     }
