@@ -21,6 +21,10 @@ also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank
 replayed on its first iterate for both forms (2 timed steps), with the roofline of the long-K Ozaki
 kernel that carries its F1 products and Schur inner products.
 
+also_measured_batch: the first --batch problems of the seeded batch workload (512 small SDPs, 1-3
+dense blocks of U[8,40], m in U[10,120]) solved to gap <= 1e-60 as concurrent independent solves
+(--batch-threads host threads, one context and stream each): solves/s and latency p50 / p99.
+
 cpu_baseline: the oracle (C restatement of the reference's mpf path, serial Cholesky as the
 reference's Rpotrf) on the same iterate, dumped by the GPU run: Make_bMat in full and the Cholesky of
 the leading --cpu-n rows/columns of B, scaled by (m / cpu_n)^3 to the full factorisation.
@@ -101,6 +105,45 @@ def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve:
     return info, sha, res
 
 
+def measure_batch(count: int, threads: int, gemm_name: str):
+    """The batched configuration as independent solves: the first `count` problems of the seeded batch
+    (1-3 dense blocks of U[8,40], m in U[10,120]) at 256 bits and gap <= 1e-60, `threads` host threads
+    each driving its own context and stream, so that the GPU runs several problems' kernels at once."""
+    import concurrent.futures
+
+    import gen_problems
+    import sdpg
+
+    tmp = tempfile.mkdtemp(prefix="sdpg_batch_")
+    t0 = time.time()
+    paths = gen_problems.generate_batch(tmp, count)
+    t_gen = time.time() - t0
+    mode = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}[gemm_name]
+
+    def one(path):
+        t = time.time()
+        r = sdpg.solve(path, PARAM, 0, None, mode)
+        return time.time() - t, r.phase.decode().strip(), r.iterations
+
+    one(paths[0])  # warm-up: first-use costs of the runtime
+    t0 = time.time()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(one, paths))
+    wall = time.time() - t0
+    lat = sorted(x[0] for x in res)
+    phases = {}
+    for _, ph, _ in res:
+        phases[ph] = phases.get(ph, 0) + 1
+    pct = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))]
+    out = {"problems": count, "host_threads": threads, "form": gemm_name, "wall_s": wall,
+           "solves_per_s": count / wall, "latency_p50_s": pct(0.50), "latency_p99_s": pct(0.99),
+           "mean_iterations": sum(x[2] for x in res) / count, "phases": phases, "generation_s": t_gen,
+           "not_pdopt": [[k, res[k][1], res[k][2]] for k in range(count) if res[k][1] != "pdOPT"]}
+    print(f"[bench] batch: {count} problems, {threads} threads: {out['solves_per_s']:.1f} solves/s, "
+          f"p50 {out['latency_p50_s']:.3f} s, p99 {out['latency_p99_s']:.3f} s, phases {phases}", flush=True)
+    return out
+
+
 def roofline(kst_entry, kernel: str, traffic=None, note: str = ""):
     """MFMA roofline of one product kernel from its live event timing (launches, ms, INT8 ops)."""
     n, ms, ops = kst_entry
@@ -126,6 +169,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the second measured workload (n=150, m=1326 dense rank-2 constraints)")
     ap.add_argument("--cpu-n", type=int, default=500, help="Cholesky sample size of the CPU baseline")
+    ap.add_argument("--batch", type=int, default=128,
+                    help="problems of the seeded 512-problem batch solved as concurrent independent solves (0: skip)")
+    ap.add_argument("--batch-threads", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding of the "
@@ -205,6 +251,12 @@ def main():
             if rl3:
                 ex["roofline"] = rl3
         out["also_measured"] = ex
+
+    if a.batch > 0:
+        b = measure_batch(a.batch, a.batch_threads, "ozaki" if "ozaki" in forms else forms[0])
+        b["note"] = ("independent solves, one context and stream per host thread; the lockstep batched form "
+                     "(persistent fused kernels per Newton phase) is not built")
+        out["also_measured_batch"] = b
 
     if not a.no_cpu and os.path.exists(dump + "X.bin"):
         import gen_problems
