@@ -244,10 +244,16 @@ def main():
         if "ozaki" in res3 and "plain" in res3:
             ex["ozaki_speedup_vs_plain"] = res3["plain"]["step_wall_ms"] / max(res3["ozaki"]["step_wall_ms"], 1e-9)
         if "ozaki" in res3:
+            traffic3 = None
+            tfile3 = os.path.join(ROOT, "profiles", "k_gemm_ozk_dense_rank2_m1326_traffic.json")
+            if os.path.exists(tfile3):
+                with open(tfile3) as f:
+                    traffic3 = json.load(f)["traffic_bytes_per_launch"]
             rl3 = roofline(res3["ozaki"]["_kst"]["ozaki_mfma_longk"],
-                           "k_gemm_ozk (Ozaki INT8 slice products, long K, v_mfma_i32_16x16x64_i8)", None,
+                           "k_gemm_ozk (Ozaki INT8 slice products, long K, v_mfma_i32_16x16x64_i8)", traffic3,
                            "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36; F1 products (K = n) and Schur "
-                           "inner products (K = n(n+1)/2)")
+                           "inner products (K = n(n+1)/2); traffic (mean of the three launches) from "
+                           "profiles/k_gemm_ozk_dense_rank2_m1326_traffic.txt")
             if rl3:
                 ex["roofline"] = rl3
         out["also_measured"] = ex
