@@ -136,6 +136,7 @@ __device__ __forceinline__ int64_t fx_at(const GemmArgs &g, int which, int b, in
 __device__ __forceinline__ bool fx_rcontig(const GemmArgs &g, int which) {
     return which == 0 ? !g.transA : g.transB;
 }
+inline bool fx_rcontig_host(const GemmArgs &g, int which) { return which == 0 ? !g.transA : g.transB; }
 
 }  // namespace sdpg
 

@@ -41,8 +41,16 @@ constexpr int OZ_LONGK_MIN = 64;  // K above which the long-K kernel (k_gemm_ozk
 // slice planes: ((((b * nch + ch) * S + p) * Rp + r) * KC + kk), Rp = R rounded up to OZ_T; KC = k per
 // chunk (16 for the short-K kernel, 64 for the long-K kernel)
 template <int L, int S, int KC>
-__global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const int32_t *eo, int8_t *out) {
+__global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const int32_t *eo, int8_t *out, int R2 = 0,
+                           int Rp2 = 0, const int32_t *eo2 = nullptr, int8_t *out2 = nullptr) {
     static_assert(S % 4 == 0 && S <= 4 * L, "slice count");
+    if (blockIdx.z) {  // both operands in one launch (op(B) in z = 1)
+        which = 1;
+        R = R2;
+        Rp = Rp2;
+        eo = eo2;
+        out = out2;
+    }
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = blockIdx.y;
     const int Kp = nch * KC;
@@ -472,6 +480,17 @@ struct OzakiWork {
             qb = eb;
         }
         int64_t sa_n = (int64_t)Mp * nch * KC, sb_n = (int64_t)Np * nch * KC;
+        if (fillA && fillB && nbA == nbB && fx_rcontig_host(g, 0) == fx_rcontig_host(g, 1)) {
+            // both operands in one launch per pass
+            dim3 gm = fix_max_grid(g, 0, std::max(g.M, g.N), nbA);
+            k_fix_max<L><<<dim3(gm.x, gm.y, 2), 256, 0, st>>>(g, 0, g.M, qa, g.N, qb);
+            dim3 gs((unsigned)((std::max(sa_n, sb_n) + 255) / 256), nbA, 2);
+            if (longk)
+                k_oz_slice<L, S, OZK_KC><<<gs, 256, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa, g.N, Np, qb, pb);
+            else
+                k_oz_slice<L, S, OZ_KC><<<gs, 256, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa, g.N, Np, qb, pb);
+            fillA = fillB = false;
+        }
         if (fillA) {
             k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, qa);
             if (longk)

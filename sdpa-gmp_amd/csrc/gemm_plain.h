@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "engine_common.h"
 #include "mpx.h"
 
@@ -150,8 +152,15 @@ __device__ __forceinline__ void fx_convert(const mpx::mp<L> &x, int32_t emax, in
 // 256 / KS rows with KS threads per row striding over k (coalesced across the rows), reduced in
 // LDS; k contiguous: a wave per row.
 constexpr int FXM_KS = 8;
+// Both operands in one launch: blockIdx.z = 1 selects op(B) with R2 rows and eo2 (same grid shape).
 template <int L>
-__global__ __launch_bounds__(256) void k_fix_max(GemmArgs g, int which, int R, int32_t *eo) {
+__global__ __launch_bounds__(256) void k_fix_max(GemmArgs g, int which, int R, int32_t *eo, int R2 = 0,
+                                                 int32_t *eo2 = nullptr) {
+    if (blockIdx.z) {
+        which = 1;
+        R = R2;
+        eo = eo2;
+    }
     int b = blockIdx.y;
     PV src = which == 0 ? g.A : g.B;
     const int K = g.K;
@@ -211,7 +220,14 @@ inline bool gemm_b_is_at(const GemmArgs &g) {
 
 // pass 2: a' = round(x * 2^(W - e_r)) + 2^(32L-1), W = 32L - 2; out[((q * K) + k) * R + r] per batch
 template <int L>
-__global__ void k_fix_conv(GemmArgs g, int which, int R, const int32_t *eo, uint32_t *out) {
+__global__ void k_fix_conv(GemmArgs g, int which, int R, const int32_t *eo, uint32_t *out, int R2 = 0,
+                           const int32_t *eo2 = nullptr, uint32_t *out2 = nullptr) {
+    if (blockIdx.z) {  // both operands in one launch (op(B) in z = 1)
+        which = 1;
+        R = R2;
+        eo = eo2;
+        out = out2;
+    }
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = blockIdx.y;
     int K = g.K;
@@ -398,6 +414,13 @@ struct PlainWork {
         }
         int64_t ta = (int64_t)g.M * g.K, tb = (int64_t)g.N * g.K;
         dim3 ga((unsigned)((ta + 255) / 256), nbA), gb((unsigned)((tb + 255) / 256), nbB);
+        if (fillA && fillB && nbA == nbB && fx_rcontig_host(g, 0) == fx_rcontig_host(g, 1)) {
+            // both operands in one launch per pass
+            dim3 gm = fix_max_grid(g, 0, std::max(g.M, g.N), nbA);
+            k_fix_max<L><<<dim3(gm.x, gm.y, 2), 256, 0, st>>>(g, 0, g.M, qa, g.N, qb);
+            k_fix_conv<L><<<dim3(std::max(ga.x, gb.x), nbA, 2), 256, 0, st>>>(g, 0, g.M, qa, fa, g.N, qb, fb);
+            fillA = fillB = false;
+        }
         if (fillA) {
             k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, qa);
             k_fix_conv<L><<<ga, 256, 0, st>>>(g, 0, g.M, qa, fa);
