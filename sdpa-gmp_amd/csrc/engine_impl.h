@@ -18,6 +18,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -647,14 +648,7 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
             c = (uint32_t)(s >> 32);
         }
         rg[L] += c;
-#pragma unroll
-        for (int s = S0; s <= 2 * L - 2; s++) {
-#pragma unroll
-            for (int q = 0; q < L; q++) {
-                int r = s - q;
-                if (r >= 0 && r < L) mpx::mac(lo[s - S0], hi[s - S0], a[q], g[r]);
-            }
-        }
+        mpx::col_products<L, S0>(a, g, lo, hi);
     }
     uint32_t ras[L + 1];
 #pragma unroll
@@ -1706,7 +1700,12 @@ struct Engine : EngineBase {
     size_t potrf_shbytes = 0;
 
     int alloc_factor_space() {
+        // diagonal block edge of the blocked Cholesky factorisations (SDPG_NBK = 8, 16 or 32 overrides)
         NBK = 32;
+        if (const char *ev = getenv("SDPG_NBK")) {
+            const int want = atoi(ev);
+            if (want == 8 || want == 16 || want == 32) NBK = want;
+        }
         while (NBK > 8 && ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4 > lds_max) NBK /= 2;
         potrf_shbytes = ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4;
         HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1743,7 +1742,9 @@ struct Engine : EngineBase {
             int nb = std::min(NBK, n - k);
             int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
             hipEvent_t ev = kprof.begin(st);
-            k_potrf_diag<L><<<1, 1024, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(), lo, d_info);
+            // one thread per updated entry of the busiest step (150 at nb = 16, 558 at nb = 32)
+            k_potrf_diag<L><<<1, nb <= 16 ? 256 : 1024, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff, linv.pv(),
+                                                                             lo, d_info);
             kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0);
             if (k + nb < n) {
                 int r = n - k - nb;

@@ -12,7 +12,8 @@
 //    (S(S+1)/2 slice products instead of S^2).  Two kernels:
 //    - k_gemm_oz (K <= 64, the Cholesky panels and trailing updates): KC = 16; every
 //      v_mfma_i32_16x16x64_i8 issues four (p, q) pairs of one d at once (lane group g = lane / 16
-//      carries pair 4u + g, each over the same 16 k), and each chunk's T_d is carried at once;
+//      carries pair (4u + g, q' - g), each over the same 16 k) into a per-digit INT32 accumulator
+//      that lives across the (at most 4) chunks;
 //    - k_gemm_ozk (long K: the F1 products and Schur inner products): KC = 64; one MFMA per
 //      (p, q) over 64 k into a per-digit INT32 accumulator that lives across chunks and is
 //      carried every 32 chunks.
@@ -87,40 +88,6 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
     }
 }
 
-// one output digit position d of the Horner recombination (compile-time d, S-1 down to 0)
-template <int S, int D>
-struct OzDigit {
-    static __device__ __forceinline__ void run(const oz_v4i *__restrict__ lA, const oz_v4i *__restrict__ lB, int ar,
-                                               int bc, int g, uint32_t (&R)[4][S / 4], int64_t (&c)[4]) {
-        oz_v4i acc = {0, 0, 0, 0};
-#pragma unroll
-        for (int u = 0; u < (D + 4) / 4; u++) {
-            int p = 4 * u + g;
-            int pc = p <= D ? p : D;
-            oz_v4i av = lA[pc * OZ_T + ar];
-            oz_v4i bv = lB[(D - pc) * OZ_T + bc];
-            if (p > D) {
-                av = oz_v4i{0, 0, 0, 0};
-                bv = oz_v4i{0, 0, 0, 0};
-            }
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv, av, acc, 0, 0, 0);
-        }
-        constexpr int w = D / 4, sh = 8 * (3 - D % 4);
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            int64_t v = (int64_t)acc[e] + c[e] + (int64_t)((R[e][w] >> sh) & 255u);
-            R[e][w] = (R[e][w] & ~(255u << sh)) | (((uint32_t)v & 255u) << sh);
-            c[e] = v >> 8;
-        }
-        OzDigit<S, D - 1>::run(lA, lB, ar, bc, g, R, c);
-    }
-};
-template <int S>
-struct OzDigit<S, -1> {
-    static __device__ __forceinline__ void run(const oz_v4i *, const oz_v4i *, int, int, int, uint32_t (&)[4][S / 4],
-                                               int64_t (&)[4]) {}
-};
-
 // Both kernels form the transposed tile, MFMA(B fragment, A fragment) = (A B)^T, so that lane l holds
 // C[i0 + l % 16, j0 + 4 (l / 16) + e]: the 16 lanes of a group touch 16 consecutive rows of a
 // column-major C (coalesced 64-byte accesses per limb plane in the read-modify-write of C).
@@ -129,12 +96,8 @@ struct OzDigit<S, -1> {
 // two's complement over NW + 2 words) is rounded once to a limb float with exponent
 // e_i + e_j - 12 - 8 (S-1) and applied to C[i, j] (set / subtract / add) in full width.
 template <int L, int S>
-__device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[S / 4], int64_t RI, int i, int j, int b,
-                                         int bA, int bB, const int32_t *__restrict__ ea,
-                                         const int32_t *__restrict__ eb) {
+__device__ __forceinline__ mpx::mp<L> oz_value(const uint32_t (&R)[S / 4], int64_t RI, int32_t ei, int32_t ej) {
     constexpr int NW = S / 4;
-    if (i >= g.M || j >= g.N) return;
-    if (g.lower && i < j) return;
     constexpr int NX = NW + 2;
     uint32_t x[NX];
 #pragma unroll
@@ -158,11 +121,21 @@ __device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[
         uint32_t y[L + 1];
 #pragma unroll
         for (int q = 0; q <= L; q++) y[q] = x[NX - 1 - L + q];
-        int32_t E = ea[(int64_t)bA * g.M + i] + eb[(int64_t)bB * g.N + j];
+        int32_t E = ei + ej;
         E = (E < -(1 << 28) ? 0 : E) - 12 - 8 * (S - 1);
         int32_t ex = 32 * NX - (int32_t)lz + E;
         mpx::round_store<L>(res, y, ex, neg);
     }
+    return res;
+}
+
+template <int L, int S>
+__device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[S / 4], int64_t RI, int i, int j, int b,
+                                         int bA, int bB, const int32_t *__restrict__ ea,
+                                         const int32_t *__restrict__ eb) {
+    if (i >= g.M || j >= g.N) return;
+    if (g.lower && i < j) return;
+    mpx::mp<L> res = oz_value<L, S>(R, RI, ea[(int64_t)bA * g.M + i], eb[(int64_t)bB * g.N + j]);
     int64_t cidx = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
     if (g.op == 1)
         res = mpx::sub<L>(ldm<L>(g.C, cidx), res);
@@ -171,50 +144,123 @@ __device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[
     stm<L>(g.C, cidx, res);
 }
 
+// Short-K product kernel: one workgroup per 32 x 32 output tile, 2 x 2 waves of 16 x 16, the
+// slices of the tile's rows and columns staged in LDS chunk by chunk.
+// Pair schedule: the MFMA numbered (u, q') gives lane group g = lane / 16 the slice pair
+// (p, q) = (4u + g, q' - g), so the four groups of a wave all add into the same digit position
+// d = p + q = 4u + q' and that MFMA accumulates into the INT32 digit sum T_d (groups with q < 0 feed a
+// zero fragment).  Every pair with p + q < S occurs once (g = p mod 4, u = p / 4, q' = q + g).  A wave
+// therefore reads its S/4 row fragments (p = g mod 4) and each column fragment once per chunk, and
+// keeps all S digit sums across the chunks: a digit sum over one chunk is bounded by
+// S * 16 * 2^14 (|digits| <= 128), and K <= 64 gives at most 4 chunks, below 2^31 for S <= 48.
+// The digit carry runs once per output at the end.
 template <int L, int S>
 __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__restrict__ as,
                                                  const int8_t *__restrict__ bs, const int32_t *__restrict__ ea,
                                                  const int32_t *__restrict__ eb, int nbA, int nbB, int Mp, int Np,
                                                  int nch) {
-    constexpr int NW = S / 4;
+    constexpr int NW = S / 4, NU = (S + 3) / 4;
     static_assert(NW + 2 >= L + 1, "accumulator narrower than the result");
+    static_assert(S <= 48, "digit sums of four chunks must stay below 2^31");
     __shared__ oz_v4i lA[S * OZ_T], lB[S * OZ_T];
-    const int M = g.M, N = g.N;
     const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T, b = blockIdx.z;
     if (g.lower && i0 + OZ_T - 1 < j0) return;  // tile strictly above the diagonal
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv & 1, wc = wv >> 1, grp = lane >> 4;
     const int ar = wr * 16 + (lane & 15), bc = wc * 16 + (lane & 15);
     const int bA = nbA > 1 ? b : 0, bB = nbB > 1 ? b : 0;
-    uint32_t R[4][NW];
-    int64_t RI[4];
+    // The four outputs of this lane, C[i, j0c + e]: their exponents and, for op != SET, the raw words
+    // of their old C values are requested before the products (at clamped, always valid addresses,
+    // without a branch, and decoded only at the end) so that the loads' latency hides under them.
+    const int i = i0 + ar, j0c = j0 + wc * 16 + 4 * grp;
+    const int ic = i < g.M ? i : g.M - 1;
+    const int32_t ei = ea[(int64_t)bA * g.M + ic];
+    int32_t ej[4];
+    uint32_t craw[4][L + 1];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        RI[e] = 0;
+        const int jc = j0c + e < g.N ? j0c + e : g.N - 1;
+        ej[e] = eb[(int64_t)bB * g.N + jc];
+        if (g.op != GEMM_SET) {
+            const int64_t ci = g.offC + (int64_t)b * g.sC + ic + (int64_t)g.ldc * jc;
 #pragma unroll
-        for (int w = 0; w < NW; w++) R[e][w] = 0;
+            for (int q = 0; q <= L; q++) craw[e][q] = g.C.p[(int64_t)q * g.C.n + ci];
+        }
     }
+    oz_v4i acc[S];
+#pragma unroll
+    for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
     const oz_v4i *sa = (const oz_v4i *)as, *sb = (const oz_v4i *)bs;
     for (int ch = 0; ch < nch; ch++) {
         const oz_v4i *pa = sa + (((int64_t)bA * nch + ch) * S) * Mp + i0;
         const oz_v4i *pb = sb + (((int64_t)bB * nch + ch) * S) * Np + j0;
-        for (int t = tid; t < S * OZ_T; t += 256) {
-            int p = t / OZ_T, r = t % OZ_T;
-            lA[t] = pa[(int64_t)p * Mp + r];
-            lB[t] = pb[(int64_t)p * Np + r];
+        constexpr int NST = (S * OZ_T + 255) / 256;  // staged fragments per thread and operand
+        oz_v4i sta[NST], stb[NST];
+#pragma unroll
+        for (int k = 0; k < NST; k++) {
+            const int t = tid + 256 * k;
+            if (t < S * OZ_T) {
+                sta[k] = pa[(int64_t)(t / OZ_T) * Mp + t % OZ_T];
+                stb[k] = pb[(int64_t)(t / OZ_T) * Np + t % OZ_T];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NST; k++) {
+            const int t = tid + 256 * k;
+            if (t < S * OZ_T) {
+                lA[t] = sta[k];
+                lB[t] = stb[k];
+            }
         }
         __syncthreads();
-        int64_t c[4];
+        oz_v4i af[NU];
 #pragma unroll
-        for (int e = 0; e < 4; e++) c[e] = 0;
-        OzDigit<S, S - 1>::run(lA, lB, ar, bc, grp, R, c);
+        for (int u = 0; u < NU; u++) {
+            const int p = 4 * u + grp;
+            af[u] = lA[(p < S ? p : S - 1) * OZ_T + ar];
+        }
 #pragma unroll
-        for (int e = 0; e < 4; e++) RI[e] += c[e];
+        for (int qq = 0; qq < S; qq++) {
+            const int q = qq - grp;
+            oz_v4i bv = lB[(q >= 0 ? q : 0) * OZ_T + bc];
+            if (q < 0) bv = oz_v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < NU; u++) {
+                if (4 * u + qq < S)
+                    acc[4 * u + qq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv, af[u], acc[4 * u + qq], 0, 0, 0);
+            }
+        }
         __syncthreads();
     }
+    // sum_d T_d 256^-d carried from d = S-1 up: digit d goes to byte 3 - d % 4 of word d / 4 of R,
+    // the carry out of d = 0 is the integer part.  |T_d| < 2^25.2 and the carry stays below 2^18,
+    // so the carry runs in 32-bit arithmetic.
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-        oz_store<L, S>(g, R[e], RI[e], i0 + wr * 16 + (lane & 15), j0 + wc * 16 + 4 * grp + e, b, bA, bB, ea, eb);
+    for (int e = 0; e < 4; e++) {
+        const int j = j0c + e;
+        uint32_t R[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) R[w] = 0;
+        int32_t c = 0;
+#pragma unroll
+        for (int d = S - 1; d >= 0; d--) {
+            const int32_t v = acc[d][e] + c;
+            R[d / 4] |= ((uint32_t)v & 255u) << (8 * (3 - d % 4));
+            c = v >> 8;
+        }
+        mpx::mp<L> res = oz_value<L, S>(R, (int64_t)c, ei, ej[e]);
+        if (g.op != GEMM_SET) {
+            mpx::mp<L> cold;
+#pragma unroll
+            for (int q = 0; q < L; q++) cold.m[q] = craw[e][q];
+            cold.e = (int32_t)craw[e][L] >> 1;
+            cold.s = craw[e][L] & 1u;
+            res = mpx::add<L>(cold, g.op == GEMM_SUB ? mpx::neg<L>(res) : res);
+        }
+        // every output is formed (padded rows and columns from zero slices and clamped C words), so the
+        // four chains interleave; only the stores are predicated
+        if (i < g.M && j < g.N && !(g.lower && i < j)) stm<L>(g.C, g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j, res);
+    }
 }
 
 // ---- long-K form ----

@@ -60,16 +60,22 @@ MPX_HD uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t r) {
 #endif
 }
 
-// lo64 += a*b with carry into hi32 (one v_mad_u64_u32 + one v_addc on the GPU).  Not volatile:
-// the compiler may interleave independent accumulator chains, which is where the speed of the
-// products below comes from (a dependent chain of these pairs costs about 28 cycles per pair in
-// one wave, independent chains issue back to back).
+// lo64 += a*b with the carry out of the low 64 bits added into hi32: one v_mad_u64_u32 and one
+// v_addc_co_u32 on the GPU, the carry passing through an SGPR pair.  On this target a VALU write
+// of an SGPR needs two wait states before a VALU reads it (the compiler pads its own carry chains
+// with s_nop for the same reason), so every helper below issues its multiply-adds first and its
+// carry-adds after them: in groups of three or four independent multiply-adds each carry-add is
+// at least two instructions behind the multiply-add that produced its carry, and the groups of
+// one or two pad the gap with s_nop.  Not volatile: the compiler may interleave independent
+// groups.
 MPX_HD void mac(uint64_t &lo, uint32_t &hi, uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32 %1, vcc, 0, %1, vcc"
-        : "+v"(lo), "+v"(hi)
-        : "v"(a), "v"(b)
-        : "vcc");
+    uint64_t c;
+    asm("v_mad_u64_u32 %0, %2, %3, %4, %0\n\t"
+        "s_nop 1\n\t"
+        "v_addc_co_u32 %1, %2, 0, %1, %2"
+        : "+v"(lo), "+v"(hi), "=&s"(c)
+        : "v"(a), "v"(b));
 #else
     uint64_t p = (uint64_t)a * b;
     lo += p;
@@ -77,14 +83,14 @@ MPX_HD void mac(uint64_t &lo, uint32_t &hi, uint32_t a, uint32_t b) {
 #endif
 }
 
-// Two independent multiply-adds (lo0 += a0*b0, lo1 += a1*b1, carries into hi0 / hi1) with their
-// own carry registers, so the second multiply issues before the first carry is consumed.
+// two independent multiply-adds (lo0 += a0*b0, lo1 += a1*b1, carries into hi0 / hi1)
 MPX_HD void mac2(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_t &lo1, uint32_t &hi1, uint32_t a1,
                  uint32_t b1) {
 #if defined(__HIP_DEVICE_COMPILE__)
     uint64_t c0, c1;
     asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
         "v_mad_u64_u32 %1, %5, %8, %9, %1\n\t"
+        "s_nop 0\n\t"
         "v_addc_co_u32 %2, %4, 0, %2, %4\n\t"
         "v_addc_co_u32 %3, %5, 0, %3, %5"
         : "+v"(lo0), "+v"(lo1), "+v"(hi0), "+v"(hi1), "=&s"(c0), "=&s"(c1)
@@ -95,22 +101,72 @@ MPX_HD void mac2(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_
 #endif
 }
 
-// Column sums of a[0..N) x b[0..N) for the columns S0..2N-2 into acc[s - S0]: pairs of
-// multiply-adds on adjacent columns (p, q) and (p, q + 1) go through mac2.
+// three independent multiply-adds
+MPX_HD void mac3(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_t &lo1, uint32_t &hi1, uint32_t a1,
+                 uint32_t b1, uint64_t &lo2, uint32_t &hi2, uint32_t a2, uint32_t b2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t c0, c1, c2;
+    asm("v_mad_u64_u32 %0, %6, %9, %10, %0\n\t"
+        "v_mad_u64_u32 %1, %7, %11, %12, %1\n\t"
+        "v_mad_u64_u32 %2, %8, %13, %14, %2\n\t"
+        "v_addc_co_u32 %3, %6, 0, %3, %6\n\t"
+        "v_addc_co_u32 %4, %7, 0, %4, %7\n\t"
+        "v_addc_co_u32 %5, %8, 0, %5, %8"
+        : "+v"(lo0), "+v"(lo1), "+v"(lo2), "+v"(hi0), "+v"(hi1), "+v"(hi2), "=&s"(c0), "=&s"(c1), "=&s"(c2)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2));
+#else
+    mac(lo0, hi0, a0, b0);
+    mac(lo1, hi1, a1, b1);
+    mac(lo2, hi2, a2, b2);
+#endif
+}
+
+// four independent multiply-adds
+MPX_HD void mac4(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_t &lo1, uint32_t &hi1, uint32_t a1,
+                 uint32_t b1, uint64_t &lo2, uint32_t &hi2, uint32_t a2, uint32_t b2, uint64_t &lo3, uint32_t &hi3,
+                 uint32_t a3, uint32_t b3) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t c0, c1, c2, c3;
+    asm("v_mad_u64_u32 %0, %8, %12, %13, %0\n\t"
+        "v_mad_u64_u32 %1, %9, %14, %15, %1\n\t"
+        "v_mad_u64_u32 %2, %10, %16, %17, %2\n\t"
+        "v_mad_u64_u32 %3, %11, %18, %19, %3\n\t"
+        "v_addc_co_u32 %4, %8, 0, %4, %8\n\t"
+        "v_addc_co_u32 %5, %9, 0, %5, %9\n\t"
+        "v_addc_co_u32 %6, %10, 0, %6, %10\n\t"
+        "v_addc_co_u32 %7, %11, 0, %7, %11"
+        : "+v"(lo0), "+v"(lo1), "+v"(lo2), "+v"(lo3), "+v"(hi0), "+v"(hi1), "+v"(hi2), "+v"(hi3), "=&s"(c0),
+          "=&s"(c1), "=&s"(c2), "=&s"(c3)
+        : "v"(a0), "v"(b0), "v"(a1), "v"(b1), "v"(a2), "v"(b2), "v"(a3), "v"(b3));
+#else
+    mac(lo0, hi0, a0, b0);
+    mac(lo1, hi1, a1, b1);
+    mac(lo2, hi2, a2, b2);
+    mac(lo3, hi3, a3, b3);
+#endif
+}
+
+// Column sums of a[0..N) x b[0..N) for the columns S0..2N-2 into acc[s - S0]: for each a[p] the
+// multiply-adds into the adjacent columns p + q, p + q + 1, ... go out in groups of four.
 template <int N, int S0>
 MPX_HD void col_products(const uint32_t (&a)[N], const uint32_t (&b)[N], uint64_t (&lo)[2 * N - 1 - S0],
                          uint32_t (&hi)[2 * N - 1 - S0]) {
 #pragma unroll
     for (int p = 0; p < N; p++) {
+        const int q0 = S0 > p ? S0 - p : 0;
 #pragma unroll
-        for (int q = 0; q < N; q += 2) {
-            const int s0 = p + q, s1 = p + q + 1;
-            if (q + 1 < N && s0 >= S0) {
-                mac2(lo[s0 - S0], hi[s0 - S0], a[p], b[q], lo[s1 - S0], hi[s1 - S0], a[p], b[q + 1]);
-            } else {
-                if (s0 >= S0) mac(lo[s0 - S0], hi[s0 - S0], a[p], b[q]);
-                if (q + 1 < N && s1 >= S0) mac(lo[s1 - S0], hi[s1 - S0], a[p], b[q + 1]);
-            }
+        for (int q = q0; q < N; q += 4) {
+            const int c = p + q - S0, rem = N - q;
+            if (rem >= 4)
+                mac4(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1], lo[c + 2], hi[c + 2], a[p],
+                     b[q + 2], lo[c + 3], hi[c + 3], a[p], b[q + 3]);
+            else if (rem == 3)
+                mac3(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1], lo[c + 2], hi[c + 2], a[p],
+                     b[q + 2]);
+            else if (rem == 2)
+                mac2(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1]);
+            else
+                mac(lo[c], hi[c], a[p], b[q]);
         }
     }
 }
@@ -256,55 +312,63 @@ MPX_HD mp<L> absv(mp<L> a) {
     return a;
 }
 
+// |b| > |a| for non-zero a, b: the exponents decide, and at equal exponents the borrow out of
+// the mantissa subtraction a - b (one carry chain instead of a limb-by-limb comparison)
+template <int L>
+MPX_HD bool gt_abs(const mp<L> &b, const mp<L> &a) {
+    uint32_t br = 0;
+#pragma unroll
+    for (int k = 0; k < L; k++) {
+        uint64_t t = (uint64_t)a.m[k] - b.m[k] - br;
+        br = (uint32_t)(t >> 63);
+    }
+    return b.e > a.e || (b.e == a.e && br != 0);
+}
+
+// a + b, rounded to nearest.  One code path for every sign and zero case (no divergence between
+// lanes that add and lanes that subtract): x = the larger magnitude, y = the smaller (a zero has the
+// smallest exponent, so it always lands in y and aligns to nothing); y is aligned to x with a
+// sticky bit; W = x + y or x - y (two's complement, borrow-in from the sticky bit) over L + 2 words
+// [guard, L limbs, top]; one left shift by the leading zeros of W (31 after a carry into the top
+// word, 32 without, 32 + the cancelled bits after a subtraction) normalises it, and the result has
+// exponent e_x + 32 - lz.
 template <int L>
 MPX_HD mp<L> add(mp<L> a, mp<L> b) {
-    if (is_zero(a)) return b;
-    if (is_zero(b)) return a;
-    if (cmp_abs(b, a) > 0) {
-        mp<L> t = a;
-        a = b;
-        b = t;
-    }
-    uint32_t d = (uint32_t)(a.e - b.e);
-    uint32_t x[L + 1], y[L + 1];
-    x[0] = 0;
+    // a zero orders by the zero exponent whatever exponent it carries
+    if (is_zero(a)) a.e = EZERO;
+    if (is_zero(b)) b.e = EZERO;
+    const bool sw = gt_abs<L>(b, a);
+    const int32_t ex = sw ? b.e : a.e, ey = sw ? a.e : b.e;
+    const uint32_t sx = sw ? b.s : a.s, sy = sw ? a.s : b.s;
+    uint32_t W[L + 2], y[L + 1];
+    W[0] = 0;
     y[0] = 0;
 #pragma unroll
     for (int k = 0; k < L; k++) {
-        x[k + 1] = a.m[k];
-        y[k + 1] = b.m[k];
+        W[k + 1] = sw ? b.m[k] : a.m[k];
+        y[k + 1] = sw ? a.m[k] : b.m[k];
     }
+    W[L + 1] = 0;
     uint32_t sticky = 0;
-    shr<L + 1>(y, d, sticky);
-    int32_t e = a.e;
-    if (a.s == b.s) {
-        uint32_t c = 0;
+    shr<L + 1>(y, (uint32_t)(ex - ey), sticky);
+    // subtraction: W - y - [sticky != 0] = W + ~y + [sticky == 0]; the top word ends at 0 there, since
+    // |x| >= |y| keeps the difference non-negative
+    const uint32_t neg = (sx != sy) ? 0xFFFFFFFFu : 0u;
+    uint32_t c = neg ? (sticky ? 0u : 1u) : 0u;
 #pragma unroll
-        for (int k = 0; k <= L; k++) {
-            uint64_t t = (uint64_t)x[k] + y[k] + c;
-            x[k] = (uint32_t)t;
-            c = (uint32_t)(t >> 32);
-        }
-        if (c) {
-#pragma unroll
-            for (int k = 0; k <= L; k++) x[k] = funnel(k < L ? x[k + 1] : 1u, x[k], 1);
-            e += 1;
-        }
-    } else {
-        uint32_t br = sticky ? 1u : 0u;  // the exact subtrahend exceeds the truncated one
-#pragma unroll
-        for (int k = 0; k <= L; k++) {
-            uint64_t t = (uint64_t)x[k] - y[k] - br;
-            x[k] = (uint32_t)t;
-            br = (uint32_t)(t >> 63);
-        }
-        uint32_t lz = lead_zeros<L + 1>(x);
-        if (lz >= 32u * (L + 1)) return zero<L>();
-        shl<L + 1>(x, lz);
-        e -= (int32_t)lz;
+    for (int k = 0; k <= L + 1; k++) {
+        const uint64_t t = (uint64_t)W[k] + ((k <= L ? y[k] : 0u) ^ neg) + c;
+        W[k] = (uint32_t)t;
+        c = (uint32_t)(t >> 32);
     }
+    const uint32_t lz = lead_zeros<L + 2>(W);
+    if (lz >= 32u * (L + 2)) return zero<L>();
+    shl<L + 2>(W, lz);
+    uint32_t x[L + 1];
+#pragma unroll
+    for (int k = 0; k <= L; k++) x[k] = W[k + 1];
     mp<L> r;
-    round_store<L>(r, x, e, a.s);
+    round_store<L>(r, x, ex + 32 - (int32_t)lz, sx);
     return r;
 }
 
@@ -347,23 +411,21 @@ MPX_HD void mant_mul(const uint32_t (&a)[L], const uint32_t (&b)[L], uint32_t (&
     col_carry<NC>(lo, hi, P + S0);
 }
 
+// a * b, rounded to nearest; one code path (a zero operand selects zero at the end)
 template <int L>
 MPX_HD mp<L> mul(const mp<L> &a, const mp<L> &b) {
-    if (is_zero(a) || is_zero(b)) return zero<L>();
     uint32_t P[2 * L];
     mant_mul<L>(a.m, b.m, P);
     uint32_t x[L + 1];
 #pragma unroll
     for (int k = 0; k <= L; k++) x[k] = P[L - 1 + k];
-    int32_t e = a.e + b.e;
-    if (!(x[L] & 0x80000000u)) {
+    // the product of two normalised mantissas has its top bit at 2^(64L-1) or 2^(64L-2)
+    const bool top = (x[L] & 0x80000000u) != 0;
 #pragma unroll
-        for (int k = L; k >= 0; k--) x[k] = funnel(x[k], k > 0 ? x[k - 1] : P[L - 2 >= 0 ? L - 2 : 0], 31);
-        e -= 1;
-    }
+    for (int k = L; k >= 0; k--) x[k] = top ? x[k] : funnel(x[k], k > 0 ? x[k - 1] : P[L - 2 >= 0 ? L - 2 : 0], 31);
     mp<L> r;
-    round_store<L>(r, x, e, a.s ^ b.s);
-    return r;
+    round_store<L>(r, x, a.e + b.e - (top ? 0 : 1), a.s ^ b.s);
+    return (is_zero(a) || is_zero(b)) ? zero<L>() : r;
 }
 
 // value * 2^k

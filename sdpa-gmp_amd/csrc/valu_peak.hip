@@ -15,14 +15,8 @@ __global__ __launch_bounds__(256) void k_peak(uint32_t *out, int iters, uint32_t
     uint64_t l0 = 0, l1 = 0, l2 = 0, l3 = 0, l4 = 0, l5 = 0, l6 = 0, l7 = 0;
     uint32_t h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0, h5 = 0, h6 = 0, h7 = 0;
     for (int i = 0; i < iters; i++) {
-        mpx::mac(l0, h0, a0, b0);
-        mpx::mac(l1, h1, a1, b0);
-        mpx::mac(l2, h2, a2, b0);
-        mpx::mac(l3, h3, a3, b0);
-        mpx::mac(l4, h4, a0, b1);
-        mpx::mac(l5, h5, a1, b1);
-        mpx::mac(l6, h6, a2, b1);
-        mpx::mac(l7, h7, a3, b1);
+        mpx::mac4(l0, h0, a0, b0, l1, h1, a1, b0, l2, h2, a2, b0, l3, h3, a3, b0);
+        mpx::mac4(l4, h4, a0, b1, l5, h5, a1, b1, l6, h6, a2, b1, l7, h7, a3, b1);
     }
     uint64_t s = l0 ^ l1 ^ l2 ^ l3 ^ l4 ^ l5 ^ l6 ^ l7;
     uint32_t hs = h0 + h1 + h2 + h3 + h4 + h5 + h6 + h7;
