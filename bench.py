@@ -10,7 +10,9 @@ For each GEMM form the solver first runs to convergence (untimed here; its per-i
 times over the whole trajectory are reported), then a second run stops at iteration `--at`
 and a "step" = one Make_bMat + Cholesky (sdpg_schur + sdpg_potrf) on that iterate, resident in
 HBM, is run `--warmup` times untimed and `--steps` times timed.  value = wall seconds per step of
-the Ozaki form.
+the Ozaki form, timed with nothing but the step's own kernels in the stream; the per-kernel times
+(events around every product and diagonal-block launch) come from a second pass over the same
+number of steps.
 
 roofline: the Ozaki MFMA product kernel, INT8 operations = 2 * S(S+1)/2 per mp multiply-add of the
 reference loop (S slices), over its event-timed average launch duration in the timed steps, against

@@ -39,6 +39,26 @@ constexpr int OZ_T = 32;   // output tile edge of one workgroup (2 x 2 waves of 
 constexpr int OZ_KC = 16;  // k per chunk (one 16-byte operand per lane and slice)
 constexpr int OZ_LONGK_MIN = 64;  // K above which the long-K kernel (k_gemm_ozk) is used
 
+// Balanced base-256 digits of x in the block fixed point of its row: Y = round(x * 2^(8S - 2 - emax))
+// (|Y| < 2^(8S-2), two's complement over L limbs) = sum_u d_u 256^u, d_u in [-128, 127]; the digit of
+// slice p is d_(S-1-p).  emax is the row's largest exponent (the zero exponent for an all-zero row).
+// Z = Y + sum_u 128 * 256^u lies in [0, 2^(8S)) and its plain base-256 digits are d_u + 128 (the
+// balanced representation is unique), so one multi-word addition and an XOR with 0x80 per byte give
+// every digit: byte u % 4 of zw[u / 4] is d_u as a signed byte.
+template <int L, int S>
+__device__ __forceinline__ void oz_digit_words(const mpx::mp<L> &x, int32_t emax, uint32_t (&zw)[S / 4]) {
+    if (emax == mpx::EZERO) emax = 0;
+    uint32_t a[L];
+    fx_convert<L>(x, emax, 32 * L - 8 * S + 2, a, false);
+    uint32_t c = 0;
+#pragma unroll
+    for (int t = 0; t < S / 4; t++) {
+        const uint64_t v = (uint64_t)a[t] + 0x80808080u + c;
+        zw[t] = (uint32_t)v ^ 0x80808080u;
+        c = (uint32_t)(v >> 32);
+    }
+}
+
 // slice planes: ((((b * nch + ch) * S + p) * Rp + r) * KC + kk), Rp = R rounded up to OZ_T; KC = k per
 // chunk (16 for the short-K kernel, 64 for the long-K kernel)
 template <int L, int S, int KC>
@@ -69,22 +89,12 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
         return;
     }
     PV src = which == 0 ? g.A : g.B;
-    int32_t emax = eo[(int64_t)b * R + r];
-    if (emax == mpx::EZERO) emax = 0;
-    uint32_t a[L];
-    // Y = round(x * 2^(8S - 2 - emax)) in two's complement over L limbs
-    fx_convert<L>(ldm<L>(src, fx_at(g, which, b, r, k)), emax, 32 * L - 8 * S + 2, a, false);
-    uint32_t carry = 0;
+    uint32_t zw[S / 4];
+    oz_digit_words<L, S>(ldm<L>(src, fx_at(g, which, b, r, k)), eo[(int64_t)b * R + r], zw);
 #pragma unroll
-    for (int u = 0; u < S; u++) {  // u = weight 256^u, digit index p = S-1-u
-        int v = (int)((a[u >> 2] >> (8 * (u & 3))) & 255u) + (int)carry;
-        int dg = v;
-        carry = 0;
-        if (v >= 128) {
-            dg = v - 256;
-            carry = 1;
-        }
-        o[(S - 1 - u) * pstride] = (int8_t)dg;
+    for (int p = 0; p < S; p++) {
+        const int u = S - 1 - p;
+        o[p * pstride] = (int8_t)(zw[u >> 2] >> (8 * (u & 3)));
     }
 }
 
@@ -259,7 +269,8 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
         }
         // every output is formed (padded rows and columns from zero slices and clamped C words), so the
         // four chains interleave; only the stores are predicated
-        if (i < g.M && j < g.N && !(g.lower && i < j)) stm<L>(g.C, g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j, res);
+        if (i < g.M && j < g.N && !(g.lower && i < j))
+            stm<L>(g.C, g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j, res);
     }
 }
 
