@@ -350,9 +350,9 @@ __device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, 
 //     dense A_j : A_j . G_i with G_i = X F_i
 // F3: Newton::calF3 (sdpa_newton.cpp:939-1005)
 template <int L>
-__global__ void k_pairs(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc, const int32_t *ri,
-                        const int32_t *ci, PV sv, PV dv, PV W, int64_t wstride, PV Fw, PV X, PV invZ, int64_t boff,
-                        PV B, int m) {
+__global__ __launch_bounds__(64) void k_pairs(int npairs, const Pair *pairs, int l, int nsdp, const Desc *desc,
+                                              const int32_t *ri, const int32_t *ci, PV sv, PV dv, PV W,
+                                              int64_t wstride, PV Fw, PV X, PV invZ, int64_t boff, PV B, int m) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npairs) return;
     Pair p = pairs[t];
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 // Xt_ii = c_i.  The true pivot is S_jj = d_j / c_j, so with r_j = 1/sqrt(c_j d_j), formed for
 // all j in parallel after the chain,
 //     L_ic = At_ic r_c,   inv(L)_ic = Xt_ic r_i,   1 / L_jj = c_j r_j.
-// Each step is one multiply-multiply-subtract per entry and one barrier; r_j is one fixed-point
+// Each step is one multiply-multiply-subtract per entry (rounded once) and one barrier; r_j is one fixed-point
 // Newton reciprocal square root (mpx::rsqrt_fx).  The reference divides each column by L_jj
 // after a square root per pivot (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:264-270); the
 // factor is the same up to rounding.  Since c_j > 0, d_j <= 0 exactly when the pivot S_jj <= 0:
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 template <int L>
 __device__ __forceinline__ mp<L> fms_scaled(const mp<L> &a, const mp<L> &b, const mp<L> &c, const mp<L> &d,
                                             int32_t e) {
-    mp<L> v = sub<L>(mul<L>(a, b), mul<L>(c, d));
+    mp<L> v = fms<L>(a, b, c, d);  // one rounding of a b - c d
     if (!is_zero<L>(v)) v.e -= e;
     return v;
 }

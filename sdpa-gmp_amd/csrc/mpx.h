@@ -428,6 +428,63 @@ MPX_HD mp<L> mul(const mp<L> &a, const mp<L> &b) {
     return (is_zero(a) || is_zero(b)) ? zero<L>() : r;
 }
 
+// a * b - c * d rounded once to nearest.  Both products keep their top L + 2 words (the low product
+// columns that mant_mul skips weigh below 2^-(32L+28) of the larger product), the one with the smaller
+// exponent is aligned to the other with a sticky bit, one two's-complement pass over L + 3 words forms
+// the sum or difference (negated when a difference of products with close exponents comes out
+// negative), and one normalising shift and one rounding give the result.  Compared with
+// sub(mul(a, b), mul(c, d)) this skips two normalisations and roundings, and under cancellation it
+// is more accurate (no rounding error of the products is amplified).
+template <int L>
+MPX_HD mp<L> fms(const mp<L> &a, const mp<L> &b, const mp<L> &c, const mp<L> &d) {
+    constexpr int N = L + 2;
+    uint32_t P[2 * L], Q[2 * L];
+    mant_mul<L>(a.m, b.m, P);
+    mant_mul<L>(c.m, d.m, Q);
+    // product value = (top N words) * 2^(e - 32N); a zero product orders by the zero exponent
+    const int32_t ep = (is_zero(a) || is_zero(b)) ? EZERO : a.e + b.e;
+    const int32_t eq = (is_zero(c) || is_zero(d)) ? EZERO : c.e + d.e;
+    const uint32_t sp = a.s ^ b.s, sq = c.s ^ d.s ^ 1u;
+    const bool sw = eq > ep;
+    const int32_t ex = sw ? eq : ep, ey = sw ? ep : eq;
+    const uint32_t sx = sw ? sq : sp, sy = sw ? sp : sq;
+    uint32_t W[N + 1], y[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        W[k] = sw ? Q[L - 2 + k] : P[L - 2 + k];
+        y[k] = sw ? P[L - 2 + k] : Q[L - 2 + k];
+    }
+    W[N] = 0;
+    uint32_t sticky = 0;
+    shr<N>(y, (uint32_t)(ex - ey), sticky);
+    const uint32_t neg = (sx != sy) ? 0xFFFFFFFFu : 0u;
+    uint32_t cy = neg ? (sticky ? 0u : 1u) : 0u;
+#pragma unroll
+    for (int k = 0; k <= N; k++) {
+        const uint64_t t = (uint64_t)W[k] + ((k < N ? y[k] : 0u) ^ neg) + cy;
+        W[k] = (uint32_t)t;
+        cy = (uint32_t)(t >> 32);
+    }
+    // magnitude and sign of the result
+    const uint32_t wneg = W[N] >> 31, wm = 0u - wneg;
+    cy = wneg;
+#pragma unroll
+    for (int k = 0; k <= N; k++) {
+        const uint64_t t = (uint64_t)(W[k] ^ wm) + cy;
+        W[k] = (uint32_t)t;
+        cy = (uint32_t)(t >> 32);
+    }
+    const uint32_t lz = lead_zeros<N + 1>(W);
+    if (lz >= 32u * (N + 1)) return zero<L>();
+    shl<N + 1>(W, lz);
+    uint32_t x[L + 1];
+#pragma unroll
+    for (int k = 0; k <= L; k++) x[k] = W[k + 2];
+    mp<L> r;
+    round_store<L>(r, x, ex + 32 - (int32_t)lz, sx ^ wneg);
+    return r;
+}
+
 // value * 2^k
 template <int L>
 MPX_HD mp<L> ldexp2(mp<L> a, int k) {

@@ -40,6 +40,11 @@ static int op_L(int op, const unsigned *a, const unsigned *b, unsigned *r) {
     return 0;
 }
 template <int L>
+static int fms_L(const unsigned *a, const unsigned *b, const unsigned *c, const unsigned *d, unsigned *r) {
+    st<L>(mpx::fms<L>(ld<L>(a), ld<L>(b), ld<L>(c), ld<L>(d)), r);
+    return 0;
+}
+template <int L>
 static int parse_L(const char *s, unsigned *r) {
     mpx::mp<L> x;
     if (mpx::parse_decimal<L>(s, x) <= 0) return -1;
@@ -65,6 +70,13 @@ static int fmt_L(const unsigned *a, int digits, char *buf, int n) {
 
 extern "C" int mpx_probe_op(int L, int op, const unsigned *a, const unsigned *b, unsigned *r) {
 #define C_(LL) op_L<LL>(op, a, b, r)
+    DISPATCH(L, C_)
+#undef C_
+}
+// r = a * b - c * d (mpx::fms)
+extern "C" int mpx_probe_fms(int L, const unsigned *a, const unsigned *b, const unsigned *c, const unsigned *d,
+                             unsigned *r) {
+#define C_(LL) fms_L<LL>(a, b, c, d, r)
     DISPATCH(L, C_)
 #undef C_
 }

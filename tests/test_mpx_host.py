@@ -91,3 +91,43 @@ def test_ops_against_exact(L):
                 assert abs(got * got - a) <= 2 * bars[op] * ulp * got, (op,)
             else:  # rsqrt: |got^2 a - 1| <= 2 * bar * ulp / got
                 assert abs(got * got * a - 1) <= 2 * bars[op] * ulp / got, (op, float((got * got * a - 1) * got / ulp))
+
+
+@pytest.mark.parametrize("L", [4, 10, 16])
+def test_fms_against_exact(L):
+    """mpx::fms, a*b - c*d with one rounding (the Bareiss update of the Cholesky diagonal blocks):
+    within half an ulp of the exact value plus the products' truncated low columns, which weigh below
+    2^-(32L+28) of the larger product; exact zero when the products cancel exactly."""
+    lib = _lib()
+    lib.mpx_probe_fms.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_uint32)] * 5
+    rnd = random.Random(100 + L)
+    A = ctypes.c_uint32 * (L + 2)
+
+    def rand_mp(spread=40):
+        return (Fraction(rnd.getrandbits(32 * L) | 1 << (32 * L - 1), 1 << (32 * L))
+                * Fraction(2) ** rnd.randint(-spread, spread) * (-1 if rnd.random() < 0.5 else 1))
+
+    for t in range(400):
+        a, b, c, d = rand_mp(), rand_mp(), rand_mp(), rand_mp()
+        kind = t % 5
+        if kind == 1:  # exact cancellation
+            c, d = a, b
+        elif kind == 2:  # near cancellation: c d = a b (1 + tiny)
+            c = a * (1 + Fraction(rnd.randint(1, 99), 1 << rnd.randint(8, 32 * L - 8)))
+            d = b
+        elif kind == 3:  # a zero product on either side
+            if rnd.random() < 0.5:
+                a = Fraction(0)
+            else:
+                d = Fraction(0)
+        w = [_words(x, L) for x in (a, b, c, d)]
+        a, b, c, d = (_val(x, L)[0] for x in w)
+        r = A()
+        assert lib.mpx_probe_fms(L, *(A(*x) for x in w), r) == 0
+        got, e = _val(list(r), L)
+        ex = a * b - c * d
+        if ex == 0:
+            assert got == 0, t
+            continue
+        bar = Fraction(2) ** (e - 32 * L) / 2 + (abs(a * b) + abs(c * d)) * Fraction(2) ** -(32 * L + 28)
+        assert abs(got - ex) <= bar, (t, kind, float((got - ex) / Fraction(2) ** (e - 32 * L)))
