@@ -221,9 +221,10 @@ def main():
     if dom == "chol_diag":
         n, ms, work = kst["chol_diag"]
         out["step_breakdown"]["dominant_kernel"] = {
-            "kernel": "k_potrf_diag (32 x 32 diagonal blocks of the Cholesky, one workgroup each)",
-            "bound": "latency: a serial chain of m pivots, one multiply-multiply-subtract in 320-bit limb "
-                     "arithmetic per pivot (one CU)",
+            "kernel": "k_potrf_diag (32 x 32 diagonal blocks of the Cholesky, 8 workgroups each: the pivot "
+                      "chain in every one, the inverse's columns split over them)",
+            "bound": "VALU issue of one CU per pivot step, a serial chain of m pivots: one fused "
+                     "multiply-multiply-subtract in 320-bit limb arithmetic per entry, in rounds of 256 entries",
             "avg_launch_us": 1e3 * ms / max(n, 1), "us_per_pivot": 1e3 * ms / max(n, 1) / 32}
     for f in res.values():
         f.pop("_kst", None)

@@ -128,7 +128,7 @@ __device__ __forceinline__ void lds_st(uint32_t *sh, int plane, int t, const mp<
 
 template <int L>
 __device__ __forceinline__ mp<L> fma_acc(const mp<L> &acc, const mp<L> &a, const mp<L> &b) {
-    return mpx::add<L>(acc, mpx::mul<L>(a, b));
+    return mpx::add_z<L>(acc, mpx::mul<L>(a, b));
 }
 
 }  // namespace sdpg

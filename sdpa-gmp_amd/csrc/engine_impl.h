@@ -313,10 +313,10 @@ struct GvPair {
 template <int L>
 __device__ __forceinline__ void bacc(PV B, int m, int i, int j, const mp<L> &v) {
     int64_t a = i + (int64_t)m * j;
-    stm<L>(B, a, add<L>(ldm<L>(B, a), v));
+    stm<L>(B, a, add_z<L>(ldm<L>(B, a), v));
     if (i != j) {
         int64_t b = j + (int64_t)m * i;
-        stm<L>(B, b, add<L>(ldm<L>(B, b), v));
+        stm<L>(B, b, add_z<L>(ldm<L>(B, b), v));
     }
 }
 
@@ -336,7 +336,7 @@ __device__ mp<L> blk_inner(const Desc &d, const int32_t *ri, const int32_t *ci, 
                 temp = add<L>(ldm<L>(D, doff + i + (int64_t)d.n * j), ldm<L>(D, doff + j + (int64_t)d.n * i));
                 temp = mul<L>(temp, v);
             }
-            ret = add<L>(ret, temp);
+            ret = add_z<L>(ret, temp);
         }
     } else if (d.kind == 2) {
         int64_t nn = (int64_t)d.n * d.n;
@@ -370,13 +370,13 @@ __global__ __launch_bounds__(64) void k_pairs(int npairs, const Pair *pairs, int
             mp<L> v2 = zero<L>();
             for (int k = 0; k < n; k++)
                 v2 = fma_acc<L>(v2, ldm<L>(X, boff + alpha + (int64_t)n * k), ldm<L>(Fw, foff + k + (int64_t)n * beta));
-            value = add<L>(value, mul<L>(v1, v2));
+            value = add_z<L>(value, mul<L>(v1, v2));
             if (alpha != beta) {
                 v2 = zero<L>();
                 for (int k = 0; k < n; k++)
                     v2 = fma_acc<L>(v2, ldm<L>(X, boff + beta + (int64_t)n * k),
                                     ldm<L>(Fw, foff + k + (int64_t)n * alpha));
-                value = add<L>(value, mul<L>(v1, v2));
+                value = add_z<L>(value, mul<L>(v1, v2));
             }
         }
     } else {
@@ -393,14 +393,14 @@ __global__ __launch_bounds__(64) void k_pairs(int npairs, const Pair *pairs, int
                 mp<L> v2 = ldm<L>(sv, e2);
                 mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * beta)),
                                    ldm<L>(X, boff + alpha + (int64_t)n * gamma));
-                sum = add<L>(sum, plu);
+                sum = add_z<L>(sum, plu);
                 if (gamma != delta) {
                     plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * beta)),
                                  ldm<L>(X, boff + alpha + (int64_t)n * delta));
-                    sum = add<L>(sum, plu);
+                    sum = add_z<L>(sum, plu);
                 }
             }
-            local = add<L>(local, mul<L>(v1, sum));
+            local = add_z<L>(local, mul<L>(v1, sum));
             if (alpha == beta) continue;
             sum = zero<L>();
             for (int i2 = 0; i2 < di.nnz; i2++) {
@@ -409,14 +409,14 @@ __global__ __launch_bounds__(64) void k_pairs(int npairs, const Pair *pairs, int
                 mp<L> v2 = ldm<L>(sv, e2);
                 mp<L> plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + delta + (int64_t)n * alpha)),
                                    ldm<L>(X, boff + beta + (int64_t)n * gamma));
-                sum = add<L>(sum, plu);
+                sum = add_z<L>(sum, plu);
                 if (gamma != delta) {
                     plu = mul<L>(mul<L>(v2, ldm<L>(invZ, boff + gamma + (int64_t)n * alpha)),
                                  ldm<L>(X, boff + beta + (int64_t)n * delta));
-                    sum = add<L>(sum, plu);
+                    sum = add_z<L>(sum, plu);
                 }
             }
-            local = add<L>(local, mul<L>(v1, sum));
+            local = add_z<L>(local, mul<L>(v1, sum));
         }
         value = local;
     }

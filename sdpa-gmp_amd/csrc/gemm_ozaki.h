@@ -38,6 +38,8 @@ typedef int oz_v16i __attribute__((ext_vector_type(16)));
 constexpr int OZ_T = 32;   // output tile edge of one workgroup (2 x 2 waves of 16 x 16)
 constexpr int OZ_KC = 16;  // k per chunk (one 16-byte operand per lane and slice)
 constexpr int OZ_LONGK_MIN = 64;  // K above which the long-K kernel (k_gemm_ozk) is used
+constexpr int OZ_FOLD = 64;       // chunks of the short-K kernel between folds of its digit sums
+constexpr int OZ_QB = 8;          // column fragments per block of the short-K kernel's pair loop
 
 // Balanced base-256 digits of x in the block fixed point of its row: Y = round(x * 2^(8S - 2 - emax))
 // (|Y| < 2^(8S-2), two's complement over L limbs) = sum_u d_u 256^u, d_u in [-128, 127]; the digit of
@@ -154,6 +156,27 @@ __device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[
     stm<L>(g.C, cidx, res);
 }
 
+// Exact fold of the per-digit INT32 sums into the fixed-point accumulator of each output: sum_d T_d 256^-d
+// carried from d = S-1 up, digit d in byte 3 - d % 4 of word d / 4 of R, the carry out of d = 0 into
+// the integer part RI; the sums restart at zero.
+template <int S>
+__device__ __forceinline__ void oz_fold(oz_v4i (&acc)[S], uint32_t (&R)[4][S / 4], int64_t (&RI)[4]) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        int32_t c = 0;
+#pragma unroll
+        for (int d = S - 1; d >= 0; d--) {
+            const int sh = 8 * (3 - d % 4);
+            const int32_t v = acc[d][e] + c + (int32_t)((R[e][d / 4] >> sh) & 255u);
+            R[e][d / 4] = (R[e][d / 4] & ~(255u << sh)) | (((uint32_t)v & 255u) << sh);
+            c = v >> 8;
+        }
+        RI[e] += c;
+    }
+#pragma unroll
+    for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
+}
+
 // Short-K product kernel: one workgroup per 32 x 32 output tile, 2 x 2 waves of 16 x 16, the
 // slices of the tile's rows and columns staged in LDS chunk by chunk.
 // Pair schedule: the MFMA numbered (u, q') gives lane group g = lane / 16 the slice pair
@@ -162,8 +185,8 @@ __device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[
 // zero fragment).  Every pair with p + q < S occurs once (g = p mod 4, u = p / 4, q' = q + g).  A wave
 // therefore reads its S/4 row fragments (p = g mod 4) and each column fragment once per chunk, and
 // keeps all S digit sums across the chunks: a digit sum over one chunk is bounded by
-// S * 16 * 2^14 (|digits| <= 128), and K <= 64 gives at most 4 chunks, below 2^31 for S <= 48.
-// The digit carry runs once per output at the end.
+// S * 16 * 2^14 (|digits| <= 128) <= 2^23.6 for S <= 48, so the sums are folded into the exact
+// fixed-point accumulator every OZ_FOLD chunks (and once at the end: the only fold for K <= 64).
 template <int L, int S>
 __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__restrict__ as,
                                                  const int8_t *__restrict__ bs, const int32_t *__restrict__ ea,
@@ -171,7 +194,7 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
                                                  int nch) {
     constexpr int NW = S / 4, NU = (S + 3) / 4;
     static_assert(NW + 2 >= L + 1, "accumulator narrower than the result");
-    static_assert(S <= 48, "digit sums of four chunks must stay below 2^31");
+    static_assert(S <= 48, "digit sums of OZ_FOLD chunks must stay below 2^31");
     __shared__ oz_v4i lA[S * OZ_T], lB[S * OZ_T];
     const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T, b = blockIdx.z;
     if (g.lower && i0 + OZ_T - 1 < j0) return;  // tile strictly above the diagonal
@@ -185,13 +208,15 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
     const int i = i0 + ar, j0c = j0 + wc * 16 + 4 * grp;
     const int ic = i < g.M ? i : g.M - 1;
     const int32_t ei = ea[(int64_t)bA * g.M + ic];
+    // (for S > 40 the C words are read in the epilogue: the registers go to the S digit sums)
+    constexpr bool CPRE = S <= 40;
     int32_t ej[4];
     uint32_t craw[4][L + 1];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         const int jc = j0c + e < g.N ? j0c + e : g.N - 1;
         ej[e] = eb[(int64_t)bB * g.N + jc];
-        if (g.op != GEMM_SET) {
+        if (CPRE && g.op != GEMM_SET) {
             const int64_t ci = g.offC + (int64_t)b * g.sC + ic + (int64_t)g.ldc * jc;
 #pragma unroll
             for (int q = 0; q <= L; q++) craw[e][q] = g.C.p[(int64_t)q * g.C.n + ci];
@@ -200,8 +225,17 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
     oz_v4i acc[S];
 #pragma unroll
     for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
+    uint32_t R[4][NW];
+    int64_t RI[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        RI[e] = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) R[e][w] = 0;
+    }
     const oz_v4i *sa = (const oz_v4i *)as, *sb = (const oz_v4i *)bs;
     for (int ch = 0; ch < nch; ch++) {
+        if (ch > 0 && ch % OZ_FOLD == 0) oz_fold<S>(acc, R, RI);
         const oz_v4i *pa = sa + (((int64_t)bA * nch + ch) * S) * Mp + i0;
         const oz_v4i *pb = sb + (((int64_t)bB * nch + ch) * S) * Np + j0;
         constexpr int NST = (S * OZ_T + 255) / 256;  // staged fragments per thread and operand
@@ -229,37 +263,57 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
             const int p = 4 * u + grp;
             af[u] = lA[(p < S ? p : S - 1) * OZ_T + ar];
         }
+        // column fragments in blocks of OZ_QB, block b + 1 read while block b's MFMAs run; the scheduling
+        // fences keep the compiler from hoisting every block's reads (S fragments would not fit
+        // beside the S digit sums for the larger S)
+        constexpr int NQB = (S + OZ_QB - 1) / OZ_QB;
+        oz_v4i bcur[OZ_QB], bnxt[OZ_QB];
 #pragma unroll
-        for (int qq = 0; qq < S; qq++) {
-            const int q = qq - grp;
-            oz_v4i bv = lB[(q >= 0 ? q : 0) * OZ_T + bc];
-            if (q < 0) bv = oz_v4i{0, 0, 0, 0};
+        for (int x = 0; x < OZ_QB; x++) {
+            const int q = x - grp;
+            bcur[x] = lB[(q >= 0 && x < S ? q : 0) * OZ_T + bc];
+        }
 #pragma unroll
-            for (int u = 0; u < NU; u++) {
-                if (4 * u + qq < S)
-                    acc[4 * u + qq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv, af[u], acc[4 * u + qq], 0, 0, 0);
+        for (int qb = 0; qb < NQB; qb++) {
+            if (qb + 1 < NQB) {
+#pragma unroll
+                for (int x = 0; x < OZ_QB; x++) {
+                    const int qq = (qb + 1) * OZ_QB + x, q = qq - grp;
+                    bnxt[x] = lB[(q >= 0 && qq < S ? q : 0) * OZ_T + bc];
+                }
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int x = 0; x < OZ_QB; x++) {
+                const int qq = qb * OZ_QB + x;
+                if (qq >= S) break;
+                const oz_v4i bv = qq - grp >= 0 ? bcur[x] : oz_v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int u = 0; u < NU; u++) {
+                    if (4 * u + qq < S)
+                        acc[4 * u + qq] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv, af[u], acc[4 * u + qq], 0, 0, 0);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int x = 0; x < OZ_QB; x++) bcur[x] = bnxt[x];
         }
         __syncthreads();
     }
-    // sum_d T_d 256^-d carried from d = S-1 up: digit d goes to byte 3 - d % 4 of word d / 4 of R,
-    // the carry out of d = 0 is the integer part.  |T_d| < 2^25.2 and the carry stays below 2^18,
-    // so the carry runs in 32-bit arithmetic.
+    oz_fold<S>(acc, R, RI);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
+        // the four outputs interleave where registers allow; for the larger S one after another
+        if constexpr (S > 40) __builtin_amdgcn_sched_barrier(0);
         const int j = j0c + e;
-        uint32_t R[NW];
-#pragma unroll
-        for (int w = 0; w < NW; w++) R[w] = 0;
-        int32_t c = 0;
-#pragma unroll
-        for (int d = S - 1; d >= 0; d--) {
-            const int32_t v = acc[d][e] + c;
-            R[d / 4] |= ((uint32_t)v & 255u) << (8 * (3 - d % 4));
-            c = v >> 8;
-        }
-        mpx::mp<L> res = oz_value<L, S>(R, (int64_t)c, ei, ej[e]);
+        mpx::mp<L> res = oz_value<L, S>(R[e], RI[e], ei, ej[e]);
         if (g.op != GEMM_SET) {
+            if constexpr (!CPRE) {
+                const int jc = j < g.N ? j : g.N - 1;
+                const int64_t ci = g.offC + (int64_t)b * g.sC + ic + (int64_t)g.ldc * jc;
+#pragma unroll
+                for (int q = 0; q <= L; q++) craw[e][q] = g.C.p[(int64_t)q * g.C.n + ci];
+            }
             mpx::mp<L> cold;
 #pragma unroll
             for (int q = 0; q < L; q++) cold.m[q] = craw[e][q];

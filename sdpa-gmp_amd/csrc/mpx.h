@@ -372,6 +372,15 @@ MPX_HD mp<L> add(mp<L> a, mp<L> b) {
     return r;
 }
 
+// add() with an early exit when either operand is zero: for accumulations that start from zero, where
+// the exit is taken by whole waves at once
+template <int L>
+MPX_HD mp<L> add_z(const mp<L> &a, const mp<L> &b) {
+    if (is_zero(a)) return b;
+    if (is_zero(b)) return a;
+    return add<L>(a, b);
+}
+
 template <int L>
 MPX_HD mp<L> sub(const mp<L> &a, const mp<L> &b) {
     return add<L>(a, neg<L>(b));

@@ -157,3 +157,26 @@ def test_rgemm_rejects_bad_slices():
     A = _pack2([[Fraction(1)]])
     with pytest.raises(sdpg.SdpgError):
         sdpg.rgemm(PREC, A, A, None, gemm=sdpg.GEMM_OZAKI, slices=7)
+
+
+@pytest.mark.gpu
+def test_rgemm_ozaki_320bit_long_k():
+    """At 320 bits (L = 12, S = 44) every Ozaki product runs on the short-K kernel, whatever K; K = 1100
+    (69 chunks of 16 k) passes its fold of the INT32 digit sums into the exact accumulator, which
+    happens every 64 chunks.  Small integers: the exact product sum (no rounding anywhere)."""
+    prec, k = 320, 1100
+    LL = sdpg.limbs_for_bits(prec)
+    rnd = random.Random(23)
+    m, n = 33, 35
+    print(f"case rgemm-320 ozaki k={k}", flush=True)
+    Av = [[Fraction(rnd.randint(-1000, 1000)) for _ in range(k)] for _ in range(m)]
+    Bv = [[Fraction(rnd.randint(-1000, 1000)) for _ in range(n)] for _ in range(k)]
+    _, _, ex = _exact(Av, Bv, None, False, False, 0)
+
+    def pack(vals):
+        return np.array([[sdpg.frac_to_words(v, LL) for v in row] for row in vals], dtype=np.uint32)
+
+    got, _ = sdpg.rgemm(prec, pack(Av), pack(Bv), None, False, False, 0, sdpg.GEMM_OZAKI, 0)
+    for i in range(m):
+        for j in range(n):
+            assert sdpg.words_to_frac(got[i, j], LL) == ex[i][j], (i, j)
