@@ -187,10 +187,12 @@ int sdpg_sync(sdpg_ctx *ctx);
 /*
  * Per-kernel device time (no reference counterpart; measurement support):
  * on = 1 clears and starts recording events around each launch of the product
- * kernels, on = 0 stops.  out (may be NULL; n >= 12) receives, for kind k =
- * 0 Ozaki MFMA product, 1 plain limb product, 2 Cholesky diagonal block:
+ * and diagonal-block kernels, on = 0 stops.  out (may be NULL) receives, for each
+ * kind k with 3k + 2 < n (n = 12 for all four) = 0 Ozaki MFMA product (short K),
+ * 1 plain limb product, 2 Cholesky diagonal block, 3 Ozaki MFMA product (long K):
  * out[3k] launches, out[3k+1] milliseconds, out[3k+2] algorithmic work
- * (k = 0: INT8 operations; 1: mp multiply-adds; 2: mp multiply-adds).
+ * (k = 0, 3: INT8 operations; 1, 2: mp multiply-adds).  The events sit in the
+ * engine's stream between the kernels and lengthen the wall time a little.
  */
 int sdpg_profile(sdpg_ctx *ctx, int on, double *out, int n);
 /*
