@@ -100,6 +100,67 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
     }
 }
 
+// K <= 32 (at most two 16-k chunks, the Cholesky products): row exponents and slices in one pass, the
+// same values as k_fix_max followed by k_oz_slice.  A workgroup covers 32 rows: thread (row rl, k = kt)
+// reads its element (consecutive threads take consecutive rows, which are contiguous for column-major
+// operands), the row's largest exponent is reduced in LDS, each thread writes its S digits into an LDS
+// copy of the output block (per chunk and slice the block's 32 rows x 16 k are contiguous in the slice
+// planes), and the block leaves in 16-byte stores.  blockIdx.z = 1: the second operand (R2, Rp2, ...).
+template <int L, int S>
+__global__ __launch_bounds__(1024) void k_oz_fixslice(GemmArgs g, int which, int R, int Rp, int nch, int32_t *eo,
+                                                      int8_t *out, int R2 = 0, int Rp2 = 0, int32_t *eo2 = nullptr,
+                                                      int8_t *out2 = nullptr) {
+    static_assert(S % 4 == 0 && S <= 4 * L, "slice count");
+    if (blockIdx.z) {
+        which = 1;
+        R = R2;
+        Rp = Rp2;
+        eo = eo2;
+        out = out2;
+    }
+    const int r0 = blockIdx.x * 32, b = blockIdx.y;
+    if (r0 >= Rp) return;  // past this operand's rows (the grid covers the larger operand)
+    __shared__ int32_t red[32][33];
+    __shared__ uint4 sout[2 * S * 32];  // [ch][p][row][16 k] as 16-byte rows
+    const int rl = threadIdx.x & 31, kt = threadIdx.x >> 5, r = r0 + rl;
+    const bool live = r < R && kt < g.K;
+    const mpx::mp<L> x = live ? ldm<L>(which == 0 ? g.A : g.B, fx_at(g, which, b, r, kt)) : mpx::zero<L>();
+    red[rl][kt] = live ? x.e : mpx::EZERO;
+    __syncthreads();
+    if (kt == 0) {
+        int32_t e = mpx::EZERO;
+#pragma unroll 8
+        for (int q = 0; q < 32; q++) e = red[rl][q] > e ? red[rl][q] : e;
+        red[rl][32] = e;
+        if (r < R) eo[(int64_t)b * R + r] = e;
+    }
+    __syncthreads();
+    const int ch = kt >> 4, kk = kt & 15;
+    if (ch < nch) {
+        uint32_t zw[S / 4];
+        if (live) {
+            oz_digit_words<L, S>(x, red[rl][32], zw);
+        } else {
+#pragma unroll
+            for (int w = 0; w < S / 4; w++) zw[w] = 0;
+        }
+        int8_t *so = (int8_t *)sout;
+#pragma unroll
+        for (int p = 0; p < S; p++) {
+            const int u = S - 1 - p;
+            so[((ch * S + p) * 32 + rl) * 16 + kk] = (int8_t)(zw[u >> 2] >> (8 * (u & 3)));
+        }
+    }
+    __syncthreads();
+    // per (ch, p): 32 rows x 16 bytes at ((((b * nch + ch) * S + p) * Rp + r0) * 16
+    const int nvec = nch * S * 32;
+    for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
+        const int row = v & 31, cp = v >> 5;  // cp = ch * S + p
+        uint4 *dst = (uint4 *)(out + ((((int64_t)b * nch) * S + cp) * Rp + r0 + row) * 16);
+        *dst = sout[cp * 32 + row];
+    }
+}
+
 // Both kernels form the transposed tile, MFMA(B fragment, A fragment) = (A B)^T, so that lane l holds
 // C[i0 + l % 16, j0 + 4 (l / 16) + e]: the 16 lanes of a group touch 16 consecutive rows of a
 // column-major C (coalesced 64-byte accesses per limb plane in the read-modify-write of C).
@@ -547,8 +608,8 @@ struct OzakiWork {
     OpCache<int8_t> kept;  // slices and exponents of operands that never change
     template <int S>
     int run_s(const GemmArgs &g, hipStream_t st, int keep) {
-        // long K: digit sums kept across 64-k chunks (k_gemm_ozk); short K: 16-k chunks, four slice pairs
-        // per MFMA and a carry per chunk (k_gemm_oz)
+        // long K: one slice pair per MFMA over 64-k chunks (k_gemm_ozk); otherwise 16-k chunks, four slice
+        // pairs of one digit per MFMA (k_gemm_oz); both keep per-digit sums across chunks
         int Mp = (g.M + OZ_T - 1) / OZ_T * OZ_T, Np = (g.N + OZ_T - 1) / OZ_T * OZ_T;
         // the long-K kernel addresses one batch's slice planes with 32-bit buffer offsets
         const int64_t nch64 = (g.K + OZK_KC - 1) / OZK_KC;
@@ -591,6 +652,19 @@ struct OzakiWork {
             qb = eb;
         }
         int64_t sa_n = (int64_t)Mp * nch * KC, sb_n = (int64_t)Np * nch * KC;
+        if (!longk && g.K <= 32 && (fillA || fillB)) {
+            // exponents and slices in one pass (both operands in one launch when both are needed)
+            if (fillA && fillB && nbA == nbB) {
+                k_oz_fixslice<L, S><<<dim3((unsigned)(std::max(Mp, Np) / 32), nbA, 2), 1024, 0, st>>>(
+                    g, 0, g.M, Mp, nch, qa, pa, g.N, Np, qb, pb);
+            } else {
+                if (fillA)
+                    k_oz_fixslice<L, S><<<dim3((unsigned)(Mp / 32), nbA), 1024, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa);
+                if (fillB)
+                    k_oz_fixslice<L, S><<<dim3((unsigned)(Np / 32), nbB), 1024, 0, st>>>(g, 1, g.N, Np, nch, qb, pb);
+            }
+            fillA = fillB = false;
+        }
         if (fillA && fillB && nbA == nbB && fx_rcontig_host(g, 0) == fx_rcontig_host(g, 1)) {
             // both operands in one launch per pass
             dim3 gm = fix_max_grid(g, 0, std::max(g.M, g.N), nbA);

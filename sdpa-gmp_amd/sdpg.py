@@ -48,11 +48,11 @@ _lib = None
 
 
 def load(path: str | None = None):
-    """Load libsdpg.so (raises if it is not built)."""
+    """Load libsdpg.so (raises if it is not built); SDPG_LIB names another build of it."""
     global _lib
     if _lib is not None:
         return _lib
-    path = path or os.path.join(LIBDIR, "libsdpg.so")
+    path = path or os.environ.get("SDPG_LIB") or os.path.join(LIBDIR, "libsdpg.so")
     if not os.path.exists(path):
         raise SdpgError(f"{path} not built; run sdpa-gmp_amd/build.py")
     lib = ctypes.CDLL(path)
