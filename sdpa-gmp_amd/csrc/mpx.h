@@ -146,8 +146,8 @@ MPX_HD void mac4(uint64_t &lo0, uint32_t &hi0, uint32_t a0, uint32_t b0, uint64_
 #endif
 }
 
-// Column sums of a[0..N) x b[0..N) for the columns S0..2N-2 into acc[s - S0]: for each a[p] the
-// multiply-adds into the adjacent columns p + q, p + q + 1, ... go out in groups of four.
+// Column sums of a[0..N) x b[0..N) for the columns S0..2N-2 added into lo/hi[s - S0]: the products of
+// each a[p] go to the adjacent columns p + q, p + q + 1, ... in groups of four.
 template <int N, int S0>
 MPX_HD void col_products(const uint32_t (&a)[N], const uint32_t (&b)[N], uint64_t (&lo)[2 * N - 1 - S0],
                          uint32_t (&hi)[2 * N - 1 - S0]) {
@@ -157,6 +157,42 @@ MPX_HD void col_products(const uint32_t (&a)[N], const uint32_t (&b)[N], uint64_
 #pragma unroll
         for (int q = q0; q < N; q += 4) {
             const int c = p + q - S0, rem = N - q;
+            if (rem >= 4)
+                mac4(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1], lo[c + 2], hi[c + 2], a[p],
+                     b[q + 2], lo[c + 3], hi[c + 3], a[p], b[q + 3]);
+            else if (rem == 3)
+                mac3(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1], lo[c + 2], hi[c + 2], a[p],
+                     b[q + 2]);
+            else if (rem == 2)
+                mac2(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1]);
+            else
+                mac(lo[c], hi[c], a[p], b[q]);
+        }
+    }
+}
+
+// The same column sums written into lo/hi (no initialisation needed): each column starts with its
+// first product as a plain 64-bit product (it cannot carry), a[0] b[q] for the columns q < N and
+// a[p] b[N-1] for the columns p + N - 1 above; the other products accumulate as in col_products.
+template <int N, int S0>
+MPX_HD void col_products_set(const uint32_t (&a)[N], const uint32_t (&b)[N], uint64_t (&lo)[2 * N - 1 - S0],
+                             uint32_t (&hi)[2 * N - 1 - S0]) {
+#pragma unroll
+    for (int q = S0; q < N; q++) {
+        lo[q - S0] = (uint64_t)a[0] * b[q];
+        hi[q - S0] = 0;
+    }
+#pragma unroll
+    for (int p = 1; p < N; p++) {
+        lo[p + N - 1 - S0] = (uint64_t)a[p] * b[N - 1];
+        hi[p + N - 1 - S0] = 0;
+    }
+#pragma unroll
+    for (int p = 1; p < N; p++) {
+        const int q0 = S0 > p ? S0 - p : 0;
+#pragma unroll
+        for (int q = q0; q < N - 1; q += 4) {
+            const int c = p + q - S0, rem = N - 1 - q;
             if (rem >= 4)
                 mac4(lo[c], hi[c], a[p], b[q], lo[c + 1], hi[c + 1], a[p], b[q + 1], lo[c + 2], hi[c + 2], a[p],
                      b[q + 2], lo[c + 3], hi[c + 3], a[p], b[q + 3]);
@@ -409,12 +445,7 @@ MPX_HD void mant_mul(const uint32_t (&a)[L], const uint32_t (&b)[L], uint32_t (&
     constexpr int NC = 2 * L - 1 - S0;
     uint64_t lo[NC];
     uint32_t hi[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        lo[c] = 0;
-        hi[c] = 0;
-    }
-    col_products<L, S0>(a, b, lo, hi);
+    col_products_set<L, S0>(a, b, lo, hi);
 #pragma unroll
     for (int s = 0; s < S0; s++) P[s] = 0;
     col_carry<NC>(lo, hi, P + S0);
@@ -659,12 +690,7 @@ MPX_HD void fx2_mulhi(const uint32_t (&a)[N], const uint32_t (&b)[N], uint32_t (
     constexpr int NC = 2 * N - 1 - S0;
     uint64_t lo[NC];
     uint32_t hi[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        lo[c] = 0;
-        hi[c] = 0;
-    }
-    col_products<N, S0>(a, b, lo, hi);
+    col_products_set<N, S0>(a, b, lo, hi);
     uint32_t P[2 * N + 1];
 #pragma unroll
     for (int s = 0; s < S0; s++) P[s] = 0;
