@@ -469,7 +469,7 @@ MPX_HD mp<L> mul(const mp<L> &a, const mp<L> &b) {
 }
 
 // a * b - c * d rounded once to nearest.  Both products keep their top L + 2 words (the low product
-// columns that mant_mul skips weigh below 2^-(32L+28) of the larger product), the one with the smaller
+// columns skipped weigh below 2^-(32L+26) of the larger product for L = 10), the one with the smaller
 // exponent is aligned to the other with a sticky bit, one two's-complement pass over L + 3 words forms
 // the sum or difference (negated when a difference of products with close exponents comes out
 // negative), and one normalising shift and one rounding give the result.  Compared with
@@ -478,9 +478,20 @@ MPX_HD mp<L> mul(const mp<L> &a, const mp<L> &b) {
 template <int L>
 MPX_HD mp<L> fms(const mp<L> &a, const mp<L> &b, const mp<L> &c, const mp<L> &d) {
     constexpr int N = L + 2;
+    // product columns from L - 2 up (the lowest kept word); the skipped ones weigh below
+    // 2 (L - 2) 2^-(32L+30) of the product
+    constexpr int S0 = (L >= 2) ? L - 2 : 0, NC = 2 * L - 1 - S0;
     uint32_t P[2 * L], Q[2 * L];
-    mant_mul<L>(a.m, b.m, P);
-    mant_mul<L>(c.m, d.m, Q);
+    {
+        uint64_t lo[NC], lo2[NC];
+        uint32_t hi[NC], hi2[NC];
+        col_products_set<L, S0>(a.m, b.m, lo, hi);
+        col_products_set<L, S0>(c.m, d.m, lo2, hi2);
+#pragma unroll
+        for (int s = 0; s < S0; s++) P[s] = Q[s] = 0;
+        col_carry<NC>(lo, hi, P + S0);
+        col_carry<NC>(lo2, hi2, Q + S0);
+    }
     // product value = (top N words) * 2^(e - 32N); a zero product orders by the zero exponent
     const int32_t ep = (is_zero(a) || is_zero(b)) ? EZERO : a.e + b.e;
     const int32_t eq = (is_zero(c) || is_zero(d)) ? EZERO : c.e + d.e;

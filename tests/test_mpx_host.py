@@ -96,8 +96,9 @@ def test_ops_against_exact(L):
 @pytest.mark.parametrize("L", [4, 10, 16])
 def test_fms_against_exact(L):
     """mpx::fms, a*b - c*d with one rounding (the Bareiss update of the Cholesky diagonal blocks):
-    within half an ulp of the exact value plus the products' truncated low columns, which weigh below
-    2^-(32L+28) of the larger product; exact zero when the products cancel exactly."""
+    within half an ulp of the exact value plus the products' skipped low columns (below L - 2),
+    which weigh below 2 (L - 2) 2^-(32L+30) <= 2^-(32L+24) of the larger product for L <= 18; exact
+    zero when the products cancel exactly."""
     lib = _lib()
     lib.mpx_probe_fms.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_uint32)] * 5
     rnd = random.Random(100 + L)
@@ -129,5 +130,5 @@ def test_fms_against_exact(L):
         if ex == 0:
             assert got == 0, t
             continue
-        bar = Fraction(2) ** (e - 32 * L) / 2 + (abs(a * b) + abs(c * d)) * Fraction(2) ** -(32 * L + 28)
+        bar = Fraction(2) ** (e - 32 * L) / 2 + (abs(a * b) + abs(c * d)) * Fraction(2) ** -(32 * L + 24)
         assert abs(got - ex) <= bar, (t, kind, float((got - ex) / Fraction(2) ** (e - 32 * L)))
