@@ -3,17 +3,18 @@
 // 1. Block fixed point as in the plain form: row r of op(A) gets the exponent e_r of its
 //    largest element and x becomes the integer Y = round(x * 2^(8S - 2 - e_r)),
 //    |Y| < 2^(8S-2); the same for the columns of op(B).
-// 2. Slicing (k_oz_slice): Y = sum_{p<S} y_p 256^(S-1-p) with balanced digits
-//    y_p in [-128, 127] (y_0 in [-64, 64]).  The digits are stored as int8 planes
-//    s[chunk][p][row][KC]: KC consecutive k of one row for one slice.
+// 2. Slicing (k_oz_slice; k_oz_fixslice does exponents and slices in one pass for K <= 32):
+//    Y = sum_{p<S} y_p 256^(S-1-p) with balanced digits y_p in [-128, 127] (y_0 in [-64, 64]).
+//    The digits are stored as int8 planes s[chunk][p][row][KC]: KC consecutive k of one row for one
+//    slice.
 // 3. Products: for the output digit position d = p + q the slice products
 //    T_d = sum_{p+q=d} sum_k a_p[i,k] b_q[k,j] are exact INT32 sums.  Pairs with
 //    p + q >= S lie below 2^-(8S-8) of the row/column scale and are not formed
 //    (S(S+1)/2 slice products instead of S^2).  Two kernels:
-//    - k_gemm_oz (K <= 64, the Cholesky panels and trailing updates): KC = 16; every
-//      v_mfma_i32_16x16x64_i8 issues four (p, q) pairs of one d at once (lane group g = lane / 16
-//      carries pair (4u + g, q' - g), each over the same 16 k) into a per-digit INT32 accumulator
-//      that lives across the (at most 4) chunks;
+//    - k_gemm_oz (K <= 64, the Cholesky panels and trailing updates; every K when S > 40): KC = 16;
+//      every v_mfma_i32_16x16x64_i8 issues four (p, q) pairs of one d at once (lane group
+//      g = lane / 16 carries pair (4u + g, q' - g), each over the same 16 k) into a per-digit INT32
+//      accumulator that lives across chunks and is folded every 64 chunks;
 //    - k_gemm_ozk (long K: the F1 products and Schur inner products): KC = 64; one MFMA per
 //      (p, q) over 64 k into a per-digit INT32 accumulator that lives across chunks and is
 //      carried every 32 chunks.
