@@ -46,9 +46,11 @@ int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_overr
  * Replay benchmark of the hot path on one iterate of the solve: runs the solver
  * for `warm_iters` iterations, then times `steps` executions of
  * [Make_bMat + Cholesky] (sdpg_schur + sdpg_potrf) on that iterate after
- * `warmup` untimed ones.  Returns per-step device milliseconds of each phase.
- * kstats (may be NULL, else 12 doubles): sdpg_profile figures over the timed
- * steps.  dump_prefix (may be NULL): the iterate is written to
+ * `warmup` untimed ones, with nothing but the step's kernels in the stream.
+ * Returns per-step device milliseconds of each phase and the wall time.
+ * kstats (may be NULL, else 12 doubles): sdpg_profile figures over a second
+ * pass of `steps` steps with events around the product and diagonal-block
+ * kernels.  dump_prefix (may be NULL): the iterate is written to
  * <dump_prefix>X.bin and <dump_prefix>invZ.bin (packed mp, raw uint32) so that a
  * CPU implementation can time the same step on the same data.
  */
