@@ -461,6 +461,8 @@ __device__ __forceinline__ void ozk_carry(oz_v4i (&acc)[S], uint32_t (*sR)[64], 
 
 // s_waitcnt lgkmcnt(0) alone (vmcnt and expcnt at their maxima), gfx9 encoding
 constexpr int OZK_WAIT_LGKM0 = 0xC07F;
+// s_waitcnt vmcnt(n) alone (expcnt and lgkmcnt at their maxima), gfx9 encoding
+constexpr int ozk_wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 // column fragments q < OzkQL of the next chunk are prefetched into LDS (at most 28 KB per wave, so
 // that four waves fit in a CU's LDS with their accumulators); the others, first used by the last
 // batches of the descending order, are loaded at the start of their chunk
@@ -523,23 +525,31 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
     };
 #pragma unroll
     for (int q = 0; q < QL; q++) ldB_lds(q, q);
-    for (int ch = 0; ch < nch; ch++) {
+    // the row-fragment ring: slot k % PD holds the row fragment of batch k = S-1-p; the first PD of a
+    // chunk are requested during the last PD batches of the chunk before (chunk 0's here, after the
+    // column prefetch: at the top of every chunk only these PD loads follow the prefetch)
+    __builtin_amdgcn_sched_barrier(0);
+    oz_v4i ap[PD];
+#pragma unroll
+    for (int t = 0; t < PD; t++) ap[t] = ldA(S - 1 - t);
+    for (int cg = 0; cg < nch; cg += OZK_CARRY) {
+    const int cg1 = cg + OZK_CARRY < nch ? cg + OZK_CARRY : nch;
+    for (int ch = cg; ch < cg1; ch++) {
         const int c0 = ch * S;
-        oz_v4i bq[S], ap[PD];
+        oz_v4i bq[S];
         // the early column fragments were prefetched into LDS during the previous chunk (wait for those
-        // loads before this chunk issues its own, so that the wait covers nothing else)
-        __builtin_amdgcn_s_waitcnt(0);
-        // the late column fragments and the first row fragments from global memory
+        // loads alone, before this chunk issues its own)
+        __builtin_amdgcn_s_waitcnt(ozk_wait_vm(PD));
+        // the late column fragments from global memory
 #pragma unroll
         for (int q = QL; q < S; q++) bq[q] = ldB(c0 + q);
-#pragma unroll
-        for (int t = 0; t < PD; t++) ap[t] = ldA(c0 + S - 1 - t);
 #pragma unroll
         for (int q = 0; q < OZK_LA && q < QL; q++) bq[q] = sB[q][lane];
         // batches by descending row slice p (batch p forms the pairs (p, q <= S-1-p), first using column
         // fragment S-1-p); column fragments come from LDS OZK_LA batches before their first use, and
         // once the last one is read the next chunk's prefetch into LDS starts; the row fragments
-        // stream through a ring of PD registers, each reloaded PD batches ahead
+        // stream through the ring, each requested PD batches ahead (the last PD batches request the
+        // next chunk's first PD; the last chunk re-reads one of its own planes instead)
 #pragma unroll
         for (int p = S - 1; p >= 0; p--) {
             {
@@ -553,12 +563,19 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
             }
             const int t = (S - 1 - p) % PD;
             const oz_v4i a = ap[t];
-            if (p - PD >= 0) ap[t] = ldA(c0 + p - PD);
+            if (p - PD >= 0)
+                ap[t] = ldA(c0 + p - PD);
+            else
+                ap[t] = ldA(ch + 1 < nch ? c0 + S + (S - 1 - t) : c0);
+            // keep the request here: left to itself the scheduler sinks every load next to its first use
+            // (the register file is nearly full), and the ring's lookahead is lost
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q <= S - 1 - p; q++)
                 acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bq[q], a, acc[p + q], 0, 0, 0);
         }
-        if ((ch % OZK_CARRY) == OZK_CARRY - 1 || ch == nch - 1) ozk_carry<S>(acc, sR, lane);
+    }
+    ozk_carry<S>(acc, sR, lane);
     }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
