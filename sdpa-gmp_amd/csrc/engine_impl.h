@@ -427,13 +427,19 @@ __global__ __launch_bounds__(64) void k_pairs(int npairs, const Pair *pairs, int
 // t = b n - b (b - 1) / 2 + (a - b) for a >= b): dst[t] = src[a, b] (sum = 0), or
 // src[a, a] on the diagonal and src[a, b] + src[b, a] below it (sum = 1).  For symmetric A,
 // A . G = sum_t packA[t] packG[t] with the second form for G, which halves the inner-product length.
-// One workgroup per 16 x 16 tile (a-block >= b-block) of one matrix: the tile and its mirror are
-// read with coalesced loads into LDS, so the transposed operand costs no strided HBM reads.
-constexpr int SYMP_T = 16;
+// One workgroup per T x T tile (a-block >= b-block) of one matrix: every thread reads its own element
+// (a row of the tile is T consecutive words of each limb plane), and the mirror tile, read the same
+// way, is transposed through LDS, so the transposed operand costs no strided HBM reads.  T = 32
+// (128-byte rows) while the mirror tile fits 64 KB of LDS, else 16.
 template <int L>
-__global__ __launch_bounds__(256) void k_sym_pack(PV src, int64_t soff, int64_t sstride, PV dst, int64_t doff,
-                                                  int64_t dstride, int n, int sum) {
-    __shared__ uint32_t t1[L + 1][SYMP_T][SYMP_T + 1], t2[L + 1][SYMP_T][SYMP_T + 1];
+struct SymPackTile {
+    static constexpr int T = (L + 1) * 32 * 33 * 4 <= 65536 ? 32 : 16;
+};
+template <int L>
+__global__ __launch_bounds__(SymPackTile<L>::T *SymPackTile<L>::T) void k_sym_pack(
+    PV src, int64_t soff, int64_t sstride, PV dst, int64_t doff, int64_t dstride, int n, int sum) {
+    constexpr int T = SymPackTile<L>::T;
+    __shared__ uint32_t t2[L + 1][T][T + 1];
     // lower tile (ia, ib), ia >= ib, from the linear tile index
     int r = blockIdx.x, ia = 0;
     while (r > ia) {
@@ -441,36 +447,24 @@ __global__ __launch_bounds__(256) void k_sym_pack(PV src, int64_t soff, int64_t 
         ia++;
     }
     const int ib = r;
-    const int c = blockIdx.y, tx = threadIdx.x % SYMP_T, ty = threadIdx.x / SYMP_T;
+    const int c = blockIdx.y, tx = threadIdx.x % T, ty = threadIdx.x / T;
     const int64_t so = soff + (int64_t)c * sstride;
-    const int a0 = ia * SYMP_T, b0 = ib * SYMP_T;
-    {
-        int a = a0 + tx, b = b0 + ty;  // t1[ty][tx] = src[a, b]
-        if (a < n && b < n) {
-            mp<L> v = ldm<L>(src, so + a + (int64_t)n * b);
-#pragma unroll
-            for (int q = 0; q < L; q++) t1[q][ty][tx] = v.m[q];
-            t1[L][ty][tx] = (uint32_t)((v.e << 1) | (int32_t)(v.s & 1u));
-        }
-        int bb = b0 + tx, aa = a0 + ty;  // t2[ty][tx] = src[b0 + tx, a0 + ty]
-        if (sum && aa < n && bb < n) {
-            mp<L> v = ldm<L>(src, so + bb + (int64_t)n * aa);
-#pragma unroll
-            for (int q = 0; q < L; q++) t2[q][ty][tx] = v.m[q];
-            t2[L][ty][tx] = (uint32_t)((v.e << 1) | (int32_t)(v.s & 1u));
-        }
-    }
-    __syncthreads();
+    const int a0 = ia * T, b0 = ib * T;
     const int a = a0 + tx, b = b0 + ty;
-    if (a >= n || b >= n || a < b) return;
-    mp<L> v;
+    const bool in = a < n && b < n;
+    mp<L> v = zero<L>();
+    if (in) v = ldm<L>(src, so + a + (int64_t)n * b);
+    if (sum) {
+        const int bb = b0 + tx, aa = a0 + ty;  // t2[ty][tx] = src[b0 + tx, a0 + ty]
+        if (aa < n && bb < n) {
+            mp<L> w = ldm<L>(src, so + bb + (int64_t)n * aa);
 #pragma unroll
-    for (int q = 0; q < L; q++) v.m[q] = t1[q][ty][tx];
-    {
-        int32_t h = (int32_t)t1[L][ty][tx];
-        v.e = h >> 1;
-        v.s = (uint32_t)h & 1u;
+            for (int q = 0; q < L; q++) t2[q][ty][tx] = w.m[q];
+            t2[L][ty][tx] = (uint32_t)((w.e << 1) | (int32_t)(w.s & 1u));
+        }
+        __syncthreads();
     }
+    if (!in || a < b) return;
     if (sum && a != b) {  // src[b, a] = t2[tx][ty]
         mp<L> w;
 #pragma unroll
@@ -484,10 +478,16 @@ __global__ __launch_bounds__(256) void k_sym_pack(PV src, int64_t soff, int64_t 
     stm<L>(dst, doff + (int64_t)c * dstride + pt, v);
 }
 
-// launch shape of k_sym_pack for n x n matrices: lower 16 x 16 tiles
+// launch shape of k_sym_pack for n x n matrices: lower T x T tiles of T^2 threads
+template <int L>
 inline unsigned sym_pack_tiles(int n) {
-    unsigned nb = (unsigned)((n + SYMP_T - 1) / SYMP_T);
+    constexpr int T = SymPackTile<L>::T;
+    unsigned nb = (unsigned)((n + T - 1) / T);
     return nb * (nb + 1) / 2;
+}
+template <int L>
+constexpr int sym_pack_threads() {
+    return SymPackTile<L>::T * SymPackTile<L>::T;
 }
 
 // dense-partner F1/F2 pairs: B[i,j] += Cs[idx] (and the mirror)
@@ -1500,7 +1500,7 @@ struct Engine : EngineBase {
                     const BlockPlan &P = plan[l];
                     if (!P.npairs_gv) continue;
                     int n = sizes[l];
-                    k_sym_pack<L><<<dim3(sym_pack_tiles(n), P.gv_N), 256, 0, st>>>(
+                    k_sym_pack<L><<<dim3(sym_pack_tiles<L>(n), P.gv_N), sym_pack_threads<L>(), 0, st>>>(
                         dval.pv(), P.gv_boff, (int64_t)n * n, dpk.pv(), P.gv_pkoff, (int64_t)n * (n + 1) / 2, n, 0);
                 }
                 HIPCHK(hipGetLastError());
@@ -1704,8 +1704,8 @@ struct Engine : EngineBase {
                 // inner products with the dense partners as one GEMM over packed lower triangles (F is no
                 // longer needed, its slots take the packed G):  Cs[s, r] = packG_s . packA_r
                 const int np = n * (n + 1) / 2;
-                k_sym_pack<L><<<dim3(sym_pack_tiles(n), P.gv_M), 256, 0, st>>>(Gw.pv(), 0, wstride, Fw.pv(), 0,
-                                                                              wstride, n, 1);
+                k_sym_pack<L><<<dim3(sym_pack_tiles<L>(n), P.gv_M), sym_pack_threads<L>(), 0, st>>>(
+                    Gw.pv(), 0, wstride, Fw.pv(), 0, wstride, n, 1);
                 if ((rc = gemm.gemm(P.gv_M, P.gv_N, np, Fw.pv(), 0, (int)wstride, true, dpk.pv(), P.gv_pkoff, np, false,
                                     Csw.pv(), 0, P.gv_M, GEMM_SET, 1, 0, 0, 0, gemm_mode, gemm_slices, P.gv_lower,
                                     GEMM_KEEP_B)))

@@ -100,8 +100,9 @@ FORM_IDS = ["plain", "element", "ozaki", "ozaki-wide"]
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
 @pytest.mark.parametrize("shape", [(70, 67, 37, False, True, 0), (33, 129, 16, True, False, 1),
                                    (1, 5, 1, False, False, 2), (96, 96, 32, False, True, 1),
-                                   (50, 40, 300, False, True, 1), (17, 20, 2100, True, False, 2)],
-                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub", "NT-sub-K300", "TN-add-K2100"])
+                                   (50, 40, 300, False, True, 1), (17, 20, 2100, True, False, 2),
+                                   (40, 36, 150, True, True, 1)],
+                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub", "NT-sub-K300", "TN-add-K2100", "TT-sub-K150"])
 def test_rgemm_random(form, shape):
     mode, slices, bits_lost = form
     m, n, k, ta, tb, op = shape
@@ -136,9 +137,10 @@ def test_rgemm_lower_update(form, k):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
-@pytest.mark.parametrize("k", [40, 300], ids=["K40", "K300"])
+@pytest.mark.parametrize("k", [40, 70, 150, 176, 300], ids=["K40", "K70", "K150", "K176", "K300"])
 def test_rgemm_small_integers_exact(form, k):
-    """K = 40 runs the short-K Ozaki kernel, K = 300 the long-K one (digit sums kept across chunks)"""
+    """K = 40 runs the short-K Ozaki kernel, the others the long-K one (digit sums kept across chunks);
+    K = 70, 150, 176 and 300 end in a chunk of 6, 22, 48 and 44 k (zero padding to 64 k)"""
     mode, slices, _ = form
     rnd = random.Random(11 + k)
     m, n = 65, 70
