@@ -532,9 +532,8 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
     oz_v4i ap[PD];
 #pragma unroll
     for (int t = 0; t < PD; t++) ap[t] = ldA(S - 1 - t);
-    for (int cg = 0; cg < nch; cg += OZK_CARRY) {
-    const int cg1 = cg + OZK_CARRY < nch ? cg + OZK_CARRY : nch;
-    for (int ch = cg; ch < cg1; ch++) {
+    // one chunk of 64 k
+    auto run_chunk = [&](int ch) {
         const int c0 = ch * S;
         oz_v4i bq[S];
         // the early column fragments were prefetched into LDS during the previous chunk (wait for those
@@ -574,8 +573,18 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
             for (int q = 0; q <= S - 1 - p; q++)
                 acc[p + q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bq[q], a, acc[p + q], 0, 0, 0);
         }
-    }
-    ozk_carry<S>(acc, sR, lane);
+    };
+    // chunks in carry groups of OZK_CARRY, two per loop iteration: with one, the allocator renames the
+    // digit accumulators between the loop's entry and its back edge and moves all of them every chunk
+    for (int cg = 0; cg < nch; cg += OZK_CARRY) {
+        const int cg1 = cg + OZK_CARRY < nch ? cg + OZK_CARRY : nch;
+        int ch = cg;
+        for (; ch + 1 < cg1; ch += 2) {
+            run_chunk(ch);
+            run_chunk(ch + 1);
+        }
+        if (ch < cg1) run_chunk(ch);
+        ozk_carry<S>(acc, sR, lane);
     }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
