@@ -203,21 +203,6 @@ __device__ __forceinline__ mpx::mp<L> oz_value(const uint32_t (&R)[S / 4], int64
     return res;
 }
 
-template <int L, int S>
-__device__ __forceinline__ void oz_store(const GemmArgs &g, const uint32_t (&R)[S / 4], int64_t RI, int i, int j, int b,
-                                         int bA, int bB, const int32_t *__restrict__ ea,
-                                         const int32_t *__restrict__ eb) {
-    if (i >= g.M || j >= g.N) return;
-    if (g.lower && i < j) return;
-    mpx::mp<L> res = oz_value<L, S>(R, RI, ea[(int64_t)bA * g.M + i], eb[(int64_t)bB * g.N + j]);
-    int64_t cidx = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
-    if (g.op == 1)
-        res = mpx::sub<L>(ldm<L>(g.C, cidx), res);
-    else if (g.op == 2)
-        res = mpx::add<L>(ldm<L>(g.C, cidx), res);
-    stm<L>(g.C, cidx, res);
-}
-
 // Exact fold of the per-digit INT32 sums into the fixed-point accumulator of each output: sum_d T_d 256^-d
 // carried from d = S-1 up, digit d in byte 3 - d % 4 of word d / 4 of R, the carry out of d = 0 into
 // the integer part RI; the sums restart at zero.
@@ -586,13 +571,32 @@ __global__ __launch_bounds__(64) void k_gemm_ozk(GemmArgs g, const int8_t *__res
         if (ch < cg1) run_chunk(ch);
         ozk_carry<S>(acc, sR, lane);
     }
+    // the lane's row exponent and its four column exponents are requested together (clamped indices);
+    // every output is formed and only the stores (and for SUB / ADD the old C reads) are predicated
+    const int i = i0 + (lane & 15), ic = i < g.M ? i : g.M - 1;
+    const int32_t ei = ea[(int64_t)bA * g.M + ic];
+    int32_t ej[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int j = j0 + 4 * grp + e, jc = j < g.N ? j : g.N - 1;
+        ej[e] = eb[(int64_t)bB * g.N + jc];
+    }
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         uint32_t R[NW];
 #pragma unroll
         for (int q = 0; q < NW; q++) R[q] = sR[e * NW + q][lane];
         const int64_t RI = (int64_t)(((uint64_t)sR[4 * NW + 2 * e + 1][lane] << 32) | sR[4 * NW + 2 * e][lane]);
-        oz_store<L, S>(g, R, RI, i0 + (lane & 15), j0 + 4 * grp + e, b, bA, bB, ea, eb);
+        mpx::mp<L> res = oz_value<L, S>(R, RI, ei, ej[e]);
+        const int j = j0 + 4 * grp + e;
+        if (i < g.M && j < g.N && !(g.lower && i < j)) {
+            const int64_t ci = g.offC + (int64_t)b * g.sC + i + (int64_t)g.ldc * j;
+            if (g.op == GEMM_SUB)
+                res = mpx::sub<L>(ldm<L>(g.C, ci), res);
+            else if (g.op == GEMM_ADD)
+                res = mpx::add<L>(ldm<L>(g.C, ci), res);
+            stm<L>(g.C, ci, res);
+        }
     }
 }
 
