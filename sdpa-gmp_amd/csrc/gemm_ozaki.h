@@ -63,11 +63,14 @@ __device__ __forceinline__ void oz_digit_words(const mpx::mp<L> &x, int32_t emax
 }
 
 // slice planes: ((((b * nch + ch) * S + p) * Rp + r) * KC + kk), Rp = R rounded up to OZ_T; KC = k per
-// chunk (16 for the short-K kernel, 64 for the long-K kernel)
+// chunk (16 for the short-K kernel, 64 for the long-K kernel).  One element per thread (consecutive
+// threads take consecutive k, one coalesced read per limb plane); each group of four lanes (four
+// consecutive k of one row) transposes its digit words in registers, so that lane q holds the four
+// k-consecutive bytes of digits 4w + q, and writes 4 bytes per store: S/4 stores per lane.
 template <int L, int S, int KC>
 __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const int32_t *eo, int8_t *out, int R2 = 0,
                            int Rp2 = 0, const int32_t *eo2 = nullptr, int8_t *out2 = nullptr) {
-    static_assert(S % 4 == 0 && S <= 4 * L, "slice count");
+    static_assert(S % 4 == 0 && S <= 4 * L && KC % 4 == 0, "slice count");
     if (blockIdx.z) {  // both operands in one launch (op(B) in z = 1)
         which = 1;
         R = R2;
@@ -77,29 +80,40 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
     }
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = blockIdx.y;
-    const int Kp = nch * KC;
-    if (t >= (int64_t)Rp * Kp) return;
+    // whole waves leave here (Rp * KC is a multiple of 64), so every quad below is complete
+    if (t >= (int64_t)Rp * nch * KC) return;
     int kk = (int)(t % KC);
     int64_t rest = t / KC;
     int r = (int)(rest % Rp);
     int ch = (int)(rest / Rp);
     int k = ch * KC + kk;
-    int8_t *o = out + ((((int64_t)b * nch + ch) * S) * Rp + r) * KC + kk;
-    const int64_t pstride = (int64_t)Rp * KC;
-    if (r >= R || k >= g.K) {
-#pragma unroll
-        for (int p = 0; p < S; p++) o[p * pstride] = 0;
-        return;
-    }
-    PV src = which == 0 ? g.A : g.B;
     uint32_t zw[S / 4];
-    oz_digit_words<L, S>(ldm<L>(src, fx_at(g, which, b, r, k)), eo[(int64_t)b * R + r], zw);
+    if (r < R && k < g.K) {
+        PV src = which == 0 ? g.A : g.B;
+        oz_digit_words<L, S>(ldm<L>(src, fx_at(g, which, b, r, k)), eo[(int64_t)b * R + r], zw);
+    } else {
 #pragma unroll
-    for (int p = 0; p < S; p++) {
-        const int u = S - 1 - p;
-        o[p * pstride] = (int8_t)(zw[u >> 2] >> (8 * (u & 3)));
+        for (int w = 0; w < S / 4; w++) zw[w] = 0;
+    }
+    const int q = kk & 3;
+    int8_t *ob = out + ((((int64_t)b * nch + ch) * S) * Rp + r) * KC + (kk - q);
+    const int64_t pstride = (int64_t)Rp * KC;
+    // 4 x 4 byte transpose over the quad: lanes 0, 1 take the low halves of their own word and of
+    // lane q ^ 2's, lanes 2, 3 the high halves; then bytes 0 and 2 pair with lane q ^ 1's
+    const uint32_t sel1 = q < 2 ? 0x01000504u : 0x07060302u, sel2 = (q & 1) ? 0x07030501u : 0x02060004u;
+#pragma unroll
+    for (int w = 0; w < S / 4; w++) {
+        uint32_t x = zw[w];
+        uint32_t y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);  // quad_perm 2,3,0,1
+        x = __builtin_amdgcn_perm(x, y, sel1);
+        y = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);  // quad_perm 1,0,3,2
+        x = __builtin_amdgcn_perm(x, y, sel2);
+        const int p = S - 1 - (4 * w + q);  // digit u = 4w + q is slice S-1-u
+        *(uint32_t *)(ob + p * pstride) = x;
     }
 }
+// workgroups of 256 threads of k_oz_slice for n slice bytes per plane set (rows x padded K)
+inline unsigned oz_slice_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 // K <= 32 (at most two 16-k chunks, the Cholesky products): row exponents and slices in one pass, the
 // same values as k_fix_max followed by k_oz_slice.  A workgroup covers 32 rows: thread (row rl, k = kt)
@@ -700,7 +714,7 @@ struct OzakiWork {
             // both operands in one launch per pass
             dim3 gm = fix_max_grid(g, 0, std::max(g.M, g.N), nbA);
             k_fix_max<L><<<dim3(gm.x, gm.y, 2), 256, 0, st>>>(g, 0, g.M, qa, g.N, qb);
-            dim3 gs((unsigned)((std::max(sa_n, sb_n) + 255) / 256), nbA, 2);
+            dim3 gs(oz_slice_blocks(std::max(sa_n, sb_n)), nbA, 2);
             if (longk)
                 k_oz_slice<L, S, OZK_KC><<<gs, 256, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa, g.N, Np, qb, pb);
             else
@@ -710,20 +724,16 @@ struct OzakiWork {
         if (fillA) {
             k_fix_max<L><<<fix_max_grid(g, 0, g.M, nbA), 256, 0, st>>>(g, 0, g.M, qa);
             if (longk)
-                k_oz_slice<L, S, OZK_KC><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch,
-                                                                                                 qa, pa);
+                k_oz_slice<L, S, OZK_KC><<<dim3(oz_slice_blocks(sa_n), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa);
             else
-                k_oz_slice<L, S, OZ_KC><<<dim3((unsigned)((sa_n + 255) / 256), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, qa,
-                                                                                                pa);
+                k_oz_slice<L, S, OZ_KC><<<dim3(oz_slice_blocks(sa_n), nbA), 256, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa);
         }
         if (fillB) {
             k_fix_max<L><<<fix_max_grid(g, 1, g.N, nbB), 256, 0, st>>>(g, 1, g.N, qb);
             if (longk)
-                k_oz_slice<L, S, OZK_KC><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch,
-                                                                                                 qb, pb);
+                k_oz_slice<L, S, OZK_KC><<<dim3(oz_slice_blocks(sb_n), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, qb, pb);
             else
-                k_oz_slice<L, S, OZ_KC><<<dim3((unsigned)((sb_n + 255) / 256), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, qb,
-                                                                                                pb);
+                k_oz_slice<L, S, OZ_KC><<<dim3(oz_slice_blocks(sb_n), nbB), 256, 0, st>>>(g, 1, g.N, Np, nch, qb, pb);
         }
         if (same) {
             pb = pa;
