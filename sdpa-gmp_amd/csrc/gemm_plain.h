@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void k_fix_max(GemmArgs g, int which, int R, i
         int lane = threadIdx.x & 63;
         int32_t e = mpx::EZERO;
         if (r < R)
+#pragma unroll 8
             for (int k = lane; k < K; k += 64) {
                 int32_t x = ((int32_t)head[fx_at(g, which, b, r, k)]) >> 1;
                 e = x > e ? x : e;
