@@ -402,7 +402,8 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 //    a few batches before its first use, so register pressure grows with the batches; the last
 //    column fragments are first used late in a chunk and load from global memory at its start;
 //  - batches run by descending row slice p (batch p forms the pairs (p, q <= S-1-p)); the row
-//    fragments stream through a ring of registers, each requested several batches ahead.
+//    fragments stream through a ring of registers, each requested several batches ahead, the last
+//    batches of a chunk requesting the next chunk's first ones.
 // Tile mapping: the tile grid of every batch entry is cut into units of BR x BC tiles (16 x 8 when the
 // grid is large); unit u runs on XCD u mod 8 (workgroups are dispatched to XCDs by id mod 8), whose
 // waves of one unit run together and share its BR row and BC column fragments in that XCD's L2
