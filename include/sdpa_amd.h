@@ -69,6 +69,14 @@ struct sdpg_ctx;
 int sdpa_amd_ctx_from_file(const char *data_path, int prec_bits, int device, int gemm_mode, struct sdpg_ctx **out,
                            int *m, long long *nd, int *formula);
 
+/*
+ * Newton::computeFormula_SDP (sdpa_newton.cpp:694-819) of a problem file, on the host
+ * only (no device): *m, *n_sdp and formula[i * n_sdp + l] in {1 = F1, 2 = F2, 3 = F3},
+ * 0 where A_i has no block l (cap = capacity of formula in ints).  This is the table
+ * that sdpa_amd_solve and sdpa_amd_ctx_from_file upload (sdpg_set_formula).
+ */
+int sdpa_amd_formula(const char *data_path, int *m, int *n_sdp, int *formula, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,7 +5,7 @@
  * Each routine names the reference file:line it follows.  The mplapack
  * kernels (Rgemm, Rpotrf, Rpotrf2, Rsyrk, Rtrsm, Rtrsv, Rtrmv, Rtrmm, Rgemv,
  * Rdot, Raxpy, Rscal) are restated from the reference-BLAS/LAPACK loop
- * orders used by mplapack/*.cpp.  Two eigen routines are replaced by
+ * orders used by the mplapack sources.  Two eigen routines are replaced by
  * textbook methods of equal accuracy (their outputs are mathematically
  * determined, only rounding differs):
  *   Rsyev("N","L") (used for blocks <= 20, sdpa_linear.cpp:126-158) -> cyclic
@@ -2417,5 +2417,70 @@ int orc_format(int prec_bits, int L, int32_t s, int32_t e, const uint32_t *l, in
     F a;
     ompf_import32(&a, L, s, e, l);
     ompf_get_str(buf, bufsz, &a, digits);
+    return 0;
+}
+
+/* ================= structure and factorisation probes for tests ================= */
+
+/*
+ * The formula table of Newton::computeFormula_SDP (sdpa_newton.cpp:694-819) for a
+ * problem read at prec_bits: out[i * nsdp + l] in {1, 2, 3}, 0 where A_i has no block l.
+ */
+int orc_formula(const char *data_path, int prec_bits, int *m, int *nsdp, int *out, int cap) {
+    ompf_set_prec_bits(prec_bits);
+    Prob P;
+    int rc = read_problem(data_path, &P);
+    if (rc) return rc;
+    *m = P.m;
+    *nsdp = P.nsdp;
+    size_t need = (size_t)P.m * P.nsdp;
+    if (need > (size_t)cap) {
+        free_problem(&P);
+        return -10;
+    }
+    Newton N;
+    memset(&N, 0, sizeof N);
+    N.m = P.m;
+    N.useFormula = (int *)xcalloc(need ? need : 1, sizeof(int));
+    compute_formula(&N, &P);
+    memcpy(out, N.useFormula, need * sizeof(int));
+    free(N.useFormula);
+    free_problem(&P);
+    return 0;
+}
+
+/*
+ * Rpotrf("Lower") (mplapack/Rpotrf.cpp:32-154 through potrf_l) of an n x n column-major
+ * matrix given in the packed exchange format of the engine (L+2 words per element:
+ * exponent, sign 0/1, L limbs); *info as Rpotrf2 reports it (0, or the first
+ * non-positive pivot, 1-based).  lo (may be NULL) receives the factor's lower
+ * triangle in the same format (strict upper part zero).
+ */
+int orc_potrf(int prec_bits, int n, int L, const uint32_t *a, uint32_t *lo, int *info) {
+    ompf_set_prec_bits(prec_bits);
+    F *A = (F *)xcalloc((size_t)n * n, sizeof(F));
+    for (size_t t = 0; t < (size_t)n * n; t++) {
+        const uint32_t *w = a + t * (size_t)(L + 2);
+        int zero = 1;
+        for (int k = 0; k < L; k++)
+            if (w[2 + k]) zero = 0;
+        ompf_import32(&A[t], L, zero ? 0 : ((w[1] & 1u) ? -1 : 1), (int32_t)w[0], w + 2);
+    }
+    *info = potrf_l(n, A, n);
+    if (lo) {
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                uint32_t *w = lo + ((size_t)i + (size_t)n * j) * (size_t)(L + 2);
+                int32_t s = 0, e = 0;
+                if (i >= j) {
+                    ompf_export32(&A[i + (size_t)n * j], L, &s, &e, w + 2);
+                } else {
+                    for (int k = 0; k < L; k++) w[2 + k] = 0;
+                }
+                w[0] = (uint32_t)(s == 0 ? -(1 << 29) : e);
+                w[1] = s < 0 ? 1u : 0u;
+            }
+    }
+    free(A);
     return 0;
 }

@@ -81,6 +81,11 @@ int orc_arith(int prec_bits, int op, int L, int32_t as, int32_t ae, const uint32
 int orc_parse(int prec_bits, const char *text, int L, int32_t *rs, int32_t *re, uint32_t *rl);
 int orc_format(int prec_bits, int L, int32_t s, int32_t e, const uint32_t *l, int digits, char *buf, int bufsz);
 
+/* formula table of Newton::computeFormula_SDP: out[i * nsdp + l] in {1,2,3}, 0 = no block */
+int orc_formula(const char *data_path, int prec_bits, int *m, int *nsdp, int *out, int cap);
+/* Rpotrf("Lower") of a packed n x n matrix (engine exchange format); *info = first failing pivot */
+int orc_potrf(int prec_bits, int n, int L, const uint32_t *a, uint32_t *lo, int *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -919,6 +919,24 @@ extern "C" int sdpa_amd_ctx_from_file(const char *data_path, int prec_bits, int 
 #undef CALL_
 }
 
+extern "C" int sdpa_amd_formula(const char *data_path, int *m, int *n_sdp, int *formula, int cap) {
+    RawProblem R;
+    int rc = read_raw(data_path, R);
+    if (rc) return rc;
+    // the table depends only on the problem's structure (stored entries, block sizes), not on
+    // the precision: read the values at 256 bits
+    Solver<10> S;
+    if ((rc = S.load(R))) return rc;
+    S.compute_formula();
+    if (m) *m = S.m;
+    if (n_sdp) *n_sdp = S.nsdp;
+    if (formula) {
+        if ((int64_t)S.useFormula.size() > (int64_t)cap) return SDPG_EARG;
+        std::copy(S.useFormula.begin(), S.useFormula.end(), formula);
+    }
+    return 0;
+}
+
 extern "C" int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_override, int gemm_mode,
                               int slices, int device, int warm_iters, int warmup, int steps, double *ms_schur,
                               double *ms_chol, double *ms_wall, double *kstats, const char *dump_prefix) {

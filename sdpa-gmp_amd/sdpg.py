@@ -36,7 +36,7 @@ EXPORTS = [
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile",
-    "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file",
+    "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula",
 ]
 
 
@@ -89,6 +89,7 @@ def load(path: str | None = None):
                                    P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                    ctypes.c_char_p]
     lib.sdpg_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
+    lib.sdpa_amd_formula.argtypes = [ctypes.c_char_p, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int]
     _lib = lib
     return lib
 
@@ -318,6 +319,22 @@ def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path
     if rc:
         raise SdpgError(f"sdpa_amd_solve rc={rc}: {lib.sdpg_last_error().decode()}")
     return r
+
+
+def formula(data_path: str) -> np.ndarray:
+    """The formula table (Newton::computeFormula_SDP) the host driver uploads, computed on the host
+    without a device: array (m, n_sdp) of 1 / 2 / 3 (F1 / F2 / F3), 0 where A_i has no block."""
+    lib = load()
+    m, ns = ctypes.c_int(), ctypes.c_int()
+    rc = lib.sdpa_amd_formula(data_path.encode(), ctypes.byref(m), ctypes.byref(ns), None, 0)
+    if rc or m.value <= 0:
+        raise SdpgError(f"sdpa_amd_formula rc={rc}")
+    out = np.zeros(m.value * max(ns.value, 1), dtype=np.int32)
+    rc = lib.sdpa_amd_formula(data_path.encode(), None, None, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              out.size)
+    if rc:
+        raise SdpgError(f"sdpa_amd_formula rc={rc}")
+    return out.reshape(m.value, max(ns.value, 1))[:, :ns.value]
 
 
 def bench(data_path: str, param_path: str | None, prec: int, gemm: int, slices: int, warm_iters: int,
