@@ -27,6 +27,7 @@ SDPG_INVCHOLX, SDPG_INVCHOLZ = 14, 15
 (SDPG_IP_XZ, SDPG_IP_XDZ, SDPG_IP_DXZ, SDPG_IP_DXDZ, SDPG_IP_CX, SDPG_IP_CDX, SDPG_IP_BY, SDPG_IP_BDY,
  SDPG_IP_TRX, SDPG_IP_TRZ) = range(1, 11)
 GEMM_ELEM, GEMM_PLAIN, GEMM_OZAKI = -1, 0, 1
+SDPG_NOINFO, SDPG_PFEAS, SDPG_DFEAS, SDPG_PDFEAS = 0, 1, 2, 3
 EZERO = -(1 << 29)
 
 # every exported symbol of include/sdpg.h and include/sdpa_amd.h

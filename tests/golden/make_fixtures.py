@@ -7,7 +7,9 @@ Produces
   inputs.json             sha256 of the generated workload files (gen_problems.py)
   newton_w{0,1}_it*.npz   Newton-step fixtures: the oracle's 256-bit iterate (inputs, exact
                           in 12 interchange limbs) and the 512-bit oracle outputs on that same
-                          iterate (16 limbs), with the 256-bit oracle's normwise error per output
+                          iterate (16 limbs), with the 256-bit oracle's normwise error per output;
+                          workload 1 keeps SAMPLE entries of B and of L's lower triangle (indices in
+                          ref_B_idx / ref_L_idx, column-major positions in the m x m matrix)
   solves.json             end-to-end oracle summaries of workloads 0 and 1 at epsilonStar =
                           epsilonDash = 1e-60: at 256 bits (GMP truncation: the primal residual
                           stalls near 1e-49, so the run ends pFEAS) and at 512 bits (pdOPT; the
@@ -41,6 +43,8 @@ import oracle_lib  # noqa: E402
 
 PARAM = os.path.join(ROOT, "sdpa-gmp_amd", "data", "param_256_gap1e-60.sdpa")
 REF = "/root/reference"
+
+SAMPLE = 3000  # sampled entries of B and L kept for workload 1 (m = 351: the full matrices are ~9 MB each)
 
 SDPLIB = ["control1", "control2", "truss1", "truss4", "hinf1", "hinf3", "theta1", "infp1", "infd1", "qap5"]
 
@@ -101,15 +105,23 @@ def newton_fixtures(tmp, name, iters):
             arrs["in_" + key] = oracle_lib.read_dump(os.path.join(d, key + ".bin"))
         with open(os.path.join(d, "phase.txt")) as f:
             arrs["phase"] = np.array([int(f.read().strip())])
-        outs = ["Rp", "Rd", "g", "dy", "dX", "dZ"] + (["B", "L"] if name == "w0" else [])
-        for key in outs:
+        for key in ("Rp", "Rd", "g", "dy", "dX", "dZ", "B", "L"):
             r512 = oracle_lib.read_dump(os.path.join(o512, key + ".bin"))
             r256 = oracle_lib.read_dump(os.path.join(o256, key + ".bin"))
-            if key == "L":  # only the lower triangle is the factor
+            if key in ("B", "L"):
                 m = int(round(len(r512) ** 0.5))
-                mask = np.array([i >= j for j in range(m) for i in range(m)])
-                r512 = r512[mask]
-                r256 = r256[mask]
+                if key == "L":  # only the lower triangle is the factor
+                    pos = np.array([i + m * j for j in range(m) for i in range(j, m)], dtype=np.int64)
+                else:
+                    pos = np.arange(m * m, dtype=np.int64)
+                if name != "w0":
+                    rng = np.random.default_rng(1000 + it + (7 if key == "L" else 0))
+                    diag = np.array([i + m * i for i in range(m)], dtype=np.int64)
+                    pick = rng.choice(pos, size=min(SAMPLE, len(pos)), replace=False)
+                    pos = np.unique(np.concatenate([pick, diag[:: max(1, m // 64)]]))
+                    arrs["ref_" + key + "_idx"] = pos
+                r512 = r512[pos]
+                r256 = r256[pos]
             arrs["ref_" + key] = r512
             arrs["err256_" + key] = np.array([normwise_err(r256, r512)])
             print(f"{name} it{it} {key}: oracle-256 normwise error {arrs['err256_' + key][0]:.3e}", flush=True)
@@ -163,6 +175,7 @@ def main():
     ap.add_argument("--skip-sdplib", action="store_true")
     ap.add_argument("--skip-newton", action="store_true")
     ap.add_argument("--solves-only", action="store_true", help="recompute solves.json only")
+    ap.add_argument("--newton-only", choices=["w0", "w1"], help="recompute the Newton fixtures of one workload")
     a = ap.parse_args()
     oracle_lib.build_oracle()
     if a.solves_only:
@@ -173,6 +186,14 @@ def main():
                       for w in ("w0", "w1")}
             with open(os.path.join(HERE, "solves.json"), "w") as f:
                 json.dump(solves, f, indent=1)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        return
+    if a.newton_only:
+        tmp = tempfile.mkdtemp(prefix="sdpg_fix_")
+        try:
+            gen_inputs(tmp)
+            newton_fixtures(tmp, a.newton_only, {"w0": [0, 1, 5, 20, 40, 60], "w1": [0, 10, 45]}[a.newton_only])
         finally:
             shutil.rmtree(tmp, ignore_errors=True)
         return
