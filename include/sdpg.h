@@ -206,6 +206,21 @@ int sdpg_profile(sdpg_ctx *ctx, int on, double *out, int n);
 int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
                const uint32_t *B, int ldb, uint32_t *C, int ldc, int op_c, int mode, int slices, int lower,
                double *ms);
+/*
+ * No counterpart in the reference: sdpg_rgemm for callers that cannot allocate device memory
+ * (code under HIP graph capture of the surrounding work, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as sdpg_rgemm (the same kernels run), but the operand
+ * copies, the staging buffer and the product workspace are carved from a static device array of
+ * 128 MiB per device, held from the first use until the process ends; the call makes no hipMalloc,
+ * hipFree, stream or event, and runs on the null stream.  Sizes whose operands and workspace do not
+ * fit in the array return SDPG_EARG (before any device work; C is untouched).  Calls on one device
+ * share its array: the library serializes them with a per-device lock, so calls that overlap from
+ * several host threads wait for one another.  *ms (may be NULL) receives the host-measured
+ * milliseconds of the product.
+ */
+int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A,
+                       int lda, const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices,
+                       int lower, double *ms);
 /* last error message of this thread */
 const char *sdpg_last_error(void);
 

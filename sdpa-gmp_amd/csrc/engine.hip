@@ -2,9 +2,43 @@
 // (kernels, Engine<L>) is in engine_impl.h, instantiated per limb count in engine_inst.hip.
 #include "engine_impl.h"
 
+#include <mutex>
+
 namespace sdpg {
 static thread_local std::string g_err;
 void set_err(const std::string &s) { g_err = s; }
+
+// Device scratch of the no-allocation GEMM entry point (sdpg_rgemm_noalloc): operand copies,
+// staging and the product workspace of one call are carved from this array.  Each device that
+// loads the library holds its own copy until the process ends.  In host code the array's name is
+// not a device address: its per-device address comes from hipGetSymbolAddress, taken once per
+// device with that device current, and only that address is handed to kernels and copies.
+constexpr size_t SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAXDEV = 64;
+static std::mutex g_scratch_mu[SCRATCH_MAXDEV];  // one call at a time per device (they share its array)
+static uint8_t *g_scratch_ptr[SCRATCH_MAXDEV];
+
+// the device address of this device's scratch array (looked up on the first call on the device);
+// the caller holds g_scratch_mu[device]
+static int scratch_address(int device, uint8_t **out) {
+    if (!g_scratch_ptr[device]) {
+        if (hipSetDevice(device) != hipSuccess) {
+            set_err("sdpg_rgemm_noalloc: bad device");
+            return SDPG_EARG;
+        }
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err(std::string("sdpg_rgemm_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return SDPG_EHIP;
+        }
+        g_scratch_ptr[device] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return SDPG_OK;
+}
 }  // namespace sdpg
 
 // ------------------------------------------------------------------ C-ABI
@@ -299,10 +333,43 @@ int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, 
     int L = mpx::limbs_for_bits(prec_bits);
 #define RG_(LL)                                                                                                    \
     if (L == LL) return rgemm_##LL(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, \
-                                   ms);
+                                   ms, nullptr, 0);
     SDPG_FOR_EACH_L(RG_)
 #undef RG_
     set_err("sdpg_rgemm: unsupported precision");
+    return SDPG_EPREC;
+}
+
+int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A,
+                       int lda, const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices,
+                       int lower, double *ms) {
+    if (m < 0 || n < 0 || k < 0 || op < 0 || op > 2 || mode < -1 || mode > 1 || slices < 0 || !C ||
+        (k > 0 && (!A || !B)) || ldc < std::max(1, m) || lda < std::max(1, transa ? k : m) ||
+        ldb < std::max(1, transb ? n : k)) {
+        set_err("sdpg_rgemm_noalloc: bad arguments");
+        return SDPG_EARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err("sdpg_rgemm_noalloc: no HIP device");
+        return SDPG_ENODEV;
+    }
+    if (device < 0 || device >= ndev || device >= SCRATCH_MAXDEV) {
+        set_err("sdpg_rgemm_noalloc: bad device ordinal");
+        return SDPG_EARG;
+    }
+    if (m == 0 || n == 0) return SDPG_OK;
+    int L = mpx::limbs_for_bits(prec_bits);
+    std::lock_guard<std::mutex> lock(g_scratch_mu[device]);
+    uint8_t *scr = nullptr;
+    int rc = scratch_address(device, &scr);
+    if (rc) return rc;
+#define RGN_(LL)                                                                                                   \
+    if (L == LL) return rgemm_##LL(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, \
+                                   ms, scr, SCRATCH_BYTES);
+    SDPG_FOR_EACH_L(RGN_)
+#undef RGN_
+    set_err("sdpg_rgemm_noalloc: unsupported precision");
     return SDPG_EPREC;
 }
 

@@ -40,6 +40,20 @@ __device__ __forceinline__ void stm(PV v, int64_t i, const mp<L> &x) {
     v.p[(int64_t)L * v.n + i] = (uint32_t)((x.e << 1) | (int32_t)(x.s & 1u));
 }
 
+// A bump allocator over device memory that the caller owns (the no-allocation GEMM entry point
+// hands it a static device array): take() returns 256-byte aligned pieces and never frees, so a
+// workspace that draws from it calls no hipMalloc / hipFree.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes) {
+        size_t at = (used + 255) & ~(size_t)255;
+        if (at > cap || bytes > cap - at) return nullptr;
+        used = at + bytes;
+        return base + at;
+    }
+};
+
 // Per-kernel device time over a measured region: events recorded on the launching stream around
 // each launch of the kernels of interest, summed after the region ends.
 // kinds: 0 Ozaki MFMA product kernel (short K), 1 plain limb product kernel, 2 Cholesky diagonal

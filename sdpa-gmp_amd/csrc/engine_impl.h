@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -48,6 +49,9 @@ void set_err(const std::string &s);
     } while (0)
 
 static inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+static inline double host_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 // ------------------------------------------------------------------ kernels
 
@@ -2154,10 +2158,15 @@ struct Engine : EngineBase {
 
 // Standalone multiple-precision GEMM on host buffers (packed words), one call = one stream:
 // C = op(A) op(B), C - op(A) op(B) or C + op(A) op(B), column-major as Rgemm.
+// Rgemm on host buffers.  scratch == nullptr: device copies, staging and the GEMM workspace are
+// allocated for the call and freed after it, on a stream of its own.  scratch != nullptr: all of
+// them are carved from the scratch_bytes of device memory at scratch (the static array of the
+// no-allocation entry point), the work runs on the null stream, and the call makes no hipMalloc,
+// hipFree, stream or event; sizes that do not fit return SDPG_EARG before any device work.
 template <int L>
 static int rgemm_L(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,
                    const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower,
-                   double *ms) {
+                   double *ms, uint8_t *scratch = nullptr, size_t scratch_bytes = 0) {
     const int W = L + 2;
     int64_t na = (int64_t)lda * (transa ? m : k), nb = (int64_t)ldb * (transb ? k : n), nc = (int64_t)ldc * n;
     if (na <= 0) na = 1;
@@ -2167,8 +2176,10 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = SDPG_OK;
     int64_t stg_n = std::max(na, std::max(nb, nc)) * W;
+    const bool fixed = scratch != nullptr;
+    Arena arena{scratch, scratch_bytes, 0};
     auto fail = [&](const char *what) {
-        set_err(std::string("sdpg_rgemm: ") + what);
+        set_err(std::string(fixed ? "sdpg_rgemm_noalloc: " : "sdpg_rgemm: ") + what);
         rc = SDPG_EHIP;
     };
     {
@@ -2177,16 +2188,30 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
             set_err("sdpg_rgemm: bad device");
             return SDPG_EARG;
         }
-        if (hipStreamCreate(&st) != hipSuccess) {
-            fail("stream");
-            return rc;
-        }
-        if (hipMalloc((void **)&dA, sizeof(uint32_t) * (L + 1) * na) != hipSuccess ||
-            hipMalloc((void **)&dB, sizeof(uint32_t) * (L + 1) * nb) != hipSuccess ||
-            hipMalloc((void **)&dC, sizeof(uint32_t) * (L + 1) * nc) != hipSuccess ||
-            hipMalloc((void **)&stg, sizeof(uint32_t) * stg_n) != hipSuccess) {
-            set_err("sdpg_rgemm: out of device memory");
-            rc = SDPG_ENOMEM;
+        if (fixed) {
+            ge.plain.arena = &arena;
+            ge.ozaki.arena = &arena;
+            dA = (uint32_t *)arena.take(sizeof(uint32_t) * (L + 1) * na);
+            dB = (uint32_t *)arena.take(sizeof(uint32_t) * (L + 1) * nb);
+            dC = (uint32_t *)arena.take(sizeof(uint32_t) * (L + 1) * nc);
+            stg = (uint32_t *)arena.take(sizeof(uint32_t) * stg_n);
+            if (!dA || !dB || !dC || !stg) {
+                set_err("sdpg_rgemm_noalloc: operands exceed the " + std::to_string(scratch_bytes >> 20) +
+                        " MiB static device scratch");
+                return SDPG_EARG;
+            }
+        } else {
+            if (hipStreamCreate(&st) != hipSuccess) {
+                fail("stream");
+                return rc;
+            }
+            if (hipMalloc((void **)&dA, sizeof(uint32_t) * (L + 1) * na) != hipSuccess ||
+                hipMalloc((void **)&dB, sizeof(uint32_t) * (L + 1) * nb) != hipSuccess ||
+                hipMalloc((void **)&dC, sizeof(uint32_t) * (L + 1) * nc) != hipSuccess ||
+                hipMalloc((void **)&stg, sizeof(uint32_t) * stg_n) != hipSuccess) {
+                set_err("sdpg_rgemm: out of device memory");
+                rc = SDPG_ENOMEM;
+            }
         }
         PV pa{dA, na}, pb{dB, nb}, pc{dC, nc};
         auto upload = [&](const uint32_t *src, int64_t cnt, PV dst) {
@@ -2203,18 +2228,32 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
         upload(C, nc, pc);
         if (!rc) {
             ge.init(st);
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
-            (void)hipEventRecord(e0, st);
+            double t0 = 0;
+            if (fixed) {
+                t0 = host_ms();
+            } else {
+                (void)hipEventCreate(&e0);
+                (void)hipEventCreate(&e1);
+                (void)hipEventRecord(e0, st);
+            }
             int g = ge.gemm(m, n, k, pa, 0, lda, transa != 0, pb, 0, ldb, transb != 0, pc, 0, ldc, op, 1, 0, 0, 0,
                             mode, slices, lower != 0);
-            (void)hipEventRecord(e1, st);
+            if (fixed) {
+                if (hipStreamSynchronize(st) != hipSuccess) g = -2;
+                if (ms) *ms = host_ms() - t0;
+            } else {
+                (void)hipEventRecord(e1, st);
+            }
             if (g == -1) {
                 set_err("sdpg_rgemm: slice count not available at this precision");
                 rc = SDPG_EARG;
             } else if (g == -5) {
                 set_err("sdpg_rgemm: the Ozaki form is built for up to 384-bit limbs");
                 rc = SDPG_EPREC;
+            } else if (g == -4 && fixed) {
+                set_err("sdpg_rgemm_noalloc: operands and product workspace exceed the " +
+                        std::to_string(scratch_bytes >> 20) + " MiB static device scratch");
+                rc = SDPG_EARG;
             } else if (g) {
                 fail("gemm launch");
             }
@@ -2225,9 +2264,10 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
                 hipStreamSynchronize(st) != hipSuccess)
                 fail("download");
             float t = 0;
-            if (!rc && ms && hipEventElapsedTime(&t, e0, e1) == hipSuccess) *ms = t;
+            if (!rc && !fixed && ms && hipEventElapsedTime(&t, e0, e1) == hipSuccess) *ms = t;
         }
     }
+    if (fixed) return rc;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(dA);
@@ -2244,7 +2284,7 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
     int init_engine_##LL(EngineBase *e);                                                                    \
     int rgemm_##LL(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,      \
                    const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower, \
-                   double *ms);
+                   double *ms, uint8_t *scratch, size_t scratch_bytes);
 SDPG_FOR_EACH_L(SDPG_DECL_L_)
 #undef SDPG_DECL_L_
 

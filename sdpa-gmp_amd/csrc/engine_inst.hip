@@ -17,9 +17,9 @@ int SDPG_CAT_(init_engine_, SDPG_INST_L)(EngineBase *e) { return static_cast<Eng
 
 int SDPG_CAT_(rgemm_, SDPG_INST_L)(int device, int transa, int transb, int m, int n, int k, const uint32_t *A,
                                    int lda, const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode,
-                                   int slices, int lower, double *ms) {
+                                   int slices, int lower, double *ms, uint8_t *scratch, size_t scratch_bytes) {
     return rgemm_L<SDPG_INST_L>(device, transa, transb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower,
-                                ms);
+                                ms, scratch, scratch_bytes);
 }
 
 }  // namespace sdpg

@@ -631,15 +631,22 @@ struct OzakiWork {
     int8_t *as = nullptr, *bs = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
     size_t cap_as = 0, cap_bs = 0, cap_ea = 0, cap_eb = 0;
+    Arena *arena = nullptr;  // when set, the workspace is carved from it (no hipMalloc / hipFree)
     ~OzakiWork() {
+        if (arena) return;
         (void)hipFree(as);
         (void)hipFree(bs);
         (void)hipFree(ea);
         (void)hipFree(eb);
     }
     template <class T>
-    static int grow(T **p, size_t &cap, size_t need) {
+    int grow(T **p, size_t &cap, size_t need) {
         if (need <= cap) return 0;
+        if (arena) {
+            *p = (T *)arena->take(sizeof(T) * (need ? need : 1));
+            cap = *p ? need : 0;
+            return *p ? 0 : -4;
+        }
         if (*p) (void)hipFree(*p);
         *p = nullptr;
         size_t n = need + need / 4 + 256;

@@ -361,15 +361,22 @@ struct PlainWork {
     uint32_t *af = nullptr, *bf = nullptr;
     int32_t *ea = nullptr, *eb = nullptr;
     size_t cap_af = 0, cap_bf = 0, cap_ea = 0, cap_eb = 0;
+    Arena *arena = nullptr;  // when set, the workspace is carved from it (no hipMalloc / hipFree)
     ~PlainWork() {
+        if (arena) return;
         hipFree(af);
         hipFree(bf);
         hipFree(ea);
         hipFree(eb);
     }
     template <class T>
-    static int grow(T **p, size_t &cap, size_t need) {
+    int grow(T **p, size_t &cap, size_t need) {
         if (need <= cap) return 0;
+        if (arena) {
+            *p = (T *)arena->take(sizeof(T) * (need ? need : 1));
+            cap = *p ? need : 0;
+            return *p ? 0 : -4;
+        }
         if (*p) hipFree(*p);
         *p = nullptr;
         size_t n = need + need / 4 + 64;

@@ -36,13 +36,15 @@ EXPORTS = [
     "sdpg_set_formula", "sdpg_commit", "sdpg_set_gemm", "sdpg_set_point", "sdpg_residuals", "sdpg_gvec",
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
-    "sdpg_inner", "sdpg_min_eigen", "sdpg_profile",
+    "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula",
 ]
 
 
 class SdpgError(RuntimeError):
-    pass
+    def __init__(self, msg, rc=None):
+        super().__init__(msg)
+        self.rc = rc
 
 
 _lib = None
@@ -80,6 +82,7 @@ def load(path: str | None = None):
     lib.sdpg_min_eigen.argtypes = [ctypes.c_void_p, ctypes.c_int, u32p]
     lib.sdpg_rgemm.argtypes = [ctypes.c_int] * 7 + [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p] + \
         [ctypes.c_int] * 5 + [P(ctypes.c_double)]
+    lib.sdpg_rgemm_noalloc.argtypes = lib.sdpg_rgemm.argtypes
     lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            P(ctypes.c_void_p), P(ctypes.c_int), P(ctypes.c_longlong),
                                            P(ctypes.c_int)]
@@ -166,9 +169,10 @@ def _ptr(a: np.ndarray):
 
 def rgemm(prec_bits: int, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = None, transa: bool = False,
           transb: bool = False, op: int = 0, gemm: int = GEMM_PLAIN, slices: int = 0, lower: bool = False,
-          device: int = 0):
+          device: int = 0, noalloc: bool = False):
     """C op= op(A) op(B) on the device.  A, B, C: packed arrays of shape (rows, cols, L+2), column-major
-    in the sense of Rgemm (element (i, j) = X[i, j]).  Returns (C, device milliseconds)."""
+    in the sense of Rgemm (element (i, j) = X[i, j]).  Returns (C, device milliseconds).  noalloc: through
+    sdpg_rgemm_noalloc (static device scratch, no allocation)."""
     lib = load()
     L = limbs_for_bits(prec_bits)
     ra, ca = A.shape[:2]
@@ -184,10 +188,11 @@ def rgemm(prec_bits: int, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = N
         C[:, :, 0] = EZERO & 0xFFFFFFFF
     c = colmaj(C)
     ms = ctypes.c_double(0)
-    rc = lib.sdpg_rgemm(prec_bits, device, int(transa), int(transb), m, n, k, _ptr(a), max(1, ra), _ptr(b),
-                        max(1, rb), _ptr(c), max(1, m), op, gemm, slices, int(lower), ctypes.byref(ms))
+    fn = lib.sdpg_rgemm_noalloc if noalloc else lib.sdpg_rgemm
+    rc = fn(prec_bits, device, int(transa), int(transb), m, n, k, _ptr(a), max(1, ra), _ptr(b),
+            max(1, rb), _ptr(c), max(1, m), op, gemm, slices, int(lower), ctypes.byref(ms))
     if rc:
-        raise SdpgError(f"sdpg_rgemm: {rc}: {lib.sdpg_last_error().decode()}")
+        raise SdpgError(f"{fn.__name__}: {rc}: {lib.sdpg_last_error().decode()}", rc)
     return np.ascontiguousarray(np.transpose(c, (1, 0, 2))), ms.value
 
 
