@@ -31,13 +31,15 @@ typedef struct {
     double t_eig_ms;      /* step-length minimum eigenvalues (sdpg_min_eigen), device time */
     char objP70[112];     /* objValPrimal with 70 significant digits */
     char objD70[112];
+    int slices;           /* Ozaki slice count used (autotuned when the caller passes 0), 0 for other forms */
 } sdpa_amd_result;
 
 /*
  * param_path may be NULL (defaults of Parameter::setDefaultParameter,
  * sdpa_parts.cpp:165-178).  prec_override > 0 replaces the precision line.
  * log_path: NULL, "-" (stdout) or a file.  gemm_mode: SDPG_GEMM_PLAIN or
- * SDPG_GEMM_OZAKI.  Returns 0 on success, < 0 on input or device errors.
+ * SDPG_GEMM_OZAKI; slices 0 with the Ozaki form: the count sdpg_tune_slices picks for the run's
+ * precision.  Returns 0 on success, < 0 on input or device errors.
  */
 int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
                    int gemm_mode, int slices, int device, sdpa_amd_result *res);

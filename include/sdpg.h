@@ -123,7 +123,9 @@ int sdpg_commit(sdpg_ctx *ctx);
  * GEMM form used by the Schur precompute, the Cholesky panel / trailing updates
  * and dX: SDPG_GEMM_PLAIN (limb MADs on the vector ALUs), SDPG_GEMM_OZAKI (INT8
  * slices on MFMA), SDPG_GEMM_ELEMENT (one output per thread, the Rgemm order).
- * slices (Ozaki only): 0 = default 4L-4, or 4L; other counts return SDPG_EARG.
+ * slices (Ozaki only): 0 = default 4L-4, or a multiple of 4 in [max(8, 4L-12), 4L] (8S-2 fraction
+ * bits per row / column of the block fixed point); other counts return SDPG_EARG.  sdpg_tune_slices
+ * picks the count for a requested precision.
  */
 int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices);
 
@@ -221,6 +223,19 @@ int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, 
 int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A,
                        int lda, const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices,
                        int lower, double *ms);
+/*
+ * Slice-count autotuner of the Ozaki form (no reference counterpart: the reference's precision is a
+ * run parameter, sdpa_parts.cpp:199-200, and its GEMMs run at the full mpf width).  For the limb
+ * count of prec_bits it measures every slice count built (multiples of 4 in [max(8, 4L-12), 4L]):
+ * effective bits of the product on calibration tiles against a product at L+2 limbs, and device time
+ * on two representative products; *slices receives the fastest count whose effective bits reach
+ * target_bits + 8 (target_bits 0 = prec_bits).  table (may be NULL) receives 4 doubles per count:
+ * slices, effective bits, ms of a 512^3 product, ms of a 1024 x 1024 x 32 lower update.  Cached per
+ * (device, limb count, target).  SDPG_EPREC when no count reaches the target or the reference limb
+ * count is not built.
+ */
+int sdpg_tune_slices(int prec_bits, int device, int target_bits, int *slices, double *table, int ntable);
+
 /* last error message of this thread */
 const char *sdpg_last_error(void);
 

@@ -218,7 +218,7 @@ int sdpg_set_gemm(sdpg_ctx *ctx, int mode, int slices) {
     if (mode == SDPG_GEMM_OZAKI && !ctx->eng->slices_ok(slices ? slices : 4 * ctx->eng->nlimb - 4)) {
         set_err(ctx->eng->nlimb > OZ_LMAX
                     ? "sdpg_set_gemm: the Ozaki form is built for up to 384-bit limbs"
-                    : "sdpg_set_gemm: slice count not available at this precision (use 0, 4L-4 or 4L)");
+                    : "sdpg_set_gemm: slice count not available at this precision (0, or a multiple of 4 in [4L-12, 4L])");
         return ctx->eng->nlimb > OZ_LMAX ? SDPG_EPREC : SDPG_EARG;
     }
     ctx->eng->gemm_mode = mode;
@@ -338,6 +338,198 @@ int sdpg_rgemm(int prec_bits, int device, int transa, int transb, int m, int n, 
 #undef RG_
     set_err("sdpg_rgemm: unsupported precision");
     return SDPG_EPREC;
+}
+
+// ---- slice-count autotuner ----
+// The Ozaki form keeps 8S - 2 fraction bits of every row / column, so its product error relative to
+// K max|A_i.| max|B_.j| falls by 2^8 per slice; the MFMA work grows as S(S+1)/2.  For a requested
+// precision the tuner measures, for every slice count built at that limb count, the effective bits
+// -log2(max |C - exact| / (K a_i b_j)) on calibration products (full-width random mantissas, exponents
+// spread over 40 binades, K = 32 as in the Cholesky updates and K = 300 as in the long products)
+// against the plain form at L + 2 limbs (error below 2^-(32L+54), exact at these scales), times each
+// count on a 512 x 512 x 512 product and a 1024 x 1024 x 32 lower-triangle update (operands resident
+// on the device, averaged over 10 runs), and keeps the fastest count whose effective bits reach the
+// request plus 8 guard bits.  Results are cached per
+// (device, limb count, target).
+namespace {
+struct TuneKey {
+    int device, L, target;
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<TuneKey, std::vector<double>>> g_tune_cache;  // rows of (S, bits, ms_long, ms_short)
+
+int rgemm_any(int L, int device, int ta, int tb, int m, int n, int k, const uint32_t *A, int lda, const uint32_t *B,
+              int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower, double *ms) {
+#define RGA_(LL) \
+    if (L == LL) return rgemm_##LL(device, ta, tb, m, n, k, A, lda, B, ldb, C, ldc, op, mode, slices, lower, ms, nullptr, 0);
+    SDPG_FOR_EACH_L(RGA_)
+#undef RGA_
+    return SDPG_EPREC;
+}
+
+bool limb_count_built(int L) {
+#define LB_(LL) \
+    if (L == LL) return true;
+    SDPG_FOR_EACH_L(LB_)
+#undef LB_
+    return false;
+}
+
+constexpr int TL = 20;  // host comparison width (limbs): holds both operands' mantissas exactly
+
+mpx::mp<TL> widen(const uint32_t *w, int L) {
+    mpx::mp<TL> x = mpx::zero<TL>();
+    bool nz = false;
+    for (int q = 0; q < L; q++) nz |= w[2 + q] != 0;
+    if (!nz) return x;
+    x.e = (int32_t)w[0];
+    x.s = w[1] & 1u;
+    for (int q = 0; q < L; q++) x.m[TL - L + q] = w[2 + q];
+    return x;
+}
+
+double log2abs(const mpx::mp<TL> &x) {  // -inf for zero
+    if (mpx::is_zero<TL>(x)) return -1e300;
+    return (double)x.e - 1.0 + std::log2((double)x.m[TL - 1] / 2147483648.0);
+}
+
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        return s;
+    }
+};
+
+// packed r x c column-major matrix of random full-width values with exponents in [-20, 20]
+std::vector<uint32_t> calib_matrix(Rng &g, int r, int c, int L) {
+    const int W = L + 2;
+    std::vector<uint32_t> v((size_t)r * c * W);
+    for (size_t t = 0; t < (size_t)r * c; t++) {
+        uint32_t *w = &v[t * W];
+        w[0] = (uint32_t)((int)(g.next() % 41) - 20);
+        w[1] = (uint32_t)(g.next() & 1);
+        for (int q = 0; q < L; q++) w[2 + q] = (uint32_t)g.next();
+        w[L + 1] |= 0x80000000u;
+    }
+    return v;
+}
+
+std::vector<uint32_t> repad(const std::vector<uint32_t> &v, int L, int L2) {
+    const size_t n = v.size() / (L + 2);
+    std::vector<uint32_t> o(n * (L2 + 2), 0);
+    for (size_t t = 0; t < n; t++) {
+        o[t * (L2 + 2)] = v[t * (L + 2)];
+        o[t * (L2 + 2) + 1] = v[t * (L + 2) + 1];
+        for (int q = 0; q < L; q++) o[t * (L2 + 2) + 2 + (L2 - L) + q] = v[t * (L + 2) + 2 + q];
+    }
+    return o;
+}
+
+// effective bits of the Ozaki product with S slices on one calibration shape (C = A B^T if tb)
+int calib_bits(int L, int device, int S, int m, int n, int k, int tb, uint64_t seed, double *bits) {
+    Rng g{seed};
+    const int W = L + 2, L2 = L + 2, W2 = L2 + 2;
+    std::vector<uint32_t> A = calib_matrix(g, m, k, L), B = tb ? calib_matrix(g, n, k, L) : calib_matrix(g, k, n, L);
+    std::vector<uint32_t> C((size_t)m * n * W, 0), R((size_t)m * n * W2, 0);
+    for (size_t t = 0; t < (size_t)m * n; t++) C[t * W] = (uint32_t)SDPG_EZERO, R[t * W2] = (uint32_t)SDPG_EZERO;
+    const int ldb = tb ? n : k;
+    int rc = rgemm_any(L, device, 0, tb, m, n, k, A.data(), m, B.data(), ldb, C.data(), m, 0, SDPG_GEMM_OZAKI, S, 0,
+                       nullptr);
+    if (rc) return rc;
+    std::vector<uint32_t> A2 = repad(A, L, L2), B2 = repad(B, L, L2);
+    rc = rgemm_any(L2, device, 0, tb, m, n, k, A2.data(), m, B2.data(), ldb, R.data(), m, 0, SDPG_GEMM_PLAIN, 0, 0,
+                   nullptr);
+    if (rc) return rc;
+    // row maxima of A, column maxima of op(B) (log2)
+    std::vector<double> la(m, -1e300), lb(n, -1e300);
+    for (int i = 0; i < m; i++)
+        for (int q = 0; q < k; q++) la[i] = std::max(la[i], log2abs(widen(&A[((size_t)i + (size_t)m * q) * W], L)));
+    for (int j = 0; j < n; j++)
+        for (int q = 0; q < k; q++) {
+            const size_t t = tb ? ((size_t)j + (size_t)n * q) : ((size_t)q + (size_t)k * j);
+            lb[j] = std::max(lb[j], log2abs(widen(&B[t * W], L)));
+        }
+    double worst = -1e300;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            const size_t t = (size_t)i + (size_t)m * j;
+            const mpx::mp<TL> d = mpx::sub<TL>(widen(&C[t * W], L), widen(&R[t * W2], L2));
+            worst = std::max(worst, log2abs(d) - (std::log2((double)k) + la[i] + lb[j]));
+        }
+    *bits = -worst;
+    return 0;
+}
+
+int time_slices(int L, int device, int S, const std::vector<uint32_t> &host, double *ms_long, double *ms_short) {
+#define TT_(LL) \
+    if (L == LL) return tune_time_##LL(device, S, host.data(), ms_long, ms_short);
+    SDPG_FOR_EACH_L(TT_)
+#undef TT_
+    return SDPG_EPREC;
+}
+}  // namespace
+
+int sdpg_tune_slices(int prec_bits, int device, int target_bits, int *slices, double *table, int ntable) {
+    if (!slices || prec_bits <= 0 || target_bits < 0) {
+        set_err("sdpg_tune_slices: bad arguments");
+        return SDPG_EARG;
+    }
+    const int L = mpx::limbs_for_bits(prec_bits);
+    const int target = target_bits ? target_bits : prec_bits;
+    if (L > OZ_LMAX || !limb_count_built(L)) {
+        set_err("sdpg_tune_slices: no Ozaki form at this precision");
+        return SDPG_EPREC;
+    }
+    if (!limb_count_built(L + 2)) {
+        set_err("sdpg_tune_slices: the calibration reference needs the " + std::to_string(L + 2) + "-limb build");
+        return SDPG_EPREC;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        set_err("sdpg_tune_slices: bad device ordinal");
+        return SDPG_ENODEV;
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const std::vector<double> *rows = nullptr;
+    for (auto &e : g_tune_cache)
+        if (e.first.device == device && e.first.L == L && e.first.target == target) rows = &e.second;
+    if (!rows) {
+        std::vector<double> r;
+        Rng g{77};
+        const std::vector<uint32_t> host = calib_matrix(g, 1024, 1024, L);  // operands of the timed products
+        for (int S = std::max(8, 4 * L - 12); S <= 4 * L; S += 4) {
+            double b1 = 0, b2 = 0, tl = 0, ts = 0;
+            int rc = calib_bits(L, device, S, 64, 64, 32, 1, 1000 + S, &b1);
+            if (!rc) rc = calib_bits(L, device, S, 48, 40, 300, 0, 2000 + S, &b2);
+            if (!rc) rc = time_slices(L, device, S, host, &tl, &ts);
+            if (rc) return rc;
+            r.insert(r.end(), {(double)S, std::min(b1, b2), tl, ts});
+        }
+        g_tune_cache.push_back({TuneKey{device, L, target}, r});
+        rows = &g_tune_cache.back().second;
+    }
+    // the fastest qualifying count; counts within 3 % of one another are ties, settled by the smaller
+    // count (less MFMA work, rows in increasing S)
+    int best = 0;
+    double best_t = 0;
+    for (size_t q = 0; q + 3 < rows->size(); q += 4) {
+        const double S = (*rows)[q], bits = (*rows)[q + 1], t = (*rows)[q + 2] + (*rows)[q + 3];
+        if (bits >= target + 8 && (!best || t < 0.97 * best_t)) {
+            best = (int)S;
+            best_t = t;
+        }
+    }
+    if (table)
+        for (int q = 0; q < ntable && q < (int)rows->size(); q++) table[q] = (*rows)[q];
+    if (!best) {
+        set_err("sdpg_tune_slices: no slice count built at this limb count reaches the target precision");
+        return SDPG_EPREC;
+    }
+    *slices = best;
+    return SDPG_OK;
 }
 
 int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A,

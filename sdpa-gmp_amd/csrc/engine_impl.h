@@ -2278,13 +2278,65 @@ static int rgemm_L(int device, int transa, int transb, int m, int n, int k, cons
     return rc;
 }
 
+// Device time of the Ozaki product with S slices on two representative shapes (operands resident,
+// one stream, 3 untimed and 10 timed runs each, averaged): a 512 x 512 x 512 product (long-K kernel)
+// and the lower-triangle update C -= P P^T with P 1024 x 32 (short-K kernel, the Cholesky's update).
+// host holds packed random values for the operands (at least 1024 * 1024 elements).
+template <int L>
+static int tune_time_L(int device, int S, const uint32_t *host, double *ms_long, double *ms_short) {
+    const int W = L + 2;
+    const int64_t n1 = 512, n2 = 1024, k2 = 32;
+    const int64_t cnt[4] = {n1 * n1, n1 * n1, n2 * k2, n2 * n2};  // A, C1, P, C2
+    uint32_t *d[4] = {nullptr, nullptr, nullptr, nullptr}, *stg = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SDPG_OK;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&st) != hipSuccess) return SDPG_EHIP;
+    {
+        GemmEngine<L> ge;
+        ge.init(st);
+        bool ok = hipMalloc((void **)&stg, sizeof(uint32_t) * W * n2 * n2) == hipSuccess &&
+                  hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+        for (int q = 0; q < 4 && ok; q++) {
+            ok = hipMalloc((void **)&d[q], sizeof(uint32_t) * (L + 1) * cnt[q]) == hipSuccess &&
+                 hipMemcpyAsync(stg, host, sizeof(uint32_t) * W * cnt[q], hipMemcpyHostToDevice, st) == hipSuccess;
+            if (ok) k_unpack<L><<<cdiv(cnt[q], 256), 256, 0, st>>>(stg, PV{d[q], cnt[q]}, 0, cnt[q]);
+            ok = ok && hipStreamSynchronize(st) == hipSuccess;
+        }
+        if (!ok) rc = SDPG_ENOMEM;
+        PV A{d[0], cnt[0]}, C1{d[1], cnt[1]}, P{d[2], cnt[2]}, C2{d[3], cnt[3]};
+        for (int shape = 0; shape < 2 && !rc; shape++) {
+            float tot = 0;
+            for (int r = 0; r < 13 && !rc; r++) {
+                if (r == 3) (void)hipEventRecord(e0, st);
+                int g = shape == 0 ? ge.gemm((int)n1, (int)n1, (int)n1, A, 0, (int)n1, false, A, 0, (int)n1, false, C1, 0,
+                                             (int)n1, GEMM_SET, 1, 0, 0, 0, SDPG_GEMM_OZAKI, S)
+                                   : ge.gemm((int)n2, (int)n2, (int)k2, P, 0, (int)n2, false, P, 0, (int)n2, true, C2, 0,
+                                             (int)n2, GEMM_SUB, 1, 0, 0, 0, SDPG_GEMM_OZAKI, S, true);
+                if (g) rc = g == -1 ? SDPG_EARG : SDPG_EHIP;
+            }
+            if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&tot, e0, e1) != hipSuccess))
+                rc = SDPG_EHIP;
+            if (!rc) *(shape == 0 ? ms_long : ms_short) = tot / 10.0;
+        }
+    }
+    for (int q = 0; q < 4; q++) (void)hipFree(d[q]);
+    (void)hipFree(stg);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
 // per-limb-count entry points, defined in engine_inst.hip
 #define SDPG_DECL_L_(LL)                                                                                      \
     EngineBase *new_engine_##LL();                                                                          \
     int init_engine_##LL(EngineBase *e);                                                                    \
     int rgemm_##LL(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,      \
                    const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower, \
-                   double *ms, uint8_t *scratch, size_t scratch_bytes);
+                   double *ms, uint8_t *scratch, size_t scratch_bytes);                                     \
+    int tune_time_##LL(int device, int S, const uint32_t *host, double *ms_long, double *ms_short);
 SDPG_FOR_EACH_L(SDPG_DECL_L_)
 #undef SDPG_DECL_L_
 

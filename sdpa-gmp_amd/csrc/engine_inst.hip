@@ -22,4 +22,8 @@ int SDPG_CAT_(rgemm_, SDPG_INST_L)(int device, int transa, int transb, int m, in
                                 ms, scratch, scratch_bytes);
 }
 
+int SDPG_CAT_(tune_time_, SDPG_INST_L)(int device, int S, const uint32_t *host, double *ms_long, double *ms_short) {
+    return tune_time_L<SDPG_INST_L>(device, S, host, ms_long, ms_short);
+}
+
 }  // namespace sdpg

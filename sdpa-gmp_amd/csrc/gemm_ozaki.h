@@ -206,9 +206,11 @@ __device__ __forceinline__ mpx::mp<L> oz_value(const uint32_t (&R)[S / 4], int64
     uint32_t lz = mpx::lead_zeros<NX>(x);
     if (lz < 32u * NX) {
         mpx::shl<NX>(x, lz);
+        // the top L + 1 words of the accumulator; below an accumulator narrower than that (S < 4L - 4:
+        // fewer fraction bits than the result carries) the result's low words are zero
         uint32_t y[L + 1];
 #pragma unroll
-        for (int q = 0; q <= L; q++) y[q] = x[NX - 1 - L + q];
+        for (int q = 0; q <= L; q++) y[q] = NX - 1 - L + q >= 0 ? x[NX - 1 - L + q >= 0 ? NX - 1 - L + q : 0] : 0u;
         int32_t E = ei + ej;
         E = (E < -(1 << 28) ? 0 : E) - 12 - 8 * (S - 1);
         int32_t ex = 32 * NX - (int32_t)lz + E;
@@ -254,7 +256,6 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
                                                  const int32_t *__restrict__ eb, int nbA, int nbB, int Mp, int Np,
                                                  int nch) {
     constexpr int NW = S / 4, NU = (S + 3) / 4;
-    static_assert(NW + 2 >= L + 1, "accumulator narrower than the result");
     static_assert(S <= 48, "digit sums of OZ_FOLD chunks must stay below 2^31");
     __shared__ oz_v4i lA[S * OZ_T], lB[S * OZ_T];
     const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T, b = blockIdx.z;
@@ -410,7 +411,8 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 // (without this, each wave streams its row fragments from HBM: about 8 TB/s at the MFMA rate).
 constexpr int OZK_KC = 64;
 constexpr int OZK_CARRY = 32;
-constexpr int OZK_SMAX = 40;  // register budget: S accumulators + S column fragments per lane
+constexpr int OZK_SMAX = 48;  // register budget: S accumulators (AGPRs) + S column fragments per lane
+constexpr int OZK_SMIN = 12;  // at least 4 column fragments prefetched into LDS
 
 // row-fragment prefetch distance (batches): smaller for the wide slice count, whose column fragments
 // take more registers
@@ -468,7 +470,10 @@ constexpr int ozk_wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4)
 // batches of the descending order, are loaded at the start of their chunk
 template <int S>
 struct OzkQL {
-    static constexpr int value = (S - 8 < 28) ? S - 8 : 28;
+    // LDS per wave: QL KB of prefetch + (S + 8) 256-byte accumulator rows, at most 40 KB (four waves
+    // per CU in its 160 KB)
+    static constexpr int lds_cap = (40960 - (S + 8) * 256) / 1024;
+    static constexpr int value = (S - 8 < 28 ? S - 8 : 28) < lds_cap ? (S - 8 < 28 ? S - 8 : 28) : lds_cap;
 };
 constexpr int OZK_LA = 4;  // batches between reading a column fragment from LDS and its first use
 
@@ -623,6 +628,10 @@ template <int L>
 struct OzSlices {
     static constexpr int DEF = 4 * L - 4;
     static constexpr int WIDE = 4 * L;
+    // the slice counts built (multiples of 4, at least 8): 4L - 12, 4L - 8, 4L - 4, 4L; a request
+    // below the full limb width (a 200-bit request runs on 320-bit limbs) needs fewer than 4L - 4
+    static constexpr int MIN = 4 * L - 12 >= 8 ? 4 * L - 12 : 8;
+    static constexpr bool built(int s) { return s % 4 == 0 && s >= MIN && s <= WIDE; }
 };
 
 template <int L>
@@ -657,7 +666,7 @@ struct OzakiWork {
         cap = n;
         return 0;
     }
-    static int slices_ok(int s) { return L <= OZ_LMAX && (s == OzSlices<L>::DEF || s == OzSlices<L>::WIDE); }
+    static int slices_ok(int s) { return L <= OZ_LMAX && OzSlices<L>::built(s); }
     OpCache<int8_t> kept;  // slices and exponents of operands that never change
     template <int S>
     int run_s(const GemmArgs &g, hipStream_t st, int keep) {
@@ -666,7 +675,7 @@ struct OzakiWork {
         int Mp = (g.M + OZ_T - 1) / OZ_T * OZ_T, Np = (g.N + OZ_T - 1) / OZ_T * OZ_T;
         // the long-K kernel addresses one batch's slice planes with 32-bit buffer offsets
         const int64_t nch64 = (g.K + OZK_KC - 1) / OZK_KC;
-        const bool longk = S <= OZK_SMAX && g.K > OZ_LONGK_MIN &&
+        const bool longk = S <= OZK_SMAX && S >= OZK_SMIN && g.K > OZ_LONGK_MIN &&
                            nch64 * S * (int64_t)std::max(Mp, Np) * OZK_KC < ((int64_t)1 << 32);
         const int KC = longk ? OZK_KC : OZ_KC;
         int nbA = g.sA ? g.batch : 1, nbB = g.sB ? g.batch : 1;
@@ -750,7 +759,7 @@ struct OzakiWork {
         hipEvent_t ev = prof ? prof->begin(st) : nullptr;
         if (longk) {
             const int64_t nwg = ozk_workgroups(Mp / 16, Np / 16, g.batch);
-            if constexpr (S <= OZK_SMAX)
+            if constexpr (S <= OZK_SMAX && S >= OZK_SMIN)
                 k_gemm_ozk<L, S><<<dim3((unsigned)nwg, 1), 64, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp,
                                                                         same ? Mp : Np, nch);
         } else {
@@ -770,8 +779,13 @@ struct OzakiWork {
             return -5;
         } else {
             if (slices <= 0) slices = OzSlices<L>::DEF;
-            if (slices == OzSlices<L>::WIDE) return run_s<OzSlices<L>::WIDE>(g, st, keep);
-            if (slices == OzSlices<L>::DEF) return run_s<OzSlices<L>::DEF>(g, st, keep);
+            if (!OzSlices<L>::built(slices)) return -1;
+            if (slices == 4 * L) return run_s<4 * L>(g, st, keep);
+            if (slices == 4 * L - 4) return run_s<4 * L - 4>(g, st, keep);
+            if constexpr (4 * L - 8 >= 8)
+                if (slices == 4 * L - 8) return run_s<4 * L - 8>(g, st, keep);
+            if constexpr (4 * L - 12 >= 8)
+                if (slices == 4 * L - 12) return run_s<4 * L - 12>(g, st, keep);
             return -1;
         }
     }

@@ -205,6 +205,7 @@ struct Solver {
     std::vector<T> X, Z, invZ, y;  // initial point only; the iterates live on the device
     std::vector<int> useFormula;
     sdpg_ctx *ctx = nullptr;
+    int used_slices = 0;  // Ozaki slice count in use (0: another GEMM form)
     Parameter prm;
     FILE *logf = nullptr;
     std::string err;
@@ -419,6 +420,13 @@ struct Solver {
         if ((rc = sdpg_set_b(ctx, pb.data()))) return rc;
         if ((rc = sdpg_set_formula(ctx, useFormula.data()))) return rc;
         if ((rc = sdpg_commit(ctx))) return rc;
+        // Ozaki form with the slice count left open: the autotuned count for the requested precision
+        // (the default 4L - 4 when the tuner cannot run, e.g. without the L + 2 limb build)
+        if (gemm_mode == SDPG_GEMM_OZAKI && slices == 0) {
+            int tuned = 0;
+            if (sdpg_tune_slices(prm.precision, device, prm.precision, &tuned, nullptr, 0) == SDPG_OK) slices = tuned;
+        }
+        used_slices = gemm_mode == SDPG_GEMM_OZAKI ? (slices ? slices : 4 * L - 4) : 0;
         return sdpg_set_gemm(ctx, gemm_mode, slices);
     }
 
@@ -766,6 +774,7 @@ struct Solver {
             res->t_eig_ms = tm[7];
             res->t_total_s = t1 - t0;
             res->limbs = L;
+            res->slices = used_slices;
         }
         if (logf) {
             fprintf(logf, "\nphase.value = %s\n", phase_str(rphase));

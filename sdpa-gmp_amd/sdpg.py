@@ -37,6 +37,7 @@ EXPORTS = [
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
+    "sdpg_tune_slices",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula",
 ]
 
@@ -83,6 +84,8 @@ def load(path: str | None = None):
     lib.sdpg_rgemm.argtypes = [ctypes.c_int] * 7 + [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p] + \
         [ctypes.c_int] * 5 + [P(ctypes.c_double)]
     lib.sdpg_rgemm_noalloc.argtypes = lib.sdpg_rgemm.argtypes
+    lib.sdpg_tune_slices.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
+                                     ctypes.c_int]
     lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            P(ctypes.c_void_p), P(ctypes.c_int), P(ctypes.c_longlong),
                                            P(ctypes.c_int)]
@@ -194,6 +197,19 @@ def rgemm(prec_bits: int, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = N
     if rc:
         raise SdpgError(f"{fn.__name__}: {rc}: {lib.sdpg_last_error().decode()}", rc)
     return np.ascontiguousarray(np.transpose(c, (1, 0, 2))), ms.value
+
+
+def tune_slices(prec_bits: int, target_bits: int = 0, device: int = 0):
+    """sdpg_tune_slices: (chosen slice count, [(S, effective bits, ms of a 512^3 product, ms of a
+    1024 x 1024 x 32 lower update) for every slice count built])"""
+    lib = load()
+    S = ctypes.c_int()
+    tab = (ctypes.c_double * 64)()
+    rc = lib.sdpg_tune_slices(prec_bits, device, target_bits, ctypes.byref(S), tab, 64)
+    if rc:
+        raise SdpgError(f"sdpg_tune_slices: {rc}: {lib.sdpg_last_error().decode()}", rc)
+    rows = [tuple(tab[q:q + 4]) for q in range(0, 64, 4) if tab[q] > 0]
+    return S.value, rows
 
 
 # ----------------------------------------------------------------- context
@@ -312,7 +328,8 @@ class Result(ctypes.Structure):
                 ("t_schur_ms", ctypes.c_double), ("t_chol_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_gvec_ms", ctypes.c_double), ("t_dxdz_ms", ctypes.c_double), ("t_res_ms", ctypes.c_double),
                 ("t_total_s", ctypes.c_double), ("limbs", ctypes.c_int), ("t_inv_ms", ctypes.c_double),
-                ("t_eig_ms", ctypes.c_double), ("objP70", ctypes.c_char * 112), ("objD70", ctypes.c_char * 112)]
+                ("t_eig_ms", ctypes.c_double), ("objP70", ctypes.c_char * 112), ("objD70", ctypes.c_char * 112),
+                ("slices", ctypes.c_int)]
 
 
 def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path: str | None = None,

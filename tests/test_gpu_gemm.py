@@ -162,10 +162,73 @@ def test_rgemm_rejects_bad_slices():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("S", [4 * L - 12, 4 * L - 8, 4 * L - 4, 4 * L])
+@pytest.mark.parametrize("k", [32, 300], ids=["K32", "K300"])
+def test_rgemm_every_slice_count(S, k):
+    """every slice count built at 256 bits (28 .. 40), short-K and long-K kernels, within
+    K a_i b_j 2^-(8S-12); below S = 4L - 4 the accumulator is narrower than the result's limbs"""
+    rnd = random.Random(31 * S + k)
+    m, n = 45, 38
+    print(f"case rgemm-slices S={S} k={k}", flush=True)
+    Av = _rand_matrix(rnd, m, k)
+    Bv = _rand_matrix(rnd, n, k)
+    Cv = _rand_matrix(rnd, m, n)
+    A, B, ex = _exact(Av, Bv, Cv, False, True, 1)
+    got, _ = sdpg.rgemm(PREC, _pack2(Av), _pack2(Bv), _pack2(Cv), False, True, 1, sdpg.GEMM_OZAKI, S)
+    worst = _check(got, A, B, Cv, ex, 1, 8 * S - 12)
+    print(f"  worst error / bound = {worst:.3e}")
+
+
+def _check_prec(got, Av, Bv, Cv, op, LL, bits_lost):
+    """the bar of _check at another limb count"""
+    m, n, k = len(Av), len(Bv[0]), len(Bv)
+    amax = [max(abs(v) for v in Av[i]) for i in range(m)]
+    bmax = [max(abs(Bv[t][j]) for t in range(k)) for j in range(n)]
+    ulp = Fraction(1, 1 << (32 * LL - 2))
+    for i in range(m):
+        for j in range(n):
+            ex = sum(Av[i][t] * Bv[t][j] for t in range(k))
+            ex = Cv[i][j] - ex if op == 1 else ex
+            tol = k * amax[i] * bmax[j] * Fraction(1, 1 << bits_lost) + abs(ex) * ulp + abs(Cv[i][j]) * ulp
+            assert abs(sdpg.words_to_frac(got[i, j], LL) - ex) <= tol, (i, j)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("S", [44, 48], ids=["S44", "S48"])
+@pytest.mark.parametrize("k", [40, 150, 1326], ids=["K40", "K150", "K1326"])
+def test_rgemm_ozaki_320bit(S, k):
+    """320 bits (L = 12): the short-K kernel (K = 40) and the long-K kernel at S = 44 and 48 (K = 150
+    ends in a 22-k chunk; K = 1326 is the dense rank-2 workload's constraint count, 21 chunks)"""
+    prec = 320
+    LL = sdpg.limbs_for_bits(prec)
+    rnd = random.Random(S + k)
+    m, n = 36, 33
+    print(f"case rgemm-320 S={S} k={k}", flush=True)
+
+    def rand(r, c):
+        out = []
+        for _ in range(r):
+            row = []
+            for _ in range(c):
+                M = rnd.getrandbits(32 * LL) | (1 << (32 * LL - 1))
+                v = Fraction(M, 1 << (32 * LL)) * Fraction(2) ** rnd.randint(-20, 20)
+                row.append(-v if rnd.random() < 0.5 else v)
+            out.append(row)
+        return out
+
+    Av, Bv, Cv = rand(m, k), rand(k, n), rand(m, n)
+
+    def pack(vals):
+        return np.array([[sdpg.frac_to_words(v, LL) for v in row] for row in vals], dtype=np.uint32)
+
+    got, _ = sdpg.rgemm(prec, pack(Av), pack(Bv), pack(Cv), False, False, 1, sdpg.GEMM_OZAKI, S)
+    _check_prec(got, Av, Bv, Cv, 1, LL, 8 * S - 12)
+
+
+@pytest.mark.gpu
 def test_rgemm_ozaki_320bit_long_k():
-    """At 320 bits (L = 12, S = 44) every Ozaki product runs on the short-K kernel, whatever K; K = 1100
-    (69 chunks of 16 k) passes its fold of the INT32 digit sums into the exact accumulator, which
-    happens every 64 chunks.  Small integers: the exact product sum (no rounding anywhere)."""
+    """At 320 bits (L = 12, S = 44) a K = 1100 product runs on the long-K kernel (18 chunks of 64 k);
+    small integers: the exact product sum (no rounding anywhere)."""
     prec, k = 320, 1100
     LL = sdpg.limbs_for_bits(prec)
     rnd = random.Random(23)
