@@ -10,32 +10,38 @@ For each GEMM form the solver first runs to convergence (untimed here; its per-i
 times over the whole trajectory are reported), then a second run stops at iteration `--at`
 and a "step" = one Make_bMat + Cholesky (sdpg_schur + sdpg_potrf) on that iterate, resident in
 HBM, is run `--warmup` times untimed and `--steps` times timed.  value = wall seconds per step of
-the Ozaki form, timed with nothing but the step's own kernels in the stream; the per-kernel times
-(events around every product and diagonal-block launch) come from a second pass over the same
-number of steps.
+the Ozaki form (slice count from the autotuner), timed with nothing but the step's own kernels in the
+stream; the per-kernel times (events around every product and diagonal-block launch) come from a
+second pass over the same number of steps.
 
-roofline: the Ozaki MFMA product kernel, INT8 operations = 2 * S(S+1)/2 per mp multiply-add of the
-reference loop (S slices), over its event-timed average launch duration in the timed steps, against
-the 5.0 POP/s dense INT8 peak.  The per-kernel breakdown of the step is reported beside it, with the
-dominant kernel (the Cholesky diagonal-block pivot chain, a latency bound).
+roofline: the dominant kernel of the step, the Cholesky diagonal-block kernel k_potrf_diag (a serial
+chain of pivots in 320-bit limb arithmetic on the vector ALUs): algorithmic work = the reference's
+nb^3/6 multiple-precision multiply-adds per 32 x 32 diagonal block (Rpotrf2's work), at L^2 = 64 limb
+multiply-adds each for a 256-bit request (L = 8 limbs of the request), over its event-timed average
+launch, against the measured integer-VALU peak (sdpg_valu_peak: sustained v_mad_u64_u32 + v_addc
+limb multiply-adds of the whole device).  kernel_rooflines: the short-K Ozaki product kernel
+(INT8 peak), the plain limb product kernel (VALU peak) and, on the dense rank-2 workload, the long-K
+Ozaki kernel (INT8 peak).
 
-also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank-2 constraints, all F1),
-replayed on its first iterate for both forms (2 timed steps), with the roofline of the long-K Ozaki
-kernel that carries its F1 products and Schur inner products.
+also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank-2 constraints, all F1)
+replayed on its first iterate for both forms at 256, 320 and 200 bits (Ozaki slice counts from the
+autotuner for each precision), 2 timed steps each.
 
 also_measured_batch: the first --batch problems of the seeded batch workload (512 small SDPs, 1-3
 dense blocks of U[8,40], m in U[10,120]) solved to gap <= 1e-60 as concurrent independent solves
 (--batch-threads host threads, one context and stream each): solves/s and latency p50 / p99.
 
-cpu_baseline: the oracle (C restatement of the reference's mpf path, serial Cholesky as the
-reference's Rpotrf) on the same iterate, dumped by the GPU run: Make_bMat in full and the Cholesky of
-the leading --cpu-n rows/columns of B, scaled by (m / cpu_n)^3 to the full factorisation.
+cpu_baseline: the oracle (C restatement of the reference's mpf path: Make_bMat and the reference's
+serial Rpotrf, one core as the reference runs them) on the same iterate, dumped by the GPU run: the
+whole Make_bMat and the whole 1000 x 1000 Cholesky.  also_measured carries a bounded CPU sample of
+workload 3 (F1 precompute with the reference's OpenMP Rgemm on the host's cores, serial inner products).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -47,7 +53,6 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 PARAM = os.path.join(PKG, "data", "param_256_gap1e-60.sdpa")
 INT8_PEAK = 5.0e15  # MI355X dense INT8 MFMA, ops/s
-HBM_PEAK = 8.0e12
 WORKLOADS = {
     2: "max-cut n=m=1000, one block, A_i=e_i e_i^T, C=-L/4 of G(1000,0.01) with weights +-1",
     3: "one block n=150, m=1326, A_i = u v^T + v u^T (dense), feasible by construction",
@@ -57,24 +62,47 @@ WORKLOADS = {
 
 
 def structure(path: str) -> dict:
-    """m, block sizes and the reference's mp multiply-add count of one Make_bMat + Cholesky."""
+    """m and the block sizes of a problem file"""
     with open(path) as f:
         lines = [ln for ln in f if not ln.startswith('"') and not ln.startswith("*")]
     m = int(lines[0].split()[0])
     nb = int(lines[1].split()[0])
     blocks = [int(x) for x in lines[2].replace("=", " ").split() if x.lstrip("-").isdigit()][:nb]
-    return {"m": m, "blocks": blocks, "chol_mp_mac": m ** 3 / 6.0}
+    return {"m": m, "blocks": blocks}
 
 
-def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve: bool, dump: str | None):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_cores() -> int:
+    """cores this process may use (the box's CPU share), at most 16"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve: bool, dump: str | None,
+            prec: int = 0, data: str | None = None):
     """Replay [Make_bMat + Cholesky] steps of one workload for each GEMM form (after an optional full
     solve); returns (structure, sha256, per-form results)."""
     import gen_problems
     import sdpg
 
-    tmp = tempfile.mkdtemp(prefix="sdpg_bench_")
-    data = os.path.join(tmp, f"w{workload}.dat-s")
-    sha = gen_problems.generate(workload, data)
+    sha = None
+    if data is None:
+        tmp = tempfile.mkdtemp(prefix="sdpg_bench_")
+        data = os.path.join(tmp, f"w{workload}.dat-s")
+        sha = gen_problems.generate(workload, data)
     info = structure(data)
     modes = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}
     res = {}
@@ -83,11 +111,12 @@ def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve:
         r_out = {}
         if solve:
             t0 = time.time()
-            r = sdpg.solve(data, PARAM, 0, None, mode)
+            r = sdpg.solve(data, PARAM, prec, None, mode)
             t_solve = time.time() - t0
             it = max(r.iterations, 1)
             r_out = {"iterations": r.iterations, "phase": r.phase.decode().strip(), "relgap": r.relgap.decode(),
                      "objP": r.objP.decode(), "objD": r.objD.decode(), "solve_wall_s": t_solve,
+                     "slices": r.slices,
                      "traj_ms_per_iter": {"schur": r.t_schur_ms / it, "chol": r.t_chol_ms / it,
                                           "solve": r.t_solve_ms / it, "gvec": r.t_gvec_ms / it,
                                           "dxdz": r.t_dxdz_ms / it, "inverse": r.t_inv_ms / it,
@@ -95,16 +124,16 @@ def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve:
             print(f"[bench] w{workload} {name}: solve {r.iterations} iterations, {r_out['phase']}, relgap "
                   f"{r_out['relgap']}, {t_solve:.1f} s; schur+chol {(r.t_schur_ms + r.t_chol_ms) / it:.2f} ms/iter "
                   f"over the trajectory", flush=True)
-        ms_s, ms_c, ms_w, kst = sdpg.bench(data, PARAM, 0, mode, 0, at, warmup, steps,
+        ms_s, ms_c, ms_w, kst = sdpg.bench(data, PARAM, prec, mode, 0, at, warmup, steps,
                                            dump_prefix=dump if (idx == 0 and dump) else None)
-        print(f"[bench] w{workload} {name}: {steps} steps on iterate {at}: schur {ms_s:.3f} ms, potrf {ms_c:.3f} ms, "
-              f"wall {ms_w:.3f} ms per step; kernels {kst}", flush=True)
+        print(f"[bench] w{workload} {name} prec {prec or 256}: {steps} steps on iterate {at}: schur {ms_s:.3f} ms, "
+              f"potrf {ms_c:.3f} ms, wall {ms_w:.3f} ms per step; kernels {kst}", flush=True)
         r_out.update({"step_schur_ms": ms_s, "step_chol_ms": ms_c, "step_wall_ms": ms_w,
                       "kernels": {k: {"launches": v[0], "ms_per_step": v[1] / steps,
                                       "avg_launch_us": 1e3 * v[1] / max(v[0], 1)} for k, v in kst.items()},
                       "_kst": kst})
         res[name] = r_out
-    return info, sha, res
+    return info, sha, data, res
 
 
 def measure_batch(count: int, threads: int, gemm_name: str):
@@ -136,7 +165,7 @@ def measure_batch(count: int, threads: int, gemm_name: str):
     phases = {}
     for _, ph, _ in res:
         phases[ph] = phases.get(ph, 0) + 1
-    pct = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))]
+    pct = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))]  # noqa: E731
     out = {"problems": count, "host_threads": threads, "form": gemm_name, "wall_s": wall,
            "solves_per_s": count / wall, "latency_p50_s": pct(0.50), "latency_p99_s": pct(0.99),
            "mean_iterations": sum(x[2] for x in res) / count, "phases": phases, "generation_s": t_gen,
@@ -146,16 +175,37 @@ def measure_batch(count: int, threads: int, gemm_name: str):
     return out
 
 
-def roofline(kst_entry, kernel: str, traffic=None, note: str = ""):
-    """MFMA roofline of one product kernel from its live event timing (launches, ms, INT8 ops)."""
+def mfma_roofline(kst_entry, kernel: str, traffic=None, note: str = ""):
+    """INT8 MFMA roofline of an Ozaki product kernel from its live event timing (launches, ms, INT8 ops)."""
     n, ms, ops = kst_entry
     if not n:
         return None
     avg_s = ms / 1e3 / n
     ach = ops / n / avg_s
-    return {"kernel": kernel, "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TFLOP/s",
+    return {"kernel": kernel, "bound": "mfma", "achieved": ach / 1e12, "peak": INT8_PEAK / 1e12, "unit": "TOP/s",
             "frac": ach / INT8_PEAK, "traffic": traffic, "ops_per_launch": ops / n, "avg_launch_us": avg_s * 1e6,
             "note": note}
+
+
+def valu_roofline(kst_entry, kernel: str, peak: float, limbs_per_mac: int, traffic=None, note: str = ""):
+    """Integer-VALU roofline of a limb-arithmetic kernel: mp multiply-adds x limb multiply-adds each over
+    its event-timed average launch, against the measured limb multiply-add peak."""
+    n, ms, mp_macs = kst_entry
+    if not n:
+        return None
+    avg_s = ms / 1e3 / n
+    ach = mp_macs * limbs_per_mac / n / avg_s
+    return {"kernel": kernel, "bound": "valu", "achieved": ach / 1e12, "peak": peak / 1e12,
+            "unit": "T limb-MAD/s", "frac": ach / peak, "traffic": traffic,
+            "work_per_launch": mp_macs * limbs_per_mac / n, "avg_launch_us": avg_s * 1e6, "note": note}
+
+
+def load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    return None
 
 
 def main():
@@ -170,7 +220,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the second measured workload (n=150, m=1326 dense rank-2 constraints)")
-    ap.add_argument("--cpu-n", type=int, default=500, help="Cholesky sample size of the CPU baseline")
+    ap.add_argument("--extra-precs", default="256,320,200", help="precisions of the second workload's replays")
+    ap.add_argument("--cpu-n", type=int, default=0, help="Cholesky sample size of the CPU baseline (0: all of B)")
     ap.add_argument("--batch", type=int, default=128,
                     help="problems of the seeded 512-problem batch solved as concurrent independent solves (0: skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
@@ -184,9 +235,17 @@ def main():
 
     L = sdpg.limbs_for_bits(256)
     forms = a.forms.split(",")
+    peak = sdpg.valu_peak()
+    print(f"[bench] integer-VALU peak: {peak / 1e12:.2f} T limb-MAD/s", flush=True)
+    tuned = {}
+    for prec in sorted({256} | {int(p) for p in a.extra_precs.split(",") if p}):
+        S, rows = sdpg.tune_slices(prec)
+        tuned[prec] = {"slices": S, "candidates": [{"slices": int(r[0]), "effective_bits": round(r[1], 1),
+                                                    "ms_512cube": r[2], "ms_1024sq_k32_lower": r[3]} for r in rows]}
+        print(f"[bench] slice autotuner, {prec} bits: S = {S} ({tuned[prec]['candidates']})", flush=True)
     tmp = tempfile.mkdtemp(prefix="sdpg_dump_")
     dump = os.path.join(tmp, "it_")
-    info, sha, res = measure(a.workload, forms, a.at, a.warmup, a.steps, not a.no_solve, dump)
+    info, sha, data2, res = measure(a.workload, forms, a.at, a.warmup, a.steps, not a.no_solve, dump)
 
     main_form = "ozaki" if "ozaki" in res else list(res)[0]
     step_ms = res[main_form]["step_wall_ms"]
@@ -200,32 +259,34 @@ def main():
         "dtype": f"mp{32 * L}",
         "data": "synthetic (seeded instance from gen_problems.py; no external data)",
         "config": {"workload": f"{wl}, 256-bit, epsilonStar=epsilonDash=1e-60", "form": main_form,
-                   "replay_iteration": a.at, "input_sha256": sha},
+                   "replay_iteration": a.at, "input_sha256": sha, "ozaki_slices": tuned[256]["slices"],
+                   "slice_autotuner": tuned},
     }
     kst = res[main_form]["_kst"]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_gemm_oz_maxcut1000_traffic.json")
-    if a.workload == 2 and os.path.exists(tfile):  # rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated
-        with open(tfile) as f:
-            traffic = json.load(f)["traffic_bytes_per_launch"]
-    if main_form == "ozaki":
-        rl = roofline(kst["ozaki_mfma"], "k_gemm_oz (Ozaki INT8 slice products, short K, v_mfma_i32_16x16x64_i8)",
-                      traffic, "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36 slices; the Cholesky panel and "
-                               "trailing-update products (K = 32); traffic from "
-                               "profiles/k_gemm_oz_maxcut1000_traffic.txt")
-        if rl:
-            out["roofline"] = rl
     shares = {k: v[1] / a.steps / step_ms for k, v in kst.items()}
     dom = max(shares, key=shares.get)
+    # the mp multiply-adds of one launch are nb^3/6 (Rpotrf2's work on a 32 x 32 block); 64 limb
+    # multiply-adds each for a 256-bit request (8 limbs)
+    limbs_req = 8
+    rl = valu_roofline(kst["chol_diag"], "k_potrf_diag (Cholesky 32 x 32 diagonal blocks: Bareiss pivot chain "
+                                         "and block inverse, 8 workgroups)", peak, limbs_req * limbs_req,
+                       load_traffic("k_potrf_diag_maxcut1000_traffic.json"),
+                       "work = nb^3/6 mp multiply-adds per block x 64 limb multiply-adds (256-bit request); "
+                       "a serial chain of 32 pivots per launch, each one fused multiply-multiply-subtract per "
+                       "entry in 320-bit limb arithmetic: latency-bound, not throughput-bound")
+    if rl:
+        out["roofline"] = rl
+    kr = {}
+    if main_form == "ozaki":
+        kr["k_gemm_oz"] = mfma_roofline(kst["ozaki_mfma"], "k_gemm_oz (Ozaki INT8 slice products, short K)",
+                                        load_traffic("k_gemm_oz_maxcut1000_traffic.json"),
+                                        f"INT8 ops: 2 S(S+1)/2 per mp multiply-add, S = {tuned[256]['slices']}; "
+                                        "Cholesky panel and trailing-update products (K = 32)")
+    if "plain" in res:
+        kr["k_gemm_plain"] = valu_roofline(res["plain"]["_kst"]["plain_limb"], "k_gemm_plain (plain limb products)",
+                                           peak, limbs_req * limbs_req, None,
+                                           "plain form, max-cut Cholesky products (K = 32)")
     out["step_breakdown"] = {"share_of_step": shares, "dominant": dom}
-    if dom == "chol_diag":
-        n, ms, work = kst["chol_diag"]
-        out["step_breakdown"]["dominant_kernel"] = {
-            "kernel": "k_potrf_diag (32 x 32 diagonal blocks of the Cholesky, 8 workgroups each: the pivot "
-                      "chain in every one, the inverse's columns split over them)",
-            "bound": "VALU issue of one CU per pivot step, a serial chain of m pivots: one fused "
-                     "multiply-multiply-subtract in 320-bit limb arithmetic per entry, in rounds of 256 entries",
-            "avg_launch_us": 1e3 * ms / max(n, 1), "us_per_pivot": 1e3 * ms / max(n, 1) / 32}
     for f in res.values():
         f.pop("_kst", None)
     out["forms"] = res
@@ -236,30 +297,37 @@ def main():
             pa, po = Decimal(res["plain"]["objP"]), Decimal(res["ozaki"]["objP"])
             out["objective_rel_diff_ozaki_vs_plain"] = float(abs(pa - po) / max(abs(pa), Decimal(1)))
 
+    dump3 = os.path.join(tmp, "w3_")
     if not a.no_extra and a.workload != 3:
         # the GEMM-heavy workload: F1 products (A_i Z^-1, X F_i: 2 m n^3) and the Schur inner products
         # (m(m+1)/2 x n(n+1)/2) dominate, the regime where the INT8 form pays
-        info3, sha3, res3 = measure(3, forms, 1, 1, 2, False, None)
-        ex = {"workload": WORKLOADS[3] + ", 256-bit", "input_sha256": sha3, "replay_iteration": 1, "steps": 2}
-        for name, r in res3.items():
-            ex[name] = {"s_per_iter": r["step_wall_ms"] / 1e3, "schur_ms": r["step_schur_ms"],
-                        "chol_ms": r["step_chol_ms"], "kernels": r["kernels"]}
-        if "ozaki" in res3 and "plain" in res3:
-            ex["ozaki_speedup_vs_plain"] = res3["plain"]["step_wall_ms"] / max(res3["ozaki"]["step_wall_ms"], 1e-9)
-        if "ozaki" in res3:
-            traffic3 = None
-            tfile3 = os.path.join(ROOT, "profiles", "k_gemm_ozk_dense_rank2_m1326_traffic.json")
-            if os.path.exists(tfile3):
-                with open(tfile3) as f:
-                    traffic3 = json.load(f)["traffic_bytes_per_launch"]
-            rl3 = roofline(res3["ozaki"]["_kst"]["ozaki_mfma_longk"],
-                           "k_gemm_ozk (Ozaki INT8 slice products, long K, v_mfma_i32_16x16x64_i8)", traffic3,
-                           "INT8 ops: 2 * S(S+1)/2 per mp multiply-add, S = 36; F1 products (K = n) and Schur "
-                           "inner products (K = n(n+1)/2); traffic (mean of the three launches) from "
-                           "profiles/k_gemm_ozk_dense_rank2_m1326_traffic.txt")
-            if rl3:
-                ex["roofline"] = rl3
+        ex = {"workload": WORKLOADS[3], "replay_iteration": 1, "steps": 2, "by_precision": {}}
+        data3 = None
+        for prec in [int(p) for p in a.extra_precs.split(",") if p]:
+            info3, sha3, data3, res3 = measure(3, forms, 1, 1, 2, False, dump3 if prec == 256 else None,
+                                               prec=prec, data=data3)
+            if sha3:
+                ex["input_sha256"] = sha3
+            e = {"slices": tuned[prec]["slices"]}
+            for name, r in res3.items():
+                e[name] = {"s_per_iter": r["step_wall_ms"] / 1e3, "schur_ms": r["step_schur_ms"],
+                           "chol_ms": r["step_chol_ms"], "kernels": r["kernels"]}
+            if "ozaki" in res3 and "plain" in res3:
+                e["ozaki_speedup_vs_plain"] = res3["plain"]["step_wall_ms"] / max(res3["ozaki"]["step_wall_ms"], 1e-9)
+            if "ozaki" in res3:
+                S = tuned[prec]["slices"]
+                e["roofline_k_gemm_ozk"] = mfma_roofline(
+                    res3["ozaki"]["_kst"]["ozaki_mfma_longk"], "k_gemm_ozk (Ozaki INT8 slice products, long K)",
+                    load_traffic("k_gemm_ozk_dense_rank2_m1326_traffic.json") if prec == 256 else None,
+                    f"INT8 ops: 2 S(S+1)/2 per mp multiply-add, S = {S}; F1 products (K = n) and Schur inner "
+                    f"products (K = n(n+1)/2)")
+            if "plain" in res3:
+                e["roofline_k_gemm_plain"] = valu_roofline(res3["plain"]["_kst"]["plain_limb"],
+                                                           "k_gemm_plain (plain limb products)", peak,
+                                                           ((prec + 31) // 32) ** 2)
+            ex["by_precision"][str(prec)] = e
         out["also_measured"] = ex
+    out["kernel_rooflines"] = kr
 
     if a.batch > 0:
         b = measure_batch(a.batch, a.batch_threads, "ozaki" if "ozaki" in forms else forms[0])
@@ -268,21 +336,33 @@ def main():
         out["also_measured_batch"] = b
 
     if not a.no_cpu and os.path.exists(dump + "X.bin"):
-        import gen_problems
         import oracle_lib
 
         if not os.path.exists(oracle_lib.ORC_LIB):
             oracle_lib.build_oracle()
-        data = os.path.join(tmp, f"w{a.workload}.dat-s")
-        gen_problems.generate(a.workload, data)
-        n = min(a.cpu_n, info["m"])
-        tb, tc = oracle_lib.time_point(data, 256, dump + "X.bin", dump + "invZ.bin", L, n, 1)
+        n = info["m"] if a.cpu_n <= 0 else min(a.cpu_n, info["m"])
+        tb, tc = oracle_lib.time_point(data2, 256, dump + "X.bin", dump + "invZ.bin", L, n, 1)
         scale = (info["m"] / n) ** 3
         out["cpu_baseline"] = {"value": tb + tc * scale, "unit": "s/iter", "cores": 1, "kind": "port",
-                               "sample": f"oracle (GMP-mpf-semantics C restatement, serial Rpotrf structure) on "
-                                         f"the GPU's iterate {a.at}: Make_bMat in full ({tb:.2f} s) and the "
-                                         f"Cholesky of B's leading {n}x{n} block ({tc:.2f} s) scaled by "
-                                         f"({info['m']}/{n})^3"}
+                               "cpu_model": cpu_model(),
+                               "sample": f"oracle (GMP-mpf-semantics C restatement of the reference's Make_bMat and "
+                                         f"serial Rpotrf; the reference runs both on one core) on the GPU's iterate "
+                                         f"{a.at}: Make_bMat in full ({tb:.2f} s) and the Cholesky of B's leading "
+                                         f"{n}x{n} block ({tc:.2f} s)" + (f" scaled by ({info['m']}/{n})^3"
+                                                                         if n < info["m"] else " (all of B)")}
+        if "also_measured" in out and os.path.exists(dump3 + "X.bin"):
+            cores = host_cores()
+            nsamp = 6
+            tcon, tpair = oracle_lib.time_f1_sample(data3, 256, dump3 + "X.bin", dump3 + "invZ.bin", L, nsamp, cores)
+            m3 = 1326
+            t_schur = m3 * tcon + m3 * (m3 + 1) / 2 * tpair
+            t_chol = tc * scale * (m3 / info["m"]) ** 3
+            out["also_measured"]["cpu_baseline_256"] = {
+                "value": t_schur + t_chol, "unit": "s/iter", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+                "sample": f"oracle on workload 3's iterate 1: F1 precompute of {nsamp} constraints (OpenMP Rgemm NN on "
+                          f"{cores} cores, {tcon:.3f} s each) and {nsamp * nsamp} inner products (serial, "
+                          f"{tpair * 1e3:.3f} ms each), scaled to 1326 constraints and 879801 pairs ({t_schur:.1f} s); "
+                          f"Cholesky from the max-cut measurement scaled by (1326/1000)^3 ({t_chol:.1f} s)"}
     print(json.dumps(out), flush=True)
 
 

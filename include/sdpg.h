@@ -236,6 +236,13 @@ int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m,
  */
 int sdpg_tune_slices(int prec_bits, int device, int target_bits, int *slices, double *table, int ntable);
 
+/*
+ * Measurement support (no reference counterpart): the sustained device rate of 32 x 32 -> 64-bit limb
+ * multiply-adds with carry (v_mad_u64_u32 + v_addc_co_u32), the integer-VALU peak that the plain limb
+ * kernels are quoted against; about 0.2 s of device time.
+ */
+int sdpg_valu_peak(int device, double *limb_mac_per_s);
+
 /* last error message of this thread */
 const char *sdpg_last_error(void);
 

@@ -2484,3 +2484,66 @@ int orc_potrf(int prec_bits, int n, int L, const uint32_t *a, uint32_t *lo, int 
     free(A);
     return 0;
 }
+
+/*
+ * Bounded CPU sample of Make_bMat for a problem whose SDP block 0 takes formula F1 for every
+ * constraint (compute_bMat_dense_SDP, sdpa_newton.cpp:1007-1096): for nsample constraints spread over
+ * the block, the F1 precompute work1 = A_i invZ, work2 = X work1 (the reference's OpenMP Rgemm NN,
+ * `threads` threads) and the inner products A_j . work2 with nsample partners each (serial, as the
+ * reference's Rdot).  Returns seconds per constraint's precompute and per pair's inner product, from
+ * which the caller scales to the whole Make_bMat.
+ */
+int orc_time_f1_sample(const char *data_path, int prec_bits, const char *x_path, const char *invz_path, int L,
+                       int nsample, int threads, double *t_per_con, double *t_per_pair) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    ompf_set_prec_bits(prec_bits);
+    Ctx c;
+    memset(&c, 0, sizeof c);
+    int rc = read_problem(data_path, &c.P);
+    if (rc) return rc;
+    Prob *P = &c.P;
+    if (P->nsdp < 1 || P->sdpNCon[0] < 1) {
+        free_problem(P);
+        return -30;
+    }
+    dls_init(&c.S.X, P);
+    dls_init(&c.S.invZ, P);
+    dls_init(&c.W.DLS1, P);
+    dls_init(&c.W.DLS2, P);
+    if ((rc = load_packed_dls(x_path, L, &c.S.X)) || (rc = load_packed_dls(invz_path, L, &c.S.invZ))) {
+        ctx_free(&c);
+        return -100 + rc;
+    }
+    const int n = P->sdpStruct[0], nc = P->sdpNCon[0];
+    const F *X = c.S.X.b[0], *invZ = c.S.invZ.b[0];
+    F *work1 = c.W.DLS1.b[0], *work2 = c.W.DLS2.b[0];
+    F one = Fv(1.0), acc;
+    ompf_set_zero(&acc);
+    double tg = 0, td = 0;
+    long pairs = 0;
+    for (int s = 0; s < nsample; s++) {
+        int k1 = (int)((long)s * nc / nsample);
+        int i = P->sdpCon[0][k1], ib = P->sdpBIdx[0][k1];
+        double t0 = now_s();
+        sp_mul_dense(work1, &P->A[i].s[ib], invZ);
+        gemm_nn(n, n, n, &one, X, n, work1, n, 0, work2, n);
+        double t1 = now_s();
+        for (int q = 0; q < nsample; q++) {
+            int k2 = (int)((long)q * nc / nsample);
+            F v = sp_inner(&P->A[P->sdpCon[0][k2]].s[P->sdpBIdx[0][k2]], work2);
+            ompf_add(&acc, &acc, &v);
+            pairs++;
+        }
+        double t2 = now_s();
+        tg += t1 - t0;
+        td += t2 - t1;
+    }
+    *t_per_con = tg / nsample;
+    *t_per_pair = pairs ? td / pairs : 0;
+    ctx_free(&c);
+    return 0;
+}

@@ -75,6 +75,14 @@ int orc_time_hotpath(const char *data_path, const orc_param *param, int prec_ove
 int orc_time_point(const char *data_path, int prec_bits, const char *x_path, const char *invz_path, int L,
                    int chol_n, int threads, double *t_bmat, double *t_chol);
 
+/*
+ * Bounded CPU sample of Make_bMat for an all-F1 block 0: seconds per constraint's F1 precompute
+ * (OpenMP Rgemm NN with `threads` threads) and per pair's inner product (serial), on nsample
+ * constraints x nsample partners of the iterate (X, invZ) in packed files of L limbs.
+ */
+int orc_time_f1_sample(const char *data_path, int prec_bits, const char *x_path, const char *invz_path, int L,
+                       int nsample, int threads, double *t_per_con, double *t_per_pair);
+
 /* arithmetic probe for tests: op codes 0 add,1 sub,2 mul,3 div,4 sqrt,5 set_str(a as text) */
 int orc_arith(int prec_bits, int op, int L, int32_t as, int32_t ae, const uint32_t *al, int32_t bs, int32_t be,
               const uint32_t *bl, int32_t *rs, int32_t *re, uint32_t *rl);

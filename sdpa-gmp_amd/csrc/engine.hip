@@ -41,6 +41,23 @@ static int scratch_address(int device, uint8_t **out) {
 }
 }  // namespace sdpg
 
+// Sustained rate of the plain form's inner operation, one 32 x 32 -> 64-bit limb multiply-add with its
+// carry (v_mad_u64_u32 + v_addc_co_u32): eight independent chains per lane, 8 workgroups of 256 per
+// CU.  This is the integer-VALU peak that the plain limb kernels and the pivot chain are quoted
+// against (a measurement, not a specification figure).
+__global__ __launch_bounds__(256) void k_valu_peak(uint32_t *out, int iters, uint32_t seed) {
+    uint32_t a0 = seed ^ threadIdx.x, a1 = a0 * 3u + 1u, a2 = a0 * 5u + 7u, a3 = a0 * 7u + 3u;
+    uint32_t b0 = a0 + 11u, b1 = a1 + 13u;
+    uint64_t l0 = 0, l1 = 0, l2 = 0, l3 = 0, l4 = 0, l5 = 0, l6 = 0, l7 = 0;
+    uint32_t h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0, h5 = 0, h6 = 0, h7 = 0;
+    for (int i = 0; i < iters; i++) {
+        mpx::mac4(l0, h0, a0, b0, l1, h1, a1, b0, l2, h2, a2, b0, l3, h3, a3, b0);
+        mpx::mac4(l4, h4, a0, b1, l5, h5, a1, b1, l6, h6, a2, b1, l7, h7, a3, b1);
+    }
+    uint64_t s = l0 ^ l1 ^ l2 ^ l3 ^ l4 ^ l5 ^ l6 ^ l7;
+    out[blockIdx.x * blockDim.x + threadIdx.x] = (uint32_t)s ^ (uint32_t)(s >> 32) ^ (h0 + h1 + h2 + h3 + h4 + h5 + h6 + h7);
+}
+
 // ------------------------------------------------------------------ C-ABI
 
 struct sdpg_ctx {
@@ -530,6 +547,42 @@ int sdpg_tune_slices(int prec_bits, int device, int target_bits, int *slices, do
     }
     *slices = best;
     return SDPG_OK;
+}
+
+int sdpg_valu_peak(int device, double *limb_mac_per_s) {
+    if (!limb_mac_per_s) return SDPG_EARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        set_err("sdpg_valu_peak: bad device ordinal");
+        return SDPG_ENODEV;
+    }
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) return SDPG_EHIP;
+    const int blocks = prop.multiProcessorCount * 8, threads = 256, iters = 20000;
+    uint32_t *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SDPG_OK;
+    float ms = 0;
+    if (hipMalloc((void **)&out, sizeof(uint32_t) * blocks * threads) != hipSuccess || hipStreamCreate(&st) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = SDPG_EHIP;
+    } else {
+        k_valu_peak<<<blocks, threads, 0, st>>>(out, 100, 1u);
+        (void)hipEventRecord(e0, st);
+        k_valu_peak<<<blocks, threads, 0, st>>>(out, iters, 2u);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0)
+            rc = SDPG_EHIP;
+        else
+            *limb_mac_per_s = 8.0 * iters * (double)blocks * threads / (ms * 1e-3);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(out);
+    if (rc) set_err("sdpg_valu_peak: device error");
+    return rc;
 }
 
 int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m, int n, int k, const uint32_t *A,

@@ -37,7 +37,7 @@ EXPORTS = [
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
-    "sdpg_tune_slices",
+    "sdpg_tune_slices", "sdpg_valu_peak",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula",
 ]
 
@@ -84,6 +84,7 @@ def load(path: str | None = None):
     lib.sdpg_rgemm.argtypes = [ctypes.c_int] * 7 + [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p] + \
         [ctypes.c_int] * 5 + [P(ctypes.c_double)]
     lib.sdpg_rgemm_noalloc.argtypes = lib.sdpg_rgemm.argtypes
+    lib.sdpg_valu_peak.argtypes = [ctypes.c_int, P(ctypes.c_double)]
     lib.sdpg_tune_slices.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_double),
                                      ctypes.c_int]
     lib.sdpa_amd_ctx_from_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -197,6 +198,16 @@ def rgemm(prec_bits: int, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = N
     if rc:
         raise SdpgError(f"{fn.__name__}: {rc}: {lib.sdpg_last_error().decode()}", rc)
     return np.ascontiguousarray(np.transpose(c, (1, 0, 2))), ms.value
+
+
+def valu_peak(device: int = 0) -> float:
+    """sdpg_valu_peak: measured limb multiply-adds per second of the device"""
+    lib = load()
+    v = ctypes.c_double()
+    rc = lib.sdpg_valu_peak(device, ctypes.byref(v))
+    if rc:
+        raise SdpgError(f"sdpg_valu_peak: {rc}: {lib.sdpg_last_error().decode()}", rc)
+    return v.value
 
 
 def tune_slices(prec_bits: int, target_bits: int = 0, device: int = 0):

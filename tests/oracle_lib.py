@@ -62,6 +62,9 @@ def lib():
         L.orc_time_point.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
+        L.orc_time_f1_sample.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -108,6 +111,16 @@ def time_point(data: str, prec: int, x_path: str, invz_path: str, L: int, chol_n
                               ctypes.byref(a), ctypes.byref(b))
     if rc < 0:
         raise RuntimeError(f"orc_time_point rc={rc}")
+    return a.value, b.value
+
+
+def time_f1_sample(data: str, prec: int, x_path: str, invz_path: str, L: int, nsample: int, threads: int):
+    """seconds per constraint's F1 precompute and per pair's inner product (all-F1 block 0)"""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    rc = lib().orc_time_f1_sample(data.encode(), prec, x_path.encode(), invz_path.encode(), L, nsample, threads,
+                                  ctypes.byref(a), ctypes.byref(b))
+    if rc < 0:
+        raise RuntimeError(f"orc_time_f1_sample rc={rc}")
     return a.value, b.value
 
 
