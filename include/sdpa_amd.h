@@ -35,9 +35,29 @@ typedef struct {
 } sdpa_amd_result;
 
 /*
+ * Options of one run, the command line of sdpa_main.cpp:501-677 (-ds / -dd data, -is / -id initial point,
+ * -p parameter file, -pt preset, -o output).
+ */
+typedef struct {
+    const char *data_path;  /* problem file */
+    int data_sparse;        /* 1 sparse (-ds), 0 dense (-dd), -1 by the name (a name ending in "-s" is sparse) */
+    const char *init_path;  /* initial point (sdpa_io.cpp:150-274); NULL: X = Z = lambda* I, y = 0 */
+    int init_sparse;        /* 1 (-is), 0 (-id), -1 by the name */
+    const char *param_path; /* param.sdpa; NULL: the preset */
+    int preset;             /* Parameter::setDefaultParameter: 0 default, 1 unstable but fast, 2 stable but slow */
+    int prec_override;      /* > 0 replaces the precision */
+    const char *log_path;   /* NULL, "-" (stdout) or a file: iteration log, summary, parameters and solution */
+    int gemm_mode, slices, device;
+} sdpa_amd_options;
+
+/* pinpal (sdpa_main.cpp:43-461) with the options above; returns 0, < 0 on input or device errors */
+int sdpa_amd_run(const sdpa_amd_options *opt, sdpa_amd_result *res);
+
+/*
  * param_path may be NULL (defaults of Parameter::setDefaultParameter,
  * sdpa_parts.cpp:165-178).  prec_override > 0 replaces the precision line.
- * log_path: NULL, "-" (stdout) or a file.  gemm_mode: SDPG_GEMM_PLAIN or
+ * log_path: NULL, "-" (stdout) or a file.  The data file's format follows its name (sdpa_amd_run
+ * chooses explicitly).  gemm_mode: SDPG_GEMM_PLAIN or
  * SDPG_GEMM_OZAKI; slices 0 with the Ozaki form: the count sdpg_tune_slices picks for the run's
  * precision.  Returns 0 on success, < 0 on input or device errors.
  */

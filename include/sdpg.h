@@ -60,6 +60,8 @@ extern "C" {
 #define SDPG_RZINV 13  /* r_zinv of the last sdpg_gvec call (dls) */
 #define SDPG_INVCHOLX 14 /* inv(chol(X)) of the last sdpg_factor_point, lower (dls) */
 #define SDPG_INVCHOLZ 15 /* inv(chol(Z)) of the last sdpg_factor_point, lower (dls) */
+#define SDPG_X0 16     /* initial point of an initial-point file (-id / -is), put once (dls) */
+#define SDPG_Z0 17
 
 /* sdpg_inner selectors: scalar inner products over the dense linear space / vectors */
 #define SDPG_IP_XZ 1    /* X . Z   */
@@ -72,6 +74,8 @@ extern "C" {
 #define SDPG_IP_BDY 8   /* b . dy  */
 #define SDPG_IP_TRX 9   /* trace X (SDP diagonals and LP entries) */
 #define SDPG_IP_TRZ 10  /* trace Z */
+#define SDPG_IP_X0Z 11  /* X0 . Z  (SolveInfo::update with an initial point, sdpa_parts.cpp:747-796) */
+#define SDPG_IP_XZ0 12  /* X . Z0  */
 
 /* GEMM forms (sdpg_set_gemm) */
 #define SDPG_GEMM_PLAIN 0  /* multi-limb integer MAD on the vector ALUs */

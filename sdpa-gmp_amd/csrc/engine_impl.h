@@ -1012,6 +1012,7 @@ struct Engine : EngineBase {
         PV pv() const { return PV{p, n}; }
     };
     Arr X, Z, invZ, Rd, rzinv, xrd, dX, dZ, W1, W2, DLS1, Cfull;
+    Arr X0, Z0;  // initial point of an initial-point file (allocated on first put)
     Arr y, Rp, g, dy, b, rdiag, scal;
     Arr Bm;
     Arr sval, dval, lpval;  // constraint data
@@ -1923,6 +1924,14 @@ struct Engine : EngineBase {
         case SDPG_IP_BDY: rc = dot(b, 0, &dy, 0, nullptr, m, v); break;
         case SDPG_IP_TRX: rc = dot(X, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
         case SDPG_IP_TRZ: rc = dot(Z, 0, nullptr, 0, d_diag_idx, ndiag, v); break;
+        case SDPG_IP_X0Z:
+        case SDPG_IP_XZ0:
+            if (!X0.p || !Z0.p) {
+                set_err("sdpg_inner: no initial point (sdpg_put SDPG_X0 / SDPG_Z0)");
+                return SDPG_ESTATE;
+            }
+            rc = what == SDPG_IP_X0Z ? dot(X0, 0, &Z, 0, nullptr, Nd, v) : dot(X, 0, &Z0, 0, nullptr, Nd, v);
+            break;
         case SDPG_IP_CX:
         case SDPG_IP_CDX: {
             const Arr &M_ = what == SDPG_IP_CX ? X : dX;
@@ -2117,20 +2126,24 @@ struct Engine : EngineBase {
         case SDPG_RZINV: cnt = Nd; return &rzinv;
         case SDPG_INVCHOLX: cnt = Nd; return &icX;
         case SDPG_INVCHOLZ: cnt = Nd; return &icZ;
+        case SDPG_X0: cnt = Nd; return &X0;
+        case SDPG_Z0: cnt = Nd; return &Z0;
         }
         return nullptr;
     }
     int get(int what, uint32_t *out) override {
         int64_t cnt = 0;
         Arr *a = sel(what, cnt);
-        if (!a) return SDPG_EARG;
+        if (!a || !a->p) return SDPG_EARG;
         return get_packed(*a, 0, out, cnt);
     }
     int put(int what, const uint32_t *in) override {
         int64_t cnt = 0;
         Arr *a = sel(what, cnt);
         if (!a) return SDPG_EARG;
-        int rc = put_packed(*a, 0, in, cnt);
+        int rc = 0;
+        if ((what == SDPG_X0 || what == SDPG_Z0) && !a->p && (rc = aalloc(*a, Nd))) return rc;
+        rc = put_packed(*a, 0, in, cnt);
         if (rc) return rc;
         HIPCHK(hipStreamSynchronize(st));
         return 0;

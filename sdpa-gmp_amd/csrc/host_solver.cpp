@@ -8,6 +8,7 @@
 // C-ABI calls into the device engine (include/sdpg.h).
 #include <stdio.h>
 #include <stdlib.h>
+#include <ctype.h>
 #include <string.h>
 #include <sys/time.h>
 
@@ -39,6 +40,28 @@ struct Parameter {
     double epsilonStar = 1.0e-30, lambdaStar = 1.0e+4, omegaStar = 2.0, lowerBound = -1.0e+5, upperBound = 1.0e+5;
     double betaStar = 0.1, betaBar = 0.3, gammaStar = 0.9, epsilonDash = 1.0e-30;
     int precision = 200;
+    // print formats of the solution (param.sdpa lines 12-15; defaults of sdpa_parts.cpp:177-180)
+    std::string xPrint = "%+50.40Fe", XPrint = "%+50.40Fe", YPrint = "%+50.40Fe", infPrint = "%+50.40Fe";
+    // Parameter::setDefaultParameter (sdpa_parts.cpp:136-181): 0 default, 1 unstable but fast, 2 stable
+    // but slow (-pt)
+    void preset(int type) {
+        *this = Parameter();
+        if (type == 2) {
+            maxIteration = 1000;
+            lambdaStar = 1.0e+2;
+            betaStar = 0.2;
+            betaBar = 0.4;
+            gammaStar = 0.5;
+            precision = 300;
+        } else if (type == 1) {
+            maxIteration = 100;
+            lambdaStar = 1.0e+2;
+            betaStar = 0.01;
+            betaBar = 0.02;
+            gammaStar = 0.98;
+            precision = 100;
+        }
+    }
     bool read(const char *path) {
         FILE *fp = fopen(path, "r");
         if (!fp) return false;
@@ -54,6 +77,12 @@ struct Parameter {
         ok &= fscanf(fp, "%lf%*[^\n]", &gammaStar) == 1;
         ok &= fscanf(fp, "%lf%*[^\n]", &epsilonDash) == 1;
         ok &= fscanf(fp, "%d%*[^\n]", &precision) == 1;
+        std::string *pf[4] = {&xPrint, &XPrint, &YPrint, &infPrint};
+        for (std::string *f : pf) {
+            char buf[64];
+            if (fscanf(fp, "%63s%*[^\n]", buf) != 1) break;  // older files end after the precision
+            *f = buf;
+        }
         fclose(fp);
         return ok;
     }
@@ -128,7 +157,24 @@ static bool read_tok(FILE *fp, std::string &s) {
     return !s.empty();
 }
 
-static int read_raw(const char *path, RawProblem &R) {
+// The reference picks the sparse reader for a data file whose name ends in "-s" when the name is given
+// positionally (sdpa_main.cpp:620-623; -ds / -dd choose explicitly); the same rule holds here.
+static bool name_is_sparse(const char *path) {
+    size_t n = strlen(path);
+    return n >= 2 && path[n - 1] == 's' && path[n - 2] == '-';
+}
+
+// a decimal token whose mantissa digits are all zero ("0", "-0.000e+3")
+static bool token_is_zero(const std::string &t) {
+    for (char c : t) {
+        if (c == 'e' || c == 'E') break;
+        if (c >= '1' && c <= '9') return false;
+    }
+    return true;
+}
+
+static int read_raw(const char *path, RawProblem &R, int dense = -1) {
+    if (dense < 0) dense = name_is_sparse(path) ? 0 : 1;
     FILE *fp = fopen(path, "r");
     if (!fp) return -1;
     char line[8192];
@@ -160,12 +206,103 @@ static int read_raw(const char *path, RawProblem &R) {
             fclose(fp);
             return -4;
         }
+    if (dense) {
+        // dense format (sdpa_io.cpp:696-751): C then A_1 .. A_m, every block in full (an SDP block as n x n
+        // values by rows, an LP block as its diagonal); entries with i <= j and a nonzero value are kept
+        for (int k = 0; k <= R.m; k++)
+            for (int l = 0; l < R.nBlock; l++) {
+                const int bs = R.blockStruct[l];
+                if (bs >= 2) {
+                    for (int i = 0; i < bs; i++)
+                        for (int j = 0; j < bs; j++) {
+                            std::string t;
+                            if (!read_tok(fp, t)) {
+                                fclose(fp);
+                                return -5;
+                            }
+                            if (i <= j && !token_is_zero(t)) R.ent.push_back(RawEntry{k, l + 1, i + 1, j + 1, t});
+                        }
+                } else {
+                    const int nl = bs < 0 ? -bs : 1;
+                    for (int j = 0; j < nl; j++) {
+                        std::string t;
+                        if (!read_tok(fp, t)) {
+                            fclose(fp);
+                            return -5;
+                        }
+                        if (!token_is_zero(t)) R.ent.push_back(RawEntry{k, l + 1, j + 1, j + 1, t});
+                    }
+                }
+            }
+        fclose(fp);
+        return 0;
+    }
     for (;;) {
         RawEntry e;
         if (!read_int(fp, &e.k) || !read_int(fp, &e.l) || !read_int(fp, &e.i) || !read_int(fp, &e.j) ||
             !read_tok(fp, e.v))
             break;
         R.ent.push_back(std::move(e));
+    }
+    fclose(fp);
+    return 0;
+}
+
+// initial-point file (sdpa_io.cpp:150-274): y with the sign flipped, then Z and X (target 1 = Z,
+// otherwise X in the sparse form; Z before X in the dense form)
+struct RawInit {
+    std::vector<std::string> y;
+    std::vector<RawEntry> ent;  // k = 1 for Z, 2 for X; l = SDP block number (1-based) or nsdp + LP position
+};
+
+static int read_init(const char *path, int sparse, int m, const std::vector<int> &blockStruct, RawInit &I) {
+    if (sparse < 0) sparse = name_is_sparse(path) ? 1 : 0;
+    FILE *fp = fopen(path, "r");
+    if (!fp) return -31;
+    I.y.resize(m);
+    for (int k = 0; k < m; k++)
+        if (!read_tok(fp, I.y[k])) {
+            fclose(fp);
+            return -32;
+        }
+    std::vector<int> sdp;
+    int nlp = 0;
+    for (int b : blockStruct) {
+        if (b >= 2) sdp.push_back(b);
+        else nlp += b < 0 ? -b : 1;
+    }
+    const int nsdp = (int)sdp.size();
+    if (sparse) {
+        for (;;) {
+            RawEntry e;
+            if (!read_int(fp, &e.k) || !read_int(fp, &e.l) || !read_int(fp, &e.i) || !read_int(fp, &e.j) ||
+                !read_tok(fp, e.v))
+                break;
+            e.k = e.k == 1 ? 1 : 2;
+            if (e.l > nsdp) e.l = nsdp + 1;  // the LP part: position i - 1 (sdpa_io.cpp:196-204)
+            I.ent.push_back(std::move(e));
+        }
+    } else {
+        for (int tgt = 1; tgt <= 2; tgt++) {
+            for (int l = 0; l < nsdp; l++)
+                for (int i = 0; i < sdp[l]; i++)
+                    for (int j = 0; j < sdp[l]; j++) {
+                        std::string t;
+                        if (!read_tok(fp, t)) {
+                            fclose(fp);
+                            return -33;
+                        }
+                        if (i <= j && !token_is_zero(t)) I.ent.push_back(RawEntry{tgt, l + 1, i + 1, j + 1, t});
+                    }
+            for (int j = 0; j < nlp; j++) {
+                std::string t;
+                if (!read_tok(fp, t)) {
+                    fclose(fp);
+                    return -33;
+                }
+                if (!token_is_zero(t)) I.ent.push_back(RawEntry{tgt, nsdp + 1, j + 1, j + 1, t});
+            }
+        }
     }
     fclose(fp);
     return 0;
@@ -206,6 +343,7 @@ struct Solver {
     std::vector<int> useFormula;
     sdpg_ctx *ctx = nullptr;
     int used_slices = 0;  // Ozaki slice count in use (0: another GEMM form)
+    const RawInit *init = nullptr;  // initial-point file (-id / -is), if any
     Parameter prm;
     FILE *logf = nullptr;
     std::string err;
@@ -431,6 +569,91 @@ struct Solver {
     }
 
     std::string fmt(const T &v, int digits) { return mpx::format_decimal<L>(v, digits); }
+
+    // one value in a printf-style exponent format of the parameter file ("%+50.40Fe": flags, width,
+    // digits after the point); the reference passes these formats to gmp_fprintf with an mpf value
+    // (sdpa_struct.cpp:93-137, 706-740)
+    std::string fmt_param(const T &v, const std::string &pf) {
+        size_t q = 1;
+        bool plus = false, space = false, left = false;
+        while (q < pf.size() && strchr("+- 0#", pf[q])) {
+            plus |= pf[q] == '+';
+            space |= pf[q] == ' ';
+            left |= pf[q] == '-';
+            q++;
+        }
+        int width = 0, prec = 6;
+        while (q < pf.size() && isdigit((unsigned char)pf[q])) width = width * 10 + (pf[q++] - '0');
+        if (q < pf.size() && pf[q] == '.') {
+            prec = 0;
+            q++;
+            while (q < pf.size() && isdigit((unsigned char)pf[q])) prec = prec * 10 + (pf[q++] - '0');
+        }
+        std::string body = mpx::format_decimal<L>(mpx::absv<L>(v), prec + 1);
+        std::string sgn = mpx::sgn<L>(v) < 0 ? "-" : (plus ? "+" : (space ? " " : ""));
+        std::string out = sgn + body;
+        if ((int)out.size() < width) out = left ? out + std::string(width - out.size(), ' ')
+                                               : std::string(width - out.size(), ' ') + out;
+        return out;
+    }
+
+    // the solution printout of printLastInfo (sdpa_io.cpp:1167-1219, REVERSE_PRIMAL_DUAL): xVec = -y,
+    // xMat = Z and yMat = X block by block (DenseMatrix::display rows; LP blocks as one row)
+    void print_solution(FILE *f) {
+        std::vector<uint32_t> buf;
+        auto fetch = [&](int what, int64_t cnt, std::vector<T> &v) {
+            buf.assign((size_t)cnt * W, 0);
+            v.assign(cnt, mpx::zero<L>());
+            if (sdpg_get(ctx, what, buf.data()) == 0)
+                for (int64_t i = 0; i < cnt; i++) v[i] = unpack1(&buf[(size_t)i * W]);
+        };
+        std::vector<T> yv, Zv, Xv;
+        fetch(SDPG_Y, m, yv);
+        fetch(SDPG_Z, Nd, Zv);
+        fetch(SDPG_X, Nd, Xv);
+        fprintf(f, "xVec = \n");
+        if (prm.xPrint == "NOPRINT") {
+            fprintf(f, "NOPRINT\n");
+        } else {
+            fprintf(f, "{");
+            for (int j = 0; j < m; j++)
+                fprintf(f, "%s%s", fmt_param(mpx::neg<L>(yv[j]), prm.xPrint).c_str(), j + 1 < m ? "," : "}\n");
+            if (m == 0) fprintf(f, "  }\n");
+        }
+        auto space = [&](const char *name, const std::vector<T> &D, const std::string &pf) {
+            fprintf(f, "%s = \n", name);
+            if (pf == "NOPRINT") {
+                fprintf(f, "NOPRINT\n");
+                return;
+            }
+            fprintf(f, "{\n");
+            for (size_t bl = 0; bl < blockType.size(); bl++) {
+                if (blockType[bl] == 1) {
+                    const int l = blockNumber[bl], n = sdpStruct[l];
+                    const T *a = &D[off[l]];
+                    fprintf(f, "{");
+                    for (int i = 0; i < n - 1; i++) {
+                        fprintf(f, "%s{", i == 0 ? " " : "  ");
+                        for (int j = 0; j < n; j++)
+                            fprintf(f, "%s%s", fmt_param(a[i + (size_t)n * j], pf).c_str(), j + 1 < n ? "," : " },\n");
+                    }
+                    if (n > 1) fprintf(f, "  {");
+                    for (int j = 0; j < n; j++)
+                        fprintf(f, "%s%s", fmt_param(a[(n - 1) + (size_t)n * j], pf).c_str(), j + 1 < n ? "," : " }");
+                    fprintf(f, n > 1 ? "   }\n" : "\n");
+                } else {
+                    const int n = blockStructAbs[bl], o = blockNumber[bl];
+                    fprintf(f, "{");
+                    for (int j = 0; j < n; j++)
+                        fprintf(f, "%s%s", fmt_param(D[off[nsdp] + o + j], pf).c_str(), j + 1 < n ? "," : "}\n");
+                    if (n <= 0) fprintf(f, "  }\n");
+                }
+            }
+            fprintf(f, "}\n");
+        };
+        space("xMat", Zv, prm.XPrint);
+        space("yMat", Xv, prm.YPrint);
+    }
     std::string fmt_signed(const T &v, int digits) {
         std::string s = fmt(v, digits);
         return (s[0] == '-') ? s : "+" + s;
@@ -561,24 +784,59 @@ struct Solver {
         identity(Z, lam);
         identity(invZ, one / lam);
         y.assign(m, mpx::zero<L>());
+        if (init) {
+            // an initial-point file overwrites the entries it lists (IO::read, sdpa_io.cpp:150-274, after
+            // Solutions::initialize, sdpa_main.cpp:234-243); y is stored with the sign flipped
+            for (int k = 0; k < m; k++) y[k] = mpx::neg<L>(parse(init->y[k]));
+            for (const RawEntry &e : init->ent) {
+                std::vector<T> &D = e.k == 1 ? Z : X;
+                const T v = parse(e.v);
+                if (e.l <= nsdp) {
+                    const int l = e.l - 1, n = sdpStruct[l];
+                    if (e.i < 1 || e.j < 1 || e.i > n || e.j > n) {
+                        err = "out of range in the initial point";
+                        return -34;
+                    }
+                    D[off[l] + (e.i - 1) + (size_t)n * (e.j - 1)] = v;
+                    D[off[l] + (e.j - 1) + (size_t)n * (e.i - 1)] = v;
+                } else {
+                    if (e.i < 1 || e.i > nlp) {
+                        err = "out of range in the initial point";
+                        return -34;
+                    }
+                    D[off[nsdp] + e.i - 1] = v;
+                }
+            }
+        }
         int rc = upload_point();
         if (rc) return rc;
+        if (init) {  // initPt_xMat / initPt_zMat (sdpa_main.cpp:246-249)
+            std::vector<uint32_t> px, pz;
+            pack(X, px);
+            pack(Z, pz);
+            if ((rc = sdpg_put(ctx, SDPG_X0, px.data())) || (rc = sdpg_put(ctx, SDPG_Z0, pz.data()))) return rc;
+        }
         X.clear();
         Z.clear();
         invZ.clear();
         X.shrink_to_fit();
         Z.shrink_to_fit();
         invZ.shrink_to_fit();
-        {  // Solutions::initialize: the inverse factors of lambda I, on the device
+        {  // Solutions::initialize / computeInverse: the inverse factors of the initial point, on the device
             int info = 0;
             if ((rc = sdpg_factor_point(ctx, &info))) return rc;
+            if (info) {
+                err = "the initial point is not positive definite";
+                return -35;
+            }
         }
         T initNrp, initNrd;
         if ((rc = residuals(initNrp, initNrd))) return rc;
         T curNrp = initNrp, curNrd = initNrd;
         T alphaP = MM::i(0), alphaD = MM::i(0), beta = MM::d(prm.betaStar);
         bool reductionOn = true;
-        T mu_initial = lam * lam, mu_current = mu_initial;
+        // AverageComplementarity: lambda*^2, or X . Z / n for an initial-point file (sdpa_parts.cpp:634-645)
+        T mu_initial = init ? ip(SDPG_IP_XZ) / MM::i(nDim) : lam * lam, mu_current = mu_initial;
         T acc = MM::d(prm.epsilonDash);
         T thetaP = (initNrp < acc) ? MM::i(0) : one;
         T thetaD = (initNrd < acc) ? MM::i(0) : one;
@@ -721,13 +979,15 @@ struct Solver {
             if ((rc = residuals(curNrp, curNrd))) return rc;
             thetaP = (initNrp > c1em10) ? curNrp / initNrp : MM::i(0);
             thetaD = (initNrd > c1em10) ? curNrd / initNrd : MM::i(0);
-            // SolveInfo::update(lambda, ...) (sdpa_parts.cpp:774-830)
+            // SolveInfo::update (sdpa_parts.cpp:747-830): with an initial-point file against that point,
+            // otherwise against lambda I
             {
                 si.objValPrimal = ip(SDPG_IP_CX);
                 si.objValDual = ip(SDPG_IP_BY);
                 T pr = thetaP, du = thetaD, x0z0 = nD * mu_initial, xz = nD * mu_current;
-                // lambda I . Z and X . lambda I (the reference sums lambda * diagonal entries)
-                T x0z = lam * ip(SDPG_IP_TRZ), xz0 = lam * ip(SDPG_IP_TRX);
+                // X0 . Z and X . Z0; for lambda I the reference sums lambda * diagonal entries
+                T x0z = init ? ip(SDPG_IP_X0Z) : lam * ip(SDPG_IP_TRZ);
+                T xz0 = init ? ip(SDPG_IP_XZ0) : lam * ip(SDPG_IP_TRX);
                 si.rho = mpx::zero<L>();
                 bool pok = curNrp <= acc, dok = curNrd <= acc;
                 if (pok && xz0 < si.etaPrimal) si.etaPrimal = xz0;
@@ -785,9 +1045,23 @@ struct Solver {
             fprintf(logf, "objValDual   = %s\n", fmt(-si.objValPrimal, 41).c_str());
             fprintf(logf, "p.feas.error = %s\n", fmt(curNrd, 17).c_str());
             fprintf(logf, "d.feas.error = %s\n", fmt(curNrp, 17).c_str());
+            // relative eps: 2^-(mpf precision bits) of the request (Rlamch "E", mplapack/Rlamch.cpp:36-45)
+            const int pbits = 64 * ((std::max(53, prm.precision) + 127) / 64) - 64;
+            fprintf(logf, "relative eps = %s\n", fmt(mpx::ldexp2<L>(MM::i(1), -pbits), 17).c_str());
+            fprintf(logf, "total time   = %.3f\n", t1 - t0);
             if (res)
                 fprintf(logf, "Make bMat    = %.3f ms (device)\nCholesky bMat= %.3f ms (device)\nmain loop    = %.3f s\n",
                         res->t_schur_ms, res->t_chol_ms, res->t_total_s);
+            fprintf(logf, "\n\nParameters are\n** Parameters **\n");
+            fprintf(logf, "maxIteration =    %d\nepsilonStar  = %8.3e\nlambdaStar   = %8.3e\nomegaStar    = %8.3e\n",
+                    prm.maxIteration, prm.epsilonStar, prm.lambdaStar, prm.omegaStar);
+            fprintf(logf, "lowerBound   = %8.3e\nupperBound   = %8.3e\nbetaStar     = %8.3e\nbetaBar      = %8.3e\n",
+                    prm.lowerBound, prm.upperBound, prm.betaStar, prm.betaBar);
+            fprintf(logf, "gammaStar    = %8.3e\nepsilonDash  = %8.3e\nprecision    =    %d\n", prm.gammaStar,
+                    prm.epsilonDash, prm.precision);
+            fprintf(logf, "xPrint       = %s \nXPrint       = %s \nYPrint       = %s \ninfPrint     = %s \n\n",
+                    prm.xPrint.c_str(), prm.XPrint.c_str(), prm.YPrint.c_str(), prm.infPrint.c_str());
+            print_solution(logf);
         }
         return 0;
     }
@@ -795,9 +1069,10 @@ struct Solver {
 
 template <int L>
 static int solve_L(const RawProblem &R, const Parameter &prm, const char *log_path, int gemm_mode, int slices,
-                   int device, sdpa_amd_result *res) {
+                   int device, sdpa_amd_result *res, const RawInit *init = nullptr) {
     Solver<L> S;
     S.prm = prm;
+    S.init = init;
     int rc = S.load(R);
     if (rc) return rc;
     S.compute_formula();
@@ -880,18 +1155,38 @@ using namespace sdpg::host;
         return SDPG_EPREC;                \
     }
 
-extern "C" int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
-                              int gemm_mode, int slices, int device, sdpa_amd_result *res) {
+extern "C" int sdpa_amd_run(const sdpa_amd_options *opt, sdpa_amd_result *res) {
+    if (!opt || !opt->data_path) return -20;
     Parameter prm;
-    if (param_path && !prm.read(param_path)) return -20;
-    if (prec_override > 0) prm.precision = prec_override;
+    prm.preset(opt->preset);
+    if (opt->param_path && !prm.read(opt->param_path)) return -20;
+    if (opt->prec_override > 0) prm.precision = opt->prec_override;
     RawProblem R;
-    int rc = read_raw(data_path, R);
+    int rc = read_raw(opt->data_path, R, opt->data_sparse < 0 ? -1 : (opt->data_sparse ? 0 : 1));
     if (rc) return rc;
+    RawInit I;
+    if (opt->init_path && (rc = read_init(opt->init_path, opt->init_sparse, R.m, R.blockStruct, I))) return rc;
     int L = sdpg_limbs(prm.precision);
-#define CALL_(LL) solve_L<LL>(R, prm, log_path, gemm_mode, slices, device, res)
+#define CALL_(LL) \
+    solve_L<LL>(R, prm, opt->log_path, opt->gemm_mode, opt->slices, opt->device, res, opt->init_path ? &I : nullptr)
     DISPATCH_L(L, CALL_)
 #undef CALL_
+}
+
+extern "C" int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_override, const char *log_path,
+                              int gemm_mode, int slices, int device, sdpa_amd_result *res) {
+    sdpa_amd_options o;
+    memset(&o, 0, sizeof o);
+    o.data_path = data_path;
+    o.data_sparse = -1;
+    o.init_sparse = -1;
+    o.param_path = param_path;
+    o.prec_override = prec_override;
+    o.log_path = log_path;
+    o.gemm_mode = gemm_mode;
+    o.slices = slices;
+    o.device = device;
+    return sdpa_amd_run(&o, res);
 }
 
 namespace sdpg {

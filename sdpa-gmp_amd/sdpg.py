@@ -24,8 +24,9 @@ LIBDIR = os.path.join(PKG, "lib")
 SDPG_X, SDPG_Z, SDPG_INVZ, SDPG_Y, SDPG_RP, SDPG_RD, SDPG_GVEC = 1, 2, 3, 4, 5, 6, 7
 SDPG_B, SDPG_L, SDPG_DY, SDPG_DZ, SDPG_DX, SDPG_RZINV = 8, 9, 10, 11, 12, 13
 SDPG_INVCHOLX, SDPG_INVCHOLZ = 14, 15
+SDPG_X0, SDPG_Z0 = 16, 17
 (SDPG_IP_XZ, SDPG_IP_XDZ, SDPG_IP_DXZ, SDPG_IP_DXDZ, SDPG_IP_CX, SDPG_IP_CDX, SDPG_IP_BY, SDPG_IP_BDY,
- SDPG_IP_TRX, SDPG_IP_TRZ) = range(1, 11)
+ SDPG_IP_TRX, SDPG_IP_TRZ, SDPG_IP_X0Z, SDPG_IP_XZ0) = range(1, 13)
 GEMM_ELEM, GEMM_PLAIN, GEMM_OZAKI = -1, 0, 1
 SDPG_NOINFO, SDPG_PFEAS, SDPG_DFEAS, SDPG_PDFEAS = 0, 1, 2, 3
 EZERO = -(1 << 29)
@@ -38,7 +39,7 @@ EXPORTS = [
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
     "sdpg_tune_slices", "sdpg_valu_peak",
-    "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula",
+    "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula", "sdpa_amd_run",
 ]
 
 
@@ -97,6 +98,7 @@ def load(path: str | None = None):
                                    P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                    ctypes.c_char_p]
     lib.sdpg_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
+    lib.sdpa_amd_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.sdpa_amd_formula.argtypes = [ctypes.c_char_p, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int]
     _lib = lib
     return lib
@@ -279,7 +281,7 @@ class Context:
     def size_of(self, what: int) -> int:
         if what in (SDPG_Y, SDPG_RP, SDPG_GVEC, SDPG_DY):
             return self.m
-        if what in (SDPG_B, SDPG_L):
+        if what in (SDPG_B, SDPG_L):  # noqa: SIM102
             return self.m * self.m
         return self.nd
 
@@ -369,6 +371,29 @@ def formula(data_path: str) -> np.ndarray:
     if rc:
         raise SdpgError(f"sdpa_amd_formula rc={rc}")
     return out.reshape(m.value, max(ns.value, 1))[:, :ns.value]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("data_path", ctypes.c_char_p), ("data_sparse", ctypes.c_int), ("init_path", ctypes.c_char_p),
+                ("init_sparse", ctypes.c_int), ("param_path", ctypes.c_char_p), ("preset", ctypes.c_int),
+                ("prec_override", ctypes.c_int), ("log_path", ctypes.c_char_p), ("gemm_mode", ctypes.c_int),
+                ("slices", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+def run(data_path: str, *, data_sparse: int = -1, init_path: str | None = None, init_sparse: int = -1,
+        param_path: str | None = None, preset: int = 0, prec: int = 0, log_path: str | None = None,
+        gemm: int = GEMM_PLAIN, slices: int = 0, device: int = 0) -> Result:
+    """sdpa_amd_run: a solve with the reference's command-line options (dense / sparse data and initial-point
+    files, parameter presets)."""
+    lib = load()
+    enc = lambda x: x.encode() if x else None  # noqa: E731
+    o = Options(enc(data_path), data_sparse, enc(init_path), init_sparse, enc(param_path), preset, prec,
+                enc(log_path), gemm, slices, device)
+    r = Result()
+    rc = lib.sdpa_amd_run(ctypes.byref(o), ctypes.byref(r))
+    if rc:
+        raise SdpgError(f"sdpa_amd_run rc={rc}: {lib.sdpg_last_error().decode()}", rc)
+    return r
 
 
 def bench(data_path: str, param_path: str | None, prec: int, gemm: int, slices: int, warm_iters: int,

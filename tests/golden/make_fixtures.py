@@ -159,7 +159,9 @@ def sdplib_fixtures():
     out_dir = os.path.join(HERE, "sdplib")
     os.makedirs(out_dir, exist_ok=True)
     gold = {}
-    shutil.copy(os.path.join(REF, "example1.dat-s"), os.path.join(out_dir, "example1.dat-s"))
+    for name in ("example1.dat-s", "example1.dat", "example1.ini", "example1.ini-s", "example2.dat",
+                 "example1.result"):
+        shutil.copy(os.path.join(REF, name), os.path.join(out_dir, name))
     with open(os.path.join(REF, "example1.result")) as f:
         gold["example1"] = parse_result(f.read())
     for name in SDPLIB:
