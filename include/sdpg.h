@@ -247,6 +247,32 @@ int sdpg_tune_slices(int prec_bits, int device, int target_bits, int *slices, do
  */
 int sdpg_valu_peak(int device, double *limb_mac_per_s);
 
+/*
+ * Batched Newton hot path over many small problems (no single reference counterpart: the reference
+ * runs one problem per process; each problem's step is Newton::Mehrotra's Make_bMat ->
+ * choleskyFactorWithAdjust -> solveSystems, sdpa_newton.cpp:1375-1414, with sdpa_linear.cpp:608-631 and
+ * 683-695).  A batch holds n committed contexts of one precision on one device whose constraint blocks
+ * are all dense and that have no LP block (at most 4 SDP blocks each); the contexts stay usable on
+ * their own.  sdpg_batch_newton takes one request per context: SDPG_BATCH_NONE, SDPG_BATCH_PREDICTOR
+ * (Schur build, Cholesky, solve: what sdpg_schur, sdpg_potrf, sdpg_potrs do) or SDPG_BATCH_CORRECTOR
+ * (solve only, with the factor of that context's last batched predictor), after the context's
+ * sdpg_gvec; it writes dy of every requested context and info[p] (Rpotrf's info for a predictor, 0
+ * otherwise; a context with info > 0 gets no dy).  Follow with each context's sdpg_dz / sdpg_dx.  The
+ * batched factor replaces the context's own: after a batched predictor, the context's sdpg_potrs is
+ * not valid until its next sdpg_potrf.  With sdpg_batch_set_graph(b, 1) the three phase kernels run as
+ * one HIP graph, captured on the first call.  sdpg_batch_time gives the device time of the last
+ * call's phases in ms.  Calls on one batch are not thread-safe.
+ */
+typedef struct sdpg_batch sdpg_batch;
+#define SDPG_BATCH_NONE 0
+#define SDPG_BATCH_PREDICTOR 1
+#define SDPG_BATCH_CORRECTOR 2
+int sdpg_batch_create(sdpg_batch **batch, int n, sdpg_ctx *const *ctxs);
+void sdpg_batch_destroy(sdpg_batch *batch);
+int sdpg_batch_newton(sdpg_batch *batch, const int *req, int *info);
+int sdpg_batch_set_graph(sdpg_batch *batch, int on);
+int sdpg_batch_time(sdpg_batch *batch, double *ms);
+
 /* last error message of this thread */
 const char *sdpg_last_error(void);
 

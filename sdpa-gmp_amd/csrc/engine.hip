@@ -64,6 +64,11 @@ struct sdpg_ctx {
     sdpg::EngineBase *eng;
 };
 
+struct sdpg_batch {
+    sdpg::BatchBase *b;
+    int n, device;
+};
+
 using namespace sdpg;
 
 extern "C" {
@@ -616,6 +621,70 @@ int sdpg_rgemm_noalloc(int prec_bits, int device, int transa, int transb, int m,
 #undef RGN_
     set_err("sdpg_rgemm_noalloc: unsupported precision");
     return SDPG_EPREC;
+}
+
+int sdpg_batch_create(sdpg_batch **batch, int n, sdpg_ctx *const *ctxs) {
+    if (!batch || n <= 0 || !ctxs) {
+        set_err("sdpg_batch_create: bad arguments");
+        return SDPG_EARG;
+    }
+    *batch = nullptr;
+    std::vector<EngineBase *> es(n);
+    for (int p = 0; p < n; p++) {
+        if (!ctxs[p] || !ctxs[p]->eng) {
+            set_err("sdpg_batch_create: null context");
+            return SDPG_EARG;
+        }
+        es[p] = ctxs[p]->eng;
+        if (es[p]->nlimb != es[0]->nlimb || es[p]->device != es[0]->device) {
+            set_err("sdpg_batch_create: contexts must share one precision and one device");
+            return SDPG_EARG;
+        }
+    }
+    if (hipSetDevice(es[0]->device) != hipSuccess) {
+        set_err("sdpg_batch_create: bad device");
+        return SDPG_EHIP;
+    }
+    const int L = es[0]->nlimb;
+    BatchBase *b = nullptr;
+    int rc = SDPG_EPREC;
+#define BC_(LL) \
+    if (L == LL) rc = batch_create_##LL(es.data(), n, &b);
+    SDPG_FOR_EACH_L(BC_)
+#undef BC_
+    if (rc) return rc;
+    *batch = new sdpg_batch{b, n, es[0]->device};
+    return SDPG_OK;
+}
+
+void sdpg_batch_destroy(sdpg_batch *batch) {
+    if (!batch) return;
+    (void)hipSetDevice(batch->device);
+    delete batch->b;
+    delete batch;
+}
+
+int sdpg_batch_newton(sdpg_batch *batch, const int *req, int *info) {
+    if (!batch || !req || !info) {
+        set_err("sdpg_batch_newton: bad arguments");
+        return SDPG_EARG;
+    }
+    for (int p = 0; p < batch->n; p++)
+        if (req[p] < SDPG_BATCH_NONE || req[p] > SDPG_BATCH_CORRECTOR) {
+            set_err("sdpg_batch_newton: bad request");
+            return SDPG_EARG;
+        }
+    return batch->b->newton(req, info);
+}
+
+int sdpg_batch_set_graph(sdpg_batch *batch, int on) {
+    if (!batch) return SDPG_EARG;
+    return batch->b->set_graph(on);
+}
+
+int sdpg_batch_time(sdpg_batch *batch, double *ms) {
+    if (!batch || !ms) return SDPG_EARG;
+    return batch->b->timing(ms);
 }
 
 }  // extern "C"

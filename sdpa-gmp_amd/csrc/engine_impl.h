@@ -2342,6 +2342,12 @@ static int tune_time_L(int device, int S, const uint32_t *host, double *ms_long,
     return rc;
 }
 
+}  // namespace sdpg
+
+#include "batch.h"
+
+namespace sdpg {
+
 // per-limb-count entry points, defined in engine_inst.hip
 #define SDPG_DECL_L_(LL)                                                                                      \
     EngineBase *new_engine_##LL();                                                                          \
@@ -2349,7 +2355,8 @@ static int tune_time_L(int device, int S, const uint32_t *host, double *ms_long,
     int rgemm_##LL(int device, int transa, int transb, int m, int n, int k, const uint32_t *A, int lda,      \
                    const uint32_t *B, int ldb, uint32_t *C, int ldc, int op, int mode, int slices, int lower, \
                    double *ms, uint8_t *scratch, size_t scratch_bytes);                                     \
-    int tune_time_##LL(int device, int S, const uint32_t *host, double *ms_long, double *ms_short);
+    int tune_time_##LL(int device, int S, const uint32_t *host, double *ms_long, double *ms_short);        \
+    int batch_create_##LL(EngineBase *const *es, int n, BatchBase **out);
 SDPG_FOR_EACH_L(SDPG_DECL_L_)
 #undef SDPG_DECL_L_
 

@@ -26,4 +26,8 @@ int SDPG_CAT_(tune_time_, SDPG_INST_L)(int device, int S, const uint32_t *host, 
     return tune_time_L<SDPG_INST_L>(device, S, host, ms_long, ms_short);
 }
 
+int SDPG_CAT_(batch_create_, SDPG_INST_L)(EngineBase *const *es, int n, BatchBase **out) {
+    return batch_create_L<SDPG_INST_L>(es, n, out);
+}
+
 }  // namespace sdpg

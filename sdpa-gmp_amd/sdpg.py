@@ -29,6 +29,7 @@ SDPG_X0, SDPG_Z0 = 16, 17
  SDPG_IP_TRX, SDPG_IP_TRZ, SDPG_IP_X0Z, SDPG_IP_XZ0) = range(1, 13)
 GEMM_ELEM, GEMM_PLAIN, GEMM_OZAKI = -1, 0, 1
 SDPG_NOINFO, SDPG_PFEAS, SDPG_DFEAS, SDPG_PDFEAS = 0, 1, 2, 3
+BATCH_NONE, BATCH_PREDICTOR, BATCH_CORRECTOR = 0, 1, 2
 EZERO = -(1 << 29)
 
 # every exported symbol of include/sdpg.h and include/sdpa_amd.h
@@ -38,7 +39,8 @@ EXPORTS = [
     "sdpg_schur", "sdpg_potrf", "sdpg_potrs", "sdpg_dz", "sdpg_dx", "sdpg_get", "sdpg_put", "sdpg_timers",
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
-    "sdpg_tune_slices", "sdpg_valu_peak",
+    "sdpg_tune_slices", "sdpg_valu_peak", "sdpg_batch_create", "sdpg_batch_destroy", "sdpg_batch_newton",
+    "sdpg_batch_set_graph", "sdpg_batch_time",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula", "sdpa_amd_run",
 ]
 
@@ -99,6 +101,12 @@ def load(path: str | None = None):
                                    ctypes.c_char_p]
     lib.sdpg_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
     lib.sdpa_amd_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.sdpg_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(ctypes.c_void_p)]
+    lib.sdpg_batch_destroy.restype = None
+    lib.sdpg_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.sdpg_batch_newton.argtypes = [ctypes.c_void_p, P(ctypes.c_int), P(ctypes.c_int)]
+    lib.sdpg_batch_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.sdpg_batch_time.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
     lib.sdpa_amd_formula.argtypes = [ctypes.c_char_p, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int]
     _lib = lib
     return lib
@@ -332,6 +340,49 @@ class Context:
         t = (ctypes.c_double * 8)()
         self.lib.sdpg_timers(self.h, t, 8)
         return list(t)
+
+
+class Batch:
+    """Lockstep Newton steps of many contexts (sdpg_batch_*): one request per context."""
+
+    def __init__(self, ctxs):
+        self.lib = load()
+        self.ctxs = list(ctxs)  # keeps the contexts alive while the batch holds them
+        arr = (ctypes.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = ctypes.c_void_p()
+        rc = self.lib.sdpg_batch_create(ctypes.byref(h), len(self.ctxs), arr)
+        if rc:
+            raise SdpgError(f"sdpg_batch_create rc={rc}: {self.lib.sdpg_last_error().decode()}", rc)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.sdpg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def newton(self, req) -> np.ndarray:
+        r = np.ascontiguousarray(req, dtype=np.int32)
+        assert r.shape == (len(self.ctxs),)
+        info = np.zeros(len(self.ctxs), dtype=np.int32)
+        rc = self.lib.sdpg_batch_newton(self.h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        info.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        if rc:
+            raise SdpgError(f"sdpg_batch_newton rc={rc}: {self.lib.sdpg_last_error().decode()}", rc)
+        return info
+
+    def set_graph(self, on: bool):
+        self.lib.sdpg_batch_set_graph(self.h, 1 if on else 0)
+
+    def time_ms(self) -> float:
+        t = ctypes.c_double()
+        self.lib.sdpg_batch_time(self.h, ctypes.byref(t))
+        return t.value
 
 
 class Result(ctypes.Structure):
