@@ -81,6 +81,22 @@ int sdpa_amd_bench(const char *data_path, const char *param_path, int prec_overr
                    double *ms_wall, double *kstats, const char *dump_prefix);
 
 /*
+ * The batched form: n problems solved in lockstep (no single reference counterpart; each problem follows
+ * pinpal, sdpa_main.cpp:43-461, as sdpa_amd_solve does).  Every problem runs the IPM loop on its own host
+ * thread with its own context; every device call of its loop (the Newton step, dZ and dX, the inner
+ * products, the step-length eigenvalues, the update, the inverse factors, the residuals) meets those of
+ * the other running problems in one batched call per kind (sdpg_batch_*), and a problem whose loop ends
+ * leaves the meetings.  The problems must be dense-constraint SDPs without LP
+ * blocks (sdpg_batch_create).  use_graph: the batched phases as a HIP graph.  res (may be NULL, else
+ * n results), latency_s (may be NULL, else n): seconds from the start of the loops to each problem's end.
+ * stats (may be NULL, else 12 doubles): wall seconds of the loops, meetings (rounds of batched calls),
+ * mean problems per meeting, then the device milliseconds of the batched calls by kind as
+ * sdpg_batch_timers reports them (9 values).  Returns 0, or the first error.
+ */
+int sdpa_amd_batch_solve(const char *const *data_paths, int n, const char *param_path, int prec_override,
+                         int device, int use_graph, sdpa_amd_result *res, double *latency_s, double *stats);
+
+/*
  * Read a problem, compute its formula table (Newton::computeFormula_SDP,
  * sdpa_newton.cpp:694-819) and return a committed engine context
  * (sdpa_main.cpp:197-225 set-up).  *m and *nd receive the number of

@@ -272,6 +272,37 @@ void sdpg_batch_destroy(sdpg_batch *batch);
 int sdpg_batch_newton(sdpg_batch *batch, const int *req, int *info);
 int sdpg_batch_set_graph(sdpg_batch *batch, int on);
 int sdpg_batch_time(sdpg_batch *batch, double *ms);
+/*
+ * The other phases of an IPM iteration, batched the same way (one request per context, nonzero = run
+ * it for that context), each the counterpart of the per-context call named:
+ *   sdpg_batch_gvec        sdpg_gvec; req 1 predictor, 2 corrector; mu, beta: n packed scalars;
+ *                          phase: n phase values (Newton::Make_gVec, sdpa_newton.cpp:821-895)
+ *   sdpg_batch_dz / _dx    sdpg_dz / sdpg_dx (sdpa_newton.cpp:1346-1373)
+ *   sdpg_batch_update      sdpg_update with n packed alphaP, alphaD (sdpa_dataset.cpp:120-133)
+ *   sdpg_batch_factor_point sdpg_factor_point; info: n ints (sdpa_dataset.cpp:96-118)
+ *   sdpg_batch_inner       sdpg_inner for the selectors SDPG_IP_XZ .. SDPG_IP_TRZ at once: out holds
+ *                          10 packed values per context, selector s at index s - 1
+ *   sdpg_batch_residuals   sdpg_residuals: out holds 4 packed values per context (max |Rp|, max |Rd|,
+ *                          C.X, b.y) (sdpa_parts.cpp:747-790)
+ *   sdpg_batch_min_eigen   sdpg_min_eigen (sdpa_jordan.cpp:55-88): out holds 1 packed value per
+ *                          context.  Blocks of at most 20 take the smallest eigenvalue (Rsyev in the
+ *                          reference) from a double-precision eigenvector refined by its Rayleigh
+ *                          quotient; larger blocks follow the reference's Lanczos process with its
+ *                          tridiagonal eigenproblem solved in double and refined the same way.
+ * sdpg_batch_wait_contexts(b, 0): the calls no longer wait for the contexts' own streams (the caller
+ * guarantees that no context work is pending).  sdpg_batch_timers: accumulated device ms of the batched
+ * calls by kind (newton, gvec, dz, dx, update, factor_point, inner, residuals, min_eigen; n <= 9).
+ */
+int sdpg_batch_gvec(sdpg_batch *batch, const int *req, const uint32_t *mu, const uint32_t *beta, const int *phase);
+int sdpg_batch_dz(sdpg_batch *batch, const int *req, const int *phase);
+int sdpg_batch_dx(sdpg_batch *batch, const int *req);
+int sdpg_batch_update(sdpg_batch *batch, const int *req, const uint32_t *alphaP, const uint32_t *alphaD);
+int sdpg_batch_factor_point(sdpg_batch *batch, const int *req, int *info);
+int sdpg_batch_inner(sdpg_batch *batch, const int *req, uint32_t *out);
+int sdpg_batch_residuals(sdpg_batch *batch, const int *req, uint32_t *out);
+int sdpg_batch_min_eigen(sdpg_batch *batch, const int *req, int which, uint32_t *out);
+int sdpg_batch_wait_contexts(sdpg_batch *batch, int on);
+int sdpg_batch_timers(sdpg_batch *batch, double *ms, int n);
 
 /* last error message of this thread */
 const char *sdpg_last_error(void);

@@ -24,6 +24,9 @@ struct BatchProb {
     int64_t blkoff[BMAXB];
     const Desc *desc;  // m x nsdp constraint descriptors (index i * nsdp + l)
     PV X, invZ, dval, B, g, dy;
+    // the context's arrays of the other phases of an iteration: the point and its inverse factors, the
+    // residuals, C (dense), b, the right-hand-side matrices of Make_gVec and two block workspaces
+    PV Z, y, b, Rp, Rd, C, rzinv, xrd, dls, dX, dZ, icX, icZ, W1, W2;
     int64_t cs_off;  // 2m values of the batch's scratch: r_j = 1/sqrt(c_j d_j), then 1/L_jj
 };
 
@@ -223,11 +226,682 @@ __global__ __launch_bounds__(256) void k_batch_potrs(const BatchProb *__restrict
     }
 }
 
+// ---------------------------------------------------------------- the other phases of an iteration
+//
+// One workgroup per SDP block of a problem (a "block task"); n <= 40 in the batch workload, so every
+// operand of a block stays in the L2-resident context arrays and the products are element-form loops.
+// Requests per problem in device memory select the problems of a call; scalars per problem (the
+// target beta mu of Make_gVec, the step lengths) come in a planar array sc (BSC_N values per problem)
+// and results go out per block task in a planar array bo (BO_N values per task).
+
+struct BatchBlk {
+    int32_t p, l;
+};
+enum { BSC_TARGET = 0, BSC_ALPHAP = 1, BSC_ALPHAD = 2, BSC_N = 3 };
+enum { BO_XZ, BO_XDZ, BO_DXZ, BO_DXDZ, BO_TRX, BO_TRZ, BO_CX, BO_CDX, BO_BY, BO_BDY, BO_MAXRD, BO_MAXRP, BO_EIG, BO_N };
+
+__device__ __forceinline__ void wg_sync() {
+    __threadfence_block();
+    __syncthreads();
+}
+
+// C = op(A) op(B) for one n x n block (column-major, element form: one rounded product and one rounded
+// sum per term); the caller synchronises before C is read by other threads
+template <int L>
+__device__ void wg_mm(int n, PV A, int64_t ao, bool ta, PV B, int64_t bo, bool tb, PV C, int64_t co) {
+    for (int o = threadIdx.x; o < n * n; o += blockDim.x) {
+        const int a = o % n, b = o / n;
+        mp<L> acc = zero<L>();
+        for (int c = 0; c < n; c++) {
+            const mp<L> x = ldm<L>(A, ao + (ta ? c + (int64_t)n * a : a + (int64_t)n * c));
+            const mp<L> y = ldm<L>(B, bo + (tb ? b + (int64_t)n * c : c + (int64_t)n * b));
+            acc = add<L>(acc, mul<L>(x, y));
+        }
+        stm<L>(C, co + o, acc);
+    }
+}
+
+// workgroup sum / max |.| of one value per thread (blockDim.x a power of two; sh: L+1 planes of blockDim.x)
+template <int L>
+__device__ mp<L> wg_sum(mp<L> v, uint32_t *sh) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    __syncthreads();
+    lds_st<L>(sh, nt, t, v);
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (t < h) lds_st<L>(sh, nt, t, add<L>(lds_ld<L>(sh, nt, t), lds_ld<L>(sh, nt, t + h)));
+        __syncthreads();
+    }
+    const mp<L> r = lds_ld<L>(sh, nt, 0);
+    __syncthreads();
+    return r;
+}
+template <int L>
+__device__ mp<L> wg_maxabs(mp<L> v, uint32_t *sh) {
+    const int t = threadIdx.x, nt = blockDim.x;
+    __syncthreads();
+    lds_st<L>(sh, nt, t, absv<L>(v));
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (t < h) {
+            const mp<L> a = lds_ld<L>(sh, nt, t), b = lds_ld<L>(sh, nt, t + h);
+            if (cmp_abs<L>(b, a) > 0) lds_st<L>(sh, nt, t, b);
+        }
+        __syncthreads();
+    }
+    const mp<L> r = lds_ld<L>(sh, nt, 0);
+    __syncthreads();
+    return r;
+}
+
+#define BATCH_BLOCK_LOOP                                                 \
+    for (int t_ = blockIdx.x; t_ < nblk; t_ += gridDim.x)                \
+        if (const BatchBlk bk = blks[t_]; req[bk.p] != 0)
+
+// Make_gVec's matrices for one block (compute_rMat and the infeasible-phase term, sdpa_newton.cpp:821-895):
+// r_zinv = (beta mu) inv(Z) - X [- dX dZ inv(Z) for the corrector]; in the infeasible phases (iarg 1)
+// the predictor forms x_rd_zinv = X Rd inv(Z) and DLS = r_zinv - x_rd_zinv, otherwise DLS = r_zinv.
+// req: 1 predictor, 2 corrector.
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_gvec(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                                 int nblk, const int32_t *__restrict__ req,
+                                                 const int32_t *__restrict__ iarg, PV sc) {
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        const mp<L> tg = ldm<L>(sc, (int64_t)bk.p * BSC_N + BSC_TARGET);
+        const bool corr = req[bk.p] == 2, infeas = iarg[bk.p] != 0;
+        for (int e = threadIdx.x; e < nn; e += blockDim.x)
+            stm<L>(P.rzinv, o + e, add<L>(mul<L>(tg, ldm<L>(P.invZ, o + e)), neg<L>(ldm<L>(P.X, o + e))));
+        if (corr) {
+            wg_mm<L>(n, P.dX, o, false, P.dZ, o, false, P.W1, o);
+            wg_sync();
+            wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o);
+            wg_sync();
+            for (int e = threadIdx.x; e < nn; e += blockDim.x)
+                stm<L>(P.rzinv, o + e, sub<L>(ldm<L>(P.rzinv, o + e), ldm<L>(P.W2, o + e)));
+        }
+        if (infeas && !corr) {
+            wg_mm<L>(n, P.X, o, false, P.Rd, o, false, P.W1, o);
+            wg_sync();
+            wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.xrd, o);
+            wg_sync();
+        }
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const mp<L> r = ldm<L>(P.rzinv, o + e);
+            stm<L>(P.dls, o + e, infeas ? sub<L>(r, ldm<L>(P.xrd, o + e)) : r);
+        }
+        wg_sync();
+    }
+}
+
+// constraint-wise inner products A_i . D over the dense blocks, one wave per constraint (p, i):
+//   mode 0: g_i = -(A_i . DLS) + Rp_i   (Make_gVec, sdpa_newton.cpp:869-895)
+//   mode 1: Rp_i = b_i - A_i . X        (Residuals::compute, sdpa_parts.cpp:760-790)
+template <int L>
+__global__ __launch_bounds__(256) void k_bc_inner(const BatchProb *__restrict__ probs, const int2 *__restrict__ tasks,
+                                                  int ntasks, const int32_t *__restrict__ req, int mode) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    const int lane = threadIdx.x & 63;
+    for (int t0 = blockIdx.x * 4; t0 < ntasks; t0 += gridDim.x * 4) {
+        const int t = t0 + (threadIdx.x >> 6);
+        mp<L> acc = zero<L>();
+        int2 tk = make_int2(0, 0);
+        bool on = false;
+        if (t < ntasks) {
+            tk = tasks[t];
+            on = req[tk.x] != 0;
+        }
+        if (on) {
+            const BatchProb &P = probs[tk.x];
+            const PV D = mode == 0 ? P.dls : P.X;
+            for (int l = 0; l < P.nsdp; l++) {
+                const Desc d = P.desc[(int64_t)tk.y * P.nsdp + l];
+                if (d.kind != 2) continue;
+                const int64_t nn = (int64_t)d.n * d.n;
+                for (int64_t k = lane; k < nn; k += 64)
+                    acc = add<L>(acc, mul<L>(ldm<L>(P.dval, d.off + k), ldm<L>(D, d.blkoff + k)));
+            }
+        }
+        const mp<L> ip = wave_sum64<L>(acc, sh);
+        if (on && lane == 0) {
+            const BatchProb &P = probs[tk.x];
+            if (mode == 0)
+                stm<L>(P.g, tk.y, add<L>(neg<L>(ip), ldm<L>(P.Rp, tk.y)));
+            else
+                stm<L>(P.Rp, tk.y, sub<L>(ldm<L>(P.b, tk.y), ip));
+        }
+        __syncthreads();
+    }
+}
+
+// dZ = -sum_i dy_i A_i (+ Rd in the infeasible phases, iarg 1) (compute_DzMat, sdpa_newton.cpp:1346-1356)
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_dz(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                               int nblk, const int32_t *__restrict__ req,
+                                               const int32_t *__restrict__ iarg) {
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            mp<L> acc = zero<L>();
+            for (int i = 0; i < P.m; i++) {
+                const Desc d = P.desc[(int64_t)i * P.nsdp + bk.l];
+                if (d.kind == 2) acc = add<L>(acc, mul<L>(ldm<L>(P.dy, i), ldm<L>(P.dval, d.off + e)));
+            }
+            mp<L> v = neg<L>(acc);
+            if (iarg[bk.p]) v = add<L>(v, ldm<L>(P.Rd, o + e));
+            stm<L>(P.dZ, o + e, v);
+        }
+        wg_sync();
+    }
+}
+
+// dX = sym(r_zinv - X dZ inv(Z)) (compute_DxMat incl. getSymmetrize, sdpa_newton.cpp:1358-1373)
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_dx(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                               int nblk, const int32_t *__restrict__ req) {
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        wg_mm<L>(n, P.X, o, false, P.dZ, o, false, P.W1, o);
+        wg_sync();
+        wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o);
+        wg_sync();
+        for (int e = threadIdx.x; e < nn; e += blockDim.x)
+            stm<L>(P.dX, o + e, add<L>(ldm<L>(P.rzinv, o + e), neg<L>(ldm<L>(P.W2, o + e))));
+        wg_sync();
+        const mp<L> half = ldexp2<L>(from_int<L>(1), -1);
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const int i = e % n, j = e / n;
+            if (i <= j) continue;
+            const mp<L> v = mul<L>(half, add<L>(ldm<L>(P.dX, o + e), ldm<L>(P.dX, o + j + (int64_t)n * i)));
+            stm<L>(P.dX, o + e, v);
+            stm<L>(P.dX, o + j + (int64_t)n * i, v);
+        }
+        wg_sync();
+    }
+}
+
+// Solutions::update (sdpa_dataset.cpp:120-133): X += alphaP dX, Z += alphaD dZ; the task of block 0
+// also does y += alphaD dy
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_update(const BatchProb *__restrict__ probs,
+                                                   const BatchBlk *__restrict__ blks, int nblk,
+                                                   const int32_t *__restrict__ req, PV sc) {
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        const mp<L> aP = ldm<L>(sc, (int64_t)bk.p * BSC_N + BSC_ALPHAP), aD = ldm<L>(sc, (int64_t)bk.p * BSC_N + BSC_ALPHAD);
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            stm<L>(P.X, o + e, add<L>(ldm<L>(P.X, o + e), mul<L>(aP, ldm<L>(P.dX, o + e))));
+            stm<L>(P.Z, o + e, add<L>(ldm<L>(P.Z, o + e), mul<L>(aD, ldm<L>(P.dZ, o + e))));
+        }
+        if (bk.l == 0)
+            for (int i = threadIdx.x; i < P.m; i += blockDim.x)
+                stm<L>(P.y, i, add<L>(ldm<L>(P.y, i), mul<L>(aD, ldm<L>(P.dy, i))));
+        wg_sync();
+    }
+}
+
+// Cholesky factor and its inverse of one block (Jal::getInvChol, sdpa_jordan.cpp:123-171): the factor by
+// the division-free elimination of the Schur kernel above in W (lower triangle, scales and reciprocal
+// diagonal in S), then inv(L) row by row: (inv L)_ij = -(sum_{k=j}^{i-1} L_ik (inv L)_kj) / L_ii.
+// Returns false (uniformly) when a pivot is not positive.
+template <int L>
+__device__ bool wg_chol_inv(int n, PV Src, PV W, PV S, int64_t o, PV Out) {
+    const int nn = n * n;
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        stm<L>(W, o + e, ldm<L>(Src, o + e));
+        stm<L>(Out, o + e, zero<L>());
+    }
+    wg_sync();
+    auto at = [&](int i, int j) -> int64_t { return o + i + (int64_t)n * j; };
+    for (int j = 0; j < n; j++) {
+        const mp<L> d = ldm<L>(W, at(j, j));
+        if (sgn<L>(d) <= 0) return false;
+        const int n1 = n - 1 - j, cnt = n1 * (n1 + 1) / 2;
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+            int ip = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while (ip > 0 && ip * (ip + 1) / 2 > t) ip--;
+            while ((ip + 1) * (ip + 2) / 2 <= t) ip++;
+            const int i = j + 1 + ip, c = j + 1 + (t - ip * (ip + 1) / 2);
+            mp<L> v = fms<L>(d, ldm<L>(W, at(i, c)), ldm<L>(W, at(i, j)), ldm<L>(W, at(c, j)));
+            if (!is_zero<L>(v)) v.e -= d.e;
+            stm<L>(W, at(i, c), v);
+        }
+        wg_sync();
+    }
+    // scales c_j (S[o + j]), then r_j = 1/sqrt(c_j d_j) (S[o + j]) and 1/L_jj = c_j r_j (S[o + n + j])
+    if (threadIdx.x == 0) {
+        mp<L> c = from_int<L>(1);
+        for (int j = 0; j < n; j++) {
+            const mp<L> d = ldm<L>(W, at(j, j));
+            stm<L>(S, o + j, c);
+            c = mul<L>(c, d);
+            c.e -= d.e;
+        }
+    }
+    wg_sync();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const mp<L> c = ldm<L>(S, o + j), r = mpx::rsqrt_fx<L>(mul<L>(c, ldm<L>(W, at(j, j))));
+        stm<L>(S, o + n + j, mul<L>(c, r));
+        stm<L>(S, o + j, r);
+    }
+    wg_sync();
+    for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+        const int i = e % n, c = e / n;
+        if (i >= c) stm<L>(W, o + e, mul<L>(ldm<L>(W, o + e), ldm<L>(S, o + c)));
+    }
+    wg_sync();
+    for (int i = 0; i < n; i++) {
+        const mp<L> rd = ldm<L>(S, o + n + i);
+        for (int j = threadIdx.x; j <= i; j += blockDim.x) {
+            mp<L> v;
+            if (j == i) {
+                v = rd;
+            } else {
+                mp<L> acc = zero<L>();
+                for (int k = j; k < i; k++) acc = add<L>(acc, mul<L>(ldm<L>(W, at(i, k)), ldm<L>(Out, at(k, j))));
+                v = neg<L>(mul<L>(acc, rd));
+            }
+            stm<L>(Out, at(i, j), v);
+        }
+        wg_sync();
+    }
+    return true;
+}
+
+// Solutions::computeInverse (sdpa_dataset.cpp:96-118): inverse Cholesky factors of X and Z and
+// inv(Z) = icZ^T icZ; info[p] = 1 when X or Z is not positive definite
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_factor(const BatchProb *__restrict__ probs,
+                                                   const BatchBlk *__restrict__ blks, int nblk,
+                                                   const int32_t *__restrict__ req, int32_t *__restrict__ info) {
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l];
+        const int64_t o = P.blkoff[bk.l];
+        bool ok = wg_chol_inv<L>(n, P.X, P.W1, P.W2, o, P.icX);
+        if (ok) ok = wg_chol_inv<L>(n, P.Z, P.W1, P.W2, o, P.icZ);
+        if (!ok) {
+            if (threadIdx.x == 0) info[bk.p] = 1;
+            wg_sync();
+            continue;
+        }
+        wg_mm<L>(n, P.icZ, o, true, P.icZ, o, false, P.invZ, o);
+        wg_sync();
+    }
+}
+
+// the scalar inner products of StepLength / DirectionParameter / SolveInfo (sdpa_parts.cpp:362-860),
+// per block (X.Z, X.dZ, dX.Z, dX.dZ, tr X, tr Z, C.X, C.dX over both triangles); the task of block 0
+// adds b.y and b.dy
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_inner(const BatchProb *__restrict__ probs,
+                                                  const BatchBlk *__restrict__ blks, int nblk,
+                                                  const int32_t *__restrict__ req, PV bo) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        mp<L> a[8];
+        for (int k = 0; k < 8; k++) a[k] = zero<L>();
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            const mp<L> x = ldm<L>(P.X, o + e), z = ldm<L>(P.Z, o + e), dx = ldm<L>(P.dX, o + e),
+                        dz = ldm<L>(P.dZ, o + e), c = ldm<L>(P.C, o + e);
+            a[0] = add<L>(a[0], mul<L>(x, z));
+            a[1] = add<L>(a[1], mul<L>(x, dz));
+            a[2] = add<L>(a[2], mul<L>(dx, z));
+            a[3] = add<L>(a[3], mul<L>(dx, dz));
+            if (e % n == e / n) {
+                a[4] = add<L>(a[4], x);
+                a[5] = add<L>(a[5], z);
+            }
+            a[6] = add<L>(a[6], mul<L>(c, x));
+            a[7] = add<L>(a[7], mul<L>(c, dx));
+        }
+        for (int k = 0; k < 8; k++) {
+            const mp<L> v = wg_sum<L>(a[k], sh);
+            if (threadIdx.x == 0) stm<L>(bo, (int64_t)t_ * BO_N + k, v);
+        }
+        if (bk.l == 0) {
+            mp<L> by = zero<L>(), bdy = zero<L>();
+            for (int i = threadIdx.x; i < P.m; i += blockDim.x) {
+                const mp<L> bi = ldm<L>(P.b, i);
+                by = add<L>(by, mul<L>(bi, ldm<L>(P.y, i)));
+                bdy = add<L>(bdy, mul<L>(bi, ldm<L>(P.dy, i)));
+            }
+            by = wg_sum<L>(by, sh);
+            bdy = wg_sum<L>(bdy, sh);
+            if (threadIdx.x == 0) {
+                stm<L>(bo, (int64_t)t_ * BO_N + BO_BY, by);
+                stm<L>(bo, (int64_t)t_ * BO_N + BO_BDY, bdy);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Residuals::compute, dual part (sdpa_parts.cpp:760-790): Rd = C - Z - sum_i y_i A_i with max |Rd| per
+// block and C.X; the task of block 0 adds max |Rp| (Rp from k_bc_inner mode 1) and b.y
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_resid(const BatchProb *__restrict__ probs,
+                                                  const BatchBlk *__restrict__ blks, int nblk,
+                                                  const int32_t *__restrict__ req, PV bo) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        mp<L> mx = zero<L>(), cx = zero<L>();
+        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
+            mp<L> acc = zero<L>();
+            for (int i = 0; i < P.m; i++) {
+                const Desc d = P.desc[(int64_t)i * P.nsdp + bk.l];
+                if (d.kind == 2) acc = add<L>(acc, mul<L>(ldm<L>(P.dval, d.off + e), ldm<L>(P.y, i)));
+            }
+            const mp<L> c = ldm<L>(P.C, o + e);
+            const mp<L> v = sub<L>(sub<L>(c, ldm<L>(P.Z, o + e)), acc);
+            stm<L>(P.Rd, o + e, v);
+            if (cmp_abs<L>(v, mx) > 0) mx = absv<L>(v);
+            cx = add<L>(cx, mul<L>(c, ldm<L>(P.X, o + e)));
+        }
+        mx = wg_maxabs<L>(mx, sh);
+        cx = wg_sum<L>(cx, sh);
+        if (threadIdx.x == 0) {
+            stm<L>(bo, (int64_t)t_ * BO_N + BO_MAXRD, mx);
+            stm<L>(bo, (int64_t)t_ * BO_N + BO_CX, cx);
+        }
+        if (bk.l == 0) {
+            mp<L> rp = zero<L>(), by = zero<L>();
+            for (int i = threadIdx.x; i < P.m; i += blockDim.x) {
+                const mp<L> v = ldm<L>(P.Rp, i);
+                if (cmp_abs<L>(v, rp) > 0) rp = absv<L>(v);
+                by = add<L>(by, mul<L>(ldm<L>(P.b, i), ldm<L>(P.y, i)));
+            }
+            rp = wg_maxabs<L>(rp, sh);
+            by = wg_sum<L>(by, sh);
+            if (threadIdx.x == 0) {
+                stm<L>(bo, (int64_t)t_ * BO_N + BO_MAXRP, rp);
+                stm<L>(bo, (int64_t)t_ * BO_N + BO_BY, by);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---- step length: minimum eigenvalue of lMat xMat lMat^T per block (Jal::getMinEigen, sdpa_jordan.cpp:55-88)
+
+// symmetric tridiagonal eigenproblem in double (implicit QL with Wilkinson shifts; d: diagonal in,
+// eigenvalues out; e: off-diagonal, destroyed; z: eigenvectors by columns, leading dimension ldz);
+// returns the index of the smallest eigenvalue, or -1 without convergence
+__device__ inline int tql_double(int n, double *d, double *e, double *z, int ldz) {
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) z[i + j * ldz] = (i == j) ? 1.0 : 0.0;
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0, m;
+        for (;;) {
+            for (m = l; m < n - 1; m++) {
+                const double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= 1.1102230246251565e-16 * dd) break;
+            }
+            if (m == l) break;
+            if (iter++ == 60 * n) return -1;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = sqrt(g * g + 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g < 0 ? -r : r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool early = false;
+            for (int i = m - 1; i >= l; i--) {
+                double f = s * e[i], b = c * e[i];
+                r = sqrt(f * f + g * g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < n; k++) {
+                    f = z[k + (i + 1) * ldz];
+                    z[k + (i + 1) * ldz] = s * z[k + i * ldz] + c * f;
+                    z[k + i * ldz] = c * z[k + i * ldz] - s * f;
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    int k = 0;
+    for (int i = 1; i < n; i++)
+        if (d[i] < d[k]) k = i;
+    return k;
+}
+
+// cyclic Jacobi in double on a symmetric n x n matrix a (destroyed); v receives the eigenvectors by
+// columns; returns the index of the smallest eigenvalue (on the diagonal of a)
+__device__ inline int jacobi_double(int n, double *a, double *v) {
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) v[i + j * n] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, dg = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) (i == j ? dg : off) += a[i + j * n] * a[i + j * n];
+        if (off == 0.0 || off <= dg * 1e-34) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = a[p + q * n];
+                if (apq == 0.0) continue;
+                const double th = (a[q + q * n] - a[p + p * n]) / (2.0 * apq);
+                double t = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
+                if (th < 0) t = -t;
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c, tau = s / (1.0 + c);
+                a[p + p * n] -= t * apq;
+                a[q + q * n] += t * apq;
+                a[p + q * n] = a[q + p * n] = 0.0;
+                for (int k = 0; k < n; k++) {
+                    if (k != p && k != q) {
+                        const double akp = a[k + p * n], akq = a[k + q * n];
+                        a[k + p * n] = a[p + k * n] = akp - s * (akq + tau * akp);
+                        a[k + q * n] = a[q + k * n] = akq + s * (akp - tau * akq);
+                    }
+                    const double vkp = v[k + p * n], vkq = v[k + q * n];
+                    v[k + p * n] = vkp - s * (vkq + tau * vkp);
+                    v[k + q * n] = vkq + s * (vkp - tau * vkq);
+                }
+            }
+    }
+    int k = 0;
+    for (int i = 1; i < n; i++)
+        if (a[i + i * n] < a[k + k * n]) k = i;
+    return k;
+}
+
+constexpr int BEIG_KMAX = 20;  // Lanczos steps of a block of at most 40 (the loop ends by sqrt(n) + 10)
+constexpr int BEIG_NMAX = 40;
+
+// which 0: (icX, dX), 1: (icZ, dZ).  M = lMat xMat lMat^T is formed in W2.  Blocks of n <= 20: the smallest
+// eigenvalue of M (Rsyev, Lal::getMinEigenValue, sdpa_linear.cpp:126-158) from the eigenvector of a double
+// Jacobi solve refined by its Rayleigh quotient in multiple precision.  Larger blocks: the Lanczos
+// process of Lal::getMinEigen (sdpa_linear.cpp:27-124) with its stopping rule, in multiple precision;
+// its tridiagonal eigenproblem (Rsteqr there) is solved in double and the smallest eigenvalue refined
+// by the Rayleigh quotient of its eigenvector in multiple precision.
+template <int L>
+__global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__ probs,
+                                                   const BatchBlk *__restrict__ blks, int nblk,
+                                                   const int32_t *__restrict__ req, int which, PV bo) {
+    __shared__ uint32_t sh[(L + 1) * 256];
+    __shared__ uint32_t vq[(L + 1) * BEIG_NMAX], vqo[(L + 1) * BEIG_NMAX], vr[(L + 1) * BEIG_NMAX],
+        vw[(L + 1) * BEIG_NMAX], tal[(L + 1) * BEIG_KMAX], tbe[(L + 1) * BEIG_KMAX], tsc[(L + 1) * 4];
+    __shared__ double dA[BEIG_NMAX * BEIG_NMAX > 400 ? 400 : BEIG_NMAX * BEIG_NMAX], dV[400];
+    __shared__ double td[BEIG_KMAX], te[BEIG_KMAX], tz[BEIG_KMAX * BEIG_KMAX];
+    __shared__ int sflag;
+    BATCH_BLOCK_LOOP {
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        const int64_t o = P.blkoff[bk.l];
+        const PV Lm = which ? P.icZ : P.icX, Xm = which ? P.dZ : P.dX;
+        wg_mm<L>(n, Xm, o, false, Lm, o, true, P.W1, o);
+        wg_sync();
+        wg_mm<L>(n, Lm, o, false, P.W1, o, false, P.W2, o);
+        wg_sync();
+        const PV M = P.W2;
+        mp<L> result;
+        if (n <= 20) {
+            for (int e = threadIdx.x; e < nn; e += blockDim.x) dA[e] = to_double<L>(ldm<L>(M, o + e));
+            __syncthreads();
+            if (threadIdx.x == 0) sflag = jacobi_double(n, dA, dV);
+            __syncthreads();
+            const int k = sflag;
+            // Rayleigh quotient of the eigenvector v: (v^T M v) / (v^T v)
+            mp<L> num = zero<L>(), den = zero<L>();
+            if (threadIdx.x < n) {
+                const int a = threadIdx.x;
+                mp<L> s = zero<L>();
+                for (int b = 0; b < n; b++)
+                    s = add<L>(s, mul<L>(ldm<L>(M, o + a + (int64_t)n * b), from_double<L>(dV[b + k * n])));
+                const mp<L> va = from_double<L>(dV[a + k * n]);
+                num = mul<L>(va, s);
+                den = mul<L>(va, va);
+            }
+            num = wg_sum<L>(num, sh);
+            den = wg_sum<L>(den, sh);
+            result = mpx::div<L>(num, den);
+        } else {
+            using namespace mpx;
+            const mp<L> one = from_int<L>(1), zr = zero<L>();
+            mp<L> mn = from_double<L>(1.0e+51), mn_old = from_double<L>(1.0e+52), error = from_double<L>(1.0e+10);
+            const mp<L> nD = from_int<L>(n);
+            mp<L> beta = sqrtv<L>(nD);
+            const mp<L> sqrtn10 = add<L>(sqrtv<L>(nD), from_int<L>(10)), sqrtn9 = sqrtv<L>(from_int<L>(n + 9));
+            const mp<L> c16 = from_double<L>(1.0e-16), c5 = from_double<L>(1.0e-5), c8 = from_double<L>(1.0e-8),
+                        c2 = from_double<L>(1.0e-2), c4 = from_double<L>(1.0e-4);
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                lds_st<L>(vq, BEIG_NMAX, i, zr);
+                lds_st<L>(vr, BEIG_NMAX, i, one);
+            }
+            __syncthreads();
+            int k = 0, kk = 0;
+            for (;;) {
+                const mp<L> kf = from_int<L>(k);
+                if (!(k < n) || !(cmp<L>(kf, sqrtn10) < 0) || !(cmp<L>(beta, c16) > 0) || k >= BEIG_KMAX) break;
+                const bool condA = cmp<L>(absv<L>(sub<L>(mn, mn_old)), add<L>(mul<L>(c5, absv<L>(mn)), c8)) > 0;
+                const bool condB = cmp<L>(absv<L>(mul<L>(error, beta)), add<L>(mul<L>(c2, absv<L>(mn)), c4)) > 0;
+                if (!(condA || condB)) break;
+                const mp<L> value = mpx::div<L>(one, beta);
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    lds_st<L>(vqo, BEIG_NMAX, i, lds_ld<L>(vq, BEIG_NMAX, i));
+                    lds_st<L>(vq, BEIG_NMAX, i, mul<L>(value, lds_ld<L>(vr, BEIG_NMAX, i)));
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    mp<L> s = zero<L>();
+                    for (int b = 0; b < n; b++)
+                        s = add<L>(s, mul<L>(ldm<L>(M, o + i + (int64_t)n * b), lds_ld<L>(vq, BEIG_NMAX, b)));
+                    lds_st<L>(vw, BEIG_NMAX, i, s);
+                }
+                __syncthreads();
+                mp<L> part = zero<L>();
+                for (int i = threadIdx.x; i < n; i += blockDim.x)
+                    part = add<L>(part, mul<L>(lds_ld<L>(vq, BEIG_NMAX, i), lds_ld<L>(vw, BEIG_NMAX, i)));
+                const mp<L> alpha = wg_sum<L>(part, sh);
+                if (threadIdx.x == 0) lds_st<L>(tal, BEIG_KMAX, k, alpha);
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    mp<L> r = add<L>(lds_ld<L>(vw, BEIG_NMAX, i), mul<L>(neg<L>(alpha), lds_ld<L>(vq, BEIG_NMAX, i)));
+                    r = add<L>(r, mul<L>(neg<L>(beta), lds_ld<L>(vqo, BEIG_NMAX, i)));
+                    lds_st<L>(vr, BEIG_NMAX, i, r);
+                }
+                __syncthreads();
+                if (cmp<L>(from_int<L>(kk), sqrtv<L>(from_int<L>(k))) >= 0 || k == n - 1 || cmp<L>(kf, sqrtn9) > 0) {
+                    kk = 0;
+                    // T = tridiag(alpha_0..alpha_k; beta_0..beta_{k-1})
+                    if (threadIdx.x == 0) {
+                        const int nt = k + 1;
+                        for (int i = 0; i < nt; i++) {
+                            td[i] = to_double<L>(lds_ld<L>(tal, BEIG_KMAX, i));
+                            te[i] = i + 1 < nt ? to_double<L>(lds_ld<L>(tbe, BEIG_KMAX, i)) : 0.0;
+                        }
+                        const int j = tql_double(nt, td, te, tz, BEIG_KMAX);
+                        sflag = j;
+                        if (j >= 0) {
+                            // Rayleigh quotient of the eigenvector in multiple precision; error = its last component
+                            mp<L> num = zero<L>(), den = zero<L>();
+                            for (int i = 0; i < nt; i++) {
+                                const mp<L> vi = from_double<L>(tz[i + j * BEIG_KMAX]);
+                                mp<L> tv = mul<L>(lds_ld<L>(tal, BEIG_KMAX, i), vi);
+                                if (i > 0)
+                                    tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i - 1),
+                                                           from_double<L>(tz[i - 1 + j * BEIG_KMAX])));
+                                if (i + 1 < nt)
+                                    tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i),
+                                                           from_double<L>(tz[i + 1 + j * BEIG_KMAX])));
+                                num = add<L>(num, mul<L>(vi, tv));
+                                den = add<L>(den, mul<L>(vi, vi));
+                            }
+                            lds_st<L>(tsc, 4, 0, mpx::div<L>(num, den));
+                            lds_st<L>(tsc, 4, 1, mpx::div<L>(from_double<L>(tz[k + j * BEIG_KMAX]), sqrtv<L>(den)));
+                        }
+                    }
+                    __syncthreads();
+                    if (sflag < 0) break;
+                    mn_old = mn;
+                    mn = lds_ld<L>(tsc, 4, 0);
+                    error = lds_ld<L>(tsc, 4, 1);
+                    __syncthreads();
+                }
+                part = zero<L>();
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    const mp<L> r = lds_ld<L>(vr, BEIG_NMAX, i);
+                    part = add<L>(part, mul<L>(r, r));
+                }
+                beta = sqrtv<L>(wg_sum<L>(part, sh));
+                if (threadIdx.x == 0) lds_st<L>(tbe, BEIG_KMAX, k, beta);
+                __syncthreads();
+                ++k;
+                ++kk;
+            }
+            result = sub<L>(mn, absv<L>(mul<L>(error, beta)));
+        }
+        if (threadIdx.x == 0) stm<L>(bo, (int64_t)t_ * BO_N + BO_EIG, result);
+        wg_sync();
+    }
+}
+#undef BATCH_BLOCK_LOOP
+
+// kinds of batched calls whose device time is accumulated (sdpg_batch_timers)
+enum { BT_NEWTON, BT_GVEC, BT_DZ, BT_DX, BT_UPDATE, BT_FACTOR, BT_INNER, BT_RESID, BT_EIG, BT_N };
+
 struct BatchBase {
     virtual ~BatchBase() {}
     virtual int newton(const int *req, int *info) = 0;
     virtual int set_graph(int on) = 0;
     virtual int timing(double *ms) = 0;
+    virtual int wait_contexts(int on) = 0;
+    virtual int timers(double *ms, int n) = 0;
+    virtual int gvec(const int *req, const uint32_t *mu, const uint32_t *beta, const int *phase) = 0;
+    virtual int dz(const int *req, const int *phase) = 0;
+    virtual int dx(const int *req) = 0;
+    virtual int update(const int *req, const uint32_t *alphaP, const uint32_t *alphaD) = 0;
+    virtual int factor_point(const int *req, int *info) = 0;
+    virtual int inner(const int *req, uint32_t *out) = 0;
+    virtual int residuals(const int *req, uint32_t *out) = 0;
+    virtual int min_eigen(const int *req, int which, uint32_t *out) = 0;
 };
 
 template <int L>
@@ -245,6 +919,18 @@ struct Batch : BatchBase {
     hipGraphExec_t gexec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double last_ms = 0;
+    bool wait_ctx = true;
+    // the other phases: block tasks, per-problem arguments, scalars in, per-task results out
+    std::vector<BatchBlk> blks;
+    std::vector<int> first_blk;  // per problem: index of its block 0 in blks
+    BatchBlk *d_blks = nullptr;
+    int32_t *d_iarg = nullptr, *d_binfo = nullptr;
+    uint32_t *d_sc = nullptr, *d_bo = nullptr;
+    int64_t sc_n = 0, bo_n = 0;
+    std::vector<uint32_t> h_sc, h_bo;
+    std::vector<int32_t> h_iarg;
+    int grid_blk = 0;
+    double tm[BT_N] = {0};
 
     ~Batch() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -254,6 +940,11 @@ struct Batch : BatchBase {
         (void)hipFree(d_req);
         (void)hipFree(d_info);
         (void)hipFree(cs);
+        (void)hipFree(d_blks);
+        (void)hipFree(d_iarg);
+        (void)hipFree(d_binfo);
+        (void)hipFree(d_sc);
+        (void)hipFree(d_bo);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (st) (void)hipStreamDestroy(st);
@@ -275,6 +966,11 @@ struct Batch : BatchBase {
                 set_err("sdpg_batch_create: the batched form takes 1 to 4 SDP blocks and no LP block");
                 return SDPG_EARG;
             }
+            for (int l = 0; l < e->nsdp; l++)
+                if (e->sizes[l] > BEIG_NMAX) {
+                    set_err("sdpg_batch_create: the batched form takes blocks of at most 40");
+                    return SDPG_EARG;
+                }
             BatchProb b{};
             b.m = e->m;
             b.nsdp = e->nsdp;
@@ -298,6 +994,22 @@ struct Batch : BatchBase {
             b.B = e->Bm.pv();
             b.g = e->g.pv();
             b.dy = e->dy.pv();
+            b.Z = e->Z.pv();
+            b.y = e->y.pv();
+            b.b = e->b.pv();
+            b.Rp = e->Rp.pv();
+            b.Rd = e->Rd.pv();
+            b.C = e->Cfull.pv();
+            b.rzinv = e->rzinv.pv();
+            b.xrd = e->xrd.pv();
+            b.dls = e->DLS1.pv();
+            b.dX = e->dX.pv();
+            b.dZ = e->dZ.pv();
+            b.icX = e->icX.pv();
+            b.icZ = e->icZ.pv();
+            b.W1 = e->W1.pv();
+            b.W2 = e->W2.pv();
+            for (int l = 0; l < e->nsdp; l++) blks.push_back(BatchBlk{p, l});
             b.cs_off = csn;
             csn += 2 * (int64_t)e->m;
             hp.push_back(b);
@@ -344,6 +1056,25 @@ struct Batch : BatchBase {
             set_err("sdpg_batch_create: out of device memory");
             return SDPG_ENOMEM;
         }
+        first_blk.assign(n, 0);
+        for (int t = (int)blks.size() - 1; t >= 0; t--)
+            if (blks[t].l == 0) first_blk[blks[t].p] = t;
+        sc_n = (int64_t)n * BSC_N;
+        bo_n = (int64_t)blks.size() * BO_N;
+        grid_blk = std::max(1, std::min((int)blks.size(), 8 * cus));
+        h_sc.assign((size_t)(L + 1) * sc_n, 0);
+        h_bo.assign((size_t)(L + 1) * bo_n, 0);
+        h_iarg.assign(n, 0);
+        if (hipMalloc((void **)&d_blks, sizeof(BatchBlk) * blks.size()) != hipSuccess ||
+            hipMalloc((void **)&d_iarg, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc((void **)&d_binfo, sizeof(int32_t) * n) != hipSuccess ||
+            hipMalloc((void **)&d_sc, sizeof(uint32_t) * h_sc.size()) != hipSuccess ||
+            hipMalloc((void **)&d_bo, sizeof(uint32_t) * h_bo.size()) != hipSuccess) {
+            set_err("sdpg_batch_create: out of device memory");
+            return SDPG_ENOMEM;
+        }
+        if (hipMemcpy(d_blks, blks.data(), sizeof(BatchBlk) * blks.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return SDPG_EHIP;
         if (hipFuncSetAttribute((const void *)k_batch_schur<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm_schur) != hipSuccess ||
             hipMemcpy(d_probs, hp.data(), sizeof(BatchProb) * n, hipMemcpyHostToDevice) != hipSuccess ||
@@ -359,11 +1090,192 @@ struct Batch : BatchBase {
         k_batch_potrs<L><<<grid_prob, 256, shm_potrs, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n}, mmax);
     }
 
+    // the contexts' own work (point updates, right-hand sides) is complete before the batch reads it
+    int begin(const int *req) {
+        if (hipSetDevice(device) != hipSuccess) return SDPG_EHIP;
+        if (wait_ctx)
+            for (Engine<L> *e : eng)
+                if (hipStreamSynchronize(e->st) != hipSuccess) return SDPG_EHIP;
+        HIPCHK(hipMemcpyAsync(d_req, req, sizeof(int32_t) * nprob, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(e0, st));
+        return SDPG_OK;
+    }
+    // after the launches: results back, wait, device time of the call
+    int end(int kind, void *dst = nullptr, const void *src = nullptr, size_t bytes = 0) {
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e1, st));
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) tm[kind] += ms;
+        return SDPG_OK;
+    }
+    // packed scalar (L+2 words) <-> planar arrays
+    static void put_planar(std::vector<uint32_t> &h, int64_t N, int64_t i, const uint32_t *w) {
+        for (int k = 0; k < L; k++) h[(size_t)k * N + i] = w[2 + k];
+        h[(size_t)L * N + i] = (uint32_t)(((int32_t)w[0] << 1) | (int32_t)(w[1] & 1u));
+    }
+    static mp<L> get_planar(const std::vector<uint32_t> &h, int64_t N, int64_t i) {
+        mp<L> x;
+        for (int k = 0; k < L; k++) x.m[k] = h[(size_t)k * N + i];
+        const int32_t hd = (int32_t)h[(size_t)L * N + i];
+        x.e = hd >> 1;
+        x.s = (uint32_t)hd & 1u;
+        return x;
+    }
+    static mp<L> unpack_w(const uint32_t *w) {
+        mp<L> x;
+        x.e = (int32_t)w[0];
+        x.s = w[1] & 1u;
+        for (int k = 0; k < L; k++) x.m[k] = w[2 + k];
+        if (x.m[L - 1] == 0) x = zero<L>();
+        return x;
+    }
+    static void pack_w(const mp<L> &x, uint32_t *w) {
+        w[0] = (uint32_t)x.e;
+        w[1] = x.s;
+        for (int k = 0; k < L; k++) w[2 + k] = x.m[k];
+    }
+    int upload_sc() {
+        HIPCHK(hipMemcpyAsync(d_sc, h_sc.data(), sizeof(uint32_t) * h_sc.size(), hipMemcpyHostToDevice, st));
+        return SDPG_OK;
+    }
+    int upload_iarg() {
+        HIPCHK(hipMemcpyAsync(d_iarg, h_iarg.data(), sizeof(int32_t) * nprob, hipMemcpyHostToDevice, st));
+        return SDPG_OK;
+    }
+    static bool infeasible(int phase) { return phase == SDPG_PFEAS || phase == SDPG_NOINFO; }
+    int grid_con() const { return std::max(1, std::min((ntasks + 3) / 4, 8 * 256)); }
+
+    int gvec(const int *req, const uint32_t *mu, const uint32_t *beta, const int *phase) override {
+        const int W = L + 2;
+        for (int p = 0; p < nprob; p++) {
+            if (!req[p]) continue;
+            const mp<L> t = mul<L>(unpack_w(beta + (size_t)p * W), unpack_w(mu + (size_t)p * W));
+            uint32_t w[L + 2];
+            pack_w(t, w);
+            put_planar(h_sc, sc_n, (int64_t)p * BSC_N + BSC_TARGET, w);
+            h_iarg[p] = infeasible(phase[p]) ? 1 : 0;
+        }
+        int rc;
+        if ((rc = begin(req)) || (rc = upload_sc()) || (rc = upload_iarg())) return rc;
+        k_bb_gvec<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg, PV{d_sc, sc_n});
+        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 0);
+        return end(BT_GVEC);
+    }
+    int dz(const int *req, const int *phase) override {
+        for (int p = 0; p < nprob; p++) h_iarg[p] = req[p] && infeasible(phase[p]) ? 1 : 0;
+        int rc;
+        if ((rc = begin(req)) || (rc = upload_iarg())) return rc;
+        k_bb_dz<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg);
+        return end(BT_DZ);
+    }
+    int dx(const int *req) override {
+        int rc;
+        if ((rc = begin(req))) return rc;
+        k_bb_dx<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req);
+        return end(BT_DX);
+    }
+    int update(const int *req, const uint32_t *alphaP, const uint32_t *alphaD) override {
+        const int W = L + 2;
+        for (int p = 0; p < nprob; p++) {
+            if (!req[p]) continue;
+            put_planar(h_sc, sc_n, (int64_t)p * BSC_N + BSC_ALPHAP, alphaP + (size_t)p * W);
+            put_planar(h_sc, sc_n, (int64_t)p * BSC_N + BSC_ALPHAD, alphaD + (size_t)p * W);
+        }
+        int rc;
+        if ((rc = begin(req)) || (rc = upload_sc())) return rc;
+        k_bb_update<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, PV{d_sc, sc_n});
+        return end(BT_UPDATE);
+    }
+    int factor_point(const int *req, int *info) override {
+        int rc;
+        if ((rc = begin(req))) return rc;
+        HIPCHK(hipMemsetAsync(d_binfo, 0, sizeof(int32_t) * nprob, st));
+        k_bb_factor<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_binfo);
+        return end(BT_FACTOR, info, d_binfo, sizeof(int32_t) * nprob);
+    }
+    int fetch_bo(int kind) { return end(kind, h_bo.data(), d_bo, sizeof(uint32_t) * h_bo.size()); }
+    mp<L> bo_at(int t, int slot) const { return get_planar(h_bo, bo_n, (int64_t)t * BO_N + slot); }
+
+    // out: per problem the ten selectors SDPG_IP_XZ .. SDPG_IP_TRZ (packed, selector - 1)
+    int inner(const int *req, uint32_t *out) override {
+        int rc;
+        if ((rc = begin(req))) return rc;
+        k_bb_inner<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, PV{d_bo, bo_n});
+        if ((rc = fetch_bo(BT_INNER))) return rc;
+        const int W = L + 2;
+        static const int slot_of[10] = {BO_XZ, BO_XDZ, BO_DXZ, BO_DXDZ, BO_CX, BO_CDX, BO_BY, BO_BDY, BO_TRX, BO_TRZ};
+        for (int p = 0; p < nprob; p++) {
+            if (!req[p]) continue;
+            const int t0 = first_blk[p], nb = eng[p]->nsdp;
+            for (int q = 0; q < 10; q++) {
+                const int sl = slot_of[q];
+                mp<L> v = bo_at(t0, sl);
+                if (sl != BO_BY && sl != BO_BDY)
+                    for (int l = 1; l < nb; l++) v = add<L>(v, bo_at(t0 + l, sl));
+                pack_w(v, out + ((size_t)p * 10 + q) * W);
+            }
+        }
+        return SDPG_OK;
+    }
+    // out: per problem max |Rp|, max |Rd|, C.X, b.y (packed)
+    int residuals(const int *req, uint32_t *out) override {
+        int rc;
+        if ((rc = begin(req))) return rc;
+        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 1);
+        k_bb_resid<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, PV{d_bo, bo_n});
+        if ((rc = fetch_bo(BT_RESID))) return rc;
+        const int W = L + 2;
+        for (int p = 0; p < nprob; p++) {
+            if (!req[p]) continue;
+            const int t0 = first_blk[p], nb = eng[p]->nsdp;
+            mp<L> rd = bo_at(t0, BO_MAXRD), cx = bo_at(t0, BO_CX);
+            for (int l = 1; l < nb; l++) {
+                const mp<L> v = bo_at(t0 + l, BO_MAXRD);
+                if (cmp_abs<L>(v, rd) > 0) rd = v;
+                cx = add<L>(cx, bo_at(t0 + l, BO_CX));
+            }
+            pack_w(bo_at(t0, BO_MAXRP), out + ((size_t)p * 4 + 0) * W);
+            pack_w(rd, out + ((size_t)p * 4 + 1) * W);
+            pack_w(cx, out + ((size_t)p * 4 + 2) * W);
+            pack_w(bo_at(t0, BO_BY), out + ((size_t)p * 4 + 3) * W);
+        }
+        return SDPG_OK;
+    }
+    int min_eigen(const int *req, int which, uint32_t *out) override {
+        int rc;
+        if ((rc = begin(req))) return rc;
+        k_bb_mineig<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, which, PV{d_bo, bo_n});
+        if ((rc = fetch_bo(BT_EIG))) return rc;
+        const int W = L + 2;
+        for (int p = 0; p < nprob; p++) {
+            if (!req[p]) continue;
+            const int t0 = first_blk[p], nb = eng[p]->nsdp;
+            mp<L> mn = from_double<L>(1.0E50);
+            for (int l = 0; l < nb; l++) {
+                const mp<L> v = bo_at(t0 + l, BO_EIG);
+                if (cmp<L>(v, mn) < 0) mn = v;
+            }
+            pack_w(mn, out + (size_t)p * W);
+        }
+        return SDPG_OK;
+    }
+    int wait_contexts(int on) override {
+        wait_ctx = on != 0;
+        return SDPG_OK;
+    }
+    int timers(double *ms, int n) override {
+        for (int k = 0; k < n && k < BT_N; k++) ms[k] = tm[k];
+        return SDPG_OK;
+    }
+
     int newton(const int *req, int *info) override {
         if (hipSetDevice(device) != hipSuccess) return SDPG_EHIP;
         // the contexts' own work (point updates, right-hand sides) is complete before the batch reads it
-        for (Engine<L> *e : eng)
-            if (hipStreamSynchronize(e->st) != hipSuccess) return SDPG_EHIP;
+        if (wait_ctx)
+            for (Engine<L> *e : eng)
+                if (hipStreamSynchronize(e->st) != hipSuccess) return SDPG_EHIP;
         HIPCHK(hipMemcpyAsync(d_req, req, sizeof(int32_t) * nprob, hipMemcpyHostToDevice, st));
         // a problem that only solves keeps the info of its factorisation (zero: it is still in the run)
         HIPCHK(hipMemsetAsync(d_info, 0, sizeof(int32_t) * nprob, st));
@@ -387,7 +1299,10 @@ struct Batch : BatchBase {
         HIPCHK(hipMemcpyAsync(info, d_info, sizeof(int32_t) * nprob, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         float ms = 0;
-        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) last_ms = ms;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            last_ms = ms;
+            tm[BT_NEWTON] += ms;
+        }
         return SDPG_OK;
     }
     int set_graph(int on) override {

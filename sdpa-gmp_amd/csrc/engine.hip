@@ -687,4 +687,82 @@ int sdpg_batch_time(sdpg_batch *batch, double *ms) {
     return batch->b->timing(ms);
 }
 
+int sdpg_batch_wait_contexts(sdpg_batch *batch, int on) {
+    if (!batch) return SDPG_EARG;
+    return batch->b->wait_contexts(on);
+}
+
+int sdpg_batch_timers(sdpg_batch *batch, double *ms, int n) {
+    if (!batch || !ms || n < 0) return SDPG_EARG;
+    return batch->b->timers(ms, n);
+}
+
+static int batch_args(sdpg_batch *batch, const int *req, const char *what) {
+    if (!batch || !req) {
+        set_err(std::string(what) + ": bad arguments");
+        return SDPG_EARG;
+    }
+    return SDPG_OK;
+}
+
+int sdpg_batch_gvec(sdpg_batch *batch, const int *req, const uint32_t *mu, const uint32_t *beta, const int *phase) {
+    int rc = batch_args(batch, req, "sdpg_batch_gvec");
+    if (rc) return rc;
+    if (!mu || !beta || !phase) return SDPG_EARG;
+    for (int p = 0; p < batch->n; p++)
+        if (req[p] < SDPG_BATCH_NONE || req[p] > SDPG_BATCH_CORRECTOR) {
+            set_err("sdpg_batch_gvec: bad request");
+            return SDPG_EARG;
+        }
+    return batch->b->gvec(req, mu, beta, phase);
+}
+
+int sdpg_batch_dz(sdpg_batch *batch, const int *req, const int *phase) {
+    int rc = batch_args(batch, req, "sdpg_batch_dz");
+    if (rc) return rc;
+    if (!phase) return SDPG_EARG;
+    return batch->b->dz(req, phase);
+}
+
+int sdpg_batch_dx(sdpg_batch *batch, const int *req) {
+    int rc = batch_args(batch, req, "sdpg_batch_dx");
+    if (rc) return rc;
+    return batch->b->dx(req);
+}
+
+int sdpg_batch_update(sdpg_batch *batch, const int *req, const uint32_t *alphaP, const uint32_t *alphaD) {
+    int rc = batch_args(batch, req, "sdpg_batch_update");
+    if (rc) return rc;
+    if (!alphaP || !alphaD) return SDPG_EARG;
+    return batch->b->update(req, alphaP, alphaD);
+}
+
+int sdpg_batch_factor_point(sdpg_batch *batch, const int *req, int *info) {
+    int rc = batch_args(batch, req, "sdpg_batch_factor_point");
+    if (rc) return rc;
+    if (!info) return SDPG_EARG;
+    return batch->b->factor_point(req, info);
+}
+
+int sdpg_batch_inner(sdpg_batch *batch, const int *req, uint32_t *out) {
+    int rc = batch_args(batch, req, "sdpg_batch_inner");
+    if (rc) return rc;
+    if (!out) return SDPG_EARG;
+    return batch->b->inner(req, out);
+}
+
+int sdpg_batch_residuals(sdpg_batch *batch, const int *req, uint32_t *out) {
+    int rc = batch_args(batch, req, "sdpg_batch_residuals");
+    if (rc) return rc;
+    if (!out) return SDPG_EARG;
+    return batch->b->residuals(req, out);
+}
+
+int sdpg_batch_min_eigen(sdpg_batch *batch, const int *req, int which, uint32_t *out) {
+    int rc = batch_args(batch, req, "sdpg_batch_min_eigen");
+    if (rc) return rc;
+    if (!out || which < 0 || which > 1) return SDPG_EARG;
+    return batch->b->min_eigen(req, which, out);
+}
+
 }  // extern "C"

@@ -12,7 +12,14 @@
 #include <string.h>
 #include <sys/time.h>
 
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_la.h"
@@ -24,6 +31,8 @@
 
 namespace sdpg {
 namespace host {
+
+constexpr int BATCH_TIMERS = 9;  // kinds of sdpg_batch_timers
 
 using mpx::mp;
 using namespace sdpg::hla;
@@ -308,6 +317,38 @@ static int read_init(const char *path, int sparse, int m, const std::vector<int>
     return 0;
 }
 
+// The device side of one problem's loop.  This one makes the context's own calls; a batched run gives
+// each problem a port that meets the other running problems' calls and issues one batched call per kind.
+struct Exec {
+    virtual ~Exec() {}
+    // Newton::Mehrotra (sdpa_newton.cpp:1375-1414) after the control's parameters: Make_gVec, then
+    // (predictor) Make_bMat and the Cholesky factorisation, the solve, dZ and dX; *info > 0 when the
+    // factorisation fails (and nothing after it runs)
+    virtual int step(sdpg_ctx *c, bool predictor, const uint32_t *mu, const uint32_t *beta, int phase, int *info) {
+        *info = 0;
+        int rc = sdpg_gvec(c, predictor ? 0 : 1, mu, beta, phase);
+        if (rc) return rc;
+        if (predictor) {
+            if ((rc = sdpg_schur(c)) || (rc = sdpg_potrf(c, info))) return rc;
+            if (*info > 0) return 0;
+        }
+        if ((rc = sdpg_potrs(c)) || (rc = sdpg_dz(c, phase))) return rc;
+        return sdpg_dx(c);
+    }
+    virtual int inner(sdpg_ctx *c, int what, uint32_t *out) { return sdpg_inner(c, what, out); }
+    // the step-length minimum eigenvalues for (X, dX) and (Z, dZ)
+    virtual int min_eigen2(sdpg_ctx *c, uint32_t *x, uint32_t *z) {
+        int rc = sdpg_min_eigen(c, 0, x);
+        return rc ? rc : sdpg_min_eigen(c, 1, z);
+    }
+    virtual int update(sdpg_ctx *c, const uint32_t *aP, const uint32_t *aD) { return sdpg_update(c, aP, aD); }
+    virtual int factor_point(sdpg_ctx *c, int *info) { return sdpg_factor_point(c, info); }
+    virtual int residuals(sdpg_ctx *c, uint32_t *nrp, uint32_t *nrd) {
+        return sdpg_residuals(c, nrp, nrd, nullptr, nullptr);
+    }
+};
+static Exec g_direct;
+
 template <int L>
 struct Solver {
     using T = mp<L>;
@@ -347,6 +388,7 @@ struct Solver {
     Parameter prm;
     FILE *logf = nullptr;
     std::string err;
+    Exec *ex = &g_direct;
 
     ~Solver() {
         if (ctx) sdpg_ctx_destroy(ctx);
@@ -671,7 +713,7 @@ struct Solver {
     // Residuals::compute on the device; returns the norms
     int residuals(T &nrp, T &nrd) {
         std::vector<uint32_t> a(W), c(W);
-        int rc = sdpg_residuals(ctx, a.data(), c.data(), nullptr, nullptr);
+        int rc = ex->residuals(ctx, a.data(), c.data());
         if (rc) return rc;
         nrp = unpack1(a.data());
         nrd = unpack1(c.data());
@@ -682,25 +724,18 @@ struct Solver {
     int mehrotra(bool predictor, const T &mu, const T &beta, int phase, bool &ok) {
         ok = true;
         auto pm = pack1(mu), pbeta = pack1(beta);
-        int rc = sdpg_gvec(ctx, predictor ? 0 : 1, pm.data(), pbeta.data(), phase);
+        int info = 0;
+        int rc = ex->step(ctx, predictor, pm.data(), pbeta.data(), phase, &info);
         if (rc) return rc;
-        if (predictor) {
-            if ((rc = sdpg_schur(ctx))) return rc;
-            int info = 0;
-            if ((rc = sdpg_potrf(ctx, &info))) return rc;
-            if (info > 0) {
-                if (logf) fprintf(logf, "cholesky miss condition :: not positive definite :: info = %d\n", info);
-                ok = false;
-                return 0;
-            }
+        if (info > 0) {
+            if (logf) fprintf(logf, "cholesky miss condition :: not positive definite :: info = %d\n", info);
+            ok = false;
         }
-        if ((rc = sdpg_potrs(ctx))) return rc;
-        if ((rc = sdpg_dz(ctx, phase))) return rc;
-        return sdpg_dx(ctx);
+        return 0;
     }
     T ip(int what) {
         std::vector<uint32_t> w(W);
-        int rc = sdpg_inner(ctx, what, w.data());
+        int rc = ex->inner(ctx, what, w.data());
         if (rc) {
             ip_err = rc;
             return mpx::zero<L>();
@@ -824,7 +859,7 @@ struct Solver {
         invZ.shrink_to_fit();
         {  // Solutions::initialize / computeInverse: the inverse factors of the initial point, on the device
             int info = 0;
-            if ((rc = sdpg_factor_point(ctx, &info))) return rc;
+            if ((rc = ex->factor_point(ctx, &info))) return rc;
             if (info) {
                 err = "the initial point is not positive definite";
                 return -35;
@@ -913,11 +948,10 @@ struct Solver {
             {
                 T mnx, mnz;
                 {
-                    std::vector<uint32_t> w(W);
-                    if ((rc = sdpg_min_eigen(ctx, 0, w.data()))) return rc;
-                    mnx = unpack1(w.data());
-                    if ((rc = sdpg_min_eigen(ctx, 1, w.data()))) return rc;
-                    mnz = unpack1(w.data());
+                    std::vector<uint32_t> wx(W), wz(W);
+                    if ((rc = ex->min_eigen2(ctx, wx.data(), wz.data()))) return rc;
+                    mnx = unpack1(wx.data());
+                    mnz = unpack1(wz.data());
                 }
                 alphaP = (-mnx > inv_alphaBD) ? -(one / mnx) : alphaBD;
                 alphaD = (-mnz > inv_alphaBD) ? -(one / mnz) : alphaBD;
@@ -956,7 +990,7 @@ struct Solver {
             }
             print_iter(pIteration, mu_current, thetaP, thetaD, si, alphaP, alphaD, beta);
             // Solutions::update (sdpa_dataset.cpp:120-143), on the device
-            if ((rc = sdpg_update(ctx, pack1(alphaP).data(), pack1(alphaD).data()))) return rc;
+            if ((rc = ex->update(ctx, pack1(alphaP).data(), pack1(alphaD).data()))) return rc;
             if (alphaP < c1em4 && alphaD < c1em4) {
                 if (logf) fprintf(logf, "Step length is too small. \ncannot move\n");
                 status = 2;
@@ -964,7 +998,7 @@ struct Solver {
             }
             {
                 int info = 0;
-                if ((rc = sdpg_factor_point(ctx, &info))) return rc;
+                if ((rc = ex->factor_point(ctx, &info))) return rc;
                 if (info) {
                     if (logf) fprintf(logf, "cannot move\n");
                     status = 2;
@@ -1141,6 +1175,261 @@ static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int
     return 0;
 }
 
+// The meeting point of the lockstep solves: every problem's thread posts its next device call and waits;
+// the last of the running problems to arrive issues one batched call per kind of call posted
+// (sdpg_batch_*) for all of them and wakes the others.  A problem whose loop has ended leaves, which may
+// complete a meeting.  The ten scalar inner products come from one batched call and are kept per problem
+// until its next step or update, so that the control's consecutive inner products cost one meeting.
+struct Meeting {
+    enum { K_STEP, K_INNER, K_EIG, K_UPDATE, K_FACTOR, K_RESID };
+    struct Post {
+        int kind = 0;
+        bool pred = false;
+        int phase = 0, what = 0;
+        const uint32_t *a = nullptr, *b = nullptr;
+        uint32_t *o1 = nullptr, *o2 = nullptr;
+        int *info = nullptr;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    sdpg_batch *batch = nullptr;
+    const int n, W;
+    int active, arrived = 0, rc = 0;
+    uint64_t gen = 0;
+    std::vector<Post> post;
+    std::vector<char> posted, ipv;
+    std::vector<uint32_t> ipc;  // per problem, the ten inner products (SDPG_IP_XZ .. SDPG_IP_TRZ)
+    std::vector<int> req, req2, phase, info;
+    std::vector<uint32_t> sa, sb, out, out2;
+    long long meetings = 0, served = 0;
+
+    Meeting(sdpg_batch *b, int n_, int W_)
+        : batch(b), n(n_), W(W_), active(n_), post(n_), posted(n_, 0), ipv(n_, 0), ipc((size_t)n_ * 10 * W_),
+          req(n_), req2(n_), phase(n_), info(n_), sa((size_t)n_ * W_), sb((size_t)n_ * W_),
+          out((size_t)n_ * 10 * W_), out2((size_t)n_ * W_) {}
+
+    int meet(int p, const Post &q) {
+        std::unique_lock<std::mutex> lk(mu);
+        post[p] = q;
+        posted[p] = 1;
+        arrived++;
+        const uint64_t g = gen;
+        if (arrived == active)
+            fire();
+        else
+            cv.wait(lk, [&] { return gen != g; });
+        return rc;
+    }
+    void leave() {
+        std::lock_guard<std::mutex> lk(mu);
+        active--;
+        if (active > 0 && arrived == active) fire();
+    }
+    bool select(int kind) {
+        bool any = false;
+        for (int p = 0; p < n; p++) {
+            req[p] = posted[p] && post[p].kind == kind;
+            any |= req[p] != 0;
+        }
+        return any;
+    }
+    void note(int r) {
+        if (r && !rc) rc = r;
+    }
+    void fire() {  // with mu held
+        if (select(K_STEP)) {
+            for (int p = 0; p < n; p++) {
+                if (!req[p]) continue;
+                req[p] = post[p].pred ? SDPG_BATCH_PREDICTOR : SDPG_BATCH_CORRECTOR;
+                phase[p] = post[p].phase;
+                memcpy(&sa[(size_t)p * W], post[p].a, sizeof(uint32_t) * W);
+                memcpy(&sb[(size_t)p * W], post[p].b, sizeof(uint32_t) * W);
+            }
+            note(sdpg_batch_gvec(batch, req.data(), sa.data(), sb.data(), phase.data()));
+            note(sdpg_batch_newton(batch, req.data(), info.data()));
+            for (int p = 0; p < n; p++) {
+                req2[p] = req[p] && !(req[p] == SDPG_BATCH_PREDICTOR && info[p] > 0);
+                if (req[p]) *post[p].info = req[p] == SDPG_BATCH_PREDICTOR ? info[p] : 0;
+            }
+            note(sdpg_batch_dz(batch, req2.data(), phase.data()));
+            note(sdpg_batch_dx(batch, req2.data()));
+        }
+        if (select(K_INNER)) {
+            note(sdpg_batch_inner(batch, req.data(), out.data()));
+            for (int p = 0; p < n; p++) {
+                if (!req[p]) continue;
+                memcpy(&ipc[(size_t)p * 10 * W], &out[(size_t)p * 10 * W], sizeof(uint32_t) * 10 * W);
+                ipv[p] = 1;
+                memcpy(post[p].o1, &ipc[((size_t)p * 10 + post[p].what - 1) * W], sizeof(uint32_t) * W);
+            }
+        }
+        if (select(K_EIG)) {
+            note(sdpg_batch_min_eigen(batch, req.data(), 0, out.data()));
+            note(sdpg_batch_min_eigen(batch, req.data(), 1, out2.data()));
+            for (int p = 0; p < n; p++) {
+                if (!req[p]) continue;
+                memcpy(post[p].o1, &out[(size_t)p * W], sizeof(uint32_t) * W);
+                memcpy(post[p].o2, &out2[(size_t)p * W], sizeof(uint32_t) * W);
+            }
+        }
+        if (select(K_UPDATE)) {
+            for (int p = 0; p < n; p++) {
+                if (!req[p]) continue;
+                memcpy(&sa[(size_t)p * W], post[p].a, sizeof(uint32_t) * W);
+                memcpy(&sb[(size_t)p * W], post[p].b, sizeof(uint32_t) * W);
+            }
+            note(sdpg_batch_update(batch, req.data(), sa.data(), sb.data()));
+        }
+        if (select(K_FACTOR)) {
+            note(sdpg_batch_factor_point(batch, req.data(), info.data()));
+            for (int p = 0; p < n; p++)
+                if (req[p]) *post[p].info = info[p];
+        }
+        if (select(K_RESID)) {
+            note(sdpg_batch_residuals(batch, req.data(), out.data()));
+            for (int p = 0; p < n; p++) {
+                if (!req[p]) continue;
+                memcpy(post[p].o1, &out[((size_t)p * 4 + 0) * W], sizeof(uint32_t) * W);
+                memcpy(post[p].o2, &out[((size_t)p * 4 + 1) * W], sizeof(uint32_t) * W);
+            }
+        }
+        meetings++;
+        served += arrived;
+        std::fill(posted.begin(), posted.end(), 0);
+        arrived = 0;
+        gen++;
+        cv.notify_all();
+    }
+};
+
+// one problem's side of the meetings
+struct Port : Exec {
+    Meeting *mt;
+    int p;
+    Port(Meeting *m, int p_) : mt(m), p(p_) {}
+    int step(sdpg_ctx *, bool predictor, const uint32_t *mu, const uint32_t *beta, int phase, int *info) override {
+        mt->ipv[p] = 0;
+        Meeting::Post q;
+        q.kind = Meeting::K_STEP;
+        q.pred = predictor;
+        q.phase = phase;
+        q.a = mu;
+        q.b = beta;
+        q.info = info;
+        return mt->meet(p, q);
+    }
+    int inner(sdpg_ctx *, int what, uint32_t *out) override {
+        if (what < SDPG_IP_XZ || what > SDPG_IP_TRZ) return SDPG_EARG;  // no initial-point products here
+        if (mt->ipv[p]) {
+            memcpy(out, &mt->ipc[((size_t)p * 10 + what - 1) * mt->W], sizeof(uint32_t) * mt->W);
+            return 0;
+        }
+        Meeting::Post q;
+        q.kind = Meeting::K_INNER;
+        q.what = what;
+        q.o1 = out;
+        return mt->meet(p, q);
+    }
+    int min_eigen2(sdpg_ctx *, uint32_t *x, uint32_t *z) override {
+        Meeting::Post q;
+        q.kind = Meeting::K_EIG;
+        q.o1 = x;
+        q.o2 = z;
+        return mt->meet(p, q);
+    }
+    int update(sdpg_ctx *, const uint32_t *aP, const uint32_t *aD) override {
+        mt->ipv[p] = 0;
+        Meeting::Post q;
+        q.kind = Meeting::K_UPDATE;
+        q.a = aP;
+        q.b = aD;
+        return mt->meet(p, q);
+    }
+    int factor_point(sdpg_ctx *, int *info) override {
+        Meeting::Post q;
+        q.kind = Meeting::K_FACTOR;
+        q.info = info;
+        return mt->meet(p, q);
+    }
+    int residuals(sdpg_ctx *, uint32_t *nrp, uint32_t *nrd) override {
+        Meeting::Post q;
+        q.kind = Meeting::K_RESID;
+        q.o1 = nrp;
+        q.o2 = nrd;
+        return mt->meet(p, q);
+    }
+};
+
+template <int L>
+static int batch_solve_L(const std::vector<RawProblem> &R, const Parameter &prm, int device, int use_graph,
+                         sdpa_amd_result *res, double *latency, double *stats) {
+    const int n = (int)R.size();
+    std::vector<std::unique_ptr<Solver<L>>> S(n);
+    std::vector<int> rcs(n, 0);
+    // set-up over a few host threads: read, formula table, committed context
+    {
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        const int nth = std::max(1, std::min(n, 16));
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&] {
+                for (int p; (p = next++) < n;) {
+                    S[p].reset(new Solver<L>());
+                    S[p]->prm = prm;
+                    int rc = S[p]->load(R[p]);
+                    if (!rc) {
+                        S[p]->compute_formula();
+                        rc = S[p]->setup_device(device, SDPG_GEMM_PLAIN, 0);
+                    }
+                    rcs[p] = rc;
+                }
+            });
+        for (auto &t : th) t.join();
+    }
+    for (int p = 0; p < n; p++)
+        if (rcs[p]) return rcs[p];
+    std::vector<sdpg_ctx *> ctxs(n);
+    for (int p = 0; p < n; p++) ctxs[p] = S[p]->ctx;
+    sdpg_batch *b = nullptr;
+    int rc = sdpg_batch_create(&b, n, ctxs.data());
+    if (rc) return rc;
+    sdpg_batch_set_graph(b, use_graph);
+    sdpg_batch_wait_contexts(b, 0);  // every device call of the loops goes through the batch
+    Meeting mt(b, n, L + 2);
+    std::vector<std::unique_ptr<Port>> ports(n);
+    for (int p = 0; p < n; p++) {
+        ports[p].reset(new Port(&mt, p));
+        S[p]->ex = ports[p].get();
+    }
+    // one host thread per problem: its own IPM control; the device calls meet
+    const double t0 = now_s();
+    {
+        std::vector<std::thread> th;
+        th.reserve(n);
+        for (int p = 0; p < n; p++)
+            th.emplace_back([&, p] {
+                rcs[p] = S[p]->run(res ? &res[p] : nullptr, -1);
+                if (latency) latency[p] = now_s() - t0;
+                mt.leave();
+            });
+        for (auto &t : th) t.join();
+    }
+    const double wall = now_s() - t0;
+    if (stats) {
+        stats[0] = wall;
+        stats[1] = (double)mt.meetings;
+        stats[2] = mt.meetings ? (double)mt.served / (double)mt.meetings : 0.0;
+        double tm[BATCH_TIMERS] = {0};
+        sdpg_batch_timers(b, tm, BATCH_TIMERS);
+        for (int k = 0; k < BATCH_TIMERS; k++) stats[3 + k] = tm[k];
+    }
+    sdpg_batch_destroy(b);
+    if (mt.rc) return mt.rc;
+    for (int p = 0; p < n; p++)
+        if (rcs[p]) return rcs[p];
+    return 0;
+}
+
 }  // namespace host
 }  // namespace sdpg
 
@@ -1254,6 +1543,24 @@ extern "C" int sdpa_amd_bench(const char *data_path, const char *param_path, int
 #define CALL_(LL)                                                                                                 \
     bench_L<LL>(R, prm, gemm_mode, slices, device, warm_iters, warmup, steps, ms_schur, ms_chol, ms_wall, kstats, \
                 dump_prefix)
+    DISPATCH_L(L, CALL_)
+#undef CALL_
+}
+
+extern "C" int sdpa_amd_batch_solve(const char *const *data_paths, int n, const char *param_path, int prec_override,
+                                    int device, int use_graph, sdpa_amd_result *res, double *latency_s,
+                                    double *stats) {
+    if (!data_paths || n <= 0) return SDPG_EARG;
+    Parameter prm;
+    if (param_path && !prm.read(param_path)) return -20;
+    if (prec_override > 0) prm.precision = prec_override;
+    std::vector<RawProblem> R(n);
+    for (int p = 0; p < n; p++) {
+        int rc = read_raw(data_paths[p], R[p]);
+        if (rc) return rc;
+    }
+    int L = sdpg_limbs(prm.precision);
+#define CALL_(LL) batch_solve_L<LL>(R, prm, device, use_graph, res, latency_s, stats)
     DISPATCH_L(L, CALL_)
 #undef CALL_
 }

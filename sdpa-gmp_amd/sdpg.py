@@ -40,8 +40,11 @@ EXPORTS = [
     "sdpg_reset_timers", "sdpg_sync", "sdpg_last_error", "sdpg_rgemm", "sdpg_update", "sdpg_factor_point",
     "sdpg_inner", "sdpg_min_eigen", "sdpg_profile", "sdpg_rgemm_noalloc",
     "sdpg_tune_slices", "sdpg_valu_peak", "sdpg_batch_create", "sdpg_batch_destroy", "sdpg_batch_newton",
-    "sdpg_batch_set_graph", "sdpg_batch_time",
+    "sdpg_batch_set_graph", "sdpg_batch_time", "sdpg_batch_gvec", "sdpg_batch_dz", "sdpg_batch_dx",
+    "sdpg_batch_update", "sdpg_batch_factor_point", "sdpg_batch_inner", "sdpg_batch_residuals",
+    "sdpg_batch_min_eigen", "sdpg_batch_wait_contexts", "sdpg_batch_timers",
     "sdpa_amd_solve", "sdpa_amd_bench", "sdpa_amd_ctx_from_file", "sdpa_amd_formula", "sdpa_amd_run",
+    "sdpa_amd_batch_solve",
 ]
 
 
@@ -101,12 +104,26 @@ def load(path: str | None = None):
                                    ctypes.c_char_p]
     lib.sdpg_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
     lib.sdpa_amd_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.sdpa_amd_batch_solve.argtypes = [P(ctypes.c_char_p), ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p, P(ctypes.c_double),
+                                         P(ctypes.c_double)]
     lib.sdpg_batch_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, P(ctypes.c_void_p)]
     lib.sdpg_batch_destroy.restype = None
     lib.sdpg_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.sdpg_batch_newton.argtypes = [ctypes.c_void_p, P(ctypes.c_int), P(ctypes.c_int)]
     lib.sdpg_batch_set_graph.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.sdpg_batch_time.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+    ip = P(ctypes.c_int)
+    lib.sdpg_batch_gvec.argtypes = [ctypes.c_void_p, ip, u32p, u32p, ip]
+    lib.sdpg_batch_dz.argtypes = [ctypes.c_void_p, ip, ip]
+    lib.sdpg_batch_dx.argtypes = [ctypes.c_void_p, ip]
+    lib.sdpg_batch_update.argtypes = [ctypes.c_void_p, ip, u32p, u32p]
+    lib.sdpg_batch_factor_point.argtypes = [ctypes.c_void_p, ip, ip]
+    lib.sdpg_batch_inner.argtypes = [ctypes.c_void_p, ip, u32p]
+    lib.sdpg_batch_residuals.argtypes = [ctypes.c_void_p, ip, u32p]
+    lib.sdpg_batch_min_eigen.argtypes = [ctypes.c_void_p, ip, ctypes.c_int, u32p]
+    lib.sdpg_batch_wait_contexts.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.sdpg_batch_timers.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]
     lib.sdpa_amd_formula.argtypes = [ctypes.c_char_p, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), ctypes.c_int]
     _lib = lib
     return lib
@@ -379,6 +396,77 @@ class Batch:
     def set_graph(self, on: bool):
         self.lib.sdpg_batch_set_graph(self.h, 1 if on else 0)
 
+    def _req(self, req):
+        r = np.ascontiguousarray(req, dtype=np.int32)
+        assert r.shape == (len(self.ctxs),)
+        return r, r.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+    def _chk(self, rc, what):
+        if rc:
+            raise SdpgError(f"{what} rc={rc}: {self.lib.sdpg_last_error().decode()}", rc)
+
+    def _scalars(self, vals):
+        L = self.ctxs[0].L
+        a = np.ascontiguousarray(np.asarray(vals, dtype=np.uint32).reshape(len(self.ctxs), L + 2))
+        return a, _ptr(a)
+
+    def gvec(self, req, mu, beta, phases):
+        r, rp = self._req(req)
+        m, mp_ = self._scalars(mu)
+        b, bp = self._scalars(beta)
+        ph, php = self._req(phases)
+        self._chk(self.lib.sdpg_batch_gvec(self.h, rp, mp_, bp, php), "sdpg_batch_gvec")
+
+    def dz(self, req, phases):
+        r, rp = self._req(req)
+        ph, php = self._req(phases)
+        self._chk(self.lib.sdpg_batch_dz(self.h, rp, php), "sdpg_batch_dz")
+
+    def dx(self, req):
+        r, rp = self._req(req)
+        self._chk(self.lib.sdpg_batch_dx(self.h, rp), "sdpg_batch_dx")
+
+    def update(self, req, alphaP, alphaD):
+        r, rp = self._req(req)
+        a, ap = self._scalars(alphaP)
+        d, dp = self._scalars(alphaD)
+        self._chk(self.lib.sdpg_batch_update(self.h, rp, ap, dp), "sdpg_batch_update")
+
+    def factor_point(self, req) -> np.ndarray:
+        r, rp = self._req(req)
+        info = np.zeros(len(self.ctxs), dtype=np.int32)
+        self._chk(self.lib.sdpg_batch_factor_point(self.h, rp, info.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                  "sdpg_batch_factor_point")
+        return info
+
+    def inner(self, req) -> np.ndarray:
+        """(n, 10, L+2): selectors SDPG_IP_XZ .. SDPG_IP_TRZ at index selector - 1"""
+        r, rp = self._req(req)
+        out = np.zeros((len(self.ctxs), 10, self.ctxs[0].L + 2), dtype=np.uint32)
+        self._chk(self.lib.sdpg_batch_inner(self.h, rp, _ptr(out)), "sdpg_batch_inner")
+        return out
+
+    def residuals(self, req) -> np.ndarray:
+        """(n, 4, L+2): max |Rp|, max |Rd|, C.X, b.y"""
+        r, rp = self._req(req)
+        out = np.zeros((len(self.ctxs), 4, self.ctxs[0].L + 2), dtype=np.uint32)
+        self._chk(self.lib.sdpg_batch_residuals(self.h, rp, _ptr(out)), "sdpg_batch_residuals")
+        return out
+
+    def min_eigen(self, req, which: int) -> np.ndarray:
+        r, rp = self._req(req)
+        out = np.zeros((len(self.ctxs), self.ctxs[0].L + 2), dtype=np.uint32)
+        self._chk(self.lib.sdpg_batch_min_eigen(self.h, rp, which, _ptr(out)), "sdpg_batch_min_eigen")
+        return out
+
+    def wait_contexts(self, on: bool):
+        self.lib.sdpg_batch_wait_contexts(self.h, 1 if on else 0)
+
+    def timers(self) -> list:
+        t = (ctypes.c_double * 9)()
+        self.lib.sdpg_batch_timers(self.h, t, 9)
+        return list(t)
+
     def time_ms(self) -> float:
         t = ctypes.c_double()
         self.lib.sdpg_batch_time(self.h, ctypes.byref(t))
@@ -406,6 +494,25 @@ def solve(data_path: str, param_path: str | None = None, prec: int = 0, log_path
     if rc:
         raise SdpgError(f"sdpa_amd_solve rc={rc}: {lib.sdpg_last_error().decode()}")
     return r
+
+
+def batch_solve(data_paths, param_path: str | None = None, prec: int = 0, device: int = 0, graph: bool = True):
+    """Lockstep solves of many problems with batched Newton steps (sdpa_amd_batch_solve): returns
+    (list of Result, latencies in seconds, stats dict)."""
+    lib = load()
+    n = len(data_paths)
+    paths = (ctypes.c_char_p * n)(*[p.encode() for p in data_paths])
+    res = (Result * n)()
+    lat = (ctypes.c_double * n)()
+    st = (ctypes.c_double * 12)()
+    rc = lib.sdpa_amd_batch_solve(paths, n, param_path.encode() if param_path else None, prec, device,
+                                  1 if graph else 0, res, lat, st)
+    if rc:
+        raise SdpgError(f"sdpa_amd_batch_solve rc={rc}: {lib.sdpg_last_error().decode()}", rc)
+    kinds = ("newton", "gvec", "dz", "dx", "update", "factor_point", "inner", "residuals", "min_eigen")
+    stats = {"wall_s": st[0], "meetings": int(st[1]), "mean_problems_per_meeting": st[2],
+             "device_ms": {k: st[3 + i] for i, k in enumerate(kinds)}}
+    return list(res), list(lat), stats
 
 
 def formula(data_path: str) -> np.ndarray:

@@ -250,3 +250,140 @@ def test_batch_graph_replays(probs):
                 assert np.array_equal(pr.ctx.get(sdpg.SDPG_DY), dy), f"replay {rep}: dy differs for {pr.path}"
     finally:
         batch.close()
+
+
+def _rel_matrix(a, b, L):
+    """normwise relative difference of two packed arrays (exact, as a float)"""
+    ai, Ea = exact.to_ints(a, L)
+    bi, Eb = exact.to_ints(b, L)
+    d = exact.diff_float(ai, Ea, bi, Eb)
+    s = float(np.max(np.abs(exact.to_float(b, L))))
+    return float(np.max(np.abs(d))) / s if s else float(np.max(np.abs(d)))
+
+
+@pytest.mark.gpu
+def test_batch_phases_match_contexts(probs):
+    """Every batched phase of an iteration against the context's own call on the same point: the inverse
+    factors, residuals, Make_gVec's g (predictor in an infeasible and a feasible phase), dZ and dX from
+    the same dy, the ten inner products (against their exact values from the device's arrays), the
+    step-length eigenvalues and the update."""
+    L = probs[0].L
+    P = 32 * L
+    n = len(probs)
+    phases = np.array([sdpg.SDPG_NOINFO if k % 2 == 0 else sdpg.SDPG_PDFEAS for k in range(n)], dtype=np.int32)
+    aP = [Fraction(7, 10), Fraction(1, 3)]
+    aD = [Fraction(9, 10), Fraction(1, 5)]
+    ref = []
+    for k, pr in enumerate(probs):
+        c = pr.ctx
+        c.set_point(pr.X, pr.Z, pr.iZ, pr.y)
+        assert c.factor_point() == 0
+        r = {"icX": c.get(sdpg.SDPG_INVCHOLX), "icZ": c.get(sdpg.SDPG_INVCHOLZ), "invZ": c.get(sdpg.SDPG_INVZ)}
+        r["res"] = c.residuals()
+        c.gvec(False, pr.mu, pr.beta, int(phases[k]))
+        r["g"] = c.get(sdpg.SDPG_GVEC)
+        c.schur()
+        assert c.potrf() == 0
+        c.potrs()
+        r["dy"] = c.get(sdpg.SDPG_DY)
+        c.dz(int(phases[k]))
+        r["dZ"] = c.get(sdpg.SDPG_DZ)
+        c.dx()
+        r["dX"] = c.get(sdpg.SDPG_DX)
+        r["ip"] = [c.inner(w) for w in range(1, 11)]
+        r["eig"] = [c.min_eigen(0), c.min_eigen(1)]
+        c.update(sdpg.pack([aP[k % 2]], L)[0], sdpg.pack([aD[k % 2]], L)[0])
+        r["X1"], r["Z1"], r["y1"] = c.get(sdpg.SDPG_X), c.get(sdpg.SDPG_Z), c.get(sdpg.SDPG_Y)
+        ref.append(r)
+    batch = sdpg.Batch([p.ctx for p in probs])
+    fails = []
+
+    def close(tag, k, got, want, bar):
+        err = _rel_matrix(got, want, L)
+        if err > bar:
+            fails.append(f"{os.path.basename(probs[k].path)} {tag}: {err:.3e} > {bar:.3e}")
+
+    try:
+        on = np.ones(n, dtype=np.int32)
+        for pr in probs:
+            pr.ctx.set_point(pr.X, pr.Z, pr.iZ, pr.y)
+        print("batched factor_point / residuals / gvec", flush=True)
+        assert not batch.factor_point(on).any()
+        for k, pr in enumerate(probs):
+            for key, sel in (("icX", sdpg.SDPG_INVCHOLX), ("icZ", sdpg.SDPG_INVCHOLZ), ("invZ", sdpg.SDPG_INVZ)):
+                close(key, k, pr.ctx.get(sel), ref[k][key], 2.0 ** -(P - 40))
+        res = batch.residuals(on)
+        for k in range(n):
+            for q in range(4):
+                close(f"residual {q}", k, res[k, q], ref[k]["res"][q], 2.0 ** -(P - 40))
+        mus = np.stack([pr.mu for pr in probs])
+        betas = np.stack([pr.beta for pr in probs])
+        batch.gvec(on, mus, betas, phases)
+        for k, pr in enumerate(probs):
+            close("g", k, pr.ctx.get(sdpg.SDPG_GVEC), ref[k]["g"], 2.0 ** -(P - 40))
+        assert not batch.newton(on).any()
+        for k, pr in enumerate(probs):
+            pr.ctx.put(sdpg.SDPG_DY, ref[k]["dy"])  # the same dy for dZ and dX
+        print("batched dz / dx / inner / min_eigen / update", flush=True)
+        batch.dz(on, phases)
+        for k, pr in enumerate(probs):
+            close("dZ", k, pr.ctx.get(sdpg.SDPG_DZ), ref[k]["dZ"], 2.0 ** -(P - 40))
+        batch.dx(on)
+        for k, pr in enumerate(probs):
+            close("dX", k, pr.ctx.get(sdpg.SDPG_DX), ref[k]["dX"], 2.0 ** -(P - 40))
+        ips = batch.inner(on)
+        for k, pr in enumerate(probs):
+            c = pr.ctx
+            arr = {w: exact.to_ints(c.get(w), L) for w in (sdpg.SDPG_X, sdpg.SDPG_Z, sdpg.SDPG_DX, sdpg.SDPG_DZ,
+                                                           sdpg.SDPG_Y, sdpg.SDPG_DY)}
+            Cm = exact.read_problem(pr.path)[4]
+            Cd = [np.full((nb, nb), Fraction(0), dtype=object) for nb in pr.sizes]
+            for blk, i, j, v in Cm:
+                Cd[blk][i, j] = v
+                Cd[blk][j, i] = v
+            Cflat = [v for mtx in Cd for v in mtx.reshape(-1, order="F")]
+            bvec = exact.read_problem(pr.path)[3]
+
+            def val(w):
+                v, E = arr[w]
+                return [Fraction(int(x)) * Fraction(2) ** E for x in v]
+
+            X, Z, dX, dZ, y, dy = (val(w) for w in (sdpg.SDPG_X, sdpg.SDPG_Z, sdpg.SDPG_DX, sdpg.SDPG_DZ,
+                                                    sdpg.SDPG_Y, sdpg.SDPG_DY))
+            offs = np.cumsum([0] + [nb * nb for nb in pr.sizes])
+            diag = [int(offs[b]) + i + nb * i for b, nb in enumerate(pr.sizes) for i in range(nb)]
+            pairs = {1: (X, Z), 2: (X, dZ), 3: (dX, Z), 4: (dX, dZ), 5: (Cflat, X), 6: (Cflat, dX), 7: (bvec, y),
+                     8: (bvec, dy)}
+            for w in range(1, 11):
+                if w in pairs:
+                    a, b = pairs[w]
+                    exact_v = sum(p_ * q_ for p_, q_ in zip(a, b))
+                    scale = float(sum(abs(p_ * q_) for p_, q_ in zip(a, b)))
+                else:
+                    src = X if w == sdpg.SDPG_IP_TRX else Z
+                    exact_v = sum(src[i] for i in diag)
+                    scale = float(sum(abs(src[i]) for i in diag))
+                got = sdpg.unpack(ips[k, w - 1], L)[0]
+                err = float(abs(got - exact_v))
+                if err > 2.0 ** -(P - 16) * max(scale, 1e-300):
+                    fails.append(f"{os.path.basename(pr.path)} inner {w}: {err:.3e} vs scale {scale:.3e}")
+        for which in (0, 1):
+            eg = batch.min_eigen(on, which)
+            for k in range(n):
+                got = float(sdpg.unpack(eg[k], L)[0])
+                want = float(sdpg.unpack(ref[k]["eig"][which], L)[0])
+                if abs(got - want) > 1e-10 * max(abs(want), 1.0):
+                    fails.append(f"{os.path.basename(probs[k].path)} min_eigen {which}: {got!r} vs {want!r}")
+        for k, pr in enumerate(probs):
+            pr.ctx.put(sdpg.SDPG_DX, ref[k]["dX"])
+            pr.ctx.put(sdpg.SDPG_DZ, ref[k]["dZ"])
+        batch.update(on, sdpg.pack([aP[k % 2] for k in range(n)], L), sdpg.pack([aD[k % 2] for k in range(n)], L))
+        for k, pr in enumerate(probs):
+            for key, sel in (("X1", sdpg.SDPG_X), ("Z1", sdpg.SDPG_Z), ("y1", sdpg.SDPG_Y)):
+                got = pr.ctx.get(sel)
+                if not np.array_equal(got, ref[k][key]):
+                    close(key, k, got, ref[k][key], 2.0 ** -(P - 8))
+        print(f"batched phase timers (ms): {[round(t, 3) for t in batch.timers()]}", flush=True)
+        assert not fails, "\n".join(fails)
+    finally:
+        batch.close()
