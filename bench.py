@@ -27,9 +27,11 @@ also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank
 replayed on its first iterate for both forms at 256, 320 and 200 bits (Ozaki slice counts from the
 autotuner for each precision), 2 timed steps each.
 
-also_measured_batch: the first --batch problems of the seeded batch workload (512 small SDPs, 1-3
-dense blocks of U[8,40], m in U[10,120]) solved to gap <= 1e-60 as concurrent independent solves
-(--batch-threads host threads, one context and stream each): solves/s and latency p50 / p99.
+also_measured_batch: the seeded batch workload (--batch = 512 small SDPs, 1-3 dense blocks of U[8,40],
+m in U[10,120]) solved to gap <= 1e-60 in lockstep (every device call of all running problems batched,
+finished problems masked out, the Newton phases replayed as a HIP graph): solves/s, latency p50 / p99
+from the start of the loops, device ms per phase; the first --batch-independent problems again as
+concurrent independent solves (--batch-threads host threads, one context and stream each).
 
 cpu_baseline: the oracle (C restatement of the reference's mpf path: Make_bMat and the reference's
 serial Rpotrf, one core as the reference runs them) on the same iterate, dumped by the GPU run: the
@@ -136,10 +138,12 @@ def measure(workload: int, forms: list, at: int, warmup: int, steps: int, solve:
     return info, sha, data, res
 
 
-def measure_batch(count: int, threads: int, gemm_name: str):
-    """The batched configuration as independent solves: the first `count` problems of the seeded batch
-    (1-3 dense blocks of U[8,40], m in U[10,120]) at 256 bits and gap <= 1e-60, `threads` host threads
-    each driving its own context and stream, so that the GPU runs several problems' kernels at once."""
+def measure_batch(count: int, independent: int, threads: int):
+    """The batched configuration: the first `count` problems of the seeded batch (1-3 dense blocks of
+    U[8,40], m in U[10,120]) at 256 bits and gap <= 1e-60 solved in lockstep (sdpa_amd_batch_solve: one
+    batched call per kind of device call of all running problems, the Newton phases as a HIP graph),
+    and the first `independent` of them as concurrent independent solves on `threads` host threads (one
+    context and stream each) for comparison on the same problems."""
     import concurrent.futures
 
     import gen_problems
@@ -149,29 +153,45 @@ def measure_batch(count: int, threads: int, gemm_name: str):
     t0 = time.time()
     paths = gen_problems.generate_batch(tmp, count)
     t_gen = time.time() - t0
-    mode = {"ozaki": sdpg.GEMM_OZAKI, "plain": sdpg.GEMM_PLAIN}[gemm_name]
-
-    def one(path):
-        t = time.time()
-        r = sdpg.solve(path, PARAM, 0, None, mode)
-        return time.time() - t, r.phase.decode().strip(), r.iterations
-
-    one(paths[0])  # warm-up: first-use costs of the runtime
     t0 = time.time()
-    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
-        res = list(ex.map(one, paths))
-    wall = time.time() - t0
-    lat = sorted(x[0] for x in res)
+    res, lat, st = sdpg.batch_solve(paths, PARAM, 0, 0, True)
+    t_all = time.time() - t0
+    wall = st["wall_s"]
+    slat = sorted(lat)
+    pct = lambda v, q: v[min(len(v) - 1, int(q * len(v)))]  # noqa: E731
     phases = {}
-    for _, ph, _ in res:
+    for r in res:
+        ph = r.phase.decode().strip()
         phases[ph] = phases.get(ph, 0) + 1
-    pct = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))]  # noqa: E731
-    out = {"problems": count, "host_threads": threads, "form": gemm_name, "wall_s": wall,
-           "solves_per_s": count / wall, "latency_p50_s": pct(0.50), "latency_p99_s": pct(0.99),
-           "mean_iterations": sum(x[2] for x in res) / count, "phases": phases, "generation_s": t_gen,
-           "not_pdopt": [[k, res[k][1], res[k][2]] for k in range(count) if res[k][1] != "pdOPT"]}
-    print(f"[bench] batch: {count} problems, {threads} threads: {out['solves_per_s']:.1f} solves/s, "
-          f"p50 {out['latency_p50_s']:.3f} s, p99 {out['latency_p99_s']:.3f} s, phases {phases}", flush=True)
+    out = {"problems": count, "form": "lockstep batched (plain limb arithmetic, element-form block kernels)",
+           "wall_s": wall, "solves_per_s": count / wall, "latency_p50_s": pct(slat, 0.50),
+           "latency_p99_s": pct(slat, 0.99), "mean_iterations": sum(r.iterations for r in res) / count,
+           "phases": phases, "meetings": st["meetings"], "mean_problems_per_meeting": st["mean_problems_per_meeting"],
+           "device_ms_by_phase": st["device_ms"], "setup_and_teardown_s": t_all - wall, "generation_s": t_gen}
+    print(f"[bench] batch lockstep: {count} problems: {out['solves_per_s']:.1f} solves/s, p50 "
+          f"{out['latency_p50_s']:.3f} s, p99 {out['latency_p99_s']:.3f} s, phases {phases}, device ms "
+          f"{ {k: round(v, 1) for k, v in st['device_ms'].items()} }", flush=True)
+    if independent > 0:
+        sub = paths[:independent]
+
+        def one(path):
+            t = time.time()
+            r = sdpg.solve(path, PARAM, 0, None, sdpg.GEMM_PLAIN)
+            return time.time() - t, r.phase.decode().strip(), r.iterations, r.objP70.decode()
+
+        one(sub[0])  # warm-up: first-use costs of the runtime
+        t0 = time.time()
+        with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+            ind = list(ex.map(one, sub))
+        w = time.time() - t0
+        il = sorted(x[0] for x in ind)
+        same = sum(1 for k, x in enumerate(ind) if x[1] == res[k].phase.decode().strip()
+                   and abs(x[2] - res[k].iterations) <= 1)
+        out["independent"] = {"problems": independent, "host_threads": threads, "wall_s": w,
+                              "solves_per_s": independent / w, "latency_p50_s": pct(il, 0.5),
+                              "latency_p99_s": pct(il, 0.99), "same_phase_and_iterations_as_lockstep": same}
+        print(f"[bench] batch independent: {independent} problems, {threads} threads: {independent / w:.1f} solves/s",
+              flush=True)
     return out
 
 
@@ -222,8 +242,10 @@ def main():
                     help="skip the second measured workload (n=150, m=1326 dense rank-2 constraints)")
     ap.add_argument("--extra-precs", default="256,320,200", help="precisions of the second workload's replays")
     ap.add_argument("--cpu-n", type=int, default=0, help="Cholesky sample size of the CPU baseline (0: all of B)")
-    ap.add_argument("--batch", type=int, default=128,
-                    help="problems of the seeded 512-problem batch solved as concurrent independent solves (0: skip)")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="problems of the seeded 512-problem batch solved in lockstep (0: skip)")
+    ap.add_argument("--batch-independent", type=int, default=64,
+                    help="of those, solved again as concurrent independent solves for comparison (0: skip)")
     ap.add_argument("--batch-threads", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -330,10 +352,7 @@ def main():
     out["kernel_rooflines"] = kr
 
     if a.batch > 0:
-        b = measure_batch(a.batch, a.batch_threads, "ozaki" if "ozaki" in forms else forms[0])
-        b["note"] = ("independent solves, one context and stream per host thread; the lockstep batched form "
-                     "(persistent fused kernels per Newton phase) is not built")
-        out["also_measured_batch"] = b
+        out["also_measured_batch"] = measure_batch(a.batch, a.batch_independent, a.batch_threads)
 
     if not a.no_cpu and os.path.exists(dump + "X.bin"):
         import oracle_lib

@@ -14,6 +14,9 @@ Produces
                           epsilonDash = 1e-60: at 256 bits (GMP truncation: the primal residual
                           stalls near 1e-49, so the run ends pFEAS) and at 512 bits (pdOPT; the
                           reference trajectory for the GPU's 256-bit run, objectives to 70 digits)
+  batch_solves.json       512-bit oracle solves (epsilonStar = epsilonDash = 1e-60) of problems
+                          BATCH_PICK of the seeded batch workload (gen_problems.generate_batch), with
+                          the sha256 of each generated input
   sdplib/*.dat-s.xz, sdplib_golden.json
                           inputs and recorded results of the reference's own regression runs
                           (SDPLIB/data/*.result.xz, example1.result): iteration count, phase,
@@ -142,6 +145,32 @@ def solve_summary(tmp, name, prec):
     return out
 
 
+BATCH_PICK = (2, 11, 14, 17, 20)  # blocks of 10..22 (both eigenvalue paths), m 10..31
+
+
+def batch_solves():
+    import hashlib
+
+    import gen_problems
+
+    tmp = tempfile.mkdtemp(prefix="sdpg_fix_b_")
+    try:
+        paths = gen_problems.generate_batch(tmp, max(BATCH_PICK) + 1)
+        out = {}
+        for k in BATCH_PICK:
+            with open(paths[k], "rb") as f:
+                sha = hashlib.sha256(f.read()).hexdigest()
+            res = oracle_lib.solve(paths[k], PARAM, 512, None)
+            out[str(k)] = {"sha256": sha, "oracle512": {
+                "iterations": res.iterations, "phase": res.phase_name.decode().strip(), "objP": res.objP70.decode(),
+                "objD": res.objD70.decode(), "relgap": res.relgap_s.decode()}}
+            print(f"batch problem {k}: {res.iterations} iterations, {res.phase_name.decode().strip()}", flush=True)
+        with open(os.path.join(HERE, "batch_solves.json"), "w") as f:
+            json.dump(out, f, indent=1)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def parse_result(text):
     g = {}
     m = re.search(r"phase.value = (\S+)", text)
@@ -178,8 +207,12 @@ def main():
     ap.add_argument("--skip-newton", action="store_true")
     ap.add_argument("--solves-only", action="store_true", help="recompute solves.json only")
     ap.add_argument("--newton-only", choices=["w0", "w1"], help="recompute the Newton fixtures of one workload")
+    ap.add_argument("--batch-solves", action="store_true", help="recompute batch_solves.json only")
     a = ap.parse_args()
     oracle_lib.build_oracle()
+    if a.batch_solves:
+        batch_solves()
+        return
     if a.solves_only:
         tmp = tempfile.mkdtemp(prefix="sdpg_fix_")
         try:
