@@ -28,31 +28,89 @@ struct BatchProb {
     // residuals, C (dense), b, the right-hand-side matrices of Make_gVec and two block workspaces
     PV Z, y, b, Rp, Rd, C, rzinv, xrd, dls, dX, dZ, icX, icZ, W1, W2;
     int64_t cs_off;  // 2m values of the batch's scratch: r_j = 1/sqrt(c_j d_j), then 1/L_jj
+    // block fixed point of the Schur build (batch arrays): the constraint matrices (element, row and
+    // matrix offsets of constraint 0; constraint i adds i Nd, i nsum, i nsdp), X by rows and inv(Z) by
+    // columns (element and row offsets)
+    int64_t fa_el, fa_row, fa_mat, fp_el, fp_row;
+    int32_t nsum, Nd;
+    int32_t rowoff[BMAXB];  // first row of block l among the problem's rows (sum of the sizes before it)
+};
+
+// a block task: SDP block l of problem p
+struct BatchBlk {
+    int32_t p, l;
 };
 
 // request flags: 0 nothing, 1 PREDICTOR (Schur build, Cholesky, solve), 2 CORRECTOR (solve only, with the
 // factor of the last predictor; Newton::Mehrotra, sdpa_newton.cpp:1375-1414)
 enum { BATCH_NONE = 0, BATCH_PREDICTOR = 1, BATCH_CORRECTOR = 2 };
 
-// Make_bMat for one constraint i of one problem per task (compute_bMat_dense_SDP with formula F1,
-// sdpa_newton.cpp:1007-1096): for every block l of constraint i, work1 = A_i inv(Z) and work2 = X work1
-// (sdpa_linear.cpp:856-955) in LDS, then B_ij += A_j . work2 for the partners j <= i of the block (the pair
-// rule for equal NonZeroEffect, sdpa_newton.cpp:1052-1054); B_ij and B_ji are written once, after the
-// blocks.  Element form: one rounded product and one rounded sum per term.  Tasks are dealt round-robin to
-// the workgroups of a grid that fills the device.
-// LDS: work1 and work2 as planar n x n arrays (plane stride PW = max(nmax^2, BS_THREADS)), then the row
-// of B (mmax values); the partial sums of the inner products reuse work1's planes.
+// Make_bMat in the plain limb form (compute_bMat_dense_SDP with formula F1, sdpa_newton.cpp:1007-1096): one
+// constraint i of one problem per task; for every block l, work1 = A_i inv(Z), work2 = X work1
+// (sdpa_linear.cpp:856-955) and B_ij += A_j . work2 for the partners j <= i of the block (the pair rule
+// for equal NonZeroEffect, sdpa_newton.cpp:1052-1054).  Every sum of products is exact: the operands are
+// in block fixed point (offset binary, W = 32L - 2 fraction bits under one exponent per row, column or
+// matrix), the limb products accumulate in 96-bit columns and one rounding gives each result.  A_i is
+// fixed by rows (work1) and by matrix (inner products) once for the batch, X by rows and inv(Z) by
+// columns per predictor (k_fx_point), work1 by columns and work2 by matrix in LDS.  Tasks are dealt
+// round-robin to a grid that fills the device.
+// LDS: R1 holds work1 or work2 as limb floats (planes of nmax^2), then the partial column sums of the
+// inner products; R2 the fixed-point work1 or work2 (planes of nmax^2); then the column exponents and
+// offset sums of work1, the exponent and offset sum of work2, and the row of B.
 constexpr int BS_THREADS = 512;
+constexpr int FX_HEAD = 2;
+
+template <int L>
+struct FxCfg {
+    static constexpr int S0 = (L >= 2) ? L - 2 : 0, NC = 2 * L - 1 - S0, W = 32 * L - FX_HEAD;
+};
+
+template <int L>
+__device__ __forceinline__ void fx_sum_add(uint32_t (&s)[L + 1], const uint32_t (&a)[L]) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q < L; q++) {
+        const uint64_t t = (uint64_t)s[q] + a[q] + c;
+        s[q] = (uint32_t)t;
+        c = (uint32_t)(t >> 32);
+    }
+    s[L] += c;
+}
+template <int L>
+__device__ __forceinline__ void fx_sum_add2(uint32_t (&s)[L + 1], const uint32_t *a) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int q = 0; q <= L; q++) {
+        const uint64_t t = (uint64_t)s[q] + a[q] + c;
+        s[q] = (uint32_t)t;
+        c = (uint32_t)(t >> 32);
+    }
+}
+// exponent of a block's product: the sum of the operand exponents, 0 when either side is all zero
+__device__ __forceinline__ int32_t fx_exp(int32_t ea, int32_t eb, int W) {
+    const int32_t e = ea + eb;
+    return (e < -(1 << 28) ? 0 : e) - 2 * W;
+}
+
 template <int L>
 __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__restrict__ probs,
                                                             const int2 *__restrict__ tasks, int ntasks,
-                                                            const int32_t *__restrict__ req, int nmax, int mmax) {
+                                                            const int32_t *__restrict__ req, int nmax, int mmax,
+                                                            const uint32_t *__restrict__ fa_r, const int32_t *__restrict__ fa_re,
+                                                            const uint32_t *__restrict__ fa_rs, const uint32_t *__restrict__ fa_g,
+                                                            const int32_t *__restrict__ fa_ge, const uint32_t *__restrict__ fa_gs,
+                                                            const uint32_t *__restrict__ fx_v, const int32_t *__restrict__ fx_e,
+                                                            const uint32_t *__restrict__ fx_s, const uint32_t *__restrict__ fz_v,
+                                                            const int32_t *__restrict__ fz_e, const uint32_t *__restrict__ fz_s) {
+    using C = FxCfg<L>;
+    constexpr int NC = C::NC, S0 = C::S0, W = C::W;
     extern __shared__ uint32_t shm[];
-    const int PW = max(nmax * nmax, BS_THREADS);
-    uint32_t *sF = shm, *sG = shm + (L + 1) * PW, *sB = shm + 2 * (L + 1) * PW, *sP = sF;
+    const int PW = nmax * nmax;
+    const int R1W = max(PW * (L + 1), BS_THREADS * 3 * NC);
+    uint32_t *R1 = shm, *R2 = shm + R1W, *sFs = R2 + PW * L, *sGs = sFs + nmax * (L + 1), *sB = sGs + (L + 1);
+    int32_t *sFe = (int32_t *)(sB + mmax * (L + 1)), *sGe = sFe + nmax;
     const int tid = threadIdx.x, nt = BS_THREADS;
-    // inner products: BS_THREADS / 8 partners at a time, 8 threads each
-    const int q = tid & 7, slot = tid >> 3, nslot = BS_THREADS / 8;
+    const int q8 = tid & 7, slot = tid >> 3, nslot = BS_THREADS / 8;
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const int2 tk = tasks[t];
         if (req[tk.x] != BATCH_PREDICTOR) continue;  // the whole workgroup skips together
@@ -60,46 +118,172 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
         const int i = tk.y, m = P.m, nsdp = P.nsdp;
         for (int j = tid; j <= i; j += nt) lds_st<L>(sB, mmax, j, zero<L>());
         for (int l = 0; l < nsdp; l++) {
-            const Desc di = P.desc[(int64_t)i * nsdp + l];
-            if (di.kind == 0) continue;
+            if (P.desc[(int64_t)i * nsdp + l].kind == 0) continue;
             const int n = P.n[l], nn = n * n;
-            const int64_t bo = P.blkoff[l];
+            const int64_t ael = P.fa_el + (int64_t)i * P.Nd + P.blkoff[l];  // A_i's block elements
+            const int64_t arw = P.fa_row + (int64_t)i * P.nsum + P.rowoff[l];
+            const int64_t pel = P.fp_el + P.blkoff[l], prw = P.fp_row + P.rowoff[l];
             __syncthreads();
             // work1 = A_i inv(Z)
             for (int o = tid; o < nn; o += nt) {
-                const int a = o % n, b = o / n;
-                mp<L> acc = zero<L>();
-                for (int c = 0; c < n; c++)
-                    acc = add<L>(acc, mul<L>(ldm<L>(P.dval, di.off + a + (int64_t)n * c),
-                                             ldm<L>(P.invZ, bo + c + (int64_t)n * b)));
-                lds_st<L>(sF, PW, o, acc);
+                const int r = o % n, c = o / n;
+                uint64_t lo[NC];
+                uint32_t hi[NC];
+#pragma unroll
+                for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
+                for (int k = 0; k < n; k++) {
+                    uint32_t a[L], b[L];
+                    const uint32_t *pa = fa_r + (ael + r + (int64_t)n * k) * L, *pb = fz_v + (pel + k + (int64_t)n * c) * L;
+#pragma unroll
+                    for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = pb[q];
+                    mpx::col_products<L, S0>(a, b, lo, hi);
+                }
+                uint32_t ra[L + 1], rb[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) ra[q] = fa_rs[(arw + r) * (L + 1) + q], rb[q] = fz_s[(prw + c) * (L + 1) + q];
+                lds_st<L>(R1, PW, o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fa_re[arw + r], fz_e[prw + c], W)));
+            }
+            __syncthreads();
+            // work1 by columns: exponents, fixed point, offset sums
+            for (int c = tid; c < n; c += nt) {
+                int32_t e = EZERO;
+                for (int r = 0; r < n; r++) {
+                    const mp<L> v = lds_ld<L>(R1, PW, r + n * c);
+                    if (!is_zero<L>(v) && v.e > e) e = v.e;
+                }
+                sFe[c] = e;
+            }
+            __syncthreads();
+            for (int o = tid; o < nn; o += nt) {
+                uint32_t a[L];
+                fx_convert<L>(lds_ld<L>(R1, PW, o), sFe[o / n], FX_HEAD, a);
+#pragma unroll
+                for (int q = 0; q < L; q++) R2[q * PW + o] = a[q];
+            }
+            __syncthreads();
+            for (int c = tid; c < n; c += nt) {
+                uint32_t sm[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) sm[q] = 0;
+                for (int r = 0; r < n; r++) {
+                    uint32_t a[L];
+#pragma unroll
+                    for (int q = 0; q < L; q++) a[q] = R2[q * PW + r + n * c];
+                    fx_sum_add<L>(sm, a);
+                }
+#pragma unroll
+                for (int q = 0; q <= L; q++) sFs[c * (L + 1) + q] = sm[q];
             }
             __syncthreads();
             // work2 = X work1
             for (int o = tid; o < nn; o += nt) {
-                const int a = o % n, b = o / n;
-                mp<L> acc = zero<L>();
-                for (int c = 0; c < n; c++)
-                    acc = add<L>(acc, mul<L>(ldm<L>(P.X, bo + a + (int64_t)n * c), lds_ld<L>(sF, PW, c + n * b)));
-                lds_st<L>(sG, PW, o, acc);
+                const int r = o % n, c = o / n;
+                uint64_t lo[NC];
+                uint32_t hi[NC];
+#pragma unroll
+                for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
+                for (int k = 0; k < n; k++) {
+                    uint32_t a[L], b[L];
+                    const uint32_t *pa = fx_v + (pel + r + (int64_t)n * k) * L;
+#pragma unroll
+                    for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = R2[q * PW + k + n * c];
+                    mpx::col_products<L, S0>(a, b, lo, hi);
+                }
+                uint32_t ra[L + 1], rb[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) ra[q] = fx_s[(prw + r) * (L + 1) + q], rb[q] = sFs[c * (L + 1) + q];
+                lds_st<L>(R1, PW, o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fx_e[prw + r], sFe[c], W)));
             }
             __syncthreads();
-            // B_ij += A_j . work2 (Rdot over n^2, sdpa_linear.cpp:334-337) for the partners j <= i
+            // work2 by matrix: exponent, fixed point, offset sum
+            if (tid < 64) {
+                int32_t e = EZERO;
+                for (int o = tid; o < nn; o += 64) {
+                    const mp<L> v = lds_ld<L>(R1, PW, o);
+                    if (!is_zero<L>(v) && v.e > e) e = v.e;
+                }
+                for (int h = 32; h >= 1; h >>= 1) e = max(e, __shfl_xor(e, h, 64));
+                if (tid == 0) sGe[0] = e;
+            }
+            __syncthreads();
+            const int32_t eg = sGe[0];
+            for (int o = tid; o < nn; o += nt) {
+                uint32_t a[L];
+                fx_convert<L>(lds_ld<L>(R1, PW, o), eg, FX_HEAD, a);
+#pragma unroll
+                for (int q = 0; q < L; q++) R2[q * PW + o] = a[q];
+            }
+            __syncthreads();
+            if (tid < 64) {
+                uint32_t sm[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) sm[q] = 0;
+                for (int o = tid; o < nn; o += 64) {
+                    uint32_t a[L];
+#pragma unroll
+                    for (int q = 0; q < L; q++) a[q] = R2[q * PW + o];
+                    fx_sum_add<L>(sm, a);
+                }
+                // lanes 0..63 -> R1 rows, then lane 0 adds them up (R1 is free until the inner products)
+#pragma unroll
+                for (int q = 0; q <= L; q++) R1[tid * (L + 1) + q] = sm[q];
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t sm[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) sm[q] = R1[q];
+                for (int u = 1; u < 64; u++) fx_sum_add2<L>(sm, R1 + u * (L + 1));
+#pragma unroll
+                for (int q = 0; q <= L; q++) sGs[q] = sm[q];
+            }
+            __syncthreads();
+            // B_ij += A_j . work2 for the partners j <= i: 8 threads per partner over the n^2 entries
+            uint32_t gsum[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) gsum[q] = sGs[q];
             for (int j0 = 0; j0 <= i; j0 += nslot) {
                 const int j = j0 + slot;
-                mp<L> part = zero<L>();
-                if (j <= i) {
-                    const Desc dj = P.desc[(int64_t)j * nsdp + l];
-                    if (dj.kind != 0)
-                        for (int e = q; e < nn; e += 8)
-                            part = add<L>(part, mul<L>(ldm<L>(P.dval, dj.off + e), lds_ld<L>(sG, PW, e)));
+                const bool on = j <= i && P.desc[(int64_t)j * nsdp + l].kind != 0;
+                uint64_t lo[NC];
+                uint32_t hi[NC];
+#pragma unroll
+                for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
+                if (on) {
+                    const int64_t jel = P.fa_el + (int64_t)j * P.Nd + P.blkoff[l];
+                    for (int e = q8; e < nn; e += 8) {
+                        uint32_t a[L], b[L];
+                        const uint32_t *pa = fa_g + (jel + e) * L;
+#pragma unroll
+                        for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = R2[q * PW + e];
+                        mpx::col_products<L, S0>(a, b, lo, hi);
+                    }
                 }
-                lds_st<L>(sP, PW, tid, part);
+                uint32_t *me = R1 + tid * 3 * NC;
+#pragma unroll
+                for (int z = 0; z < NC; z++) {
+                    me[3 * z] = (uint32_t)lo[z];
+                    me[3 * z + 1] = (uint32_t)(lo[z] >> 32);
+                    me[3 * z + 2] = hi[z];
+                }
                 __syncthreads();
-                if (q == 0 && j <= i) {
-                    mp<L> s = lds_ld<L>(sP, PW, tid);
-                    for (int u = 1; u < 8; u++) s = add<L>(s, lds_ld<L>(sP, PW, tid + u));
-                    lds_st<L>(sB, mmax, j, add<L>(lds_ld<L>(sB, mmax, j), s));
+                if (q8 == 0 && on) {
+                    for (int u = 1; u < 8; u++) {
+                        const uint32_t *o = R1 + (tid + u) * 3 * NC;
+#pragma unroll
+                        for (int z = 0; z < NC; z++) {
+                            const uint64_t v = (uint64_t)o[3 * z] | ((uint64_t)o[3 * z + 1] << 32);
+                            const uint64_t sm = lo[z] + v;
+                            hi[z] += o[3 * z + 2] + (sm < v ? 1u : 0u);
+                            lo[z] = sm;
+                        }
+                    }
+                    const int64_t jm = P.fa_mat + (int64_t)j * nsdp + l;
+                    uint32_t ra[L + 1];
+#pragma unroll
+                    for (int q = 0; q <= L; q++) ra[q] = fa_gs[jm * (L + 1) + q];
+                    const mp<L> v = fx_finish<L>(lo, hi, nn, ra, gsum, fx_exp(fa_ge[jm], eg, W));
+                    lds_st<L>(sB, mmax, j, add<L>(lds_ld<L>(sB, mmax, j), v));
                 }
                 __syncthreads();
             }
@@ -111,6 +295,135 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
             stm<L>(P.B, j + (int64_t)m * i, v);
         }
         __syncthreads();
+    }
+}
+
+// LDS words of k_batch_schur
+template <int L>
+inline size_t schur_lds_words(int nmax, int mmax) {
+    const size_t PW = (size_t)nmax * nmax;
+    const size_t R1W = std::max(PW * (L + 1), (size_t)BS_THREADS * 3 * FxCfg<L>::NC);
+    return R1W + PW * L + (size_t)nmax * (L + 1) + (L + 1) + (size_t)mmax * (L + 1) + nmax + 1;
+}
+
+// the constraint matrices in block fixed point, once per batch: per task (p, i) and block, by rows
+// (exponent, fixed value and offset sum per row) and by matrix (one exponent, fixed values, offset sum)
+template <int L>
+__global__ __launch_bounds__(256) void k_fx_constraints(const BatchProb *__restrict__ probs,
+                                                        const int2 *__restrict__ tasks, int ntasks, uint32_t *fa_r,
+                                                        int32_t *fa_re, uint32_t *fa_rs, uint32_t *fa_g,
+                                                        int32_t *fa_ge, uint32_t *fa_gs) {
+    __shared__ int32_t se[256];
+    __shared__ uint32_t ssum[256 * (L + 1)];
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const int2 tk = tasks[t];
+        const BatchProb &P = probs[tk.x];
+        const int i = tk.y;
+        for (int l = 0; l < P.nsdp; l++) {
+            const Desc d = P.desc[(int64_t)i * P.nsdp + l];
+            const int n = P.n[l], nn = n * n;
+            const int64_t el = P.fa_el + (int64_t)i * P.Nd + P.blkoff[l];
+            const int64_t rw = P.fa_row + (int64_t)i * P.nsum + P.rowoff[l];
+            const int64_t mt = P.fa_mat + (int64_t)i * P.nsdp + l;
+            auto val = [&](int r, int k) -> mp<L> {
+                return d.kind == 2 ? ldm<L>(P.dval, d.off + r + (int64_t)n * k) : zero<L>();
+            };
+            // by rows
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                int32_t e = EZERO;
+                for (int k = 0; k < n; k++) {
+                    const mp<L> v = val(r, k);
+                    if (!is_zero<L>(v) && v.e > e) e = v.e;
+                }
+                uint32_t sm[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) sm[q] = 0;
+                for (int k = 0; k < n; k++) {
+                    uint32_t a[L];
+                    fx_convert<L>(val(r, k), e, FX_HEAD, a);
+#pragma unroll
+                    for (int q = 0; q < L; q++) fa_r[(el + r + (int64_t)n * k) * L + q] = a[q];
+                    fx_sum_add<L>(sm, a);
+                }
+                fa_re[rw + r] = e;
+#pragma unroll
+                for (int q = 0; q <= L; q++) fa_rs[(rw + r) * (L + 1) + q] = sm[q];
+            }
+            // by matrix
+            int32_t e = EZERO;
+            for (int o = threadIdx.x; o < nn; o += blockDim.x) {
+                const mp<L> v = val(o % n, o / n);
+                if (!is_zero<L>(v) && v.e > e) e = v.e;
+            }
+            se[threadIdx.x] = e;
+            __syncthreads();
+            for (int h = blockDim.x / 2; h >= 1; h >>= 1) {
+                if ((int)threadIdx.x < h) se[threadIdx.x] = max(se[threadIdx.x], se[threadIdx.x + h]);
+                __syncthreads();
+            }
+            e = se[0];
+            uint32_t sm[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) sm[q] = 0;
+            for (int o = threadIdx.x; o < nn; o += blockDim.x) {
+                uint32_t a[L];
+                fx_convert<L>(val(o % n, o / n), e, FX_HEAD, a);
+#pragma unroll
+                for (int q = 0; q < L; q++) fa_g[(el + o) * L + q] = a[q];
+                fx_sum_add<L>(sm, a);
+            }
+#pragma unroll
+            for (int q = 0; q <= L; q++) ssum[threadIdx.x * (L + 1) + q] = sm[q];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int u = 1; u < (int)blockDim.x; u++) fx_sum_add2<L>(sm, ssum + u * (L + 1));
+                fa_ge[mt] = e;
+#pragma unroll
+                for (int q = 0; q <= L; q++) fa_gs[mt * (L + 1) + q] = sm[q];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// X by rows and inv(Z) by columns in block fixed point, per predictor (one workgroup per block task)
+template <int L>
+__global__ __launch_bounds__(256) void k_fx_point(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                                  int nblk, const int32_t *__restrict__ req, uint32_t *fx_v,
+                                                  int32_t *fx_e, uint32_t *fx_s, uint32_t *fz_v, int32_t *fz_e,
+                                                  uint32_t *fz_s) {
+    for (int t = blockIdx.x; t < nblk; t += gridDim.x) {
+        const BatchBlk bk = blks[t];
+        if (req[bk.p] != BATCH_PREDICTOR) continue;
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l];
+        const int64_t o = P.blkoff[bk.l], el = P.fp_el + o, rw = P.fp_row + P.rowoff[bk.l];
+        for (int u = threadIdx.x; u < 2 * n; u += blockDim.x) {
+            const bool isx = u < n;
+            const int r = isx ? u : u - n;  // row of X, or column of inv(Z)
+            auto at = [&](int k) -> int64_t { return isx ? r + (int64_t)n * k : k + (int64_t)n * r; };
+            const PV S = isx ? P.X : P.invZ;
+            int32_t e = EZERO;
+            for (int k = 0; k < n; k++) {
+                const mp<L> v = ldm<L>(S, o + at(k));
+                if (!is_zero<L>(v) && v.e > e) e = v.e;
+            }
+            uint32_t sm[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) sm[q] = 0;
+            uint32_t *dst = isx ? fx_v : fz_v;
+            for (int k = 0; k < n; k++) {
+                uint32_t a[L];
+                fx_convert<L>(ldm<L>(S, o + at(k)), e, FX_HEAD, a);
+#pragma unroll
+                for (int q = 0; q < L; q++) dst[(el + at(k)) * L + q] = a[q];
+                fx_sum_add<L>(sm, a);
+            }
+            (isx ? fx_e : fz_e)[rw + r] = e;
+            uint32_t *ds = isx ? fx_s : fz_s;
+#pragma unroll
+            for (int q = 0; q <= L; q++) ds[(rw + r) * (L + 1) + q] = sm[q];
+        }
     }
 }
 
@@ -234,9 +547,6 @@ __global__ __launch_bounds__(256) void k_batch_potrs(const BatchProb *__restrict
 // target beta mu of Make_gVec, the step lengths) come in a planar array sc (BSC_N values per problem)
 // and results go out per block task in a planar array bo (BO_N values per task).
 
-struct BatchBlk {
-    int32_t p, l;
-};
 enum { BSC_TARGET = 0, BSC_ALPHAP = 1, BSC_ALPHAD = 2, BSC_N = 3 };
 enum { BO_XZ, BO_XDZ, BO_DXZ, BO_DXDZ, BO_TRX, BO_TRZ, BO_CX, BO_CDX, BO_BY, BO_BDY, BO_MAXRD, BO_MAXRP, BO_EIG, BO_N };
 
@@ -934,6 +1244,10 @@ struct Batch : BatchBase {
     std::vector<int32_t> h_iarg;
     int grid_blk = 0;
     double tm[BT_N] = {0};
+    // block fixed point of the Schur build (k_fx_constraints, k_fx_point)
+    uint32_t *fa_r = nullptr, *fa_rs = nullptr, *fa_g = nullptr, *fa_gs = nullptr, *fx_v = nullptr, *fx_s = nullptr,
+             *fz_v = nullptr, *fz_s = nullptr;
+    int32_t *fa_re = nullptr, *fa_ge = nullptr, *fx_e = nullptr, *fz_e = nullptr;
 
     ~Batch() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -948,6 +1262,9 @@ struct Batch : BatchBase {
         (void)hipFree(d_binfo);
         (void)hipFree(d_sc);
         (void)hipFree(d_bo);
+        for (void *q : {(void *)fa_r, (void *)fa_rs, (void *)fa_g, (void *)fa_gs, (void *)fx_v, (void *)fx_s,
+                        (void *)fz_v, (void *)fz_s, (void *)fa_re, (void *)fa_ge, (void *)fx_e, (void *)fz_e})
+            (void)hipFree(q);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (st) (void)hipStreamDestroy(st);
@@ -956,6 +1273,7 @@ struct Batch : BatchBase {
     int create(EngineBase *const *es, int n) {
         if (n <= 0) return SDPG_EARG;
         device = es[0]->device;
+        int64_t fa_el_n = 0, fa_row_n = 0, fa_mat_n = 0, fp_el_n = 0, fp_row_n = 0;
         std::vector<BatchProb> hp;
         std::vector<int2> tasks;
         int64_t csn = 0;
@@ -1013,6 +1331,22 @@ struct Batch : BatchBase {
             b.W1 = e->W1.pv();
             b.W2 = e->W2.pv();
             for (int l = 0; l < e->nsdp; l++) blks.push_back(BatchBlk{p, l});
+            b.Nd = (int32_t)e->Nd;
+            b.nsum = 0;
+            for (int l = 0; l < e->nsdp; l++) {
+                b.rowoff[l] = b.nsum;
+                b.nsum += e->sizes[l];
+            }
+            b.fa_el = fa_el_n;
+            b.fa_row = fa_row_n;
+            b.fa_mat = fa_mat_n;
+            b.fp_el = fp_el_n;
+            b.fp_row = fp_row_n;
+            fa_el_n += (int64_t)e->m * e->Nd;
+            fa_row_n += (int64_t)e->m * b.nsum;
+            fa_mat_n += (int64_t)e->m * e->nsdp;
+            fp_el_n += e->Nd;
+            fp_row_n += b.nsum;
             b.cs_off = csn;
             csn += 2 * (int64_t)e->m;
             hp.push_back(b);
@@ -1036,7 +1370,7 @@ struct Batch : BatchBase {
             if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
                 lds_max = (size_t)v;
         }
-        shm_schur = (size_t)(L + 1) * 4 * (2 * (size_t)std::max(nmax * nmax, BS_THREADS) + mmax);
+        shm_schur = 4 * schur_lds_words<L>(nmax, mmax);
         shm_potrs = (size_t)(L + 1) * 4 * mmax;
         if (shm_schur > lds_max) {
             set_err("sdpg_batch_create: blocks too large for the batched form at this precision");
@@ -1078,17 +1412,37 @@ struct Batch : BatchBase {
         }
         if (hipMemcpy(d_blks, blks.data(), sizeof(BatchBlk) * blks.size(), hipMemcpyHostToDevice) != hipSuccess)
             return SDPG_EHIP;
+        // block fixed point of the Schur build: the constraint matrices (filled once below), X and inv(Z)
+        auto dm = [&](void **ptr, size_t bytes) { return hipMalloc(ptr, std::max<size_t>(bytes, 4)) == hipSuccess; };
+        if (!dm((void **)&fa_r, 4 * (size_t)L * fa_el_n) || !dm((void **)&fa_re, 4 * (size_t)fa_row_n) ||
+            !dm((void **)&fa_rs, 4 * (size_t)(L + 1) * fa_row_n) || !dm((void **)&fa_g, 4 * (size_t)L * fa_el_n) ||
+            !dm((void **)&fa_ge, 4 * (size_t)fa_mat_n) || !dm((void **)&fa_gs, 4 * (size_t)(L + 1) * fa_mat_n) ||
+            !dm((void **)&fx_v, 4 * (size_t)L * fp_el_n) || !dm((void **)&fx_e, 4 * (size_t)fp_row_n) ||
+            !dm((void **)&fx_s, 4 * (size_t)(L + 1) * fp_row_n) || !dm((void **)&fz_v, 4 * (size_t)L * fp_el_n) ||
+            !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n)) {
+            set_err("sdpg_batch_create: out of device memory");
+            return SDPG_ENOMEM;
+        }
         if (hipFuncSetAttribute((const void *)k_batch_schur<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm_schur) != hipSuccess ||
             hipMemcpy(d_probs, hp.data(), sizeof(BatchProb) * n, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(d_tasks, tasks.data(), sizeof(int2) * ntasks, hipMemcpyHostToDevice) != hipSuccess)
             return SDPG_EHIP;
+        for (Engine<L> *e : eng) HIPCHK(hipStreamSynchronize(e->st));  // the constraint values are in place
+        k_fx_constraints<L><<<std::max(1, std::min(ntasks, 8 * cus)), 256, 0, st>>>(d_probs, d_tasks, ntasks, fa_r,
+                                                                                   fa_re, fa_rs, fa_g, fa_ge, fa_gs);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
         return SDPG_OK;
     }
 
     // the three phases on the batch's stream (what the graph holds)
     void launch() {
-        k_batch_schur<L><<<grid_schur, BS_THREADS, shm_schur, st>>>(d_probs, d_tasks, ntasks, d_req, nmax, mmax);
+        k_fx_point<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, fx_v, fx_e, fx_s, fz_v, fz_e,
+                                                fz_s);
+        k_batch_schur<L><<<grid_schur, BS_THREADS, shm_schur, st>>>(d_probs, d_tasks, ntasks, d_req, nmax, mmax, fa_r,
+                                                                    fa_re, fa_rs, fa_g, fa_ge, fa_gs, fx_v, fx_e, fx_s,
+                                                                    fz_v, fz_e, fz_s);
         k_batch_potrf<L><<<grid_prob, 256, 0, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n});
         k_batch_potrs<L><<<grid_prob, 256, shm_potrs, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n}, mmax);
     }
