@@ -948,12 +948,10 @@ __global__ __launch_bounds__(256) void k_bb_resid(const BatchProb *__restrict__ 
 
 // ---- step length: minimum eigenvalue of lMat xMat lMat^T per block (Jal::getMinEigen, sdpa_jordan.cpp:55-88)
 
-// symmetric tridiagonal eigenproblem in double (implicit QL with Wilkinson shifts; d: diagonal in,
-// eigenvalues out; e: off-diagonal, destroyed; z: eigenvectors by columns, leading dimension ldz);
-// returns the index of the smallest eigenvalue, or -1 without convergence
-__device__ inline int tql_double(int n, double *d, double *e, double *z, int ldz) {
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) z[i + j * ldz] = (i == j) ? 1.0 : 0.0;
+// eigenvalues of the symmetric tridiagonal (d, e) in double by implicit QL with Wilkinson shifts (d in:
+// diagonal, out: eigenvalues; e: off-diagonal e[0..n-2], destroyed); returns the index of the smallest,
+// or -1 without convergence
+__device__ inline int tql_values(int n, double *d, double *e) {
     e[n - 1] = 0.0;
     for (int l = 0; l < n; l++) {
         int iter = 0, m;
@@ -970,7 +968,7 @@ __device__ inline int tql_double(int n, double *d, double *e, double *z, int ldz
             double s = 1.0, c = 1.0, p = 0.0;
             bool early = false;
             for (int i = m - 1; i >= l; i--) {
-                double f = s * e[i], b = c * e[i];
+                const double f = s * e[i], b = c * e[i];
                 r = sqrt(f * f + g * g);
                 e[i + 1] = r;
                 if (r == 0.0) {
@@ -986,11 +984,6 @@ __device__ inline int tql_double(int n, double *d, double *e, double *z, int ldz
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
-                for (int k = 0; k < n; k++) {
-                    f = z[k + (i + 1) * ldz];
-                    z[k + (i + 1) * ldz] = s * z[k + i * ldz] + c * f;
-                    z[k + i * ldz] = c * z[k + i * ldz] - s * f;
-                }
             }
             if (early) continue;
             d[l] -= p;
@@ -1004,57 +997,113 @@ __device__ inline int tql_double(int n, double *d, double *e, double *z, int ldz
     return k;
 }
 
-// cyclic Jacobi in double on a symmetric n x n matrix a (destroyed); v receives the eigenvectors by
-// columns; returns the index of the smallest eigenvalue (on the diagonal of a).  The sweeps end when the
-// off-diagonal mass is below 1e-26 of the diagonal's: rotations leave rounding errors of about
-// n^2 u^2 |a|^2 off the diagonal, so a tighter test would never pass; the Rayleigh quotient in multiple
-// precision squares the eigenvector's error of about 1e-13.
-__device__ inline int jacobi_double(int n, double *a, double *v) {
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) v[i + j * n] = (i == j) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 60; sweep++) {
-        double off = 0.0, dg = 0.0;
-        for (int j = 0; j < n; j++)
-            for (int i = 0; i < n; i++) (i == j ? dg : off) += a[i + j * n] * a[i + j * n];
-        if (off == 0.0 || off <= dg * 1e-26) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                const double apq = a[p + q * n];
-                if (apq == 0.0) continue;
-                const double th = (a[q + q * n] - a[p + p * n]) / (2.0 * apq);
-                double t = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
-                if (th < 0) t = -t;
-                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c, tau = s / (1.0 + c);
-                a[p + p * n] -= t * apq;
-                a[q + q * n] += t * apq;
-                a[p + q * n] = a[q + p * n] = 0.0;
-                for (int k = 0; k < n; k++) {
-                    if (k != p && k != q) {
-                        const double akp = a[k + p * n], akq = a[k + q * n];
-                        a[k + p * n] = a[p + k * n] = akp - s * (akq + tau * akp);
-                        a[k + q * n] = a[q + k * n] = akq + s * (akp - tau * akq);
-                    }
-                    const double vkp = v[k + p * n], vkq = v[k + q * n];
-                    v[k + p * n] = vkp - s * (vkq + tau * vkp);
-                    v[k + q * n] = vkq + s * (vkp - tau * vkq);
-                }
-            }
+// unit eigenvector x of the tridiagonal (a: diagonal, b: off-diagonal) for its eigenvalue lam: two steps
+// of inverse iteration, (T - sigma I) x_new = x by the tridiagonal elimination (sigma just below lam)
+__device__ inline void tri_inverse_iteration(int n, const double *a, const double *b, double lam, double *x, double *w) {
+    double scale = 0.0;
+    for (int i = 0; i < n; i++) scale = fmax(scale, fabs(a[i]) + (i + 1 < n ? fabs(b[i]) : 0.0));
+    const double sigma = lam - 1e-12 * fmax(scale, 1e-300);
+    for (int i = 0; i < n; i++) x[i] = 1.0;
+    for (int it = 0; it < 2; it++) {
+        // forward elimination: w holds the modified super-diagonal
+        double den = a[0] - sigma;
+        if (den == 0.0) den = 1e-300;
+        w[0] = (n > 1 ? b[0] : 0.0) / den;
+        x[0] = x[0] / den;
+        for (int i = 1; i < n; i++) {
+            den = (a[i] - sigma) - b[i - 1] * w[i - 1];
+            if (den == 0.0) den = 1e-300;
+            w[i] = (i + 1 < n ? b[i] : 0.0) / den;
+            x[i] = (x[i] - b[i - 1] * x[i - 1]) / den;
+        }
+        for (int i = n - 2; i >= 0; i--) x[i] -= w[i] * x[i + 1];
+        double nr = 0.0;
+        for (int i = 0; i < n; i++) nr += x[i] * x[i];
+        nr = 1.0 / sqrt(nr);
+        for (int i = 0; i < n; i++) x[i] *= nr;
     }
-    int k = 0;
-    for (int i = 1; i < n; i++)
-        if (a[i + i * n] < a[k + k * n]) k = i;
-    return k;
 }
 
 constexpr int BEIG_KMAX = 20;  // Lanczos steps of a block of at most 40 (the loop ends by sqrt(n) + 10)
 constexpr int BEIG_NMAX = 40;
+constexpr int BEIG_JMAX = 20;  // blocks taking the full eigenvalue problem (Rsyev in the reference)
+constexpr int BEIG_LD = BEIG_JMAX + 1;
+
+// one sweep of the parallel cyclic Jacobi method on the symmetric matrix a (leading dimension BEIG_LD,
+// padded to an even order N with a zero row and column), eigenvector columns accumulated in v: N - 1
+// rounds of N/2 disjoint rotations in round-robin order (index N - 1 fixed, the others rotating), each
+// round applied as A <- A J (columns), then A <- J^T A (rows), V <- V J.  cs: N/2 (c, s) pairs.
+__device__ inline void jacobi_sweep(int N, double *a, double *v, double *cs, int *pr) {
+    const int tid = threadIdx.x, half = N / 2;
+    for (int round = 0; round < N - 1; round++) {
+        if (tid < half) {
+            // pair tid of this round: players (round + tid) mod (N-1) and (round - tid) mod (N-1) for
+            // tid > 0, and (round, N - 1) for tid = 0
+            int p, q;
+            if (tid == 0) {
+                p = round;
+                q = N - 1;
+            } else {
+                p = (round + tid) % (N - 1);
+                q = (round - tid + (N - 1)) % (N - 1);
+            }
+            if (p > q) {
+                const int t = p;
+                p = q;
+                q = t;
+            }
+            double c = 1.0, s = 0.0;
+            const double apq = a[p + q * BEIG_LD];
+            if (apq != 0.0) {
+                const double th = (a[q + q * BEIG_LD] - a[p + p * BEIG_LD]) / (2.0 * apq);
+                double t = 1.0 / (fabs(th) + sqrt(th * th + 1.0));
+                if (th < 0) t = -t;
+                c = 1.0 / sqrt(t * t + 1.0);
+                s = t * c;
+            }
+            cs[2 * tid] = c;
+            cs[2 * tid + 1] = s;
+            pr[2 * tid] = p;
+            pr[2 * tid + 1] = q;
+        }
+        __syncthreads();
+        for (int u = tid; u < half * N; u += blockDim.x) {  // columns: A <- A J, V <- V J
+            const int k = u / N, r = u % N;
+            const int p = pr[2 * k], q = pr[2 * k + 1];
+            const double c = cs[2 * k], s = cs[2 * k + 1];
+            const double arp = a[r + p * BEIG_LD], arq = a[r + q * BEIG_LD];
+            a[r + p * BEIG_LD] = c * arp - s * arq;
+            a[r + q * BEIG_LD] = s * arp + c * arq;
+            const double vrp = v[r + p * BEIG_LD], vrq = v[r + q * BEIG_LD];
+            v[r + p * BEIG_LD] = c * vrp - s * vrq;
+            v[r + q * BEIG_LD] = s * vrp + c * vrq;
+        }
+        __syncthreads();
+        for (int u = tid; u < half * N; u += blockDim.x) {  // rows: A <- J^T A
+            const int k = u / N, r = u % N;
+            const int p = pr[2 * k], q = pr[2 * k + 1];
+            const double c = cs[2 * k], s = cs[2 * k + 1];
+            const double apr = a[p + r * BEIG_LD], aqr = a[q + r * BEIG_LD];
+            a[p + r * BEIG_LD] = c * apr - s * aqr;
+            a[q + r * BEIG_LD] = s * apr + c * aqr;
+        }
+        __syncthreads();
+        if (tid < half) {  // the annihilated pair is zero up to rounding: make it exact
+            const int p = pr[2 * tid], q = pr[2 * tid + 1];
+            a[p + q * BEIG_LD] = 0.0;
+            a[q + p * BEIG_LD] = 0.0;
+        }
+        __syncthreads();
+    }
+}
 
 // which 0: (icX, dX), 1: (icZ, dZ).  M = lMat xMat lMat^T is formed in W2.  Blocks of n <= 20: the smallest
-// eigenvalue of M (Rsyev, Lal::getMinEigenValue, sdpa_linear.cpp:126-158) from the eigenvector of a double
-// Jacobi solve refined by its Rayleigh quotient in multiple precision.  Larger blocks: the Lanczos
-// process of Lal::getMinEigen (sdpa_linear.cpp:27-124) with its stopping rule, in multiple precision;
-// its tridiagonal eigenproblem (Rsteqr there) is solved in double and the smallest eigenvalue refined
-// by the Rayleigh quotient of its eigenvector in multiple precision.
+// eigenvalue of M (Rsyev, Lal::getMinEigenValue, sdpa_linear.cpp:126-158) from the eigenvector of a parallel
+// Jacobi solve in double refined by its Rayleigh quotient in multiple precision.  Larger blocks: the
+// Lanczos process of Lal::getMinEigen (sdpa_linear.cpp:27-124) with its stopping rule, in multiple
+// precision; its tridiagonal eigenproblem (Rsteqr there) is solved in double (eigenvalues by QL, the
+// smallest one's eigenvector by inverse iteration) and the eigenvalue refined by the Rayleigh quotient in
+// multiple precision.
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__ probs,
                                                    const BatchBlk *__restrict__ blks, int nblk,
@@ -1062,8 +1111,9 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
     __shared__ uint32_t sh[(L + 1) * 256];
     __shared__ uint32_t vq[(L + 1) * BEIG_NMAX], vqo[(L + 1) * BEIG_NMAX], vr[(L + 1) * BEIG_NMAX],
         vw[(L + 1) * BEIG_NMAX], tal[(L + 1) * BEIG_KMAX], tbe[(L + 1) * BEIG_KMAX], tsc[(L + 1) * 4];
-    __shared__ double dA[BEIG_NMAX * BEIG_NMAX > 400 ? 400 : BEIG_NMAX * BEIG_NMAX], dV[400];
-    __shared__ double td[BEIG_KMAX], te[BEIG_KMAX], tz[BEIG_KMAX * BEIG_KMAX];
+    __shared__ double dA[BEIG_LD * BEIG_LD], dV[BEIG_LD * BEIG_LD], dcs[BEIG_LD + 1];
+    __shared__ int dpr[BEIG_LD + 1];
+    __shared__ double td[BEIG_KMAX], te[BEIG_KMAX], tx[BEIG_KMAX], tw[BEIG_KMAX];
     __shared__ int sflag;
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
@@ -1076,10 +1126,40 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
         wg_sync();
         const PV M = P.W2;
         mp<L> result;
-        if (n <= 20) {
-            for (int e = threadIdx.x; e < nn; e += blockDim.x) dA[e] = to_double<L>(ldm<L>(M, o + e));
+        if (n <= BEIG_JMAX) {
+            const int N = n + (n & 1);
+            for (int e = threadIdx.x; e < BEIG_LD * BEIG_LD; e += blockDim.x) {
+                const int r = e % BEIG_LD, c = e / BEIG_LD;
+                dA[e] = (r < n && c < n) ? to_double<L>(ldm<L>(M, o + r + (int64_t)n * c)) : 0.0;
+                dV[e] = (r == c) ? 1.0 : 0.0;
+            }
             __syncthreads();
-            if (threadIdx.x == 0) sflag = jacobi_double(n, dA, dV);
+            for (int sweep = 0; sweep < 40; sweep++) {
+                // off-diagonal against diagonal mass (1e-26: rotations leave about n^2 u^2 |a|^2 off the
+                // diagonal; the Rayleigh quotient squares the eigenvector's error of about 1e-13)
+                double off = 0.0, dg = 0.0;
+                if (threadIdx.x < 64) {
+                    for (int e = threadIdx.x; e < N * N; e += 64) {
+                        const int r = e % N, c = e / N;
+                        const double x = dA[r + c * BEIG_LD];
+                        (r == c ? dg : off) += x * x;
+                    }
+                    for (int h = 32; h >= 1; h >>= 1) {
+                        off += __shfl_xor(off, h, 64);
+                        dg += __shfl_xor(dg, h, 64);
+                    }
+                    if (threadIdx.x == 0) sflag = (off == 0.0 || off <= dg * 1e-26) ? 1 : 0;
+                }
+                __syncthreads();
+                if (sflag) break;
+                jacobi_sweep(N, dA, dV, dcs, dpr);
+            }
+            if (threadIdx.x == 0) {
+                int k = 0;
+                for (int i = 1; i < n; i++)
+                    if (dA[i + i * BEIG_LD] < dA[k + k * BEIG_LD]) k = i;
+                sflag = k;
+            }
             __syncthreads();
             const int k = sflag;
             // Rayleigh quotient of the eigenvector v: (v^T M v) / (v^T v)
@@ -1088,8 +1168,8 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
                 const int a = threadIdx.x;
                 mp<L> s = zero<L>();
                 for (int b = 0; b < n; b++)
-                    s = add<L>(s, mul<L>(ldm<L>(M, o + a + (int64_t)n * b), from_double<L>(dV[b + k * n])));
-                const mp<L> va = from_double<L>(dV[a + k * n]);
+                    s = add<L>(s, mul<L>(ldm<L>(M, o + a + (int64_t)n * b), from_double<L>(dV[b + k * BEIG_LD])));
+                const mp<L> va = from_double<L>(dV[a + k * BEIG_LD]);
                 num = mul<L>(va, s);
                 den = mul<L>(va, va);
             }
@@ -1097,7 +1177,6 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
             den = wg_sum<L>(den, sh);
             result = mpx::div<L>(num, den);
         } else {
-            using namespace mpx;
             const mp<L> one = from_int<L>(1), zr = zero<L>();
             mp<L> mn = from_double<L>(1.0e+51), mn_old = from_double<L>(1.0e+52), error = from_double<L>(1.0e+10);
             const mp<L> nD = from_int<L>(n);
@@ -1110,6 +1189,8 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
                 lds_st<L>(vr, BEIG_NMAX, i, one);
             }
             __syncthreads();
+            // matrix-vector products: row i split over MV_PARTS threads, partial sums through sh
+            constexpr int MV_PARTS = 6;
             int k = 0, kk = 0;
             for (;;) {
                 const mp<L> kf = from_int<L>(k);
@@ -1123,10 +1204,17 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
                     lds_st<L>(vq, BEIG_NMAX, i, mul<L>(value, lds_ld<L>(vr, BEIG_NMAX, i)));
                 }
                 __syncthreads();
-                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                if (threadIdx.x < n * MV_PARTS) {
+                    const int i = threadIdx.x / MV_PARTS, part = threadIdx.x % MV_PARTS;
                     mp<L> s = zero<L>();
-                    for (int b = 0; b < n; b++)
+                    for (int b = part; b < n; b += MV_PARTS)
                         s = add<L>(s, mul<L>(ldm<L>(M, o + i + (int64_t)n * b), lds_ld<L>(vq, BEIG_NMAX, b)));
+                    lds_st<L>(sh, 256, threadIdx.x, s);
+                }
+                __syncthreads();
+                for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                    mp<L> s = lds_ld<L>(sh, 256, i * MV_PARTS);
+                    for (int part = 1; part < MV_PARTS; part++) s = add<L>(s, lds_ld<L>(sh, 256, i * MV_PARTS + part));
                     lds_st<L>(vw, BEIG_NMAX, i, s);
                 }
                 __syncthreads();
@@ -1150,25 +1238,27 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
                             td[i] = to_double<L>(lds_ld<L>(tal, BEIG_KMAX, i));
                             te[i] = i + 1 < nt ? to_double<L>(lds_ld<L>(tbe, BEIG_KMAX, i)) : 0.0;
                         }
-                        const int j = tql_double(nt, td, te, tz, BEIG_KMAX);
+                        const int j = tql_values(nt, td, te);
                         sflag = j;
                         if (j >= 0) {
+                            const double lam = td[j];
+                            for (int i = 0; i < nt; i++) {
+                                td[i] = to_double<L>(lds_ld<L>(tal, BEIG_KMAX, i));
+                                te[i] = i + 1 < nt ? to_double<L>(lds_ld<L>(tbe, BEIG_KMAX, i)) : 0.0;
+                            }
+                            tri_inverse_iteration(nt, td, te, lam, tx, tw);
                             // Rayleigh quotient of the eigenvector in multiple precision; error = its last component
                             mp<L> num = zero<L>(), den = zero<L>();
                             for (int i = 0; i < nt; i++) {
-                                const mp<L> vi = from_double<L>(tz[i + j * BEIG_KMAX]);
+                                const mp<L> vi = from_double<L>(tx[i]);
                                 mp<L> tv = mul<L>(lds_ld<L>(tal, BEIG_KMAX, i), vi);
-                                if (i > 0)
-                                    tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i - 1),
-                                                           from_double<L>(tz[i - 1 + j * BEIG_KMAX])));
-                                if (i + 1 < nt)
-                                    tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i),
-                                                           from_double<L>(tz[i + 1 + j * BEIG_KMAX])));
+                                if (i > 0) tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i - 1), from_double<L>(tx[i - 1])));
+                                if (i + 1 < nt) tv = add<L>(tv, mul<L>(lds_ld<L>(tbe, BEIG_KMAX, i), from_double<L>(tx[i + 1])));
                                 num = add<L>(num, mul<L>(vi, tv));
                                 den = add<L>(den, mul<L>(vi, vi));
                             }
                             lds_st<L>(tsc, 4, 0, mpx::div<L>(num, den));
-                            lds_st<L>(tsc, 4, 1, mpx::div<L>(from_double<L>(tz[k + j * BEIG_KMAX]), sqrtv<L>(den)));
+                            lds_st<L>(tsc, 4, 1, mpx::div<L>(from_double<L>(tx[k]), sqrtv<L>(den)));
                         }
                     }
                     __syncthreads();
