@@ -16,7 +16,8 @@
 
 namespace sdpg {
 
-constexpr int BMAXB = 4;  // SDP blocks per problem
+constexpr int BMAXB = 4;        // SDP blocks per problem
+constexpr int BATCH_MMAX = 4096;  // constraints per problem
 
 struct BatchProb {
     int32_t m, nsdp;
@@ -434,8 +435,9 @@ __global__ __launch_bounds__(256) void k_fx_point(const BatchProb *__restrict__ 
 // pivot is d_j / c_j and c_{j+1} = 2^-e_j c_j d_j.  After the chain, r_j = 1/sqrt(c_j d_j) gives
 // L_ic = At_ic r_c (so L_jj = d_j r_j) and 1/L_jj = c_j r_j.  info = the first non-positive pivot
 // (1-based, as Rpotrf2's info), and the problem's solve is skipped.
+constexpr int BPOTRF_THREADS = 1024;
 template <int L>
-__global__ __launch_bounds__(256) void k_batch_potrf(const BatchProb *__restrict__ probs, int nprob,
+__global__ __launch_bounds__(BPOTRF_THREADS) void k_batch_potrf(const BatchProb *__restrict__ probs, int nprob,
                                                      const int32_t *__restrict__ req, int32_t *__restrict__ info,
                                                      PV cs) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -468,20 +470,33 @@ __global__ __launch_bounds__(256) void k_batch_potrf(const BatchProb *__restrict
         }
         if (tid == 0) info[p] = fail;
         if (fail) continue;
-        // running scales (sequential, one product per pivot), then r_j and 1/L_jj in parallel
-        if (tid == 0) {
-            mp<L> c = from_int<L>(1);
-            for (int j = 0; j < m; j++) {
-                const mp<L> d = ldm<L>(B, at(j, j));
-                stm<L>(cs, P.cs_off + j, c);
-                c = mul<L>(c, d);
-                c.e -= d.e;
-            }
+        // running scales c_j = prod_{k<j} 2^-e_k d_k by a doubling scan over the two halves of the
+        // problem's scratch (f_k first, then partial products), then r_j and 1/L_jj in parallel
+        int cur = 0;
+        for (int j = tid; j < m; j += nt) {
+            mp<L> d = ldm<L>(B, at(j, j));
+            d.e = 0;
+            stm<L>(cs, P.cs_off + j, d);
         }
         __threadfence_block();
         __syncthreads();
-        for (int j = tid; j < m; j += nt) {
-            const mp<L> c = ldm<L>(cs, P.cs_off + j), d = ldm<L>(B, at(j, j));
+        for (int s = 1; s < m; s <<= 1) {
+            for (int j = tid; j < m; j += nt) {
+                mp<L> x = ldm<L>(cs, P.cs_off + (int64_t)cur * m + j);
+                if (j >= s) x = mul<L>(x, ldm<L>(cs, P.cs_off + (int64_t)cur * m + j - s));
+                stm<L>(cs, P.cs_off + (int64_t)(1 - cur) * m + j, x);
+            }
+            cur = 1 - cur;
+            __threadfence_block();
+            __syncthreads();
+        }
+        // c_j = (inclusive product up to j - 1); read every c_j before the scratch is overwritten
+        mp<L> cj[(BATCH_MMAX + BPOTRF_THREADS - 1) / BPOTRF_THREADS];
+        for (int j = tid, q = 0; j < m; j += nt, q++)
+            cj[q] = j == 0 ? from_int<L>(1) : ldm<L>(cs, P.cs_off + (int64_t)cur * m + j - 1);
+        __syncthreads();
+        for (int j = tid, q = 0; j < m; j += nt, q++) {
+            const mp<L> c = cj[q], d = ldm<L>(B, at(j, j));
             const mp<L> r = mpx::rsqrt_fx<L>(mul<L>(c, d));
             stm<L>(cs, P.cs_off + m + j, mul<L>(c, r));  // 1 / L_jj
             stm<L>(cs, P.cs_off + j, r);
@@ -758,71 +773,62 @@ __global__ __launch_bounds__(256) void k_bb_update(const BatchProb *__restrict__
     }
 }
 
-// Cholesky factor and its inverse of one block (Jal::getInvChol, sdpa_jordan.cpp:123-171): the factor by
-// the division-free elimination of the Schur kernel above in W (lower triangle, scales and reciprocal
-// diagonal in S), then inv(L) row by row: (inv L)_ij = -(sum_{k=j}^{i-1} L_ik (inv L)_kj) / L_ii.
+// Inverse Cholesky factor of one block (Jal::getInvChol, sdpa_jordan.cpp:123-171) by the division-free
+// elimination of the Schur kernel above applied to [A | I]: step j updates the trailing lower triangle
+// of At (in W) and, with the same pivot, the identity-started Xt (in Out): Xt_ic <- 2^-e_j (d_j Xt_ic -
+// At_ij Xt_jc) for c <= j < i, and Xt_ii <- 2^-e_j d_j Xt_ii, so that Xt_ii ends as the running scale c_i
+// and inv(L)_ic = Xt_ic / sqrt(c_i d_i).  The inverse costs no steps beyond the factorisation's n.
 // Returns false (uniformly) when a pivot is not positive.
 template <int L>
 __device__ bool wg_chol_inv(int n, PV Src, PV W, PV S, int64_t o, PV Out) {
     const int nn = n * n;
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {
         stm<L>(W, o + e, ldm<L>(Src, o + e));
-        stm<L>(Out, o + e, zero<L>());
+        stm<L>(Out, o + e, (e % n == e / n) ? from_int<L>(1) : zero<L>());
     }
     wg_sync();
     auto at = [&](int i, int j) -> int64_t { return o + i + (int64_t)n * j; };
     for (int j = 0; j < n; j++) {
         const mp<L> d = ldm<L>(W, at(j, j));
         if (sgn<L>(d) <= 0) return false;
-        const int n1 = n - 1 - j, cnt = n1 * (n1 + 1) / 2;
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-            int ip = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-            while (ip > 0 && ip * (ip + 1) / 2 > t) ip--;
-            while ((ip + 1) * (ip + 2) / 2 <= t) ip++;
-            const int i = j + 1 + ip, c = j + 1 + (t - ip * (ip + 1) / 2);
-            mp<L> v = fms<L>(d, ldm<L>(W, at(i, c)), ldm<L>(W, at(i, j)), ldm<L>(W, at(c, j)));
+        const int n1 = n - 1 - j, nA = n1 * (n1 + 1) / 2, nX = n1 * (j + 1), tot = nA + nX + n1;
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int i, c;
+            PV U;
+            mp<L> v;
+            if (t < nA) {
+                int ip = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+                while (ip > 0 && ip * (ip + 1) / 2 > t) ip--;
+                while ((ip + 1) * (ip + 2) / 2 <= t) ip++;
+                i = j + 1 + ip;
+                c = j + 1 + (t - ip * (ip + 1) / 2);
+                U = W;
+                v = fms<L>(d, ldm<L>(W, at(i, c)), ldm<L>(W, at(i, j)), ldm<L>(W, at(c, j)));
+            } else if (t < nA + nX) {
+                const int u = t - nA;
+                i = j + 1 + u / (j + 1);
+                c = u % (j + 1);
+                U = Out;
+                v = fms<L>(d, ldm<L>(Out, at(i, c)), ldm<L>(W, at(i, j)), ldm<L>(Out, at(j, c)));
+            } else {
+                i = c = j + 1 + (t - nA - nX);
+                U = Out;
+                v = mul<L>(d, ldm<L>(Out, at(i, i)));
+            }
             if (!is_zero<L>(v)) v.e -= d.e;
-            stm<L>(W, at(i, c), v);
+            stm<L>(U, at(i, c), v);
         }
         wg_sync();
     }
-    // scales c_j (S[o + j]), then r_j = 1/sqrt(c_j d_j) (S[o + j]) and 1/L_jj = c_j r_j (S[o + n + j])
-    if (threadIdx.x == 0) {
-        mp<L> c = from_int<L>(1);
-        for (int j = 0; j < n; j++) {
-            const mp<L> d = ldm<L>(W, at(j, j));
-            stm<L>(S, o + j, c);
-            c = mul<L>(c, d);
-            c.e -= d.e;
-        }
-    }
-    wg_sync();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const mp<L> c = ldm<L>(S, o + j), r = mpx::rsqrt_fx<L>(mul<L>(c, ldm<L>(W, at(j, j))));
-        stm<L>(S, o + n + j, mul<L>(c, r));
-        stm<L>(S, o + j, r);
-    }
+    // r_i = 1/sqrt(c_i d_i) with c_i = Xt_ii, then inv(L)_ic = Xt_ic r_i
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        stm<L>(S, o + i, mpx::rsqrt_fx<L>(mul<L>(ldm<L>(Out, at(i, i)), ldm<L>(W, at(i, i)))));
     wg_sync();
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {
         const int i = e % n, c = e / n;
-        if (i >= c) stm<L>(W, o + e, mul<L>(ldm<L>(W, o + e), ldm<L>(S, o + c)));
+        if (i >= c) stm<L>(Out, o + e, mul<L>(ldm<L>(Out, o + e), ldm<L>(S, o + i)));
     }
     wg_sync();
-    for (int i = 0; i < n; i++) {
-        const mp<L> rd = ldm<L>(S, o + n + i);
-        for (int j = threadIdx.x; j <= i; j += blockDim.x) {
-            mp<L> v;
-            if (j == i) {
-                v = rd;
-            } else {
-                mp<L> acc = zero<L>();
-                for (int k = j; k < i; k++) acc = add<L>(acc, mul<L>(ldm<L>(W, at(i, k)), ldm<L>(Out, at(k, j))));
-                v = neg<L>(mul<L>(acc, rd));
-            }
-            stm<L>(Out, at(i, j), v);
-        }
-        wg_sync();
-    }
     return true;
 }
 
@@ -1181,7 +1187,6 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
             mp<L> mn = from_double<L>(1.0e+51), mn_old = from_double<L>(1.0e+52), error = from_double<L>(1.0e+10);
             const mp<L> nD = from_int<L>(n);
             mp<L> beta = sqrtv<L>(nD);
-            const mp<L> sqrtn10 = add<L>(sqrtv<L>(nD), from_int<L>(10)), sqrtn9 = sqrtv<L>(from_int<L>(n + 9));
             const mp<L> c16 = from_double<L>(1.0e-16), c5 = from_double<L>(1.0e-5), c8 = from_double<L>(1.0e-8),
                         c2 = from_double<L>(1.0e-2), c4 = from_double<L>(1.0e-4);
             for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1193,8 +1198,8 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
             constexpr int MV_PARTS = 6;
             int k = 0, kk = 0;
             for (;;) {
-                const mp<L> kf = from_int<L>(k);
-                if (!(k < n) || !(cmp<L>(kf, sqrtn10) < 0) || !(cmp<L>(beta, c16) > 0) || k >= BEIG_KMAX) break;
+                // k < sqrt(n) + 10 exactly: k < 10 or (k - 10)^2 < n
+                if (!(k < n) || !(k < 10 || (k - 10) * (k - 10) < n) || !(cmp<L>(beta, c16) > 0) || k >= BEIG_KMAX) break;
                 const bool condA = cmp<L>(absv<L>(sub<L>(mn, mn_old)), add<L>(mul<L>(c5, absv<L>(mn)), c8)) > 0;
                 const bool condB = cmp<L>(absv<L>(mul<L>(error, beta)), add<L>(mul<L>(c2, absv<L>(mn)), c4)) > 0;
                 if (!(condA || condB)) break;
@@ -1229,7 +1234,8 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
                     lds_st<L>(vr, BEIG_NMAX, i, r);
                 }
                 __syncthreads();
-                if (cmp<L>(from_int<L>(kk), sqrtv<L>(from_int<L>(k))) >= 0 || k == n - 1 || cmp<L>(kf, sqrtn9) > 0) {
+                // kk >= sqrt(k) and k > sqrt(n + 9), exactly in integers
+                if (kk * kk >= k || k == n - 1 || k * k > n + 9) {
                     kk = 0;
                     // T = tridiag(alpha_0..alpha_k; beta_0..beta_{k-1})
                     if (threadIdx.x == 0) {
@@ -1397,6 +1403,10 @@ struct Batch : BatchBase {
                         return SDPG_EARG;
                     }
                 }
+            if (e->m > BATCH_MMAX) {
+                set_err("sdpg_batch_create: the batched form takes at most 4096 constraints per problem");
+                return SDPG_EARG;
+            }
             mmax = std::max(mmax, e->m);
             b.desc = e->d_desc;
             b.X = e->X.pv();
@@ -1533,7 +1543,7 @@ struct Batch : BatchBase {
         k_batch_schur<L><<<grid_schur, BS_THREADS, shm_schur, st>>>(d_probs, d_tasks, ntasks, d_req, nmax, mmax, fa_r,
                                                                     fa_re, fa_rs, fa_g, fa_ge, fa_gs, fx_v, fx_e, fx_s,
                                                                     fz_v, fz_e, fz_s);
-        k_batch_potrf<L><<<grid_prob, 256, 0, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n});
+        k_batch_potrf<L><<<grid_prob, BPOTRF_THREADS, 0, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n});
         k_batch_potrs<L><<<grid_prob, 256, shm_potrs, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n}, mmax);
     }
 
