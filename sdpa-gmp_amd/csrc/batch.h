@@ -114,7 +114,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
     const int q8 = tid & 7, slot = tid >> 3, nslot = BS_THREADS / 8;
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const int2 tk = tasks[t];
-        if (req[tk.x] != BATCH_PREDICTOR) continue;  // the whole workgroup skips together
+        if (tk.x < 0 || req[tk.x] != BATCH_PREDICTOR) continue;  // the whole workgroup skips together
         const BatchProb &P = probs[tk.x];
         const int i = tk.y, m = P.m, nsdp = P.nsdp;
         for (int j = tid; j <= i; j += nt) lds_st<L>(sB, mmax, j, zero<L>());
@@ -318,6 +318,7 @@ __global__ __launch_bounds__(256) void k_fx_constraints(const BatchProb *__restr
     __shared__ uint32_t ssum[256 * (L + 1)];
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const int2 tk = tasks[t];
+        if (tk.x < 0) continue;
         const BatchProb &P = probs[tk.x];
         const int i = tk.y;
         for (int l = 0; l < P.nsdp; l++) {
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(256) void k_bc_inner(const BatchProb *__restrict__ 
         bool on = false;
         if (t < ntasks) {
             tk = tasks[t];
-            on = req[tk.x] != 0;
+            on = tk.x >= 0 && req[tk.x] != 0;
         }
         if (on) {
             const BatchProb &P = probs[tk.x];
@@ -1454,15 +1455,38 @@ struct Batch : BatchBase {
             eng.push_back(e);
         }
         nprob = n;
-        // largest tasks first (the constraints of the problems with the largest blocks), dealt round-robin
-        std::stable_sort(tasks.begin(), tasks.end(), [&](const int2 &a, const int2 &b) {
-            auto cost = [&](const int2 &t) {
+        // XCD-aware order of the constraint tasks: the workgroups of a launch go round-robin to the 8 XCDs
+        // (workgroup id mod 8), so position t of the task list runs on XCD t mod 8.  Each problem goes
+        // whole to one XCD (largest problems first, each to the least loaded XCD), so that the
+        // constraint matrices its tasks stream for the inner products stay in that XCD's L2; within a
+        // problem the tasks run from the largest i down.  Shorter XCD lists are padded with empty tasks.
+        {
+            constexpr int NX = 8;
+            auto tcost = [&](int q, int i) {
                 int64_t c = 0;
-                for (int l = 0; l < hp[t.x].nsdp; l++) c += (int64_t)hp[t.x].n[l] * hp[t.x].n[l] * (hp[t.x].n[l] + t.y);
+                for (int l = 0; l < hp[q].nsdp; l++) c += (int64_t)hp[q].n[l] * hp[q].n[l] * (2 * hp[q].n[l] + i);
                 return c;
             };
-            return cost(a) > cost(b);
-        });
+            std::vector<int> order(n);
+            std::vector<int64_t> pcost(n, 0);
+            for (int q = 0; q < n; q++) {
+                order[q] = q;
+                for (int i = 0; i < hp[q].m; i++) pcost[q] += tcost(q, i);
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pcost[a] > pcost[b]; });
+            std::vector<int64_t> load(NX, 0);
+            std::vector<std::vector<int2>> xt(NX);
+            for (int q : order) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[x] += pcost[q];
+                for (int i = hp[q].m - 1; i >= 0; i--) xt[x].push_back(int2{q, i});
+            }
+            size_t len = 0;
+            for (auto &v : xt) len = std::max(len, v.size());
+            tasks.assign(len * NX, int2{-1, 0});
+            for (int x = 0; x < NX; x++)
+                for (size_t k = 0; k < xt[x].size(); k++) tasks[k * NX + x] = xt[x][k];
+        }
         ntasks = (int)tasks.size();
         size_t lds_max = 65536;
         {
@@ -1480,7 +1504,8 @@ struct Batch : BatchBase {
         if (hipSetDevice(device) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
             return SDPG_EHIP;
-        grid_schur = std::max(1, std::min(ntasks, cus * std::max(1, (int)(lds_max / shm_schur))));
+        // a multiple of 8 workgroups, so that workgroup w and its tasks w + k grid stay on XCD w mod 8
+        grid_schur = std::max(8, (std::min(ntasks, cus * std::max(1, (int)(lds_max / shm_schur))) / 8) * 8);
         grid_prob = std::max(1, std::min(nprob, 4 * cus));
         cs_n = std::max<int64_t>(csn, 1);
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
