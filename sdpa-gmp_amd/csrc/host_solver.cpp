@@ -1178,8 +1178,11 @@ static int bench_L(const RawProblem &R, const Parameter &prm, int gemm_mode, int
 // The meeting point of the lockstep solves: every problem's thread posts its next device call and waits;
 // the last of the running problems to arrive issues one batched call per kind of call posted
 // (sdpg_batch_*) for all of them and wakes the others.  A problem whose loop has ended leaves, which may
-// complete a meeting.  The ten scalar inner products come from one batched call and are kept per problem
-// until its next step or update, so that the control's consecutive inner products cost one meeting.
+// complete a meeting.  The loop's calls come in fixed runs, so a meeting does a whole run and keeps the
+// results the problem asks for next: a step (Make_gVec, Newton, dZ, dX) is followed by the inner
+// products and, for a corrector, by the step-length eigenvalues; an update by the inverse factors, the
+// inner products and the residuals.  Each problem's cached results stay valid until it posts its next
+// step or update, so an iteration costs three meetings.
 struct Meeting {
     enum { K_STEP, K_INNER, K_EIG, K_UPDATE, K_FACTOR, K_RESID };
     struct Post {
@@ -1190,6 +1193,11 @@ struct Meeting {
         uint32_t *o1 = nullptr, *o2 = nullptr;
         int *info = nullptr;
     };
+    // per problem: results of its last meeting that it has not asked for yet
+    struct Cache {
+        bool ip = false, eig = false, fac = false, res = false;
+        int info = 0;
+    };
     std::mutex mu;
     std::condition_variable cv;
     sdpg_batch *batch = nullptr;
@@ -1197,16 +1205,19 @@ struct Meeting {
     int active, arrived = 0, rc = 0;
     uint64_t gen = 0;
     std::vector<Post> post;
-    std::vector<char> posted, ipv;
+    std::vector<Cache> cache;
+    std::vector<char> posted;
     std::vector<uint32_t> ipc;  // per problem, the ten inner products (SDPG_IP_XZ .. SDPG_IP_TRZ)
+    std::vector<uint32_t> eigc;  // per problem, the two step-length eigenvalues
+    std::vector<uint32_t> resc;  // per problem, max |Rp|, max |Rd|, C.X, b.y
     std::vector<int> req, req2, phase, info;
-    std::vector<uint32_t> sa, sb, out, out2;
+    std::vector<uint32_t> sa, sb, out;
     long long meetings = 0, served = 0;
 
     Meeting(sdpg_batch *b, int n_, int W_)
-        : batch(b), n(n_), W(W_), active(n_), post(n_), posted(n_, 0), ipv(n_, 0), ipc((size_t)n_ * 10 * W_),
-          req(n_), req2(n_), phase(n_), info(n_), sa((size_t)n_ * W_), sb((size_t)n_ * W_),
-          out((size_t)n_ * 10 * W_), out2((size_t)n_ * W_) {}
+        : batch(b), n(n_), W(W_), active(n_), post(n_), cache(n_), posted(n_, 0), ipc((size_t)n_ * 10 * W_),
+          eigc((size_t)n_ * 2 * W_), resc((size_t)n_ * 4 * W_), req(n_), req2(n_), phase(n_), info(n_),
+          sa((size_t)n_ * W_), sb((size_t)n_ * W_), out((size_t)n_ * 10 * W_) {}
 
     int meet(int p, const Post &q) {
         std::unique_lock<std::mutex> lk(mu);
@@ -1236,6 +1247,46 @@ struct Meeting {
     void note(int r) {
         if (r && !rc) rc = r;
     }
+    bool any(const std::vector<int> &r) const {
+        for (int v : r)
+            if (v) return true;
+        return false;
+    }
+    void fetch_inner(const std::vector<int> &r) {
+        note(sdpg_batch_inner(batch, r.data(), out.data()));
+        for (int p = 0; p < n; p++)
+            if (r[p]) {
+                memcpy(&ipc[(size_t)p * 10 * W], &out[(size_t)p * 10 * W], sizeof(uint32_t) * 10 * W);
+                cache[p].ip = true;
+            }
+    }
+    void fetch_eig(const std::vector<int> &r) {
+        note(sdpg_batch_min_eigen(batch, r.data(), 0, out.data()));
+        for (int p = 0; p < n; p++)
+            if (r[p]) memcpy(&eigc[(size_t)p * 2 * W], &out[(size_t)p * W], sizeof(uint32_t) * W);
+        note(sdpg_batch_min_eigen(batch, r.data(), 1, out.data()));
+        for (int p = 0; p < n; p++)
+            if (r[p]) {
+                memcpy(&eigc[((size_t)p * 2 + 1) * W], &out[(size_t)p * W], sizeof(uint32_t) * W);
+                cache[p].eig = true;
+            }
+    }
+    void fetch_factor(const std::vector<int> &r) {
+        note(sdpg_batch_factor_point(batch, r.data(), info.data()));
+        for (int p = 0; p < n; p++)
+            if (r[p]) {
+                cache[p].info = info[p];
+                cache[p].fac = true;
+            }
+    }
+    void fetch_resid(const std::vector<int> &r) {
+        note(sdpg_batch_residuals(batch, r.data(), out.data()));
+        for (int p = 0; p < n; p++)
+            if (r[p]) {
+                memcpy(&resc[(size_t)p * 4 * W], &out[(size_t)p * 4 * W], sizeof(uint32_t) * 4 * W);
+                cache[p].res = true;
+            }
+    }
     void fire() {  // with mu held
         if (select(K_STEP)) {
             for (int p = 0; p < n; p++) {
@@ -1253,24 +1304,9 @@ struct Meeting {
             }
             note(sdpg_batch_dz(batch, req2.data(), phase.data()));
             note(sdpg_batch_dx(batch, req2.data()));
-        }
-        if (select(K_INNER)) {
-            note(sdpg_batch_inner(batch, req.data(), out.data()));
-            for (int p = 0; p < n; p++) {
-                if (!req[p]) continue;
-                memcpy(&ipc[(size_t)p * 10 * W], &out[(size_t)p * 10 * W], sizeof(uint32_t) * 10 * W);
-                ipv[p] = 1;
-                memcpy(post[p].o1, &ipc[((size_t)p * 10 + post[p].what - 1) * W], sizeof(uint32_t) * W);
-            }
-        }
-        if (select(K_EIG)) {
-            note(sdpg_batch_min_eigen(batch, req.data(), 0, out.data()));
-            note(sdpg_batch_min_eigen(batch, req.data(), 1, out2.data()));
-            for (int p = 0; p < n; p++) {
-                if (!req[p]) continue;
-                memcpy(post[p].o1, &out[(size_t)p * W], sizeof(uint32_t) * W);
-                memcpy(post[p].o2, &out2[(size_t)p * W], sizeof(uint32_t) * W);
-            }
+            fetch_inner(req2);  // the step-length and corrector-parameter inner products come next
+            for (int p = 0; p < n; p++) req2[p] = req2[p] && req[p] == SDPG_BATCH_CORRECTOR;
+            if (any(req2)) fetch_eig(req2);  // after a corrector, the step-length eigenvalues
         }
         if (select(K_UPDATE)) {
             for (int p = 0; p < n; p++) {
@@ -1279,19 +1315,41 @@ struct Meeting {
                 memcpy(&sb[(size_t)p * W], post[p].b, sizeof(uint32_t) * W);
             }
             note(sdpg_batch_update(batch, req.data(), sa.data(), sb.data()));
+            // then the inverse factors, mu (X.Z), the residuals and SolveInfo's inner products
+            fetch_factor(req);
+            fetch_inner(req);
+            fetch_resid(req);
+        }
+        if (select(K_INNER)) {
+            fetch_inner(req);
+            for (int p = 0; p < n; p++)
+                if (req[p]) memcpy(post[p].o1, &ipc[((size_t)p * 10 + post[p].what - 1) * W], sizeof(uint32_t) * W);
+        }
+        if (select(K_EIG)) {
+            fetch_eig(req);
+            for (int p = 0; p < n; p++)
+                if (req[p]) {
+                    memcpy(post[p].o1, &eigc[(size_t)p * 2 * W], sizeof(uint32_t) * W);
+                    memcpy(post[p].o2, &eigc[((size_t)p * 2 + 1) * W], sizeof(uint32_t) * W);
+                    cache[p].eig = false;
+                }
         }
         if (select(K_FACTOR)) {
-            note(sdpg_batch_factor_point(batch, req.data(), info.data()));
+            fetch_factor(req);
             for (int p = 0; p < n; p++)
-                if (req[p]) *post[p].info = info[p];
+                if (req[p]) {
+                    *post[p].info = cache[p].info;
+                    cache[p].fac = false;
+                }
         }
         if (select(K_RESID)) {
-            note(sdpg_batch_residuals(batch, req.data(), out.data()));
-            for (int p = 0; p < n; p++) {
-                if (!req[p]) continue;
-                memcpy(post[p].o1, &out[((size_t)p * 4 + 0) * W], sizeof(uint32_t) * W);
-                memcpy(post[p].o2, &out[((size_t)p * 4 + 1) * W], sizeof(uint32_t) * W);
-            }
+            fetch_resid(req);
+            for (int p = 0; p < n; p++)
+                if (req[p]) {
+                    memcpy(post[p].o1, &resc[(size_t)p * 4 * W], sizeof(uint32_t) * W);
+                    memcpy(post[p].o2, &resc[((size_t)p * 4 + 1) * W], sizeof(uint32_t) * W);
+                    cache[p].res = false;
+                }
         }
         meetings++;
         served += arrived;
@@ -1302,13 +1360,15 @@ struct Meeting {
     }
 };
 
-// one problem's side of the meetings
+// one problem's side of the meetings (its cache entries are touched by its own thread, or by the thread
+// running a meeting while this one waits in it)
 struct Port : Exec {
     Meeting *mt;
     int p;
     Port(Meeting *m, int p_) : mt(m), p(p_) {}
+    void invalidate() { mt->cache[p] = Meeting::Cache(); }
     int step(sdpg_ctx *, bool predictor, const uint32_t *mu, const uint32_t *beta, int phase, int *info) override {
-        mt->ipv[p] = 0;
+        invalidate();
         Meeting::Post q;
         q.kind = Meeting::K_STEP;
         q.pred = predictor;
@@ -1320,7 +1380,7 @@ struct Port : Exec {
     }
     int inner(sdpg_ctx *, int what, uint32_t *out) override {
         if (what < SDPG_IP_XZ || what > SDPG_IP_TRZ) return SDPG_EARG;  // no initial-point products here
-        if (mt->ipv[p]) {
+        if (mt->cache[p].ip) {
             memcpy(out, &mt->ipc[((size_t)p * 10 + what - 1) * mt->W], sizeof(uint32_t) * mt->W);
             return 0;
         }
@@ -1331,6 +1391,12 @@ struct Port : Exec {
         return mt->meet(p, q);
     }
     int min_eigen2(sdpg_ctx *, uint32_t *x, uint32_t *z) override {
+        if (mt->cache[p].eig) {
+            memcpy(x, &mt->eigc[(size_t)p * 2 * mt->W], sizeof(uint32_t) * mt->W);
+            memcpy(z, &mt->eigc[((size_t)p * 2 + 1) * mt->W], sizeof(uint32_t) * mt->W);
+            mt->cache[p].eig = false;
+            return 0;
+        }
         Meeting::Post q;
         q.kind = Meeting::K_EIG;
         q.o1 = x;
@@ -1338,7 +1404,7 @@ struct Port : Exec {
         return mt->meet(p, q);
     }
     int update(sdpg_ctx *, const uint32_t *aP, const uint32_t *aD) override {
-        mt->ipv[p] = 0;
+        invalidate();
         Meeting::Post q;
         q.kind = Meeting::K_UPDATE;
         q.a = aP;
@@ -1346,12 +1412,23 @@ struct Port : Exec {
         return mt->meet(p, q);
     }
     int factor_point(sdpg_ctx *, int *info) override {
+        if (mt->cache[p].fac) {
+            *info = mt->cache[p].info;
+            mt->cache[p].fac = false;
+            return 0;
+        }
         Meeting::Post q;
         q.kind = Meeting::K_FACTOR;
         q.info = info;
         return mt->meet(p, q);
     }
     int residuals(sdpg_ctx *, uint32_t *nrp, uint32_t *nrd) override {
+        if (mt->cache[p].res) {
+            memcpy(nrp, &mt->resc[(size_t)p * 4 * mt->W], sizeof(uint32_t) * mt->W);
+            memcpy(nrd, &mt->resc[((size_t)p * 4 + 1) * mt->W], sizeof(uint32_t) * mt->W);
+            mt->cache[p].res = false;
+            return 0;
+        }
         Meeting::Post q;
         q.kind = Meeting::K_RESID;
         q.o1 = nrp;
