@@ -55,9 +55,11 @@ enum { BATCH_NONE = 0, BATCH_PREDICTOR = 1, BATCH_CORRECTOR = 2 };
 // fixed by rows (work1) and by matrix (inner products) once for the batch, X by rows and inv(Z) by
 // columns per predictor (k_fx_point), work1 by columns and work2 by matrix in LDS.  Tasks are dealt
 // round-robin to a grid that fills the device.
-// LDS: R1 holds work1 or work2 as limb floats (planes of nmax^2), then the partial column sums of the
-// inner products; R2 the fixed-point work1 or work2 (planes of nmax^2); then the column exponents and
-// offset sums of work1, the exponent and offset sum of work2, and the row of B.
+// Memory: work1 and work2 as limb floats in a global scratch slice of the workgroup (ws, nmax^2 values per
+// workgroup, L2-resident); in LDS the fixed-point work1 or work2 (planes of nmax^2), the column
+// exponents and offset sums of work1, the exponent and offset sum of work2 and the row of B (71 KB at
+// n = 40, m = 120, L = 10: two workgroups of 512 per CU).  The 8 lanes of a partner add their column
+// sums by cross-lane shuffles.
 constexpr int BS_THREADS = 512;
 constexpr int FX_HEAD = 2;
 
@@ -87,6 +89,26 @@ __device__ __forceinline__ void fx_sum_add2(uint32_t (&s)[L + 1], const uint32_t
         c = (uint32_t)(t >> 32);
     }
 }
+// sum over k = k0, k0 + step, ... < K of the limb products of the operands ld(k, a, b) delivers, into
+// the column accumulators: the operands of the next term are requested before the current term's
+// multiply-adds (two register sets, no copies), so a load's latency overlaps a term's arithmetic
+template <int L, class LD>
+__device__ __forceinline__ void fx_mac_pipelined(int K, int step, int k0, LD &&ld,
+                                                 uint64_t (&lo)[FxCfg<L>::NC], uint32_t (&hi)[FxCfg<L>::NC]) {
+    constexpr int S0 = FxCfg<L>::S0;
+    if (k0 >= K) return;
+    uint32_t a0[L], b0[L], a1[L], b1[L];
+    ld(k0, a0, b0);
+    int k = k0;
+    for (; k + step < K; k += 2 * step) {
+        ld(k + step, a1, b1);
+        mpx::col_products<L, S0>(a0, b0, lo, hi);
+        if (k + 2 * step < K) ld(k + 2 * step, a0, b0);
+        mpx::col_products<L, S0>(a1, b1, lo, hi);
+    }
+    if (k < K) mpx::col_products<L, S0>(a0, b0, lo, hi);
+}
+
 // exponent of a block's product: the sum of the operand exponents, 0 when either side is all zero
 __device__ __forceinline__ int32_t fx_exp(int32_t ea, int32_t eb, int W) {
     const int32_t e = ea + eb;
@@ -102,14 +124,16 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                                                             const int32_t *__restrict__ fa_ge, const uint32_t *__restrict__ fa_gs,
                                                             const uint32_t *__restrict__ fx_v, const int32_t *__restrict__ fx_e,
                                                             const uint32_t *__restrict__ fx_s, const uint32_t *__restrict__ fz_v,
-                                                            const int32_t *__restrict__ fz_e, const uint32_t *__restrict__ fz_s) {
+                                                            const int32_t *__restrict__ fz_e, const uint32_t *__restrict__ fz_s,
+                                                            PV ws) {
     using C = FxCfg<L>;
-    constexpr int NC = C::NC, S0 = C::S0, W = C::W;
+    constexpr int NC = C::NC, W = C::W;
     extern __shared__ uint32_t shm[];
     const int PW = nmax * nmax;
-    const int R1W = max(PW * (L + 1), BS_THREADS * 3 * NC);
-    uint32_t *R1 = shm, *R2 = shm + R1W, *sFs = R2 + PW * L, *sGs = sFs + nmax * (L + 1), *sB = sGs + (L + 1);
+    uint32_t *R2 = shm, *sFs = R2 + PW * L, *sGs = sFs + nmax * (L + 1), *sRed = sGs + (L + 1),
+             *sB = sRed + 64 * (L + 1);
     int32_t *sFe = (int32_t *)(sB + mmax * (L + 1)), *sGe = sFe + nmax;
+    const int64_t wb = (int64_t)blockIdx.x * PW;  // this workgroup's work1 / work2 limb floats in ws
     const int tid = threadIdx.x, nt = BS_THREADS;
     const int q8 = tid & 7, slot = tid >> 3, nslot = BS_THREADS / 8;
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
@@ -132,24 +156,23 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 uint32_t hi[NC];
 #pragma unroll
                 for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
-                for (int k = 0; k < n; k++) {
-                    uint32_t a[L], b[L];
+                auto ld = [&](int k, uint32_t (&a)[L], uint32_t (&b)[L]) {
                     const uint32_t *pa = fa_r + (ael + r + (int64_t)n * k) * L, *pb = fz_v + (pel + k + (int64_t)n * c) * L;
 #pragma unroll
                     for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = pb[q];
-                    mpx::col_products<L, S0>(a, b, lo, hi);
-                }
+                };
+                fx_mac_pipelined<L>(n, 1, 0, ld, lo, hi);
                 uint32_t ra[L + 1], rb[L + 1];
 #pragma unroll
                 for (int q = 0; q <= L; q++) ra[q] = fa_rs[(arw + r) * (L + 1) + q], rb[q] = fz_s[(prw + c) * (L + 1) + q];
-                lds_st<L>(R1, PW, o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fa_re[arw + r], fz_e[prw + c], W)));
+                stm<L>(ws, wb + o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fa_re[arw + r], fz_e[prw + c], W)));
             }
             __syncthreads();
             // work1 by columns: exponents, fixed point, offset sums
             for (int c = tid; c < n; c += nt) {
                 int32_t e = EZERO;
                 for (int r = 0; r < n; r++) {
-                    const mp<L> v = lds_ld<L>(R1, PW, r + n * c);
+                    const mp<L> v = ldm<L>(ws, wb + r + n * c);
                     if (!is_zero<L>(v) && v.e > e) e = v.e;
                 }
                 sFe[c] = e;
@@ -157,7 +180,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
             __syncthreads();
             for (int o = tid; o < nn; o += nt) {
                 uint32_t a[L];
-                fx_convert<L>(lds_ld<L>(R1, PW, o), sFe[o / n], FX_HEAD, a);
+                fx_convert<L>(ldm<L>(ws, wb + o), sFe[o / n], FX_HEAD, a);
 #pragma unroll
                 for (int q = 0; q < L; q++) R2[q * PW + o] = a[q];
             }
@@ -183,24 +206,23 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 uint32_t hi[NC];
 #pragma unroll
                 for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
-                for (int k = 0; k < n; k++) {
-                    uint32_t a[L], b[L];
+                auto ld = [&](int k, uint32_t (&a)[L], uint32_t (&b)[L]) {
                     const uint32_t *pa = fx_v + (pel + r + (int64_t)n * k) * L;
 #pragma unroll
                     for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = R2[q * PW + k + n * c];
-                    mpx::col_products<L, S0>(a, b, lo, hi);
-                }
+                };
+                fx_mac_pipelined<L>(n, 1, 0, ld, lo, hi);
                 uint32_t ra[L + 1], rb[L + 1];
 #pragma unroll
                 for (int q = 0; q <= L; q++) ra[q] = fx_s[(prw + r) * (L + 1) + q], rb[q] = sFs[c * (L + 1) + q];
-                lds_st<L>(R1, PW, o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fx_e[prw + r], sFe[c], W)));
+                stm<L>(ws, wb + o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fx_e[prw + r], sFe[c], W)));
             }
             __syncthreads();
             // work2 by matrix: exponent, fixed point, offset sum
             if (tid < 64) {
                 int32_t e = EZERO;
                 for (int o = tid; o < nn; o += 64) {
-                    const mp<L> v = lds_ld<L>(R1, PW, o);
+                    const mp<L> v = ldm<L>(ws, wb + o);
                     if (!is_zero<L>(v) && v.e > e) e = v.e;
                 }
                 for (int h = 32; h >= 1; h >>= 1) e = max(e, __shfl_xor(e, h, 64));
@@ -210,7 +232,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
             const int32_t eg = sGe[0];
             for (int o = tid; o < nn; o += nt) {
                 uint32_t a[L];
-                fx_convert<L>(lds_ld<L>(R1, PW, o), eg, FX_HEAD, a);
+                fx_convert<L>(ldm<L>(ws, wb + o), eg, FX_HEAD, a);
 #pragma unroll
                 for (int q = 0; q < L; q++) R2[q * PW + o] = a[q];
             }
@@ -225,16 +247,16 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                     for (int q = 0; q < L; q++) a[q] = R2[q * PW + o];
                     fx_sum_add<L>(sm, a);
                 }
-                // lanes 0..63 -> R1 rows, then lane 0 adds them up (R1 is free until the inner products)
+                // lanes 0..63 -> sRed rows, then lane 0 adds them up
 #pragma unroll
-                for (int q = 0; q <= L; q++) R1[tid * (L + 1) + q] = sm[q];
+                for (int q = 0; q <= L; q++) sRed[tid * (L + 1) + q] = sm[q];
             }
             __syncthreads();
             if (tid == 0) {
                 uint32_t sm[L + 1];
 #pragma unroll
-                for (int q = 0; q <= L; q++) sm[q] = R1[q];
-                for (int u = 1; u < 64; u++) fx_sum_add2<L>(sm, R1 + u * (L + 1));
+                for (int q = 0; q <= L; q++) sm[q] = sRed[q];
+                for (int u = 1; u < 64; u++) fx_sum_add2<L>(sm, sRed + u * (L + 1));
 #pragma unroll
                 for (int q = 0; q <= L; q++) sGs[q] = sm[q];
             }
@@ -252,33 +274,28 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 for (int z = 0; z < NC; z++) lo[z] = 0, hi[z] = 0;
                 if (on) {
                     const int64_t jel = P.fa_el + (int64_t)j * P.Nd + P.blkoff[l];
-                    for (int e = q8; e < nn; e += 8) {
-                        uint32_t a[L], b[L];
+                    auto ld = [&](int e, uint32_t (&a)[L], uint32_t (&b)[L]) {
                         const uint32_t *pa = fa_g + (jel + e) * L;
 #pragma unroll
                         for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = R2[q * PW + e];
-                        mpx::col_products<L, S0>(a, b, lo, hi);
+                    };
+                    fx_mac_pipelined<L>(nn, 8, q8, ld, lo, hi);
+                }
+                // the 8 lanes of a partner (consecutive lanes of one wave) add their column sums
+#pragma unroll
+                for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+                    for (int z = 0; z < NC; z++) {
+                        const uint32_t o0 = __shfl_xor((uint32_t)lo[z], h, 64);
+                        const uint32_t o1 = __shfl_xor((uint32_t)(lo[z] >> 32), h, 64);
+                        const uint32_t oh = __shfl_xor(hi[z], h, 64);
+                        const uint64_t v = (uint64_t)o0 | ((uint64_t)o1 << 32);
+                        const uint64_t sm = lo[z] + v;
+                        hi[z] += oh + (sm < v ? 1u : 0u);
+                        lo[z] = sm;
                     }
                 }
-                uint32_t *me = R1 + tid * 3 * NC;
-#pragma unroll
-                for (int z = 0; z < NC; z++) {
-                    me[3 * z] = (uint32_t)lo[z];
-                    me[3 * z + 1] = (uint32_t)(lo[z] >> 32);
-                    me[3 * z + 2] = hi[z];
-                }
-                __syncthreads();
                 if (q8 == 0 && on) {
-                    for (int u = 1; u < 8; u++) {
-                        const uint32_t *o = R1 + (tid + u) * 3 * NC;
-#pragma unroll
-                        for (int z = 0; z < NC; z++) {
-                            const uint64_t v = (uint64_t)o[3 * z] | ((uint64_t)o[3 * z + 1] << 32);
-                            const uint64_t sm = lo[z] + v;
-                            hi[z] += o[3 * z + 2] + (sm < v ? 1u : 0u);
-                            lo[z] = sm;
-                        }
-                    }
                     const int64_t jm = P.fa_mat + (int64_t)j * nsdp + l;
                     uint32_t ra[L + 1];
 #pragma unroll
@@ -286,7 +303,6 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                     const mp<L> v = fx_finish<L>(lo, hi, nn, ra, gsum, fx_exp(fa_ge[jm], eg, W));
                     lds_st<L>(sB, mmax, j, add<L>(lds_ld<L>(sB, mmax, j), v));
                 }
-                __syncthreads();
             }
         }
         __syncthreads();
@@ -303,8 +319,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
 template <int L>
 inline size_t schur_lds_words(int nmax, int mmax) {
     const size_t PW = (size_t)nmax * nmax;
-    const size_t R1W = std::max(PW * (L + 1), (size_t)BS_THREADS * 3 * FxCfg<L>::NC);
-    return R1W + PW * L + (size_t)nmax * (L + 1) + (L + 1) + (size_t)mmax * (L + 1) + nmax + 1;
+    return PW * L + (size_t)nmax * (L + 1) + (L + 1) + 64 * (L + 1) + (size_t)mmax * (L + 1) + nmax + 1;
 }
 
 // the constraint matrices in block fixed point, once per batch: per task (p, i) and block, by rows
@@ -1345,6 +1360,8 @@ struct Batch : BatchBase {
     uint32_t *fa_r = nullptr, *fa_rs = nullptr, *fa_g = nullptr, *fa_gs = nullptr, *fx_v = nullptr, *fx_s = nullptr,
              *fz_v = nullptr, *fz_s = nullptr;
     int32_t *fa_re = nullptr, *fa_ge = nullptr, *fx_e = nullptr, *fz_e = nullptr;
+    uint32_t *ws_schur = nullptr;  // per workgroup of k_batch_schur: work1 / work2 as limb floats
+    int64_t ws_n = 0;
 
     ~Batch() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -1359,6 +1376,7 @@ struct Batch : BatchBase {
         (void)hipFree(d_binfo);
         (void)hipFree(d_sc);
         (void)hipFree(d_bo);
+        (void)hipFree(ws_schur);
         for (void *q : {(void *)fa_r, (void *)fa_rs, (void *)fa_g, (void *)fa_gs, (void *)fx_v, (void *)fx_s,
                         (void *)fz_v, (void *)fz_s, (void *)fa_re, (void *)fa_ge, (void *)fx_e, (void *)fz_e})
             (void)hipFree(q);
@@ -1544,7 +1562,8 @@ struct Batch : BatchBase {
             !dm((void **)&fa_ge, 4 * (size_t)fa_mat_n) || !dm((void **)&fa_gs, 4 * (size_t)(L + 1) * fa_mat_n) ||
             !dm((void **)&fx_v, 4 * (size_t)L * fp_el_n) || !dm((void **)&fx_e, 4 * (size_t)fp_row_n) ||
             !dm((void **)&fx_s, 4 * (size_t)(L + 1) * fp_row_n) || !dm((void **)&fz_v, 4 * (size_t)L * fp_el_n) ||
-            !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n)) {
+            !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n) ||
+            !dm((void **)&ws_schur, 4 * (size_t)(L + 1) * (ws_n = (int64_t)grid_schur * nmax * nmax))) {
             set_err("sdpg_batch_create: out of device memory");
             return SDPG_ENOMEM;
         }
@@ -1567,7 +1586,7 @@ struct Batch : BatchBase {
                                                 fz_s);
         k_batch_schur<L><<<grid_schur, BS_THREADS, shm_schur, st>>>(d_probs, d_tasks, ntasks, d_req, nmax, mmax, fa_r,
                                                                     fa_re, fa_rs, fa_g, fa_ge, fa_gs, fx_v, fx_e, fx_s,
-                                                                    fz_v, fz_e, fz_s);
+                                                                    fz_v, fz_e, fz_s, PV{ws_schur, ws_n});
         k_batch_potrf<L><<<grid_prob, BPOTRF_THREADS, 0, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n});
         k_batch_potrs<L><<<grid_prob, 256, shm_potrs, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n}, mmax);
     }
