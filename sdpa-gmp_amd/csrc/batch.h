@@ -18,6 +18,7 @@ namespace sdpg {
 
 constexpr int BMAXB = 4;        // SDP blocks per problem
 constexpr int BATCH_MMAX = 4096;  // constraints per problem
+constexpr int BATCH_MMAX_LDS = 256;  // constraints per problem held in LDS by the fixed-point combinations
 
 struct BatchProb {
     int32_t m, nsdp;
@@ -33,7 +34,7 @@ struct BatchProb {
     // matrix offsets of constraint 0; constraint i adds i Nd, i nsum, i nsdp), X by rows and inv(Z) by
     // columns (element and row offsets)
     int64_t fa_el, fa_row, fa_mat, fp_el, fp_row;
-    int32_t nsum, Nd;
+    int32_t nsum, Nd, blk0;  // blk0: index of the problem's block 0 in the block task list
     int32_t rowoff[BMAXB];  // first row of block l among the problem's rows (sum of the sizes before it)
 };
 
@@ -41,6 +42,12 @@ struct BatchProb {
 struct BatchBlk {
     int32_t p, l;
 };
+
+// barrier after global writes that other threads of the workgroup read next
+__device__ __forceinline__ void wg_sync() {
+    __threadfence_block();
+    __syncthreads();
+}
 
 // request flags: 0 nothing, 1 PREDICTOR (Schur build, Cholesky, solve), 2 CORRECTOR (solve only, with the
 // factor of the last predictor; Newton::Mehrotra, sdpa_newton.cpp:1375-1414)
@@ -113,6 +120,111 @@ __device__ __forceinline__ void fx_mac_pipelined(int K, int step, int k0, LD &&l
 __device__ __forceinline__ int32_t fx_exp(int32_t ea, int32_t eb, int W) {
     const int32_t e = ea + eb;
     return (e < -(1 << 28) ? 0 : e) - 2 * W;
+}
+
+// the batch's block fixed-point arrays that the phases other than the Schur build read
+struct FxArrays {
+    const uint32_t *fa_g;   // constraint matrices by matrix (L words per element)
+    const int32_t *fa_ge;   // their exponents (per constraint and block)
+    const uint32_t *fa_gs;  // their offset sums (L + 1 words)
+    const uint32_t *fa_cs;  // per element of a problem's blocks: offset sum over the constraints (L + 1)
+    uint32_t *fd_v;         // a dense space (X or DLS) by block: fixed values
+    int32_t *fd_e;          // its exponents (per block task)
+    uint32_t *fd_s;         // its offset sums (per block task, L + 1)
+};
+
+// the 64 lanes of a wave add their 96-bit column sums (every lane ends with the total)
+template <int L>
+__device__ __forceinline__ void fx_wave_sum(uint64_t (&lo)[FxCfg<L>::NC], uint32_t (&hi)[FxCfg<L>::NC]) {
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+#pragma unroll
+        for (int z = 0; z < FxCfg<L>::NC; z++) {
+            const uint32_t o0 = __shfl_xor((uint32_t)lo[z], h, 64);
+            const uint32_t o1 = __shfl_xor((uint32_t)(lo[z] >> 32), h, 64);
+            const uint32_t oh = __shfl_xor(hi[z], h, 64);
+            const uint64_t v = (uint64_t)o0 | ((uint64_t)o1 << 32);
+            const uint64_t sm = lo[z] + v;
+            hi[z] += oh + (sm < v ? 1u : 0u);
+            lo[z] = sm;
+        }
+    }
+}
+
+// sum_i s_i A_i over the m constraints for block l of problem P, exactly in block fixed point: w_i =
+// s_i 2^(exponent of A_i's block) goes to fixed point under one exponent (LDS sw: m x L words, planar),
+// each element sums w_i a_i[e] over the constraints in 96-bit columns and is rounded once; f(e, value)
+// receives the elements.  sred: L + 1 planes of blockDim.x words; se: blockDim.x ints.
+template <int L, class F>
+__device__ void wg_fx_combo(const BatchProb &P, int l, PV sv, const FxArrays &fx, uint32_t *sw, uint32_t *sred,
+                            int32_t *se, F &&f) {
+    using C = FxCfg<L>;
+    const int tid = threadIdx.x, nt = blockDim.x, m = P.m, nsdp = P.nsdp, n = P.n[l], nn = n * n;
+    auto wv = [&](int i) -> mp<L> {
+        mp<L> x = ldm<L>(sv, i);
+        const int32_t ge = fx.fa_ge[P.fa_mat + (int64_t)i * nsdp + l];
+        if (is_zero<L>(x) || ge == EZERO) return zero<L>();
+        x.e += ge;
+        return x;
+    };
+    int32_t e = EZERO;
+    for (int i = tid; i < m; i += nt) {
+        const mp<L> x = wv(i);
+        if (!is_zero<L>(x) && x.e > e) e = x.e;
+    }
+    se[tid] = e;
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (tid < h) se[tid] = max(se[tid], se[tid + h]);
+        __syncthreads();
+    }
+    const int32_t ew = se[0];
+    uint32_t sm[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) sm[q] = 0;
+    for (int i = tid; i < m; i += nt) {
+        uint32_t a[L];
+        fx_convert<L>(wv(i), ew, FX_HEAD, a);
+#pragma unroll
+        for (int q = 0; q < L; q++) sw[q * m + i] = a[q];
+        fx_sum_add<L>(sm, a);
+    }
+#pragma unroll
+    for (int q = 0; q <= L; q++) sred[q * nt + tid] = sm[q];
+    __syncthreads();
+    for (int h = nt / 2; h >= 1; h >>= 1) {
+        if (tid < h) {
+            uint32_t a[L + 1], b[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) a[q] = sred[q * nt + tid], b[q] = sred[q * nt + tid + h];
+            fx_sum_add2<L>(a, b);
+#pragma unroll
+            for (int q = 0; q <= L; q++) sred[q * nt + tid] = a[q];
+        }
+        __syncthreads();
+    }
+    uint32_t ra[L + 1];
+#pragma unroll
+    for (int q = 0; q <= L; q++) ra[q] = sred[q * nt];
+    const int32_t E = fx_exp(ew, 0, C::W);
+    const int64_t el0 = P.fa_el + P.blkoff[l];
+    for (int o = tid; o < nn; o += nt) {
+        uint64_t lo[C::NC];
+        uint32_t hi[C::NC];
+#pragma unroll
+        for (int z = 0; z < C::NC; z++) lo[z] = 0, hi[z] = 0;
+        auto ld = [&](int i, uint32_t (&a)[L], uint32_t (&b)[L]) {
+            const uint32_t *pb = fx.fa_g + (el0 + (int64_t)i * P.Nd + o) * L;
+#pragma unroll
+            for (int q = 0; q < L; q++) a[q] = sw[q * m + i], b[q] = pb[q];
+        };
+        fx_mac_pipelined<L>(m, 1, 0, ld, lo, hi);
+        uint32_t rb[L + 1];
+#pragma unroll
+        for (int q = 0; q <= L; q++) rb[q] = fx.fa_cs[(P.fp_el + P.blkoff[l] + o) * (L + 1) + q];
+        f(o, fx_finish<L>(lo, hi, m, ra, rb, E));
+    }
+    wg_sync();
 }
 
 template <int L>
@@ -581,11 +693,6 @@ __global__ __launch_bounds__(256) void k_batch_potrs(const BatchProb *__restrict
 enum { BSC_TARGET = 0, BSC_ALPHAP = 1, BSC_ALPHAD = 2, BSC_N = 3 };
 enum { BO_XZ, BO_XDZ, BO_DXZ, BO_DXDZ, BO_TRX, BO_TRZ, BO_CX, BO_CDX, BO_BY, BO_BDY, BO_MAXRD, BO_MAXRP, BO_EIG, BO_N };
 
-__device__ __forceinline__ void wg_sync() {
-    __threadfence_block();
-    __syncthreads();
-}
-
 // C = op(A) op(B) for one n x n block (column-major, element form: one rounded product and one rounded
 // sum per term); the caller synchronises before C is read by other threads
 template <int L>
@@ -677,65 +784,157 @@ __global__ __launch_bounds__(256) void k_bb_gvec(const BatchProb *__restrict__ p
     }
 }
 
-// constraint-wise inner products A_i . D over the dense blocks, one wave per constraint (p, i):
+// a dense space by block in block fixed point (mode 0: DLS, mode 1: X), for the constraint-wise inner
+// products: one exponent per block, the fixed values and their offset sum
+template <int L>
+__global__ __launch_bounds__(256) void k_fx_dspace(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                                   int nblk, const int32_t *__restrict__ req, int mode, FxArrays fx) {
+    __shared__ int32_t se[256];
+    __shared__ uint32_t sred[(L + 1) * 256];
+    for (int t = blockIdx.x; t < nblk; t += gridDim.x) {
+        const BatchBlk bk = blks[t];
+        if (req[bk.p] == 0) continue;
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n, tid = threadIdx.x, nt = blockDim.x;
+        const int64_t o = P.blkoff[bk.l];
+        const PV D = mode == 0 ? P.dls : P.X;
+        int32_t e = EZERO;
+        for (int u = tid; u < nn; u += nt) {
+            const mp<L> v = ldm<L>(D, o + u);
+            if (!is_zero<L>(v) && v.e > e) e = v.e;
+        }
+        se[tid] = e;
+        __syncthreads();
+        for (int h = nt / 2; h >= 1; h >>= 1) {
+            if (tid < h) se[tid] = max(se[tid], se[tid + h]);
+            __syncthreads();
+        }
+        e = se[0];
+        uint32_t sm[L + 1];
+#pragma unroll
+        for (int q = 0; q <= L; q++) sm[q] = 0;
+        for (int u = tid; u < nn; u += nt) {
+            uint32_t a[L];
+            fx_convert<L>(ldm<L>(D, o + u), e, FX_HEAD, a);
+#pragma unroll
+            for (int q = 0; q < L; q++) fx.fd_v[(P.fp_el + o + u) * L + q] = a[q];
+            fx_sum_add<L>(sm, a);
+        }
+#pragma unroll
+        for (int q = 0; q <= L; q++) sred[q * nt + tid] = sm[q];
+        __syncthreads();
+        for (int h = nt / 2; h >= 1; h >>= 1) {
+            if (tid < h) {
+                uint32_t a[L + 1], b[L + 1];
+#pragma unroll
+                for (int q = 0; q <= L; q++) a[q] = sred[q * nt + tid], b[q] = sred[q * nt + tid + h];
+                fx_sum_add2<L>(a, b);
+#pragma unroll
+                for (int q = 0; q <= L; q++) sred[q * nt + tid] = a[q];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            fx.fd_e[t] = e;
+#pragma unroll
+            for (int q = 0; q <= L; q++) fx.fd_s[(int64_t)t * (L + 1) + q] = sred[q * nt];
+        }
+        __syncthreads();
+    }
+}
+
+// constraint-wise inner products A_i . D over the dense blocks, one wave per constraint (p, i), exactly in
+// block fixed point per block (A_i by matrix, D by block from k_fx_dspace) and one rounding per block:
 //   mode 0: g_i = -(A_i . DLS) + Rp_i   (Make_gVec, sdpa_newton.cpp:869-895)
 //   mode 1: Rp_i = b_i - A_i . X        (Residuals::compute, sdpa_parts.cpp:760-790)
 template <int L>
 __global__ __launch_bounds__(256) void k_bc_inner(const BatchProb *__restrict__ probs, const int2 *__restrict__ tasks,
-                                                  int ntasks, const int32_t *__restrict__ req, int mode) {
-    __shared__ uint32_t sh[(L + 1) * 256];
+                                                  int ntasks, const int32_t *__restrict__ req, int mode, FxArrays fx) {
+    using C = FxCfg<L>;
     const int lane = threadIdx.x & 63;
     for (int t0 = blockIdx.x * 4; t0 < ntasks; t0 += gridDim.x * 4) {
         const int t = t0 + (threadIdx.x >> 6);
-        mp<L> acc = zero<L>();
         int2 tk = make_int2(0, 0);
         bool on = false;
         if (t < ntasks) {
             tk = tasks[t];
             on = tk.x >= 0 && req[tk.x] != 0;
         }
-        if (on) {
-            const BatchProb &P = probs[tk.x];
-            const PV D = mode == 0 ? P.dls : P.X;
-            for (int l = 0; l < P.nsdp; l++) {
-                const Desc d = P.desc[(int64_t)tk.y * P.nsdp + l];
-                if (d.kind != 2) continue;
-                const int64_t nn = (int64_t)d.n * d.n;
-                for (int64_t k = lane; k < nn; k += 64)
-                    acc = add<L>(acc, mul<L>(ldm<L>(P.dval, d.off + k), ldm<L>(D, d.blkoff + k)));
-            }
+        if (!on) continue;  // whole waves: the wave's lanes share t
+        const BatchProb &P = probs[tk.x];
+        mp<L> ip = zero<L>();
+        for (int l = 0; l < P.nsdp; l++) {
+            const int n = P.n[l], nn = n * n;
+            const int64_t jm = P.fa_mat + (int64_t)tk.y * P.nsdp + l, bt = P.blk0 + l;
+            const int64_t ael = P.fa_el + (int64_t)tk.y * P.Nd + P.blkoff[l], del = P.fp_el + P.blkoff[l];
+            uint64_t lo[C::NC];
+            uint32_t hi[C::NC];
+#pragma unroll
+            for (int z = 0; z < C::NC; z++) lo[z] = 0, hi[z] = 0;
+            auto ld = [&](int e, uint32_t (&a)[L], uint32_t (&b)[L]) {
+                const uint32_t *pa = fx.fa_g + (ael + e) * L, *pb = fx.fd_v + (del + e) * L;
+#pragma unroll
+                for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = pb[q];
+            };
+            fx_mac_pipelined<L>(nn, 64, lane, ld, lo, hi);
+            fx_wave_sum<L>(lo, hi);
+            uint32_t ra[L + 1], rb[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) ra[q] = fx.fa_gs[jm * (L + 1) + q], rb[q] = fx.fd_s[bt * (L + 1) + q];
+            ip = add<L>(ip, fx_finish<L>(lo, hi, nn, ra, rb, fx_exp(fx.fa_ge[jm], fx.fd_e[bt], C::W)));
         }
-        const mp<L> ip = wave_sum64<L>(acc, sh);
-        if (on && lane == 0) {
-            const BatchProb &P = probs[tk.x];
+        if (lane == 0) {
             if (mode == 0)
                 stm<L>(P.g, tk.y, add<L>(neg<L>(ip), ldm<L>(P.Rp, tk.y)));
             else
                 stm<L>(P.Rp, tk.y, sub<L>(ldm<L>(P.b, tk.y), ip));
         }
-        __syncthreads();
+    }
+}
+
+// per element of every problem's blocks: the offset sum over the constraints of the fixed-point
+// constraint values (by matrix), once per batch (k_fx_combo's column sums)
+template <int L>
+__global__ __launch_bounds__(256) void k_fx_colsum(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
+                                                   int nblk, FxArrays fx, uint32_t *fa_cs) {
+    for (int t = blockIdx.x; t < nblk; t += gridDim.x) {
+        const BatchBlk bk = blks[t];
+        const BatchProb &P = probs[bk.p];
+        const int n = P.n[bk.l], nn = n * n;
+        for (int o = threadIdx.x; o < nn; o += blockDim.x) {
+            uint32_t sm[L + 1];
+#pragma unroll
+            for (int q = 0; q <= L; q++) sm[q] = 0;
+            for (int i = 0; i < P.m; i++) {
+                uint32_t a[L];
+                const uint32_t *pa = fx.fa_g + (P.fa_el + (int64_t)i * P.Nd + P.blkoff[bk.l] + o) * L;
+#pragma unroll
+                for (int q = 0; q < L; q++) a[q] = pa[q];
+                fx_sum_add<L>(sm, a);
+            }
+#pragma unroll
+            for (int q = 0; q <= L; q++) fa_cs[(P.fp_el + P.blkoff[bk.l] + o) * (L + 1) + q] = sm[q];
+        }
     }
 }
 
 // dZ = -sum_i dy_i A_i (+ Rd in the infeasible phases, iarg 1) (compute_DzMat, sdpa_newton.cpp:1346-1356)
+// (sum_i dy_i A_i in block fixed point, wg_fx_combo)
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_dz(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
                                                int nblk, const int32_t *__restrict__ req,
-                                               const int32_t *__restrict__ iarg) {
+                                               const int32_t *__restrict__ iarg, FxArrays fx) {
+    __shared__ uint32_t sw[L * BATCH_MMAX_LDS], sred[(L + 1) * 256];
+    __shared__ int32_t se[256];
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
-        const int n = P.n[bk.l], nn = n * n;
         const int64_t o = P.blkoff[bk.l];
-        for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-            mp<L> acc = zero<L>();
-            for (int i = 0; i < P.m; i++) {
-                const Desc d = P.desc[(int64_t)i * P.nsdp + bk.l];
-                if (d.kind == 2) acc = add<L>(acc, mul<L>(ldm<L>(P.dy, i), ldm<L>(P.dval, d.off + e)));
-            }
+        const bool useR = iarg[bk.p] != 0;
+        wg_fx_combo<L>(P, bk.l, P.dy, fx, sw, sred, se, [&](int e, const mp<L> &acc) {
             mp<L> v = neg<L>(acc);
-            if (iarg[bk.p]) v = add<L>(v, ldm<L>(P.Rd, o + e));
+            if (useR) v = add<L>(v, ldm<L>(P.Rd, o + e));
             stm<L>(P.dZ, o + e, v);
-        }
+        });
         wg_sync();
     }
 }
@@ -925,22 +1124,20 @@ __global__ __launch_bounds__(256) void k_bb_inner(const BatchProb *__restrict__ 
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_resid(const BatchProb *__restrict__ probs,
                                                   const BatchBlk *__restrict__ blks, int nblk,
-                                                  const int32_t *__restrict__ req, PV bo) {
-    __shared__ uint32_t sh[(L + 1) * 256];
+                                                  const int32_t *__restrict__ req, PV bo, FxArrays fx) {
+    __shared__ uint32_t sh[(L + 1) * 256], sw[L * BATCH_MMAX_LDS];
+    __shared__ int32_t se[256];
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
         const int n = P.n[bk.l], nn = n * n;
         const int64_t o = P.blkoff[bk.l];
+        // Rd = C - Z - sum_i y_i A_i (the sum in block fixed point)
+        wg_fx_combo<L>(P, bk.l, P.y, fx, sw, sh, se, [&](int e, const mp<L> &acc) {
+            stm<L>(P.Rd, o + e, sub<L>(sub<L>(ldm<L>(P.C, o + e), ldm<L>(P.Z, o + e)), acc));
+        });
         mp<L> mx = zero<L>(), cx = zero<L>();
         for (int e = threadIdx.x; e < nn; e += blockDim.x) {
-            mp<L> acc = zero<L>();
-            for (int i = 0; i < P.m; i++) {
-                const Desc d = P.desc[(int64_t)i * P.nsdp + bk.l];
-                if (d.kind == 2) acc = add<L>(acc, mul<L>(ldm<L>(P.dval, d.off + e), ldm<L>(P.y, i)));
-            }
-            const mp<L> c = ldm<L>(P.C, o + e);
-            const mp<L> v = sub<L>(sub<L>(c, ldm<L>(P.Z, o + e)), acc);
-            stm<L>(P.Rd, o + e, v);
+            const mp<L> c = ldm<L>(P.C, o + e), v = ldm<L>(P.Rd, o + e);
             if (cmp_abs<L>(v, mx) > 0) mx = absv<L>(v);
             cx = add<L>(cx, mul<L>(c, ldm<L>(P.X, o + e)));
         }
@@ -1362,6 +1559,9 @@ struct Batch : BatchBase {
     int32_t *fa_re = nullptr, *fa_ge = nullptr, *fx_e = nullptr, *fz_e = nullptr;
     uint32_t *ws_schur = nullptr;  // per workgroup of k_batch_schur: work1 / work2 as limb floats
     int64_t ws_n = 0;
+    uint32_t *fa_cs = nullptr, *fd_v = nullptr, *fd_s = nullptr;  // FxArrays of the other phases
+    int32_t *fd_e = nullptr;
+    FxArrays fxa() const { return FxArrays{fa_g, fa_ge, fa_gs, fa_cs, fd_v, fd_e, fd_s}; }
 
     ~Batch() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -1377,6 +1577,10 @@ struct Batch : BatchBase {
         (void)hipFree(d_sc);
         (void)hipFree(d_bo);
         (void)hipFree(ws_schur);
+        (void)hipFree(fa_cs);
+        (void)hipFree(fd_v);
+        (void)hipFree(fd_s);
+        (void)hipFree(fd_e);
         for (void *q : {(void *)fa_r, (void *)fa_rs, (void *)fa_g, (void *)fa_gs, (void *)fx_v, (void *)fx_s,
                         (void *)fz_v, (void *)fz_s, (void *)fa_re, (void *)fa_ge, (void *)fx_e, (void *)fz_e})
             (void)hipFree(q);
@@ -1422,8 +1626,8 @@ struct Batch : BatchBase {
                         return SDPG_EARG;
                     }
                 }
-            if (e->m > BATCH_MMAX) {
-                set_err("sdpg_batch_create: the batched form takes at most 4096 constraints per problem");
+            if (e->m > BATCH_MMAX_LDS) {
+                set_err("sdpg_batch_create: the batched form takes at most 256 constraints per problem");
                 return SDPG_EARG;
             }
             mmax = std::max(mmax, e->m);
@@ -1449,6 +1653,7 @@ struct Batch : BatchBase {
             b.icZ = e->icZ.pv();
             b.W1 = e->W1.pv();
             b.W2 = e->W2.pv();
+            b.blk0 = (int32_t)blks.size();
             for (int l = 0; l < e->nsdp; l++) blks.push_back(BatchBlk{p, l});
             b.Nd = (int32_t)e->Nd;
             b.nsum = 0;
@@ -1563,7 +1768,9 @@ struct Batch : BatchBase {
             !dm((void **)&fx_v, 4 * (size_t)L * fp_el_n) || !dm((void **)&fx_e, 4 * (size_t)fp_row_n) ||
             !dm((void **)&fx_s, 4 * (size_t)(L + 1) * fp_row_n) || !dm((void **)&fz_v, 4 * (size_t)L * fp_el_n) ||
             !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n) ||
-            !dm((void **)&ws_schur, 4 * (size_t)(L + 1) * (ws_n = (int64_t)grid_schur * nmax * nmax))) {
+            !dm((void **)&ws_schur, 4 * (size_t)(L + 1) * (ws_n = (int64_t)grid_schur * nmax * nmax)) ||
+            !dm((void **)&fa_cs, 4 * (size_t)(L + 1) * fp_el_n) || !dm((void **)&fd_v, 4 * (size_t)L * fp_el_n) ||
+            !dm((void **)&fd_s, 4 * (size_t)(L + 1) * blks.size()) || !dm((void **)&fd_e, 4 * blks.size())) {
             set_err("sdpg_batch_create: out of device memory");
             return SDPG_ENOMEM;
         }
@@ -1575,6 +1782,8 @@ struct Batch : BatchBase {
         for (Engine<L> *e : eng) HIPCHK(hipStreamSynchronize(e->st));  // the constraint values are in place
         k_fx_constraints<L><<<std::max(1, std::min(ntasks, 8 * cus)), 256, 0, st>>>(d_probs, d_tasks, ntasks, fa_r,
                                                                                    fa_re, fa_rs, fa_g, fa_ge, fa_gs);
+        HIPCHK(hipGetLastError());
+        k_fx_colsum<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), fxa(), fa_cs);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         return SDPG_OK;
@@ -1661,14 +1870,15 @@ struct Batch : BatchBase {
         int rc;
         if ((rc = begin(req)) || (rc = upload_sc()) || (rc = upload_iarg())) return rc;
         k_bb_gvec<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg, PV{d_sc, sc_n});
-        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 0);
+        k_fx_dspace<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, 0, fxa());
+        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 0, fxa());
         return end(BT_GVEC);
     }
     int dz(const int *req, const int *phase) override {
         for (int p = 0; p < nprob; p++) h_iarg[p] = req[p] && infeasible(phase[p]) ? 1 : 0;
         int rc;
         if ((rc = begin(req)) || (rc = upload_iarg())) return rc;
-        k_bb_dz<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg);
+        k_bb_dz<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg, fxa());
         return end(BT_DZ);
     }
     int dx(const int *req) override {
@@ -1724,8 +1934,9 @@ struct Batch : BatchBase {
     int residuals(const int *req, uint32_t *out) override {
         int rc;
         if ((rc = begin(req))) return rc;
-        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 1);
-        k_bb_resid<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, PV{d_bo, bo_n});
+        k_fx_dspace<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, 1, fxa());
+        k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 1, fxa());
+        k_bb_resid<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, PV{d_bo, bo_n}, fxa());
         if ((rc = fetch_bo(BT_RESID))) return rc;
         const int W = L + 2;
         for (int p = 0; p < nprob; p++) {
