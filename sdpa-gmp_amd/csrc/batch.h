@@ -693,20 +693,69 @@ __global__ __launch_bounds__(256) void k_batch_potrs(const BatchProb *__restrict
 enum { BSC_TARGET = 0, BSC_ALPHAP = 1, BSC_ALPHAD = 2, BSC_N = 3 };
 enum { BO_XZ, BO_XDZ, BO_DXZ, BO_DXDZ, BO_TRX, BO_TRZ, BO_CX, BO_CDX, BO_BY, BO_BDY, BO_MAXRD, BO_MAXRP, BO_EIG, BO_N };
 
-// C = op(A) op(B) for one n x n block (column-major, element form: one rounded product and one rounded
-// sum per term); the caller synchronises before C is read by other threads
+// C = op(A) op(B) for one n x n block in block fixed point: op(A) by rows and op(B) by columns (one
+// exponent each, offset binary), exact 96-bit column sums over k and one rounding per output.  The
+// fixed operands, offset sums and exponents go to the workgroup's global scratch slice scr
+// (fx_mm_words(n) words, L2-resident).  The caller synchronises before C is read by other threads.
 template <int L>
-__device__ void wg_mm(int n, PV A, int64_t ao, bool ta, PV B, int64_t bo, bool tb, PV C, int64_t co) {
-    for (int o = threadIdx.x; o < n * n; o += blockDim.x) {
-        const int a = o % n, b = o / n;
-        mp<L> acc = zero<L>();
-        for (int c = 0; c < n; c++) {
-            const mp<L> x = ldm<L>(A, ao + (ta ? c + (int64_t)n * a : a + (int64_t)n * c));
-            const mp<L> y = ldm<L>(B, bo + (tb ? b + (int64_t)n * c : c + (int64_t)n * b));
-            acc = add<L>(acc, mul<L>(x, y));
+__host__ __device__ constexpr int64_t fx_mm_words(int n) {
+    return 2 * (int64_t)n * n * L + 2 * (int64_t)n * (L + 1) + 2 * (int64_t)n;
+}
+template <int L>
+__device__ void wg_mm_fx(int n, PV A, int64_t ao, bool ta, PV B, int64_t bo, bool tb, PV C, int64_t co, uint32_t *scr) {
+    using Cf = FxCfg<L>;
+    uint32_t *Av = scr, *Bv = Av + (int64_t)n * n * L, *As = Bv + (int64_t)n * n * L, *Bs = As + n * (L + 1);
+    int32_t *Ae = (int32_t *)(Bs + n * (L + 1)), *Be = Ae + n;
+    const int tid = threadIdx.x, nt = blockDim.x, nn = n * n;
+    for (int u = tid; u < 2 * n; u += nt) {
+        const bool isA = u < n;
+        const int r = isA ? u : u - n;  // row of op(A), or column of op(B)
+        // op(A)[r, k] = ta ? A[k, r] : A[r, k]; op(B)[k, c] = tb ? B[c, k] : B[k, c] (column-major)
+        auto at = [&](int k) -> int64_t {
+            return isA ? ao + (ta ? k + (int64_t)n * r : r + (int64_t)n * k)
+                       : bo + (tb ? r + (int64_t)n * k : k + (int64_t)n * r);
+        };
+        const PV S = isA ? A : B;
+        int32_t e = EZERO;
+        for (int k = 0; k < n; k++) {
+            const mp<L> v = ldm<L>(S, at(k));
+            if (!is_zero<L>(v) && v.e > e) e = v.e;
         }
-        stm<L>(C, co + o, acc);
+        uint32_t sm[L + 1];
+#pragma unroll
+        for (int q = 0; q <= L; q++) sm[q] = 0;
+        for (int k = 0; k < n; k++) {
+            uint32_t a[L];
+            fx_convert<L>(ldm<L>(S, at(k)), e, FX_HEAD, a);
+            uint32_t *dst = isA ? Av + (r + (int64_t)n * k) * L : Bv + (k + (int64_t)n * r) * L;
+#pragma unroll
+            for (int q = 0; q < L; q++) dst[q] = a[q];
+            fx_sum_add<L>(sm, a);
+        }
+        (isA ? Ae : Be)[r] = e;
+        uint32_t *ds = (isA ? As : Bs) + r * (L + 1);
+#pragma unroll
+        for (int q = 0; q <= L; q++) ds[q] = sm[q];
     }
+    wg_sync();
+    for (int o = tid; o < nn; o += nt) {
+        const int r = o % n, c = o / n;
+        uint64_t lo[Cf::NC];
+        uint32_t hi[Cf::NC];
+#pragma unroll
+        for (int z = 0; z < Cf::NC; z++) lo[z] = 0, hi[z] = 0;
+        auto ld = [&](int k, uint32_t (&a)[L], uint32_t (&b)[L]) {
+            const uint32_t *pa = Av + (r + (int64_t)n * k) * L, *pb = Bv + (k + (int64_t)n * c) * L;
+#pragma unroll
+            for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = pb[q];
+        };
+        fx_mac_pipelined<L>(n, 1, 0, ld, lo, hi);
+        uint32_t ra[L + 1], rb[L + 1];
+#pragma unroll
+        for (int q = 0; q <= L; q++) ra[q] = As[r * (L + 1) + q], rb[q] = Bs[c * (L + 1) + q];
+        stm<L>(C, co + o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(Ae[r], Be[c], Cf::W)));
+    }
+    wg_sync();  // the scratch is reused by the next product
 }
 
 // workgroup sum / max |.| of one value per thread (blockDim.x a power of two; sh: L+1 planes of blockDim.x)
@@ -753,7 +802,8 @@ __device__ mp<L> wg_maxabs(mp<L> v, uint32_t *sh) {
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_gvec(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
                                                  int nblk, const int32_t *__restrict__ req,
-                                                 const int32_t *__restrict__ iarg, PV sc) {
+                                                 const int32_t *__restrict__ iarg, PV sc, uint32_t *fxs, int64_t fxn) {
+    uint32_t *scr = fxs + (int64_t)blockIdx.x * fxn;  // this workgroup's scratch of the fixed-point products
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
         const int n = P.n[bk.l], nn = n * n;
@@ -763,17 +813,17 @@ __global__ __launch_bounds__(256) void k_bb_gvec(const BatchProb *__restrict__ p
         for (int e = threadIdx.x; e < nn; e += blockDim.x)
             stm<L>(P.rzinv, o + e, add<L>(mul<L>(tg, ldm<L>(P.invZ, o + e)), neg<L>(ldm<L>(P.X, o + e))));
         if (corr) {
-            wg_mm<L>(n, P.dX, o, false, P.dZ, o, false, P.W1, o);
+            wg_mm_fx<L>(n, P.dX, o, false, P.dZ, o, false, P.W1, o, scr);
             wg_sync();
-            wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o);
+            wg_mm_fx<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o, scr);
             wg_sync();
             for (int e = threadIdx.x; e < nn; e += blockDim.x)
                 stm<L>(P.rzinv, o + e, sub<L>(ldm<L>(P.rzinv, o + e), ldm<L>(P.W2, o + e)));
         }
         if (infeas && !corr) {
-            wg_mm<L>(n, P.X, o, false, P.Rd, o, false, P.W1, o);
+            wg_mm_fx<L>(n, P.X, o, false, P.Rd, o, false, P.W1, o, scr);
             wg_sync();
-            wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.xrd, o);
+            wg_mm_fx<L>(n, P.W1, o, false, P.invZ, o, false, P.xrd, o, scr);
             wg_sync();
         }
         for (int e = threadIdx.x; e < nn; e += blockDim.x) {
@@ -942,14 +992,15 @@ __global__ __launch_bounds__(256) void k_bb_dz(const BatchProb *__restrict__ pro
 // dX = sym(r_zinv - X dZ inv(Z)) (compute_DxMat incl. getSymmetrize, sdpa_newton.cpp:1358-1373)
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_dx(const BatchProb *__restrict__ probs, const BatchBlk *__restrict__ blks,
-                                               int nblk, const int32_t *__restrict__ req) {
+                                               int nblk, const int32_t *__restrict__ req, uint32_t *fxs, int64_t fxn) {
+    uint32_t *scr = fxs + (int64_t)blockIdx.x * fxn;
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
         const int n = P.n[bk.l], nn = n * n;
         const int64_t o = P.blkoff[bk.l];
-        wg_mm<L>(n, P.X, o, false, P.dZ, o, false, P.W1, o);
+        wg_mm_fx<L>(n, P.X, o, false, P.dZ, o, false, P.W1, o, scr);
         wg_sync();
-        wg_mm<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o);
+        wg_mm_fx<L>(n, P.W1, o, false, P.invZ, o, false, P.W2, o, scr);
         wg_sync();
         for (int e = threadIdx.x; e < nn; e += blockDim.x)
             stm<L>(P.dX, o + e, add<L>(ldm<L>(P.rzinv, o + e), neg<L>(ldm<L>(P.W2, o + e))));
@@ -1052,7 +1103,9 @@ __device__ bool wg_chol_inv(int n, PV Src, PV W, PV S, int64_t o, PV Out) {
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_factor(const BatchProb *__restrict__ probs,
                                                    const BatchBlk *__restrict__ blks, int nblk,
-                                                   const int32_t *__restrict__ req, int32_t *__restrict__ info) {
+                                                   const int32_t *__restrict__ req, int32_t *__restrict__ info,
+                                                   uint32_t *fxs, int64_t fxn) {
+    uint32_t *scr = fxs + (int64_t)blockIdx.x * fxn;
     BATCH_BLOCK_LOOP {
         const BatchProb &P = probs[bk.p];
         const int n = P.n[bk.l];
@@ -1064,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_bb_factor(const BatchProb *__restrict__
             wg_sync();
             continue;
         }
-        wg_mm<L>(n, P.icZ, o, true, P.icZ, o, false, P.invZ, o);
+        wg_mm_fx<L>(n, P.icZ, o, true, P.icZ, o, false, P.invZ, o, scr);
         wg_sync();
     }
 }
@@ -1326,7 +1379,9 @@ __device__ inline void jacobi_sweep(int N, double *a, double *v, double *cs, int
 template <int L>
 __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__ probs,
                                                    const BatchBlk *__restrict__ blks, int nblk,
-                                                   const int32_t *__restrict__ req, int which, PV bo) {
+                                                   const int32_t *__restrict__ req, int which, PV bo, uint32_t *fxs,
+                                                   int64_t fxn) {
+    uint32_t *scr = fxs + (int64_t)blockIdx.x * fxn;
     __shared__ uint32_t sh[(L + 1) * 256];
     __shared__ uint32_t vq[(L + 1) * BEIG_NMAX], vqo[(L + 1) * BEIG_NMAX], vr[(L + 1) * BEIG_NMAX],
         vw[(L + 1) * BEIG_NMAX], tal[(L + 1) * BEIG_KMAX], tbe[(L + 1) * BEIG_KMAX], tsc[(L + 1) * 4];
@@ -1339,9 +1394,9 @@ __global__ __launch_bounds__(256) void k_bb_mineig(const BatchProb *__restrict__
         const int n = P.n[bk.l], nn = n * n;
         const int64_t o = P.blkoff[bk.l];
         const PV Lm = which ? P.icZ : P.icX, Xm = which ? P.dZ : P.dX;
-        wg_mm<L>(n, Xm, o, false, Lm, o, true, P.W1, o);
+        wg_mm_fx<L>(n, Xm, o, false, Lm, o, true, P.W1, o, scr);
         wg_sync();
-        wg_mm<L>(n, Lm, o, false, P.W1, o, false, P.W2, o);
+        wg_mm_fx<L>(n, Lm, o, false, P.W1, o, false, P.W2, o, scr);
         wg_sync();
         const PV M = P.W2;
         mp<L> result;
@@ -1562,6 +1617,8 @@ struct Batch : BatchBase {
     uint32_t *fa_cs = nullptr, *fd_v = nullptr, *fd_s = nullptr;  // FxArrays of the other phases
     int32_t *fd_e = nullptr;
     FxArrays fxa() const { return FxArrays{fa_g, fa_ge, fa_gs, fa_cs, fd_v, fd_e, fd_s}; }
+    uint32_t *fxs = nullptr;  // per workgroup of the block-task kernels: the fixed-point products' scratch
+    int64_t fxs_n = 0;
 
     ~Batch() override {
         if (st) (void)hipStreamSynchronize(st);
@@ -1578,6 +1635,7 @@ struct Batch : BatchBase {
         (void)hipFree(d_bo);
         (void)hipFree(ws_schur);
         (void)hipFree(fa_cs);
+        (void)hipFree(fxs);
         (void)hipFree(fd_v);
         (void)hipFree(fd_s);
         (void)hipFree(fd_e);
@@ -1770,7 +1828,8 @@ struct Batch : BatchBase {
             !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n) ||
             !dm((void **)&ws_schur, 4 * (size_t)(L + 1) * (ws_n = (int64_t)grid_schur * nmax * nmax)) ||
             !dm((void **)&fa_cs, 4 * (size_t)(L + 1) * fp_el_n) || !dm((void **)&fd_v, 4 * (size_t)L * fp_el_n) ||
-            !dm((void **)&fd_s, 4 * (size_t)(L + 1) * blks.size()) || !dm((void **)&fd_e, 4 * blks.size())) {
+            !dm((void **)&fd_s, 4 * (size_t)(L + 1) * blks.size()) || !dm((void **)&fd_e, 4 * blks.size()) ||
+            !dm((void **)&fxs, 4 * (size_t)grid_blk * (fxs_n = fx_mm_words<L>(nmax)))) {
             set_err("sdpg_batch_create: out of device memory");
             return SDPG_ENOMEM;
         }
@@ -1869,7 +1928,8 @@ struct Batch : BatchBase {
         }
         int rc;
         if ((rc = begin(req)) || (rc = upload_sc()) || (rc = upload_iarg())) return rc;
-        k_bb_gvec<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg, PV{d_sc, sc_n});
+        k_bb_gvec<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_iarg, PV{d_sc, sc_n}, fxs,
+                                               fxs_n);
         k_fx_dspace<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, 0, fxa());
         k_bc_inner<L><<<grid_con(), 256, 0, st>>>(d_probs, d_tasks, ntasks, d_req, 0, fxa());
         return end(BT_GVEC);
@@ -1884,7 +1944,7 @@ struct Batch : BatchBase {
     int dx(const int *req) override {
         int rc;
         if ((rc = begin(req))) return rc;
-        k_bb_dx<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req);
+        k_bb_dx<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, fxs, fxs_n);
         return end(BT_DX);
     }
     int update(const int *req, const uint32_t *alphaP, const uint32_t *alphaD) override {
@@ -1903,7 +1963,7 @@ struct Batch : BatchBase {
         int rc;
         if ((rc = begin(req))) return rc;
         HIPCHK(hipMemsetAsync(d_binfo, 0, sizeof(int32_t) * nprob, st));
-        k_bb_factor<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_binfo);
+        k_bb_factor<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, d_binfo, fxs, fxs_n);
         return end(BT_FACTOR, info, d_binfo, sizeof(int32_t) * nprob);
     }
     int fetch_bo(int kind) { return end(kind, h_bo.data(), d_bo, sizeof(uint32_t) * h_bo.size()); }
@@ -1958,7 +2018,8 @@ struct Batch : BatchBase {
     int min_eigen(const int *req, int which, uint32_t *out) override {
         int rc;
         if ((rc = begin(req))) return rc;
-        k_bb_mineig<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, which, PV{d_bo, bo_n});
+        k_bb_mineig<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, which, PV{d_bo, bo_n}, fxs,
+                                                 fxs_n);
         if ((rc = fetch_bo(BT_EIG))) return rc;
         const int W = L + 2;
         for (int p = 0; p < nprob; p++) {
