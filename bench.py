@@ -163,8 +163,8 @@ def measure_batch(count: int, independent: int, threads: int):
     for r in res:
         ph = r.phase.decode().strip()
         phases[ph] = phases.get(ph, 0) + 1
-    out = {"problems": count, "form": "lockstep batched (Schur build in block fixed point with exact limb accumulation, "
-                                           "the other phases in element-form block kernels)",
+    out = {"problems": count, "form": "lockstep batched (block kernels over a problem table; every sum of products in "
+                                           "block fixed point with exact limb accumulation)",
            "wall_s": wall, "solves_per_s": count / wall, "latency_p50_s": pct(slat, 0.50),
            "latency_p99_s": pct(slat, 0.99), "mean_iterations": sum(r.iterations for r in res) / count,
            "phases": phases, "meetings": st["meetings"], "mean_problems_per_meeting": st["mean_problems_per_meeting"],

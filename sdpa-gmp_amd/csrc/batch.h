@@ -68,6 +68,7 @@ enum { BATCH_NONE = 0, BATCH_PREDICTOR = 1, BATCH_CORRECTOR = 2 };
 // n = 40, m = 120, L = 10: two workgroups of 512 per CU).  The 8 lanes of a partner add their column
 // sums by cross-lane shuffles.
 constexpr int BS_THREADS = 512;
+constexpr int SCHUR_PK = 2;  // packed work2 values per thread: n(n+1)/2 <= 820 <= 2 x 512 for n <= 40
 constexpr int FX_HEAD = 2;
 
 template <int L>
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                                                             const int32_t *__restrict__ req, int nmax, int mmax,
                                                             const uint32_t *__restrict__ fa_r, const int32_t *__restrict__ fa_re,
                                                             const uint32_t *__restrict__ fa_rs, const uint32_t *__restrict__ fa_g,
-                                                            const int32_t *__restrict__ fa_ge, const uint32_t *__restrict__ fa_gs,
+                                                            const int32_t *__restrict__ fa_ge, const uint32_t *__restrict__ fa_ps,
                                                             const uint32_t *__restrict__ fx_v, const int32_t *__restrict__ fx_e,
                                                             const uint32_t *__restrict__ fx_s, const uint32_t *__restrict__ fz_v,
                                                             const int32_t *__restrict__ fz_e, const uint32_t *__restrict__ fz_s,
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
     const int PW = nmax * nmax;
     uint32_t *R2 = shm, *sFs = R2 + PW * L, *sGs = sFs + nmax * (L + 1), *sRed = sGs + (L + 1),
              *sB = sRed + 64 * (L + 1);
-    int32_t *sFe = (int32_t *)(sB + mmax * (L + 1)), *sGe = sFe + nmax;
+    int32_t *sFe = (int32_t *)(sB + mmax * (L + 1)), *sGe = sFe + nmax, *sMx = sGe + 1, *sIdx = sMx + BS_THREADS / 64;
     const int64_t wb = (int64_t)blockIdx.x * PW;  // this workgroup's work1 / work2 limb floats in ws
     const int tid = threadIdx.x, nt = BS_THREADS;
     const int q8 = tid & 7, slot = tid >> 3, nslot = BS_THREADS / 8;
@@ -330,30 +331,53 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 stm<L>(ws, wb + o, fx_finish<L>(lo, hi, n, ra, rb, fx_exp(fx_e[prw + r], sFe[c], W)));
             }
             __syncthreads();
-            // work2 by matrix: exponent, fixed point, offset sum
-            if (tid < 64) {
-                int32_t e = EZERO;
-                for (int o = tid; o < nn; o += 64) {
-                    const mp<L> v = ldm<L>(ws, wb + o);
-                    if (!is_zero<L>(v) && v.e > e) e = v.e;
+            // work2 + work2^T on and below the diagonal, packed column by column (t -> element a + n b, a >= b):
+            // for symmetric A_j, A_j . work2 = sum over the lower triangle of A_j[a,b] (work2[a,b] + work2[b,a])
+            // with the diagonal once; then by matrix: exponent, fixed point, offset sum
+            const int np = n * (n + 1) / 2;
+            for (int b = tid; b < n; b += nt) {
+                const int base = b * n - b * (b - 1) / 2;
+                for (int a = b; a < n; a++) sIdx[base + a - b] = a + n * b;
+            }
+            __syncthreads();
+            mp<L> pv[SCHUR_PK];
+            int32_t e = EZERO;
+#pragma unroll
+            for (int u = 0; u < SCHUR_PK; u++) {
+                const int t = tid + u * nt;
+                pv[u] = zero<L>();
+                if (t < np) {
+                    const int x = sIdx[t], a = x % n, b = x / n;
+                    pv[u] = a == b ? ldm<L>(ws, wb + x) : add<L>(ldm<L>(ws, wb + x), ldm<L>(ws, wb + b + (int64_t)n * a));
+                    if (!is_zero<L>(pv[u]) && pv[u].e > e) e = pv[u].e;
                 }
-                for (int h = 32; h >= 1; h >>= 1) e = max(e, __shfl_xor(e, h, 64));
-                if (tid == 0) sGe[0] = e;
+            }
+            for (int h = 32; h >= 1; h >>= 1) e = max(e, __shfl_xor(e, h, 64));
+            if ((tid & 63) == 0) sMx[tid >> 6] = e;
+            __syncthreads();
+            if (tid == 0) {
+                int32_t m8 = sMx[0];
+                for (int w = 1; w < BS_THREADS / 64; w++) m8 = max(m8, sMx[w]);
+                sGe[0] = m8;
             }
             __syncthreads();
             const int32_t eg = sGe[0];
-            for (int o = tid; o < nn; o += nt) {
-                uint32_t a[L];
-                fx_convert<L>(ldm<L>(ws, wb + o), eg, FX_HEAD, a);
 #pragma unroll
-                for (int q = 0; q < L; q++) R2[q * PW + o] = a[q];
+            for (int u = 0; u < SCHUR_PK; u++) {
+                const int t = tid + u * nt;
+                if (t < np) {
+                    uint32_t a[L];
+                    fx_convert<L>(pv[u], eg, FX_HEAD, a);
+#pragma unroll
+                    for (int q = 0; q < L; q++) R2[q * PW + t] = a[q];
+                }
             }
             __syncthreads();
             if (tid < 64) {
                 uint32_t sm[L + 1];
 #pragma unroll
                 for (int q = 0; q <= L; q++) sm[q] = 0;
-                for (int o = tid; o < nn; o += 64) {
+                for (int o = tid; o < np; o += 64) {
                     uint32_t a[L];
 #pragma unroll
                     for (int q = 0; q < L; q++) a[q] = R2[q * PW + o];
@@ -373,7 +397,7 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 for (int q = 0; q <= L; q++) sGs[q] = sm[q];
             }
             __syncthreads();
-            // B_ij += A_j . work2 for the partners j <= i: 8 threads per partner over the n^2 entries
+            // B_ij += A_j . work2 for the partners j <= i: 8 threads per partner over the packed entries
             uint32_t gsum[L + 1];
 #pragma unroll
             for (int q = 0; q <= L; q++) gsum[q] = sGs[q];
@@ -387,11 +411,11 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                 if (on) {
                     const int64_t jel = P.fa_el + (int64_t)j * P.Nd + P.blkoff[l];
                     auto ld = [&](int e, uint32_t (&a)[L], uint32_t (&b)[L]) {
-                        const uint32_t *pa = fa_g + (jel + e) * L;
+                        const uint32_t *pa = fa_g + (jel + sIdx[e]) * L;
 #pragma unroll
                         for (int q = 0; q < L; q++) a[q] = pa[q], b[q] = R2[q * PW + e];
                     };
-                    fx_mac_pipelined<L>(nn, 8, q8, ld, lo, hi);
+                    fx_mac_pipelined<L>(np, 8, q8, ld, lo, hi);
                 }
                 // the 8 lanes of a partner (consecutive lanes of one wave) add their column sums
 #pragma unroll
@@ -411,8 +435,8 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
                     const int64_t jm = P.fa_mat + (int64_t)j * nsdp + l;
                     uint32_t ra[L + 1];
 #pragma unroll
-                    for (int q = 0; q <= L; q++) ra[q] = fa_gs[jm * (L + 1) + q];
-                    const mp<L> v = fx_finish<L>(lo, hi, nn, ra, gsum, fx_exp(fa_ge[jm], eg, W));
+                    for (int q = 0; q <= L; q++) ra[q] = fa_ps[jm * (L + 1) + q];
+                    const mp<L> v = fx_finish<L>(lo, hi, np, ra, gsum, fx_exp(fa_ge[jm], eg, W));
                     lds_st<L>(sB, mmax, j, add<L>(lds_ld<L>(sB, mmax, j), v));
                 }
             }
@@ -431,7 +455,8 @@ __global__ __launch_bounds__(BS_THREADS) void k_batch_schur(const BatchProb *__r
 template <int L>
 inline size_t schur_lds_words(int nmax, int mmax) {
     const size_t PW = (size_t)nmax * nmax;
-    return PW * L + (size_t)nmax * (L + 1) + (L + 1) + 64 * (L + 1) + (size_t)mmax * (L + 1) + nmax + 1;
+    return PW * L + (size_t)nmax * (L + 1) + (L + 1) + 64 * (L + 1) + (size_t)mmax * (L + 1) + nmax + 1 +
+           BS_THREADS / 64 + (size_t)nmax * (nmax + 1) / 2;
 }
 
 // the constraint matrices in block fixed point, once per batch: per task (p, i) and block, by rows
@@ -440,9 +465,9 @@ template <int L>
 __global__ __launch_bounds__(256) void k_fx_constraints(const BatchProb *__restrict__ probs,
                                                         const int2 *__restrict__ tasks, int ntasks, uint32_t *fa_r,
                                                         int32_t *fa_re, uint32_t *fa_rs, uint32_t *fa_g,
-                                                        int32_t *fa_ge, uint32_t *fa_gs) {
+                                                        int32_t *fa_ge, uint32_t *fa_gs, uint32_t *fa_ps) {
     __shared__ int32_t se[256];
-    __shared__ uint32_t ssum[256 * (L + 1)];
+    __shared__ uint32_t ssum[256 * (L + 1)], psum[256 * (L + 1)];
     for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const int2 tk = tasks[t];
         if (tk.x < 0) continue;
@@ -491,24 +516,28 @@ __global__ __launch_bounds__(256) void k_fx_constraints(const BatchProb *__restr
                 __syncthreads();
             }
             e = se[0];
-            uint32_t sm[L + 1];
+            uint32_t sm[L + 1], sp[L + 1];  // all elements; the lower triangle with the diagonal
 #pragma unroll
-            for (int q = 0; q <= L; q++) sm[q] = 0;
+            for (int q = 0; q <= L; q++) sm[q] = 0, sp[q] = 0;
             for (int o = threadIdx.x; o < nn; o += blockDim.x) {
                 uint32_t a[L];
                 fx_convert<L>(val(o % n, o / n), e, FX_HEAD, a);
 #pragma unroll
                 for (int q = 0; q < L; q++) fa_g[(el + o) * L + q] = a[q];
                 fx_sum_add<L>(sm, a);
+                if (o % n >= o / n) fx_sum_add<L>(sp, a);
             }
 #pragma unroll
-            for (int q = 0; q <= L; q++) ssum[threadIdx.x * (L + 1) + q] = sm[q];
+            for (int q = 0; q <= L; q++) ssum[threadIdx.x * (L + 1) + q] = sm[q], psum[threadIdx.x * (L + 1) + q] = sp[q];
             __syncthreads();
             if (threadIdx.x == 0) {
-                for (int u = 1; u < (int)blockDim.x; u++) fx_sum_add2<L>(sm, ssum + u * (L + 1));
+                for (int u = 1; u < (int)blockDim.x; u++) {
+                    fx_sum_add2<L>(sm, ssum + u * (L + 1));
+                    fx_sum_add2<L>(sp, psum + u * (L + 1));
+                }
                 fa_ge[mt] = e;
 #pragma unroll
-                for (int q = 0; q <= L; q++) fa_gs[mt * (L + 1) + q] = sm[q];
+                for (int q = 0; q <= L; q++) fa_gs[mt * (L + 1) + q] = sm[q], fa_ps[mt * (L + 1) + q] = sp[q];
             }
             __syncthreads();
         }
@@ -1615,6 +1644,7 @@ struct Batch : BatchBase {
     uint32_t *ws_schur = nullptr;  // per workgroup of k_batch_schur: work1 / work2 as limb floats
     int64_t ws_n = 0;
     uint32_t *fa_cs = nullptr, *fd_v = nullptr, *fd_s = nullptr;  // FxArrays of the other phases
+    uint32_t *fa_ps = nullptr;  // offset sums of the constraint matrices' lower triangles (packed inner products)
     int32_t *fd_e = nullptr;
     FxArrays fxa() const { return FxArrays{fa_g, fa_ge, fa_gs, fa_cs, fd_v, fd_e, fd_s}; }
     uint32_t *fxs = nullptr;  // per workgroup of the block-task kernels: the fixed-point products' scratch
@@ -1635,6 +1665,7 @@ struct Batch : BatchBase {
         (void)hipFree(d_bo);
         (void)hipFree(ws_schur);
         (void)hipFree(fa_cs);
+        (void)hipFree(fa_ps);
         (void)hipFree(fxs);
         (void)hipFree(fd_v);
         (void)hipFree(fd_s);
@@ -1827,7 +1858,7 @@ struct Batch : BatchBase {
             !dm((void **)&fx_s, 4 * (size_t)(L + 1) * fp_row_n) || !dm((void **)&fz_v, 4 * (size_t)L * fp_el_n) ||
             !dm((void **)&fz_e, 4 * (size_t)fp_row_n) || !dm((void **)&fz_s, 4 * (size_t)(L + 1) * fp_row_n) ||
             !dm((void **)&ws_schur, 4 * (size_t)(L + 1) * (ws_n = (int64_t)grid_schur * nmax * nmax)) ||
-            !dm((void **)&fa_cs, 4 * (size_t)(L + 1) * fp_el_n) || !dm((void **)&fd_v, 4 * (size_t)L * fp_el_n) ||
+            !dm((void **)&fa_cs, 4 * (size_t)(L + 1) * fp_el_n) || !dm((void **)&fa_ps, 4 * (size_t)(L + 1) * fa_mat_n) || !dm((void **)&fd_v, 4 * (size_t)L * fp_el_n) ||
             !dm((void **)&fd_s, 4 * (size_t)(L + 1) * blks.size()) || !dm((void **)&fd_e, 4 * blks.size()) ||
             !dm((void **)&fxs, 4 * (size_t)grid_blk * (fxs_n = fx_mm_words<L>(nmax)))) {
             set_err("sdpg_batch_create: out of device memory");
@@ -1840,7 +1871,8 @@ struct Batch : BatchBase {
             return SDPG_EHIP;
         for (Engine<L> *e : eng) HIPCHK(hipStreamSynchronize(e->st));  // the constraint values are in place
         k_fx_constraints<L><<<std::max(1, std::min(ntasks, 8 * cus)), 256, 0, st>>>(d_probs, d_tasks, ntasks, fa_r,
-                                                                                   fa_re, fa_rs, fa_g, fa_ge, fa_gs);
+                                                                                   fa_re, fa_rs, fa_g, fa_ge, fa_gs,
+                                                                                   fa_ps);
         HIPCHK(hipGetLastError());
         k_fx_colsum<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), fxa(), fa_cs);
         HIPCHK(hipGetLastError());
@@ -1853,7 +1885,7 @@ struct Batch : BatchBase {
         k_fx_point<L><<<grid_blk, 256, 0, st>>>(d_probs, d_blks, (int)blks.size(), d_req, fx_v, fx_e, fx_s, fz_v, fz_e,
                                                 fz_s);
         k_batch_schur<L><<<grid_schur, BS_THREADS, shm_schur, st>>>(d_probs, d_tasks, ntasks, d_req, nmax, mmax, fa_r,
-                                                                    fa_re, fa_rs, fa_g, fa_ge, fa_gs, fx_v, fx_e, fx_s,
+                                                                    fa_re, fa_rs, fa_g, fa_ge, fa_ps, fx_v, fx_e, fx_s,
                                                                     fz_v, fz_e, fz_s, PV{ws_schur, ws_n});
         k_batch_potrf<L><<<grid_prob, BPOTRF_THREADS, 0, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n});
         k_batch_potrs<L><<<grid_prob, 256, shm_potrs, st>>>(d_probs, nprob, d_req, d_info, PV{cs, cs_n}, mmax);
