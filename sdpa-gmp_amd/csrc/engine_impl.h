@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "chol_fx.h"
 #include "engine_common.h"
 #include "gemm.h"
 #include "host_la.h"
@@ -665,130 +666,149 @@ __global__ __launch_bounds__(128) void k_pairs_fx(int npairs, const Pair *pairs,
 // mirror: B[j,i] = B[i,j] is maintained by bacc; nothing else needed.
 
 // ---- Cholesky (Rpotrf lower), blocked right-looking ----
-// k_potrf_diag: factors the nb x nb diagonal block at (k, k) (nb <= 32) and forms its inverse Linv
-// (nb x nb, leading dimension nb), with POTRF_WG workgroups of up to 1024 threads.
-//
-// Division-free elimination (the Bareiss form of LDL^T), so that the serial chain of pivots
-// carries no square root and no reciprocal.  With S^(j) the Schur complement after j pivots
-// and c_j > 0 a running scale, the block holds At^(j) = c_j S^(j) and step j does, for every
-// i > j at once,
-//     At_ic <- 2^-e_j (d_j At_ic - At_ij At_cj)     (j < c <= i)
-//     Xt_ic <- 2^-e_j (d_j Xt_ic - At_ij Xt_jc)     (c <= j, and c = i where Xt_ji = 0)
-// with d_j = At_jj and e_j its binary exponent (an exact rescaling that keeps c_j in [2^-j, 1]);
-// Xt starts as the identity and row i ends as c_i times row i of inv(unit-lower factor), and
-// Xt_ii = c_i.  The true pivot is S_jj = d_j / c_j, so with r_j = 1/sqrt(c_j d_j), formed for
-// all j in parallel after the chain,
-//     L_ic = At_ic r_c,   inv(L)_ic = Xt_ic r_i,   1 / L_jj = c_j r_j.
-// Each step is one multiply-multiply-subtract per entry (rounded once) and one barrier; r_j is one fixed-point
-// Newton reciprocal square root (mpx::rsqrt_fx).
-// A step's entries cost one update each on the workgroup's four SIMDs, in rounds of 256, so the step
-// time is set by its entry count.  The columns of Xt evolve independently of one another (column c
-// reads only itself and the pivot columns of At), so every workgroup runs the whole At chain
-// (identical in all of them) and every diagonal entry Xt_ii, but only the columns c = wg (mod
-// POTRF_WG) of Xt, and writes only those columns of inv(L): 44 instead of 63 rounds for nb = 32,
-// with no communication between the workgroups.  The last workgroup to finish writes L and the
-// pivots (by then every workgroup has read the block).  The reference divides each column by L_jj
-// after a square root per pivot (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:264-270); the
-// factor is the same up to rounding.  Since c_j > 0, d_j <= 0 exactly when the pivot S_jj <= 0:
-// info receives the first failing global pivot (1-based, as Rpotrf2's info); once info is set,
-// later launches return at once.
-// LDS (dynamic): At and Xt as (L+1) planes of nb (nb+1) words each, then r_j as (L+1) planes of nb.
-template <int L>
-__device__ __forceinline__ mp<L> fms_scaled(const mp<L> &a, const mp<L> &b, const mp<L> &c, const mp<L> &d,
-                                            int32_t e) {
-    mp<L> v = fms<L>(a, b, c, d);  // one rounding of a b - c d
-    if (!is_zero<L>(v)) v.e -= e;
-    return v;
+// info <- v unless a smaller failing pivot is already there (0 = none); several workgroups may report
+__device__ __forceinline__ void info_min(int *info, int v) {
+    int cur = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (cur == 0 || v < cur) {
+        if (__hip_atomic_compare_exchange_strong(info, &cur, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            break;
+    }
 }
 
-constexpr int POTRF_WG = 8;  // workgroups of k_potrf_diag (the inverse's columns are split over them)
-
+// LDS words of k_potrf_col for a workgroup of R panel rows
 template <int L>
-__global__ __launch_bounds__(1024) void k_potrf_diag(PV A, int64_t off, int ld, int k, int nb, PV rdiag, int64_t roff,
-                                                     PV Linv, int64_t loff, int *info, int *done) {
+constexpr size_t potrf_col_words(int nb, int R) {
+    const size_t ldv = (size_t)nb + R, r1 = R ? R : 1;
+    return ((size_t)ldv * nb + R) * ColElim<L>::SW + (size_t)nb * (nb - 1) / 2 + (size_t)R * nb +
+           2 * (size_t)(L + 1) * nb + (size_t)(nb + 1) * ldv + ldv + r1 + (size_t)nb * ldv;
+}
+
+// k_potrf_col: eliminates the block column at (k, k) of the n x n matrix A (Rpotrf lower): the nb x nb
+// diagonal block (nb <= 32) and the r rows below it, in block fixed point (chol_fx.h, ColElim: one division-free
+// step per pivot, two fixed-point products per entry).  Workgroup w takes the panel rows [w R, w R + R) and
+// runs the diagonal block's elimination itself (the same arithmetic in every workgroup, so they exchange
+// nothing); it writes its panel rows' factor entries over A.  The last workgroup to finish writes the
+// block's factor entries and the reciprocal pivots 1 / L_jj (by then every workgroup has read the block), and
+// clears the count for the next launch; each workgroup counts once on every path.  The reference factors the
+// diagonal block with a square root and a division per pivot and solves the panel (mplapack/Rpotrf2.cpp:86-97,
+// mplapack/Rtrsm.cpp:263-285); the factor is the same to rounding.  info: the first failing pivot (1-based,
+// as Rpotrf2's), or for an entry out of the fixed-point range (impossible for a positive definite matrix) the
+// first pivot not yet confirmed; once info is set, later launches do nothing.
+template <int L>
+__global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, int k, int nb, int r, int R, PV rdiag,
+                                                    int64_t roff, int *info, int *done) {
     extern __shared__ uint32_t shm[];
-    __shared__ int s_flag;
+    __shared__ int s_flag, s_ovf;
     const int tid = threadIdx.x, nt = blockDim.x, K = gridDim.x, wg = blockIdx.x;
-    // LDS leading dimension nb + 1 (odd): a wave touching consecutive columns of one row, or
-    // consecutive rows of one column, hits distinct banks
-    const int lq = nb + 1, P = nb * lq;
-    uint32_t *sA = shm, *sX = shm + (L + 1) * P, *sR = shm + 2 * (L + 1) * P;
-    auto at = [&](int i, int j) -> int64_t { return off + (int64_t)(k + i) + (int64_t)ld * (k + j); };
-    if (tid == 0) s_flag = *info;
+    const int p0 = wg * R, Rw = r > p0 ? min(R, r - p0) : 0;
+    const int LDV = nb + Rw, R1 = Rw ? Rw : 1;
+    using CE = ColElim<L>;
+    uint32_t *sV = shm, *sG = sV + (size_t)(LDV * nb + Rw) * CE::SW, *sRh = sG + (size_t)(L + 1) * nb;
+    int32_t *sE = (int32_t *)(sRh + (size_t)(L + 1) * nb), *hist = sE + nb * (nb - 1) / 2 + Rw * nb,
+            *f0 = hist + (nb + 1) * LDV, *pm = f0 + LDV, *Fpre = pm + R1;
+    const CE ce{sV, sG, sRh, sE, hist, f0, pm, Fpre, nb, Rw, LDV};
+    // element (i, j) of the block column (rows 0..nb-1 the block, nb + p this workgroup's panel row p)
+    auto at = [&](int i, int j) -> int64_t {
+        const int gi = i < nb ? i : nb + p0 + (i - nb);
+        return off + (int64_t)(k + gi) + (int64_t)ld * (k + j);
+    };
+    auto atd = [&](int p) -> int64_t { return at(nb + p, nb + p0 + p); };  // panel row p's diagonal
+    if (tid == 0) {
+        s_flag = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ovf = 0;
+    }
+    for (int t = tid; t < (nb + 1) * LDV; t += nt) hist[t] = mpx::FX_HZERO;
     __syncthreads();
     bool ok = s_flag == 0;
     if (ok) {
-        for (int t = tid; t < nb * nb; t += nt) {
-            int i = t % nb, j = t / nb;
-            lds_st<L>(sA, P, i + lq * j, i >= j ? ldm<L>(A, at(i, j)) : zero<L>());
-            lds_st<L>(sX, P, i + lq * j, i == j ? from_int<L>(1) : zero<L>());
+        for (int t = tid; t < nb; t += nt) ce.block_scale(t, ldm<L>(A, at(t, t)));
+        for (int t = tid; t < Rw; t += nt) ce.panel_scale0(t, ldm<L>(A, atd(t)));
+        for (int t = tid; t < ce.nitems(); t += nt) ce.set_item(t);
+    }
+    __syncthreads();
+    if (ok)
+        for (int t = tid; t < Rw * nb; t += nt) ce.panel_rowmax(t % Rw, t / Rw, ldm<L>(A, at(nb + t % Rw, t / Rw)));
+    __syncthreads();
+    if (ok)
+        for (int t = tid; t < Rw; t += nt) ce.panel_scale(t);
+    __syncthreads();
+    if (ok) {
+        int ov = 0;
+        for (int t = tid; t < nb * nb + Rw * nb; t += nt) {
+            int row, c;
+            if (t < nb * nb) {
+                row = t % nb;
+                c = t / nb;
+                if (row < c) continue;
+            } else {
+                row = nb + (t - nb * nb) % Rw;
+                c = (t - nb * nb) / Rw;
+            }
+            if (ce.load_entry(row, c, ldm<L>(A, at(row, c)))) ov |= row < nb ? 1 : 2;
         }
-        __syncthreads();
+        for (int t = tid; t < Rw; t += nt)
+            if (ce.load_pdiag(t, ldm<L>(A, atd(t)))) ov |= 2;
+        if (ov) atomicOr(&s_ovf, ov);
+    }
+    __syncthreads();
+    if (ok) {
+        const int nit = ce.nitems();
         for (int j = 0; j < nb; j++) {
-            // every thread reads the same pivot, so the failure exit is uniform
-            const mp<L> d = lds_ld<L>(sA, P, j + lq * j);
-            if (sgn<L>(d) <= 0) {
-                if (wg == 0 && tid == 0) *info = k + j + 1;
+            // the pivot is the same for every thread (scalar registers), and so is the failure exit
+            int32_t a[CE::N];
+            ce.ld(j + LDV * j, a);
+#pragma unroll
+            for (int q = 0; q < CE::N; q++) a[q] = __builtin_amdgcn_readfirstlane(a[q]);
+            if ((s_ovf & 1) || mpx::fx_nonpos<CE::N>(a)) {
+                if (tid == 0) info_min(info, k + j + 1);
                 ok = false;
                 break;
             }
-            const int n1 = nb - 1 - j;
-            // this workgroup's columns of Xt: c = wg (mod K), c <= j
-            const int mc = wg <= j ? (j - wg) / K + 1 : 0;
-            const int nA = n1 * (n1 + 1) / 2, nX = n1 * mc;
-            const int t = tid;
-            if (t < nA + nX + n1) {
-                // one code path for both kinds of entries (no divergence inside a wave): the updated
-                // value U, At_ij and the third operand T, with U <- 2^-e (d U - At_ij T)
-                int i, c;
-                uint32_t *pU;
-                int iT;
-                if (t < nA) {
-                    // lower triangle of the trailing block, row-major: t = ip (ip + 1) / 2 + cp, cp <= ip
-                    int ip = (int)((__builtin_sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-                    if ((ip + 1) * (ip + 2) / 2 <= t) ip++;
-                    if (ip * (ip + 1) / 2 > t) ip--;
-                    i = j + 1 + ip;
-                    c = j + 1 + (t - ip * (ip + 1) / 2);
-                    pU = sA;
-                    iT = c + lq * j;  // At_cj
-                } else {
-                    // Xt_ic for the own columns c <= j, then every diagonal Xt_ii, whose T = Xt_ji is an
-                    // upper entry (zero)
-                    const int u = t - nA;
-                    if (u < nX) {
-                        i = j + 1 + u / mc;
-                        c = wg + K * (u % mc);
-                    } else {
-                        i = j + 1 + (u - nX);
-                        c = i;
-                    }
-                    pU = sX;
-                    iT = j + lq * c + (L + 1) * P;  // Xt_jc: offset from sA to sX folded in
-                }
-                const int iU = i + lq * c;
-                mp<L> v = fms_scaled<L>(d, lds_ld<L>(pU, P, iU), lds_ld<L>(sA, P, i + lq * j), lds_ld<L>(sA, P, iT), d.e);
-                lds_st<L>(pU, P, iU, v);
-            }
+            if (j == nb - 1) break;
+            const int32_t dj = __builtin_amdgcn_readfirstlane(mpx::fx_delta(hist[j * LDV + j]));
+            int ov = 0;
+            for (int t = tid; t < nit; t += nt) ov |= ce.update(j, t, a, dj);
+            if (ov) atomicOr(&s_ovf, ov);
             __syncthreads();
         }
     }
     if (ok) {
-        if (tid < nb) {
-            const mp<L> cj = lds_ld<L>(sX, P, tid + lq * tid), dj = lds_ld<L>(sA, P, tid + lq * tid);
-            lds_st<L>(sR, nb, tid, mpx::rsqrt_fx<L>(mul<L>(cj, dj)));
-        }
+        // row scales per column, normalised pivots, their exclusive prefix products gamma_c (scan in LDS,
+        // sG and sRh alternating), rho_c, 1 / L_cc
+        for (int t = tid; t < LDV; t += nt) ce.prefix_scales(t);
+        for (int t = tid; t < nb; t += nt) ce.pivot_value(t);
         __syncthreads();
-        // this workgroup's columns of inv(L)
-        for (int t = tid; t < nb * nb; t += nt) {
-            int i = t % nb, c = t / nb;
-            if (c % K != wg) continue;
-            stm<L>(Linv, loff + t, i >= c ? mul<L>(lds_ld<L>(sX, P, i + lq * c), lds_ld<L>(sR, nb, i)) : zero<L>());
+        mp<L> vh, x;
+        if (tid < nb) vh = ce.gld(sG, tid);
+        uint32_t *cur = sG, *nxt = sRh;
+        for (int d = 1; d < nb; d <<= 1) {
+            if (tid < nb) {
+                x = ce.gld(cur, tid);
+                if (tid >= d) x = mul<L>(ce.gld(cur, tid - d), x);
+            }
+            __syncthreads();
+            if (tid < nb) ce.gst(nxt, tid, x);
+            __syncthreads();
+            uint32_t *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+        if (tid < nb) x = tid == 0 ? from_int<L>(1) : ce.gld(cur, tid - 1);
+        __syncthreads();
+        if (tid < nb) ce.gst(sG, tid, x);
+        __syncthreads();
+        if (tid < nb) ce.gst(sG, tid, ce.finish_pivot(tid, vh));  // sG[c] <- 1 / L_cc once gamma_c is read
+        __syncthreads();
+        if (s_ovf & 2) {
+            if (tid == 0) info_min(info, k + nb + 1);
+        } else {
+            for (int t = tid; t < Rw * nb; t += nt) {
+                const int p = t % Rw, c = t / Rw;
+                stm<L>(A, at(nb + p, c), ce.factor_entry(nb + p, c));
+            }
         }
     }
-    // Every workgroup has read the block by now; the last one to finish (all have the same A chain in
-    // LDS) writes L over it and the pivots, and clears the count for the next launch.  Each workgroup
-    // counts once on every path, so the count always returns to zero.
     __syncthreads();
     if (tid == 0) {
         const int old = __hip_atomic_fetch_add(done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -797,10 +817,46 @@ __global__ __launch_bounds__(1024) void k_potrf_diag(PV A, int64_t off, int ld, 
     }
     __syncthreads();
     if (!(ok && s_flag)) return;
-    if (tid < nb) stm<L>(rdiag, roff + k + tid, mul<L>(lds_ld<L>(sX, P, tid + lq * tid), lds_ld<L>(sR, nb, tid)));
+    for (int t = tid; t < nb; t += nt) stm<L>(rdiag, roff + k + t, ce.gld(sG, t));
     for (int t = tid; t < nb * nb; t += nt) {
-        int i = t % nb, c = t / nb;
-        if (i >= c) stm<L>(A, at(i, c), mul<L>(lds_ld<L>(sA, P, i + lq * c), lds_ld<L>(sR, nb, c)));
+        const int i = t % nb, c = t / nb;
+        if (i >= c) stm<L>(A, at(i, c), ce.factor_entry(i, c));
+    }
+}
+
+// k_blkinv: inverses of the diagonal blocks of a Cholesky factor (n x n in A, blocks of NBK, reciprocal
+// pivots in rdiag), one workgroup per block, all blocks in one launch: with U = L_kk diag(1 / L_cc) (unit
+// lower triangular), inv(U) by the row operations Y_i <- Y_i - U_ij Y_j (j = 0 .. nb-2, i > j) on Y = I,
+// then inv(L_kk)_ic = Y_ic / L_ii.  Block b's inverse goes to Linv at b NBK^2 (leading dimension nb).
+// LDS: U and Y as (L+1) planes of NBK^2 words each.
+template <int L>
+__global__ __launch_bounds__(256) void k_blkinv(PV A, int64_t off, int ld, int n, int NBK, PV rdiag, int64_t roff,
+                                                 PV Linv, const int *info) {
+    extern __shared__ uint32_t shm[];
+    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int tid = threadIdx.x, nt = blockDim.x, b = blockIdx.x, k = b * NBK, nb = min(NBK, n - k);
+    const int P = NBK * NBK;
+    uint32_t *sU = shm, *sY = shm + (size_t)(L + 1) * P;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, j = t / nb;
+        lds_st<L>(sU, P, t, i > j ? mul<L>(ldm<L>(A, off + (k + i) + (int64_t)ld * (k + j)), ldm<L>(rdiag, roff + k + j))
+                                  : zero<L>());
+        lds_st<L>(sY, P, t, i == j ? from_int<L>(1) : zero<L>());
+    }
+    __syncthreads();
+    for (int j = 0; j < nb - 1; j++) {
+        const int n1 = nb - 1 - j;
+        for (int t = tid; t < n1 * (j + 1); t += nt) {
+            const int i = j + 1 + t % n1, c = t / n1;
+            const mp<L> y = lds_ld<L>(sY, P, i + nb * c);
+            lds_st<L>(sY, P, i + nb * c, sub<L>(y, mul<L>(lds_ld<L>(sU, P, i + nb * j), lds_ld<L>(sY, P, j + nb * c))));
+        }
+        __syncthreads();
+    }
+    const int64_t lo = (int64_t)b * P;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, c = t / nb;
+        stm<L>(Linv, lo + t, i >= c ? mul<L>(ldm<L>(rdiag, roff + k + i), lds_ld<L>(sY, P, t)) : zero<L>());
     }
 }
 
@@ -1728,14 +1784,14 @@ struct Engine : EngineBase {
 
     // Cholesky workspaces: inverses of every diagonal block (LinvB for the Schur matrix, LinvW for
     // the blocks of X and Z), the panel below the current block, pivots of X / Z blocks
-    Arr LinvB, LinvW, Pw, rdW;
+    Arr LinvB, LinvW, rdW;
     Arr icX, icZ, Wc, Wt;        // inverse Cholesky factors of X and Z, factor / inverse workspaces
     Arr lq, lqold, lr, lw, ltmp;  // Lanczos vectors
     int64_t *d_diag_idx = nullptr;
     int64_t ndiag = 0, maxn = 1;
     size_t lds_max = 65536;
     int NBK = 32;
-    size_t potrf_shbytes = 0;
+    int col_rmax = 4;
 
     int alloc_factor_space() {
         // diagonal block edge of the blocked Cholesky factorisations (SDPG_NBK = 8, 16 or 32 overrides)
@@ -1744,17 +1800,21 @@ struct Engine : EngineBase {
             const int want = atoi(ev);
             if (want == 8 || want == 16 || want == 32) NBK = want;
         }
-        while (NBK > 8 && ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4 > lds_max) NBK /= 2;
-        potrf_shbytes = ((size_t)2 * NBK * (NBK + 1) + NBK) * (L + 1) * 4;
-        HIPCHK(hipFuncSetAttribute((const void *)k_potrf_diag<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)potrf_shbytes));
+        while (NBK > 8 && std::max(potrf_col_words<L>(NBK, 4), (size_t)2 * (L + 1) * NBK * NBK) * 4 + 256 > lds_max)
+            NBK /= 2;
+        // panel rows per workgroup of the block-column kernel: as many as the LDS holds beside its static words
+        col_rmax = 64;
+        while (col_rmax > 1 && potrf_col_words<L>(NBK, col_rmax) * 4 + 256 > lds_max) col_rmax--;
+        HIPCHK(hipFuncSetAttribute((const void *)k_potrf_col<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(potrf_col_words<L>(NBK, col_rmax) * 4)));
+        HIPCHK(hipFuncSetAttribute((const void *)k_blkinv<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)((size_t)2 * (L + 1) * NBK * NBK * 4)));
         maxn = 1;
         for (int l = 0; l < nsdp; l++) maxn = std::max<int64_t>(maxn, sizes[l]);
         int rc;
         int64_t nbl_m = (m + NBK - 1) / NBK, nbl_n = (maxn + NBK - 1) / NBK;
         if ((rc = aalloc(LinvB, nbl_m * NBK * NBK))) return rc;
         if ((rc = aalloc(LinvW, nbl_n * NBK * NBK))) return rc;
-        if ((rc = aalloc(Pw, std::max<int64_t>(m, maxn) * NBK))) return rc;
         if ((rc = aalloc(rdW, maxn))) return rc;
         if ((rc = aalloc(icX, Nd)) || (rc = aalloc(icZ, Nd))) return rc;
         if ((rc = aalloc(Wc, maxn * maxn)) || (rc = aalloc(Wt, maxn * maxn))) return rc;
@@ -1777,29 +1837,17 @@ struct Engine : EngineBase {
     int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
         int rc;
         for (int k = 0; k < n; k += NBK) {
-            int nb = std::min(NBK, n - k);
-            int64_t lo = (int64_t)(k / NBK) * NBK * NBK;
+            const int nb = std::min(NBK, n - k), r = n - k - nb;
+            // panel rows per workgroup: 4 while at most 256 workgroups hold the panel (the diagonal block's
+            // elimination is repeated in every workgroup), more for larger panels, within the LDS
+            const int R = r > 0 ? std::min(col_rmax, std::max(4, (int)cdiv(r, 256))) : 0;
+            const int G = r > 0 ? (int)cdiv(r, R) : 1;
+            const size_t sh = potrf_col_words<L>(nb, std::min(R, r)) * 4;
             hipEvent_t ev = kprof.begin(st);
-            // one thread per updated entry of the busiest step (150 at nb = 16, 558 at nb = 32)
-            // POTRF_WG workgroups, each with the whole pivot chain and a share of the inverse's columns
-            k_potrf_diag<L><<<POTRF_WG, nb <= 16 ? 256 : 1024, potrf_shbytes, st>>>(A, off, ld, k, nb, rd, roff,
-                                                                                    linv.pv(), lo, d_info, d_done);
-            kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0);
-            if (k + nb < n) {
-                int r = n - k - nb;
-                int64_t a21 = off + (k + nb) + (int64_t)ld * k;
-                // panel below the diagonal block: P = A21 * inv(L11)^T  (Rtrsm R/L/T/N).  The plain and
-                // Ozaki forms convert their operands before the product kernel, so P overwrites A21 in
-                // place; the element form reads its operands directly and goes through Pw.
-                if (gemm_mode == -1) {
-                    if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, Pw.pv(), 0, r, GEMM_SET,
-                                        1, 0, 0, 0, gemm_mode, gemm_slices)))
-                        return rc;
-                    k_copy2d<L><<<cdiv((int64_t)r * nb, 256), 256, 0, st>>>(A, a21, ld, Pw.pv(), 0, r, r, nb);
-                } else if ((rc = gemm.gemm(r, nb, nb, A, a21, ld, false, linv.pv(), lo, nb, true, A, a21, ld, GEMM_SET,
-                                           1, 0, 0, 0, gemm_mode, gemm_slices))) {
-                    return rc;
-                }
+            k_potrf_col<L><<<G, 1024, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, d_info, d_done);
+            kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0 + (double)r * nb * nb / 2.0);
+            if (r > 0) {
+                const int64_t a21 = off + (k + nb) + (int64_t)ld * k;
                 // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
                 if ((rc = gemm.gemm(r, r, nb, A, a21, ld, false, A, a21, ld, true, A,
                                     off + (k + nb) + (int64_t)ld * (k + nb), ld, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
@@ -1807,6 +1855,10 @@ struct Engine : EngineBase {
                     return rc;
             }
         }
+        // the diagonal blocks' inverses (triangular solves, inverse factors), all blocks in one launch
+        const int nblk = (int)cdiv(n, NBK);
+        k_blkinv<L><<<nblk, 256, (size_t)2 * (L + 1) * NBK * NBK * 4, st>>>(A, off, ld, n, NBK, rd, roff, linv.pv(),
+                                                                             d_info);
         HIPCHK(hipGetLastError());
         return 0;
     }

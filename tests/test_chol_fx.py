@@ -1,0 +1,156 @@
+"""Host build of the block-column elimination of the blocked Cholesky (chol_fx.h, run serially through
+libmpxprobe.so in the order the device kernel follows) against exact rational arithmetic.  No GPU needed.
+
+A block column is the nb x nb diagonal block of B and R rows of the panel below it, with those rows'
+diagonal entries.  The factor it returns is checked two ways, exactly:
+  - backward: |(L L^T)_rc - B_rc| <= bar sqrt(|B_rr| B_cc) for every entry r >= c of the block column
+    (the bar is set by the fixed point's 32L - 4 fraction bits; nb steps of truncated products);
+  - the reciprocal pivots: |rdiag_c L_cc - 1| <= 2^-(32L - 8).
+Matrices: random well-conditioned ones, and B = S U D U^T S with U unit lower triangular, pivots D over
+2^-240 .. 2 and row scales S over 2^-120 .. 2^120 (the regime of late interior-point iterates: the Schur
+matrix's pivots spread over many binades and the rows differ in scale).  Failures: B = U D U^T with
+d_f = -1 reports info = f (Rpotrf2's first non-positive pivot, mplapack/Rpotrf2.cpp:86-97) for f in the
+block, and nothing for f among the panel rows (their pivots belong to later blocks).
+"""
+import ctypes
+import math
+import os
+import random
+from fractions import Fraction
+
+import pytest
+
+from test_mpx_host import LIB, _val, _words
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _lib():
+    lib = ctypes.CDLL(LIB)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib.mpx_probe_potrf_col.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, u32p, u32p, u32p,
+                                        ctypes.POINTER(ctypes.c_int)]
+    return lib
+
+
+def _dyadic(rnd, bits, lo=-1, hi=1):
+    """a random dyadic rational in [lo, hi) with the given number of bits"""
+    return Fraction(lo) + Fraction(rnd.getrandbits(bits), 1 << bits) * (hi - lo)
+
+
+def ldl_matrix(n, rnd, pivots, scales):
+    """B = S U D U^T S (exact Fractions, row-major list of lists)"""
+    U = [[Fraction(0)] * n for _ in range(n)]
+    for i in range(n):
+        U[i][i] = Fraction(1)
+        for j in range(i):
+            U[i][j] = _dyadic(rnd, 20, -1, 1) / 4
+    B = [[Fraction(0)] * n for _ in range(n)]
+    for i in range(n):
+        for j in range(i + 1):
+            s = sum(U[i][k] * pivots[k] * U[j][k] for k in range(j + 1))
+            B[i][j] = B[j][i] = s * scales[i] * scales[j]
+    return B
+
+
+def gram_matrix(n, rnd):
+    G = [[_dyadic(rnd, 24) for _ in range(n)] for _ in range(n)]
+    B = [[sum(G[i][k] * G[j][k] for k in range(n)) + (n if i == j else 0) for j in range(n)] for i in range(n)]
+    return B
+
+
+def run(lib, B, nb, R, L):
+    """factor the block column B[:, :nb] (rows 0..nb+R-1) with the panel diagonals B[p][p]"""
+    W = L + 2
+    arr = ctypes.c_uint32 * ((nb + R) * nb * W)
+    A, Lout = arr(), arr()
+    for c in range(nb):
+        for r in range(c, nb + R):
+            A[(r + (nb + R) * c) * W:(r + (nb + R) * c + 1) * W] = _words(B[r][c], L)
+    pd = (ctypes.c_uint32 * max(1, R * W))()
+    for p in range(R):
+        pd[p * W:(p + 1) * W] = _words(B[nb + p][nb + p], L)
+    rd = (ctypes.c_uint32 * (nb * W))()
+    info = ctypes.c_int(-1)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    cast = lambda x: ctypes.cast(x, u32p)  # noqa: E731
+    assert lib.mpx_probe_potrf_col(L, nb, R, cast(A), cast(pd), cast(Lout), cast(rd), ctypes.byref(info)) == 0
+    Lm = {}
+    if info.value == 0:
+        for c in range(nb):
+            for r in range(c, nb + R):
+                Lm[r, c] = _val(Lout[(r + (nb + R) * c) * W:(r + (nb + R) * c + 1) * W], L)[0]
+    rdv = [_val(rd[c * W:(c + 1) * W], L)[0] for c in range(nb)]
+    return info.value, Lm, rdv
+
+
+def check_factor(B, Lm, rdv, nb, R, L):
+    worst = Fraction(0)
+    for c in range(nb):
+        for r in range(c, nb + R):
+            s = sum(Lm[r, k] * Lm[c, k] for k in range(c + 1))
+            sc = math.sqrt(abs(float(B[r][r])) * float(B[c][c]))
+            e = abs(s - B[r][c])
+            if e:
+                worst = max(worst, Fraction(float(e) / sc) if sc else e)
+    rworst = max(abs(rdv[c] * Lm[c, c] - 1) for c in range(nb))
+    return float(worst), float(rworst)
+
+
+def backward_bar(L, nb):
+    return 2.0 ** -(32 * L - 4) * 16 * nb
+
+
+@pytest.mark.parametrize("L", [4, 10])
+@pytest.mark.parametrize("nb,R", [(32, 4), (32, 0), (17, 3), (1, 2)])
+def test_well_conditioned(L, nb, R):
+    lib = _lib()
+    rnd = random.Random(1000 * L + nb + R)
+    B = gram_matrix(nb + R, rnd)
+    info, Lm, rdv = run(lib, B, nb, R, L)
+    assert info == 0
+    bw, rw = check_factor(B, Lm, rdv, nb, R, L)
+    print(f"L={L} nb={nb} R={R}: backward {bw:.3e} (bar {backward_bar(L, nb):.3e}), rdiag {rw:.3e}")
+    assert bw <= backward_bar(L, nb)
+    assert rw <= 2.0 ** -(32 * L - 8)
+
+
+@pytest.mark.parametrize("L", [4, 10])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_ill_conditioned_and_scaled(L, seed):
+    lib = _lib()
+    rnd = random.Random(seed)
+    nb, R = 32, 5
+    n = nb + R
+    pivots = [Fraction(2) ** rnd.randint(-240 if L >= 10 else -60, 1) * (1 + _dyadic(rnd, 16, 0, 1)) for _ in range(n)]
+    scales = [Fraction(2) ** rnd.randint(-120, 120) for _ in range(n)]
+    B = ldl_matrix(n, rnd, pivots, scales)
+    info, Lm, rdv = run(lib, B, nb, R, L)
+    assert info == 0
+    bw, rw = check_factor(B, Lm, rdv, nb, R, L)
+    print(f"L={L} seed={seed}: backward {bw:.3e} (bar {backward_bar(L, nb):.3e}), rdiag {rw:.3e}")
+    assert bw <= backward_bar(L, nb)
+    assert rw <= 2.0 ** -(32 * L - 8)
+    # the pivots themselves: L_cc^2 = d_c s_c^2 up to the cancellation B_cc / (d_c s_c^2) that forming
+    # the pivot costs (any backward-stable elimination loses that much)
+    for c in range(nb):
+        rel = abs(Lm[c, c] ** 2 / (pivots[c] * scales[c] ** 2) - 1)
+        bar = 2.0 ** -(32 * L - 16) * nb * float(B[c][c] / (pivots[c] * scales[c] ** 2))
+        assert rel <= bar, (c, float(rel), bar)
+
+
+@pytest.mark.parametrize("fail_at", [1, 2, 9, 32, 33, 36])
+def test_failing_pivot(fail_at):
+    lib = _lib()
+    L = 10
+    rnd = random.Random(fail_at)
+    nb, R = 32, 4
+    n = nb + R
+    pivots = [Fraction(1 + rnd.randint(0, 2)) for _ in range(n)]
+    pivots[fail_at - 1] = Fraction(-1)
+    B = ldl_matrix(n, rnd, pivots, [Fraction(1)] * n)
+    info, Lm, rdv = run(lib, B, nb, R, L)
+    assert info == (fail_at if fail_at <= nb else 0)
+    if info == 0:
+        bw, _ = check_factor(B, Lm, rdv, nb, R, L)
+        assert bw <= backward_bar(L, nb)
