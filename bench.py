@@ -11,17 +11,19 @@ times over the whole trajectory are reported), then a second run stops at iterat
 and a "step" = one Make_bMat + Cholesky (sdpg_schur + sdpg_potrf) on that iterate, resident in
 HBM, is run `--warmup` times untimed and `--steps` times timed.  value = wall seconds per step of
 the Ozaki form (slice count from the autotuner), timed with nothing but the step's own kernels in the
-stream; the per-kernel times (events around every product and diagonal-block launch) come from a
+stream; the per-kernel times (events around every product and block-column launch) come from a
 second pass over the same number of steps.
 
-roofline: the dominant kernel of the step, the Cholesky diagonal-block kernel k_potrf_diag (a serial
-chain of pivots in 320-bit limb arithmetic on the vector ALUs): algorithmic work = the reference's
-nb^3/6 multiple-precision multiply-adds per 32 x 32 diagonal block (Rpotrf2's work), at L^2 = 64 limb
-multiply-adds each for a 256-bit request (L = 8 limbs of the request), over its event-timed average
-launch, against the measured integer-VALU peak (sdpg_valu_peak: sustained v_mad_u64_u32 + v_addc
-limb multiply-adds of the whole device).  kernel_rooflines: the short-K Ozaki product kernel
-(INT8 peak), the plain limb product kernel (VALU peak) and, on the dense rank-2 workload, the long-K
-Ozaki kernel (INT8 peak).
+roofline: the dominant kernel of the step, the Cholesky block-column kernel k_potrf_col (one launch per
+32-column block column: the diagonal block and the panel rows below it eliminated by the division-free
+recurrence in a fixed point of signed 29-bit limbs, the diagonal block's elimination repeated in every
+workgroup; a serial chain of 32 pivots per launch on the vector ALUs): algorithmic work = the reference's
+multiple-precision multiply-adds for that block column, nb^3/6 (Rpotrf2 on the diagonal block) + r nb^2/2
+(Rtrsm on the r panel rows), at L^2 = 64 limb multiply-adds each for a 256-bit request (L = 8 limbs of the
+request), over its event-timed average launch, against the measured integer-VALU peak (sdpg_valu_peak:
+sustained v_mad_u64_u32 + v_addc limb multiply-adds of the whole device).  kernel_rooflines: the short-K
+Ozaki product kernel (INT8 peak), the plain limb product kernel (VALU peak) and, on the dense rank-2
+workload, the long-K Ozaki kernel (INT8 peak).
 
 also_measured: the GEMM-heavy workload 3 (one block n = 150, m = 1326 dense rank-2 constraints, all F1)
 replayed on its first iterate for both forms at 256, 320 and 200 bits (Ozaki slice counts from the
@@ -288,15 +290,16 @@ def main():
     kst = res[main_form]["_kst"]
     shares = {k: v[1] / a.steps / step_ms for k, v in kst.items()}
     dom = max(shares, key=shares.get)
-    # the mp multiply-adds of one launch are nb^3/6 (Rpotrf2's work on a 32 x 32 block); 64 limb
-    # multiply-adds each for a 256-bit request (8 limbs)
+    # the mp multiply-adds of a launch are nb^3/6 + r nb^2/2 (Rpotrf2 on the diagonal block and Rtrsm on the panel
+    # rows); 64 limb multiply-adds each for a 256-bit request (8 limbs)
     limbs_req = 8
-    rl = valu_roofline(kst["chol_diag"], "k_potrf_diag (Cholesky 32 x 32 diagonal blocks: Bareiss pivot chain "
-                                         "and block inverse, 8 workgroups)", peak, limbs_req * limbs_req,
-                       load_traffic("k_potrf_diag_maxcut1000_traffic.json"),
-                       "work = nb^3/6 mp multiply-adds per block x 64 limb multiply-adds (256-bit request); "
-                       "a serial chain of 32 pivots per launch, each one fused multiply-multiply-subtract per "
-                       "entry in 320-bit limb arithmetic: latency-bound, not throughput-bound")
+    rl = valu_roofline(kst["chol_col"], "k_potrf_col (Cholesky 32-column block columns: the diagonal block and the "
+                                        "panel rows below it, division-free elimination in signed 29-bit-limb "
+                                        "fixed point)", peak, limbs_req * limbs_req,
+                       load_traffic("k_potrf_col_maxcut1000_traffic.json"),
+                       "work = nb^3/6 + r nb^2/2 mp multiply-adds per block column x 64 limb multiply-adds (256-bit "
+                       "request); a serial chain of 32 pivots per launch, one update latency per pivot: "
+                       "latency-bound, not throughput-bound")
     if rl:
         out["roofline"] = rl
     kr = {}

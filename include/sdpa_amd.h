@@ -71,7 +71,7 @@ int sdpa_amd_solve(const char *data_path, const char *param_path, int prec_overr
  * `warmup` untimed ones, with nothing but the step's kernels in the stream.
  * Returns per-step device milliseconds of each phase and the wall time.
  * kstats (may be NULL, else 12 doubles): sdpg_profile figures over a second
- * pass of `steps` steps with events around the product and diagonal-block
+ * pass of `steps` steps with events around the product and block-column
  * kernels.  dump_prefix (may be NULL): the iterate is written to
  * <dump_prefix>X.bin and <dump_prefix>invZ.bin (packed mp, raw uint32) so that a
  * CPU implementation can time the same step on the same data.

@@ -193,9 +193,10 @@ int sdpg_sync(sdpg_ctx *ctx);
 /*
  * Per-kernel device time (no reference counterpart; measurement support):
  * on = 1 clears and starts recording events around each launch of the product
- * and diagonal-block kernels, on = 0 stops.  out (may be NULL) receives, for each
+ * and Cholesky block-column kernels, on = 0 stops.  out (may be NULL) receives, for each
  * kind k with 3k + 2 < n (n = 12 for all four) = 0 Ozaki MFMA product (short K),
- * 1 plain limb product, 2 Cholesky diagonal block, 3 Ozaki MFMA product (long K):
+ * 1 plain limb product, 2 Cholesky block column (diagonal block and panel rows),
+ * 3 Ozaki MFMA product (long K):
  * out[3k] launches, out[3k+1] milliseconds, out[3k+2] algorithmic work
  * (k = 0, 3: INT8 operations; 1, 2: mp multiply-adds).  The events sit in the
  * engine's stream between the kernels and lengthen the wall time a little.

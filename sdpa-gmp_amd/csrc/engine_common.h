@@ -56,8 +56,8 @@ struct Arena {
 
 // Per-kernel device time over a measured region: events recorded on the launching stream around
 // each launch of the kernels of interest, summed after the region ends.
-// kinds: 0 Ozaki MFMA product kernel (short K), 1 plain limb product kernel, 2 Cholesky diagonal
-// block, 3 Ozaki MFMA product kernel (long K)
+// kinds: 0 Ozaki MFMA product kernel (short K), 1 plain limb product kernel, 2 Cholesky block column
+// (diagonal block and panel rows), 3 Ozaki MFMA product kernel (long K)
 struct KProf {
     static constexpr int NK = 4;
     bool on = false;

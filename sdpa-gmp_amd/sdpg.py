@@ -566,6 +566,6 @@ def bench(data_path: str, param_path: str | None, prec: int, gemm: int, slices: 
                             dump_prefix.encode() if dump_prefix else None)
     if rc:
         raise SdpgError(f"sdpa_amd_bench rc={rc}: {lib.sdpg_last_error().decode()}")
-    kinds = ("ozaki_mfma", "plain_limb", "chol_diag", "ozaki_mfma_longk")
+    kinds = ("ozaki_mfma", "plain_limb", "chol_col", "ozaki_mfma_longk")
     kst = {k: (int(ks[3 * i]), ks[3 * i + 1], ks[3 * i + 2]) for i, k in enumerate(kinds)}
     return a.value, b.value, c.value, kst
