@@ -586,8 +586,8 @@ __global__ __launch_bounds__(256) void k_fx_point(const BatchProb *__restrict__ 
 }
 
 // Rpotrf("Lower") of one problem's Schur matrix per workgroup (choleskyFactorWithAdjust,
-// sdpa_linear.cpp:608-631), unblocked and division-free as the diagonal blocks of the single-problem
-// engine: with c_j > 0 a running scale, At = c_j S^(j) and step j updates every remaining entry of the lower
+// sdpa_linear.cpp:608-631), unblocked and division-free (the Bareiss form of L D L^T, in mp arithmetic with
+// one rounding per update): with c_j > 0 a running scale, At = c_j S^(j) and step j updates every remaining entry of the lower
 // triangle at once, At_ic <- 2^-e_j (d_j At_ic - At_ij At_cj) (d_j = At_jj, e_j its exponent); the true
 // pivot is d_j / c_j and c_{j+1} = 2^-e_j c_j d_j.  After the chain, r_j = 1/sqrt(c_j d_j) gives
 // L_ic = At_ic r_c (so L_jj = d_j r_j) and 1/L_jj = c_j r_j.  info = the first non-positive pivot
