@@ -356,9 +356,10 @@ MPX_HD void fx_amax(int32_t *p, int32_t v) {
 //         rows 0..nb-1 the block, nb..nb+R-1 the panel rows, columns 0..nb-1; the panel rows' diagonals
 //         follow at (LDV nb + p) SW
 //   E     the work of a step, one int per item (row | col << 16; col = 0xFFFF for a panel diagonal):
-//         the block's lower triangle column by column without the diagonal block's first column, then the
-//         panel rows' entries column by column, then the panel diagonals; step j updates the items whose
-//         column exceeds j (a panel diagonal at every step but the last)
+//         the panel diagonals first, then the columns from the last to the second, each with its block
+//         entries and its panel entries; step j updates the items whose column exceeds j (a panel diagonal
+//         at every step but the last), a prefix of this order, so that the items still active fill whole
+//         waves (the device kernel keeps its one item per lane in registers and needs no E)
 //   hist  (nb + 1) rows of LDV ints, all FX_HZERO at first: hist[j * LDV + r] sets row r's renormalisation
 //         entering step j (d_r = fx_delta of it): for a block row the leading exponent h of its diagonal;
 //         for a panel row the larger of that and 2 h + 1 over its entries, so that a panel row whose own
@@ -448,23 +449,43 @@ struct ColElim {
     MPX_HD int nitems() const {
         return nb * (nb - 1) / 2 + R * (nb - 1) + R;
     }
-    MPX_HD void set_item(int t) const {
-        const int nbk = nb * (nb - 1) / 2, npn = R * (nb - 1);
-        int row, col;
-        if (t < nbk) {
-            // column c (1..nb-1) holds rows c..nb-1: columns before c hold (c - 1)(2 nb - c) / 2 items
-            int c = 1;
-            while (c < nb - 1 && (c * (2 * nb - c - 1)) / 2 <= t) c++;
-            col = c;
-            row = c + (t - ((c - 1) * (2 * nb - c)) / 2);
-        } else if (t < nbk + npn) {
-            col = 1 + (t - nbk) / R;
-            row = nb + (t - nbk) % R;
-        } else {
-            row = nb + (t - nbk - npn);
-            col = 0xFFFF;
+    // item t: the R panel diagonals, then column c = nb-1 down to 1 with its nb - c block entries (rows
+    // c..nb-1) and its R panel entries
+    MPX_HD int item_code(int t) const {
+        if (t < R) return (nb + t) | (0xFFFF << 16);
+        t -= R;
+        int c = nb - 1;
+        while (c > 1 && t >= nb - c + R) {
+            t -= nb - c + R;
+            c--;
         }
-        E[t] = row | (col << 16);
+        const int row = t < nb - c ? c + t : nb + (t - (nb - c));
+        return row | (c << 16);
+    }
+    MPX_HD void set_item(int t) const {
+        E[t] = item_code(t);
+    }
+    // entry t of the block column as loaded at set-up: the block's lower triangle column by column
+    // (nb (nb + 1) / 2), the panel entries (R nb, column by column), the R panel diagonals.  Returns the
+    // row and sets the column (-1 for a panel diagonal).
+    MPX_HD int entry_code(int t, int &col) const {
+        const int nbl = nb * (nb + 1) / 2;
+        if (t < nbl) {
+            int c = 0;
+            while (t >= nb - c) {
+                t -= nb - c;
+                c++;
+            }
+            col = c;
+            return c + t;
+        }
+        t -= nbl;
+        if (t < R * nb) {
+            col = t / R;
+            return nb + t % R;
+        }
+        col = -1;
+        return nb + (t - R * nb);
     }
 
     // ---- elimination ----
@@ -477,7 +498,10 @@ struct ColElim {
     // item t at step j with the pivot a = V_jj and dj its renormalisation exponent.  Returns 0 (also for an
     // item not updated at this step), 1 (a block entry out of range) or 2 (a panel entry out of range).
     MPX_HD int update(int j, int t, const int32_t (&a)[N], int32_t dj) const {
-        const int32_t it = E[t];
+        return update_item(j, E[t], a, dj);
+    }
+    // the same for the item whose code (item_code) is it
+    MPX_HD int update_item(int j, int32_t it, const int32_t (&a)[N], int32_t dj) const {
         const int row = it & 0xFFFF, col = (int)((uint32_t)it >> 16);
         if (col <= j || (col == 0xFFFF && j == nb - 1)) return 0;
         const int32_t *hj = hist + j * LDV;
@@ -499,6 +523,12 @@ struct ColElim {
     }
 
     // ---- after the chain ----
+    // f_r^(c) alone (c <= r for a block row), into Fpre
+    MPX_HD void prefix_scale(int r, int c) const {
+        int32_t f = f0[r];
+        for (int q = 0; q < c; q++) f += fx_delta(hist[q * LDV + r]);
+        Fpre[c * LDV + r] = f;
+    }
     // row r: f_r^(c) for every column c it needs (block rows: c <= r; panel rows: all)
     MPX_HD void prefix_scales(int r) const {
         int32_t f = f0[r];
@@ -508,11 +538,15 @@ struct ColElim {
             f += fx_delta(hist[c * LDV + r]);
         }
     }
-    // normalised pivot Vh_c = V_cc 2^-(2 d_c) as an mp value, into Gm[c]
-    MPX_HD void pivot_value(int c) const {
+    // normalised pivot Vh_c = V_cc 2^-(2 d_c) as an mp value
+    MPX_HD mp<L> pivot_mp(int c) const {
         int32_t x[N];
         ld(c + LDV * c, x);
-        gst(Gm, c, fx_to_mp<L>(x, -2 * fx_delta(hist[c * LDV + c])));
+        return fx_to_mp<L>(x, -2 * fx_delta(hist[c * LDV + c]));
+    }
+    // the same into Gm[c]
+    MPX_HD void pivot_value(int c) const {
+        gst(Gm, c, pivot_mp(c));
     }
     // rho_c = 1/sqrt(gamma_c Vh_c) into Rh[c] (Gm holds the exclusive prefix products gamma_c, Vh in vh);
     // returns 1 / L_cc

@@ -680,40 +680,63 @@ __device__ __forceinline__ void info_min(int *info, int v) {
 template <int L>
 constexpr size_t potrf_col_words(int nb, int R) {
     const size_t ldv = (size_t)nb + R, r1 = R ? R : 1;
-    return ((size_t)ldv * nb + R) * ColElim<L>::SW + (size_t)nb * (nb - 1) / 2 + (size_t)R * nb +
-           2 * (size_t)(L + 1) * nb + (size_t)(nb + 1) * ldv + ldv + r1 + (size_t)nb * ldv;
+    return ((size_t)ldv * nb + R) * ColElim<L>::SW + 2 * (size_t)(L + 1) * nb + (size_t)(nb + 1) * ldv + ldv + r1 +
+           (size_t)nb * ldv;
+}
+// items (entries updated by the steps) and entries (loaded at set-up) of a workgroup of R panel rows
+constexpr int potrf_col_items(int nb, int R) {
+    return nb * (nb - 1) / 2 + R * nb;
+}
+constexpr int potrf_col_entries(int nb, int R) {
+    return nb * (nb + 1) / 2 + R * (nb + 1);
+}
+// threads of a k_potrf_col workgroup: one lane per item and per entry, and one more wave (the pivots' prefix
+// products); 0 when R rows do not fit 1024 threads
+constexpr int potrf_col_threads(int nb, int R) {
+    const int it = potrf_col_items(nb, R), en = potrf_col_entries(nb, R);
+    const int w = (it + 63) / 64 + 1 > (en + 63) / 64 ? (it + 63) / 64 + 1 : (en + 63) / 64;
+    return 64 * w <= 1024 ? 64 * w : 0;
 }
 
 // k_potrf_col: eliminates the block column at (k, k) of the n x n matrix A (Rpotrf lower): the nb x nb
 // diagonal block (nb <= 32) and the r rows below it, in block fixed point (chol_fx.h, ColElim: one division-free
 // step per pivot, two fixed-point products per entry).  Workgroup w takes the panel rows [w R, w R + R) and
 // runs the diagonal block's elimination itself (the same arithmetic in every workgroup, so they exchange
-// nothing); it writes its panel rows' factor entries over A.  The last workgroup to finish writes the
-// block's factor entries and the reciprocal pivots 1 / L_jj (by then every workgroup has read the block), and
-// clears the count for the next launch; each workgroup counts once on every path.  The reference factors the
-// diagonal block with a square root and a division per pivot and solves the panel (mplapack/Rpotrf2.cpp:86-97,
-// mplapack/Rtrsm.cpp:263-285); the factor is the same to rounding.  info: the first failing pivot (1-based,
-// as Rpotrf2's), or for an entry out of the fixed-point range (impossible for a positive definite matrix) the
-// first pivot not yet confirmed; once info is set, later launches do nothing.
+// nothing); it writes its panel rows' factor entries over A.  Workgroup 0 also writes the block's factor
+// (lower triangle, leading dimension nb) to Lblk at lbo, not over A (other workgroups may still be reading the
+// block), and the reciprocal pivots 1 / L_jj; k_blkinv moves the block into A.  Each thread holds at most one
+// entry at set-up and one item in the steps (blockDim.x = potrf_col_threads(nb, R)); the last wave holds no
+// item and forms the prefix products of the normalised pivots while the others update.  The reference factors
+// the diagonal block with a square root and a division per pivot and solves the panel
+// (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:263-285); the factor is the same to rounding.  info: the
+// first failing pivot (1-based, as Rpotrf2's), or for an entry out of the fixed-point range (impossible for a
+// positive definite matrix) the first pivot not yet confirmed; once info is set, later launches do nothing.
 template <int L>
 __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, int k, int nb, int r, int R, PV rdiag,
-                                                    int64_t roff, int *info, int *done) {
+                                                    int64_t roff, PV Lblk, int64_t lbo, int *info) {
     extern __shared__ uint32_t shm[];
     __shared__ int s_flag, s_ovf;
-    const int tid = threadIdx.x, nt = blockDim.x, K = gridDim.x, wg = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x, wg = blockIdx.x;
     const int p0 = wg * R, Rw = r > p0 ? min(R, r - p0) : 0;
     const int LDV = nb + Rw, R1 = Rw ? Rw : 1;
     using CE = ColElim<L>;
     uint32_t *sV = shm, *sG = sV + (size_t)(LDV * nb + Rw) * CE::SW, *sRh = sG + (size_t)(L + 1) * nb;
-    int32_t *sE = (int32_t *)(sRh + (size_t)(L + 1) * nb), *hist = sE + nb * (nb - 1) / 2 + Rw * nb,
-            *f0 = hist + (nb + 1) * LDV, *pm = f0 + LDV, *Fpre = pm + R1;
-    const CE ce{sV, sG, sRh, sE, hist, f0, pm, Fpre, nb, Rw, LDV};
+    int32_t *hist = (int32_t *)(sRh + (size_t)(L + 1) * nb), *f0 = hist + (nb + 1) * LDV, *pm = f0 + LDV,
+            *Fpre = pm + R1;
+    const CE ce{sV, sG, sRh, nullptr, hist, f0, pm, Fpre, nb, Rw, LDV};
     // element (i, j) of the block column (rows 0..nb-1 the block, nb + p this workgroup's panel row p)
     auto at = [&](int i, int j) -> int64_t {
         const int gi = i < nb ? i : nb + p0 + (i - nb);
         return off + (int64_t)(k + gi) + (int64_t)ld * (k + j);
     };
-    auto atd = [&](int p) -> int64_t { return at(nb + p, nb + p0 + p); };  // panel row p's diagonal
+    // this thread's entry (set-up) and item (steps)
+    const int nent = potrf_col_entries(nb, Rw), nit = ce.nitems();
+    int ecol = 0;
+    const int erow = tid < nent ? ce.entry_code(tid, ecol) : 0;
+    const bool has_item = tid < nit;
+    const int32_t item = has_item ? ce.item_code(tid) : 0;
+    mp<L> ev;
+    if (tid < nent) ev = ldm<L>(A, ecol >= 0 ? at(erow, ecol) : at(erow, nb + p0 + (erow - nb)));
     if (tid == 0) {
         s_flag = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_ovf = 0;
@@ -721,39 +744,24 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
     for (int t = tid; t < (nb + 1) * LDV; t += nt) hist[t] = mpx::FX_HZERO;
     __syncthreads();
     bool ok = s_flag == 0;
-    if (ok) {
-        for (int t = tid; t < nb; t += nt) ce.block_scale(t, ldm<L>(A, at(t, t)));
-        for (int t = tid; t < Rw; t += nt) ce.panel_scale0(t, ldm<L>(A, atd(t)));
-        for (int t = tid; t < ce.nitems(); t += nt) ce.set_item(t);
+    // row scales: the block rows' from their diagonals, the panel rows' from their diagonals and entries
+    if (ok && tid < nent) {
+        if (erow < nb && erow == ecol) ce.block_scale(erow, ev);
+        if (ecol < 0) ce.panel_scale0(erow - nb, ev);
     }
     __syncthreads();
-    if (ok)
-        for (int t = tid; t < Rw * nb; t += nt) ce.panel_rowmax(t % Rw, t / Rw, ldm<L>(A, at(nb + t % Rw, t / Rw)));
+    if (ok && tid < nent && erow >= nb && ecol >= 0) ce.panel_rowmax(erow - nb, ecol, ev);
     __syncthreads();
-    if (ok)
-        for (int t = tid; t < Rw; t += nt) ce.panel_scale(t);
+    if (ok && tid < Rw) ce.panel_scale(tid);
     __syncthreads();
-    if (ok) {
-        int ov = 0;
-        for (int t = tid; t < nb * nb + Rw * nb; t += nt) {
-            int row, c;
-            if (t < nb * nb) {
-                row = t % nb;
-                c = t / nb;
-                if (row < c) continue;
-            } else {
-                row = nb + (t - nb * nb) % Rw;
-                c = (t - nb * nb) / Rw;
-            }
-            if (ce.load_entry(row, c, ldm<L>(A, at(row, c)))) ov |= row < nb ? 1 : 2;
-        }
-        for (int t = tid; t < Rw; t += nt)
-            if (ce.load_pdiag(t, ldm<L>(A, atd(t)))) ov |= 2;
-        if (ov) atomicOr(&s_ovf, ov);
+    if (ok && tid < nent) {
+        const bool o = ecol >= 0 ? ce.load_entry(erow, ecol, ev) : ce.load_pdiag(erow - nb, ev);
+        if (o) atomicOr(&s_ovf, erow < nb ? 1 : 2);
     }
     __syncthreads();
+    const bool gwave = tid >= nt - 64;  // no item: the prefix products gamma_j
     if (ok) {
-        const int nit = ce.nitems();
+        mp<L> g = from_int<L>(1);
         for (int j = 0; j < nb; j++) {
             // the pivot is the same for every thread (scalar registers), and so is the failure exit
             int32_t a[CE::N];
@@ -765,40 +773,25 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
                 ok = false;
                 break;
             }
+            if (gwave && tid == nt - 64) ce.gst(sG, j, g);  // gamma_j
             if (j == nb - 1) break;
             const int32_t dj = __builtin_amdgcn_readfirstlane(mpx::fx_delta(hist[j * LDV + j]));
             int ov = 0;
-            for (int t = tid; t < nit; t += nt) ov |= ce.update(j, t, a, dj);
+            if (has_item) ov = ce.update_item(j, item, a, dj);
+            if (gwave) g = mul<L>(g, fx_to_mp<L>(a, -2 * dj));  // gamma_{j+1} = gamma_j Vh_j
             if (ov) atomicOr(&s_ovf, ov);
             __syncthreads();
         }
     }
     if (ok) {
-        // row scales per column, normalised pivots, their exclusive prefix products gamma_c (scan in LDS,
-        // sG and sRh alternating), rho_c, 1 / L_cc
-        for (int t = tid; t < LDV; t += nt) ce.prefix_scales(t);
-        for (int t = tid; t < nb; t += nt) ce.pivot_value(t);
-        __syncthreads();
-        mp<L> vh, x;
-        if (tid < nb) vh = ce.gld(sG, tid);
-        uint32_t *cur = sG, *nxt = sRh;
-        for (int d = 1; d < nb; d <<= 1) {
-            if (tid < nb) {
-                x = ce.gld(cur, tid);
-                if (tid >= d) x = mul<L>(ce.gld(cur, tid - d), x);
-            }
-            __syncthreads();
-            if (tid < nb) ce.gst(nxt, tid, x);
-            __syncthreads();
-            uint32_t *tmp = cur;
-            cur = nxt;
-            nxt = tmp;
+        // row scales per column, rho_c and 1 / L_cc (gamma_c in sG from the steps)
+        for (int t = tid; t < LDV * nb; t += nt) {
+            const int rr = t % LDV, c = t / LDV;
+            if (c <= (rr < nb ? rr : nb - 1)) ce.prefix_scale(rr, c);
         }
-        if (tid < nb) x = tid == 0 ? from_int<L>(1) : ce.gld(cur, tid - 1);
         __syncthreads();
-        if (tid < nb) ce.gst(sG, tid, x);
-        __syncthreads();
-        if (tid < nb) ce.gst(sG, tid, ce.finish_pivot(tid, vh));  // sG[c] <- 1 / L_cc once gamma_c is read
+        mp<L> rinv;
+        if (tid < nb) rinv = ce.finish_pivot(tid, ce.pivot_mp(tid));  // rho_c into sRh
         __syncthreads();
         if (s_ovf & 2) {
             if (tid == 0) info_min(info, k + nb + 1);
@@ -808,39 +801,43 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
                 stm<L>(A, at(nb + p, c), ce.factor_entry(nb + p, c));
             }
         }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int old = __hip_atomic_fetch_add(done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_flag = old == K - 1;
-        if (s_flag) __hip_atomic_store(done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    if (!(ok && s_flag)) return;
-    for (int t = tid; t < nb; t += nt) stm<L>(rdiag, roff + k + t, ce.gld(sG, t));
-    for (int t = tid; t < nb * nb; t += nt) {
-        const int i = t % nb, c = t / nb;
-        if (i >= c) stm<L>(A, at(i, c), ce.factor_entry(i, c));
+        if (wg == 0) {
+            if (tid < nb) stm<L>(rdiag, roff + k + tid, rinv);
+            for (int t = tid; t < nb * nb; t += nt) {
+                const int i = t % nb, c = t / nb;
+                if (i >= c) stm<L>(Lblk, lbo + t, ce.factor_entry(i, c));
+            }
+        }
     }
 }
 
-// k_blkinv: inverses of the diagonal blocks of a Cholesky factor (n x n in A, blocks of NBK, reciprocal
-// pivots in rdiag), one workgroup per block, all blocks in one launch: with U = L_kk diag(1 / L_cc) (unit
-// lower triangular), inv(U) by the row operations Y_i <- Y_i - U_ij Y_j (j = 0 .. nb-2, i > j) on Y = I,
-// then inv(L_kk)_ic = Y_ic / L_ii.  Block b's inverse goes to Linv at b NBK^2 (leading dimension nb).
-// LDS: U and Y as (L+1) planes of NBK^2 words each.
+// k_blkinv: after the block columns, one workgroup per diagonal block, all blocks in one launch: block b's
+// factor L_kk (written by k_potrf_col to Linv at b NBK^2, leading dimension nb) is copied into A, then
+// replaced in Linv by its inverse: with U = L_kk diag(1 / L_cc) (unit lower triangular; reciprocal pivots in
+// rdiag), inv(U) by the row operations Y_i <- Y_i - U_ij Y_j (j = 0 .. nb-2, i > j) on Y = I, then
+// inv(L_kk)_ic = Y_ic / L_ii.  When a pivot failed (info > 0), only the blocks before the failing pivot's block
+// are copied, and no inverse is formed.  LDS: U and Y as (L+1) planes of NBK^2 words each.
 template <int L>
 __global__ __launch_bounds__(256) void k_blkinv(PV A, int64_t off, int ld, int n, int NBK, PV rdiag, int64_t roff,
                                                  PV Linv, const int *info) {
     extern __shared__ uint32_t shm[];
-    if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int inf = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int tid = threadIdx.x, nt = blockDim.x, b = blockIdx.x, k = b * NBK, nb = min(NBK, n - k);
+    if (inf != 0 && k >= ((inf - 1) / NBK) * NBK) return;
     const int P = NBK * NBK;
+    const int64_t lo = (int64_t)b * P;
     uint32_t *sU = shm, *sY = shm + (size_t)(L + 1) * P;
     for (int t = tid; t < nb * nb; t += nt) {
         const int i = t % nb, j = t / nb;
-        lds_st<L>(sU, P, t, i > j ? mul<L>(ldm<L>(A, off + (k + i) + (int64_t)ld * (k + j)), ldm<L>(rdiag, roff + k + j))
-                                  : zero<L>());
+        if (i < j) continue;
+        const mp<L> l = ldm<L>(Linv, lo + t);
+        stm<L>(A, off + (k + i) + (int64_t)ld * (k + j), l);
+        if (inf == 0) lds_st<L>(sU, P, t, i > j ? mul<L>(l, ldm<L>(rdiag, roff + k + j)) : zero<L>());
+    }
+    if (inf != 0) return;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, j = t / nb;
+        if (i < j) lds_st<L>(sU, P, t, zero<L>());
         lds_st<L>(sY, P, t, i == j ? from_int<L>(1) : zero<L>());
     }
     __syncthreads();
@@ -853,7 +850,6 @@ __global__ __launch_bounds__(256) void k_blkinv(PV A, int64_t off, int ld, int n
         }
         __syncthreads();
     }
-    const int64_t lo = (int64_t)b * P;
     for (int t = tid; t < nb * nb; t += nt) {
         const int i = t % nb, c = t / nb;
         stm<L>(Linv, lo + t, i >= c ? mul<L>(ldm<L>(rdiag, roff + k + i), lds_ld<L>(sY, P, t)) : zero<L>());
@@ -1116,7 +1112,6 @@ struct Engine : EngineBase {
     LPPair *d_lppairs = nullptr;
     int nlppairs = 0;
     int *d_info = nullptr;
-    int *d_done = nullptr;  // completion count of the diagonal-block workgroups (zero between launches)
     std::vector<void *> allocs;
     KProf kprof;
     GemmEngine<L> gemm;
@@ -1416,8 +1411,7 @@ struct Engine : EngineBase {
             if ((rc = aalloc(*a, m))) return rc;
         if ((rc = aalloc(scal, 4096))) return rc;
         if ((rc = aalloc(Bm, (int64_t)m * m))) return rc;
-        if ((rc = dalloc(&d_info, 1)) || (rc = dalloc(&d_done, 1))) return rc;
-        k_fill_i32<<<1, 1, 0, st>>>(d_done, 1, 0);
+        if ((rc = dalloc(&d_info, 1))) return rc;
         for (Arr *a : dls) zero_arr(*a, 0, Nd);
         if ((rc = alloc_factor_space())) return rc;
         if (!bvec.empty() && (rc = put_packed(b, 0, bvec.data(), m))) return rc;
@@ -1830,21 +1824,29 @@ struct Engine : EngineBase {
         return 0;
     }
 
-    // Rpotrf("Lower") of the n x n matrix A[off + i + ld j], blocked right-looking: per block of NBK
-    // columns a one-workgroup factorisation of the diagonal block (which also forms its inverse,
-    // kept in linv at block index k / NBK), the panel below it as a GEMM with that inverse, and the
-    // lower-triangular trailing update as a GEMM.  Pivot failures set d_info (checked by the caller).
+    // Rpotrf("Lower") of the n x n matrix A[off + i + ld j], blocked right-looking: per block of NBK columns
+    // one k_potrf_col launch (the diagonal block and the panel below it; the block's factor to linv at block
+    // index k / NBK) and the lower-triangular trailing update as a GEMM; then k_blkinv moves the blocks' factors
+    // into A and leaves their inverses in linv.  Pivot failures set d_info (checked by the caller).
     int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
         int rc;
         for (int k = 0; k < n; k += NBK) {
             const int nb = std::min(NBK, n - k), r = n - k - nb;
             // panel rows per workgroup: 4 while at most 256 workgroups hold the panel (the diagonal block's
-            // elimination is repeated in every workgroup), more for larger panels, within the LDS
-            const int R = r > 0 ? std::min(col_rmax, std::max(4, (int)cdiv(r, 256))) : 0;
-            const int G = r > 0 ? (int)cdiv(r, R) : 1;
-            const size_t sh = potrf_col_words<L>(nb, std::min(R, r)) * 4;
+            // elimination is repeated in every workgroup), more for larger panels, within the LDS and with
+            // one item per thread
+            int R = r > 0 ? std::min(col_rmax, std::max(4, (int)cdiv(r, 256))) : 0;
+            while (R > 1 && !potrf_col_threads(nb, R)) R--;
+            R = std::min(R, r);
+            const int G = r > 0 ? (int)cdiv(r, R) : 1, nt = potrf_col_threads(nb, R);
+            if (!nt) {
+                set_err("k_potrf_col: a block column of " + std::to_string(nb) + " columns does not fit a workgroup");
+                return SDPG_EARG;
+            }
+            const size_t sh = potrf_col_words<L>(nb, R) * 4;
             hipEvent_t ev = kprof.begin(st);
-            k_potrf_col<L><<<G, 1024, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, d_info, d_done);
+            k_potrf_col<L><<<G, nt, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, linv.pv(),
+                                              (int64_t)(k / NBK) * NBK * NBK, d_info);
             kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0 + (double)r * nb * nb / 2.0);
             if (r > 0) {
                 const int64_t a21 = off + (k + nb) + (int64_t)ld * k;
@@ -1855,7 +1857,7 @@ struct Engine : EngineBase {
                     return rc;
             }
         }
-        // the diagonal blocks' inverses (triangular solves, inverse factors), all blocks in one launch
+        // the diagonal blocks into A and their inverses (triangular solves, inverse factors), one launch
         const int nblk = (int)cdiv(n, NBK);
         k_blkinv<L><<<nblk, 256, (size_t)2 * (L + 1) * NBK * NBK * 4, st>>>(A, off, ld, n, NBK, rd, roff, linv.pv(),
                                                                              d_info);

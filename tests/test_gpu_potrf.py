@@ -11,8 +11,7 @@ choleskyFactorWithAdjust turns into FAILURE (sdpa_linear.cpp:622-630).  The test
   - the factor columns finished before the failing diagonal block are the factor of B there
     (L~ L~^T = B on those columns, exact integer arithmetic);
   - a positive definite B on the same context afterwards factors with info = 0 and
-    L~ L~^T = B (a failure leaves no state behind: the pivot flag and the completion count of
-    the block-column kernel are reset).
+    L~ L~^T = B (a failure leaves no state behind: the pivot flag is reset).
 """
 import ctypes
 import math
