@@ -314,9 +314,10 @@ MPX_HD bool fx_upd(const int32_t (&a)[fx_limbs<L>()], const int32_t (&b)[fx_limb
     return fx_extract<N, NV>(U, rb, ext, r);
 }
 
-// Lowest product column for a shift s: the columns below S0 weigh at most 2N 2^(29 S0 + 29) units of the
-// product's last bit, kept below 2^-10 of the result's last bit 2^(F + s): S0 = N - 2 for s >= -10, N - 4 for
-// s >= -68, else every column.
+// Lowest product column for a shift s: the columns below S0 weigh at most 2 S0 2^(29 (S0 - 1) + 58) units of
+// 2^-2F, kept below 2^-10 of the result's last bit 2^(F + s): S0 = N - 2 for s >= -10 (tier 0), N - 4 for
+// s >= -68 (tier 1), else every column (tier 2).
+template <int L>
 MPX_HD int fx_tier(int32_t s) {
     return s >= -10 ? 0 : (s >= -68 ? 1 : 2);
 }
@@ -454,12 +455,14 @@ struct ColElim {
     MPX_HD int item_code(int t) const {
         if (t < R) return (nb + t) | (0xFFFF << 16);
         t -= R;
-        int c = nb - 1;
-        while (c > 1 && t >= nb - c + R) {
-            t -= nb - c + R;
-            c--;
-        }
-        const int row = t < nb - c ? c + t : nb + (t - (nb - c));
+        // the m = nb - 1 - c columns above column c hold m (m + 1) / 2 + m R items: the largest m with that <= t
+        const float b = 2.0f * R + 1.0f;
+        int m = (int)((sqrtf(b * b + 8.0f * t) - b) * 0.5f);
+        while (m > 0 && m * (m + 1) / 2 + m * R > t) m--;
+        while ((m + 1) * (m + 2) / 2 + (m + 1) * R <= t) m++;
+        if (m > nb - 2) m = nb - 2;
+        const int c = nb - 1 - m, i = t - (m * (m + 1) / 2 + m * R);
+        const int row = i < nb - c ? c + i : nb + (i - (nb - c));
         return row | (c << 16);
     }
     MPX_HD void set_item(int t) const {
@@ -471,13 +474,13 @@ struct ColElim {
     MPX_HD int entry_code(int t, int &col) const {
         const int nbl = nb * (nb + 1) / 2;
         if (t < nbl) {
-            int c = 0;
-            while (t >= nb - c) {
-                t -= nb - c;
-                c++;
-            }
+            // columns before c hold c nb - c (c - 1) / 2 entries: the largest c with that <= t
+            const float b = 2.0f * nb + 1.0f;
+            int c = (int)((b - sqrtf(b * b - 8.0f * t)) * 0.5f);
+            while (c > 0 && c * nb - c * (c - 1) / 2 > t) c--;
+            while ((c + 1) * nb - (c + 1) * c / 2 <= t) c++;
             col = c;
-            return c + t;
+            return c + t - (c * nb - c * (c - 1) / 2);
         }
         t -= nbl;
         if (t < R * nb) {
@@ -513,7 +516,7 @@ struct ColElim {
         ld(row + LDV * j, c);
         ld(pd ? row + LDV * j : col + LDV * j, d);
         const int32_t s = 2 * dj + fx_delta(hj[row]) + (pd ? fx_delta(hj[row]) : fx_delta(hj[col]));
-        const bool ovf = fx_upd_tier<L>(uniform_tier(fx_tier(s)), a, b, c, d, s, r);
+        const bool ovf = fx_upd_tier<L>(uniform_tier(fx_tier<L>(s)), a, b, c, d, s, r);
         st(ib, r);
         if (pd || row == col)
             fx_amax(hn + row, fx_lead<N>(r));

@@ -690,11 +690,12 @@ constexpr int potrf_col_items(int nb, int R) {
 constexpr int potrf_col_entries(int nb, int R) {
     return nb * (nb + 1) / 2 + R * (nb + 1);
 }
-// threads of a k_potrf_col workgroup: one lane per item and per entry, and one more wave (the pivots' prefix
-// products); 0 when R rows do not fit 1024 threads
+// threads of a k_potrf_col workgroup: one lane per item and per entry, and two more waves (the normalised
+// pivots and their prefix products, a lane per row); 0 when R rows do not fit 1024 threads or one wave
 constexpr int potrf_col_threads(int nb, int R) {
+    if (nb + R > 64) return 0;
     const int it = potrf_col_items(nb, R), en = potrf_col_entries(nb, R);
-    const int w = (it + 63) / 64 + 1 > (en + 63) / 64 ? (it + 63) / 64 + 1 : (en + 63) / 64;
+    const int w = (it + 63) / 64 + 2 > (en + 63) / 64 ? (it + 63) / 64 + 2 : (en + 63) / 64;
     return 64 * w <= 1024 ? 64 * w : 0;
 }
 
@@ -705,8 +706,8 @@ constexpr int potrf_col_threads(int nb, int R) {
 // nothing); it writes its panel rows' factor entries over A.  Workgroup 0 also writes the block's factor
 // (lower triangle, leading dimension nb) to Lblk at lbo, not over A (other workgroups may still be reading the
 // block), and the reciprocal pivots 1 / L_jj; k_blkinv moves the block into A.  Each thread holds at most one
-// entry at set-up and one item in the steps (blockDim.x = potrf_col_threads(nb, R)); the last wave holds no
-// item and forms the prefix products of the normalised pivots while the others update.  The reference factors
+// entry at set-up and one item in the steps (blockDim.x = potrf_col_threads(nb, R)); the last two waves hold no
+// item and form the normalised pivots and their prefix products while the others update.  The reference factors
 // the diagonal block with a square root and a division per pivot and solves the panel
 // (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:263-285); the factor is the same to rounding.  info: the
 // first failing pivot (1-based, as Rpotrf2's), or for an entry out of the fixed-point range (impossible for a
@@ -733,10 +734,10 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
     const int nent = potrf_col_entries(nb, Rw), nit = ce.nitems();
     int ecol = 0;
     const int erow = tid < nent ? ce.entry_code(tid, ecol) : 0;
-    const bool has_item = tid < nit;
-    const int32_t item = has_item ? ce.item_code(tid) : 0;
     mp<L> ev;
     if (tid < nent) ev = ldm<L>(A, ecol >= 0 ? at(erow, ecol) : at(erow, nb + p0 + (erow - nb)));
+    const bool has_item = tid < nit;
+    const int32_t item = has_item ? ce.item_code(tid) : 0;
     if (tid == 0) {
         s_flag = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_ovf = 0;
@@ -759,9 +760,14 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
         if (o) atomicOr(&s_ovf, erow < nb ? 1 : 2);
     }
     __syncthreads();
-    const bool gwave = tid >= nt - 64;  // no item: the prefix products gamma_j
+    // the last two waves hold no item: at step j one forms the normalised pivot Vh_j (into sRh) and, lane r for
+    // row r, the row scale f_r^(j) (into Fpre), the other the prefix product gamma_j = gamma_{j-1} Vh_{j-1}
+    // (into sG), each well within the time of an update
+    const bool vwave = tid >= nt - 128 && tid < nt - 64, gwave = tid >= nt - 64;
+    const int vl = tid - (nt - 128);  // lane in the pivot wave: its row (LDV <= 64)
+    int32_t fr = vwave && vl < LDV ? f0[vl] : 0;
     if (ok) {
-        mp<L> g = from_int<L>(1);
+        mp<L> g = from_int<L>(1);  // gamma_j in the last wave
         for (int j = 0; j < nb; j++) {
             // the pivot is the same for every thread (scalar registers), and so is the failure exit
             int32_t a[CE::N];
@@ -773,22 +779,28 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
                 ok = false;
                 break;
             }
-            if (gwave && tid == nt - 64) ce.gst(sG, j, g);  // gamma_j
+            if (gwave) {
+                if (j > 0) g = mul<L>(g, ce.gld(sRh, j - 1));
+                if (tid == nt - 64) ce.gst(sG, j, g);
+            }
+            if (vwave && vl < LDV) {
+                if (j > 0) fr += mpx::fx_delta(hist[(j - 1) * LDV + vl]);
+                Fpre[j * LDV + vl] = fr;  // f_r^(j)
+            }
             if (j == nb - 1) break;
             const int32_t dj = __builtin_amdgcn_readfirstlane(mpx::fx_delta(hist[j * LDV + j]));
             int ov = 0;
             if (has_item) ov = ce.update_item(j, item, a, dj);
-            if (gwave) g = mul<L>(g, fx_to_mp<L>(a, -2 * dj));  // gamma_{j+1} = gamma_j Vh_j
+            if (vwave) {
+                const mp<L> vh = fx_to_mp<L>(a, -2 * dj);
+                if (tid == nt - 128) ce.gst(sRh, j, vh);
+            }
             if (ov) atomicOr(&s_ovf, ov);
             __syncthreads();
         }
     }
     if (ok) {
-        // row scales per column, rho_c and 1 / L_cc (gamma_c in sG from the steps)
-        for (int t = tid; t < LDV * nb; t += nt) {
-            const int rr = t % LDV, c = t / LDV;
-            if (c <= (rr < nb ? rr : nb - 1)) ce.prefix_scale(rr, c);
-        }
+        // rho_c and 1 / L_cc (gamma_c in sG and the row scales in Fpre from the steps)
         __syncthreads();
         mp<L> rinv;
         if (tid < nb) rinv = ce.finish_pivot(tid, ce.pivot_mp(tid));  // rho_c into sRh
@@ -1786,6 +1798,7 @@ struct Engine : EngineBase {
     size_t lds_max = 65536;
     int NBK = 32;
     int col_rmax = 4;
+    bool chol_pairs = true;  // Ozaki form: Cholesky trailing updates in pairs of block columns (SDPG_CHOL_PAIRS=0: not)
 
     int alloc_factor_space() {
         // diagonal block edge of the blocked Cholesky factorisations (SDPG_NBK = 8, 16 or 32 overrides)
@@ -1796,6 +1809,7 @@ struct Engine : EngineBase {
         }
         while (NBK > 8 && std::max(potrf_col_words<L>(NBK, 4), (size_t)2 * (L + 1) * NBK * NBK) * 4 + 256 > lds_max)
             NBK /= 2;
+        if (const char *ev = getenv("SDPG_CHOL_PAIRS")) chol_pairs = atoi(ev) != 0;
         // panel rows per workgroup of the block-column kernel: as many as the LDS holds beside its static words
         col_rmax = 64;
         while (col_rmax > 1 && potrf_col_words<L>(NBK, col_rmax) * 4 + 256 > lds_max) col_rmax--;
@@ -1826,10 +1840,15 @@ struct Engine : EngineBase {
 
     // Rpotrf("Lower") of the n x n matrix A[off + i + ld j], blocked right-looking: per block of NBK columns
     // one k_potrf_col launch (the diagonal block and the panel below it; the block's factor to linv at block
-    // index k / NBK) and the lower-triangular trailing update as a GEMM; then k_blkinv moves the blocks' factors
-    // into A and leaves their inverses in linv.  Pivot failures set d_info (checked by the caller).
+    // index k / NBK); in the Ozaki form the trailing updates go in pairs of block columns: after the first of a
+    // pair only the next block column is updated (a GEMM of NBK columns), after the second the trailing matrix
+    // with both panels at once (a lower-triangular GEMM with K = 2 NBK: half as many passes over the trailing
+    // matrix, whose per-output work does not grow with K in that form; the plain form's does);
+    // then k_blkinv moves the blocks' factors into A and leaves their inverses in linv.  Pivot failures set
+    // d_info (checked by the caller).
     int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
         int rc;
+        int pend = 0;  // columns of the previous block column not yet applied to the trailing matrix
         for (int k = 0; k < n; k += NBK) {
             const int nb = std::min(NBK, n - k), r = n - k - nb;
             // panel rows per workgroup: 4 while at most 256 workgroups hold the panel (the diagonal block's
@@ -1848,13 +1867,24 @@ struct Engine : EngineBase {
             k_potrf_col<L><<<G, nt, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, linv.pv(),
                                               (int64_t)(k / NBK) * NBK * NBK, d_info);
             kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0 + (double)r * nb * nb / 2.0);
-            if (r > 0) {
+            if (r == 0) break;
+            const int64_t c22 = off + (k + nb) + (int64_t)ld * (k + nb);
+            if (pend == 0 && r > NBK && chol_pairs && gemm_mode == SDPG_GEMM_OZAKI) {
+                // the next block column: A[k+nb.., k+nb .. k+nb+NBK) -= P P[0 .. NBK)^T (its diagonal block's
+                // lower triangle)
                 const int64_t a21 = off + (k + nb) + (int64_t)ld * k;
-                // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T)
-                if ((rc = gemm.gemm(r, r, nb, A, a21, ld, false, A, a21, ld, true, A,
-                                    off + (k + nb) + (int64_t)ld * (k + nb), ld, GEMM_SUB, 1, 0, 0, 0, gemm_mode,
-                                    gemm_slices, true)))
+                if ((rc = gemm.gemm(r, NBK, nb, A, a21, ld, false, A, a21, ld, true, A, c22, ld, GEMM_SUB, 1, 0, 0,
+                                    0, gemm_mode, gemm_slices, true)))
                     return rc;
+                pend = nb;
+            } else {
+                // trailing update A22 -= P P^T on the lower triangle (Rsyrk / Rgemm N,T), P = the panels of
+                // this block column and of a pending one
+                const int64_t a21 = off + (k + nb) + (int64_t)ld * (k - pend);
+                if ((rc = gemm.gemm(r, r, pend + nb, A, a21, ld, false, A, a21, ld, true, A, c22, ld, GEMM_SUB, 1, 0,
+                                    0, 0, gemm_mode, gemm_slices, true)))
+                    return rc;
+                pend = 0;
             }
         }
         // the diagonal blocks into A and their inverses (triangular solves, inverse factors), one launch

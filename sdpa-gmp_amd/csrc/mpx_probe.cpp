@@ -149,7 +149,7 @@ static int fxupd_L(const unsigned *a, const unsigned *b, const unsigned *c, cons
         C[k] = (int32_t)c[k];
         D[k] = (int32_t)d[k];
     }
-    const bool ovf = mpx::fx_upd_tier<L>(mpx::fx_tier(s), A, B, C, D, s, Rr);
+    const bool ovf = mpx::fx_upd_tier<L>(mpx::fx_tier<L>(s), A, B, C, D, s, Rr);
     for (int k = 0; k < N; k++) r[k] = (unsigned)Rr[k];
     *h = ovf ? 0 : mpx::fx_lead<N>(Rr);
     return ovf ? 1 : 0;
@@ -159,6 +159,7 @@ static int fxupd_L(const unsigned *a, const unsigned *b, const unsigned *c, cons
     case 4: return CALL(4); \
     case 8: return CALL(8); \
     case 10: return CALL(10); \
+    case 12: return CALL(12); \
     case 16: return CALL(16); \
     case 18: return CALL(18); \
     }                     \

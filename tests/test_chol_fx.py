@@ -154,3 +154,65 @@ def test_failing_pivot(fail_at):
     if info == 0:
         bw, _ = check_factor(B, Lm, rdv, nb, R, L)
         assert bw <= backward_bar(L, nb)
+
+
+def _fx_words(v, N):
+    """canonical limbs of the integer v (29-bit lower limbs, signed top limb) as uint32 words"""
+    out = []
+    for k in range(N - 1):
+        out.append(v & ((1 << 29) - 1))
+        v >>= 29
+    assert -(1 << 29) <= v < (1 << 29)
+    out.append(v & 0xFFFFFFFF)
+    return out
+
+
+def _fx_int(w, N):
+    v = 0
+    for k in range(N - 1, -1, -1):
+        x = w[k] if k < N - 1 else (w[k] - (1 << 32) if w[k] >= (1 << 31) else w[k])
+        v = (v << 29) + x
+    return v
+
+
+@pytest.mark.parametrize("L", [4, 8, 10, 12])
+def test_fx_update(L):
+    """one step's update r = (a b - c d) 2^-s in the fixed point (chol_fx.h fx_upd, the product tier chosen
+    from s) against exact integers: within 1 unit of the last bit where partial products are dropped (they
+    weigh below 2^-10 of a unit; the result is the floor), exactly the floor with every column kept
+    (s < -68); out-of-range results reported."""
+    lib = ctypes.CDLL(LIB)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib.mpx_probe_fx_upd.argtypes = [ctypes.c_int, u32p, u32p, u32p, u32p, ctypes.c_int, u32p,
+                                     ctypes.POINTER(ctypes.c_int)]
+    N = lib.mpx_probe_fx_limbs(L)
+    F = 29 * N - 3
+    rnd = random.Random(L)
+    arr = ctypes.c_uint32 * N
+    worst = {}
+    for it in range(600):
+        s = rnd.choice([4, 2, 1, 0, 0, -1, -3, -10, -11, -40, -68, -69, -100, -200])
+        a = rnd.randrange(1 << (F - 1), 1 << (F + 1))  # pivot in [1/2, 2)
+        b = rnd.randrange(-(1 << (F + 1)), 1 << (F + 1))
+        d = rnd.randrange(1 << (F - 1), 1 << (F + 1)) * rnd.choice([-1, 1])
+        # c so that a b - c d is about t 2^(F + s) units of 2^-2F with |t| < 4 (|result| < 4)
+        t = rnd.randrange(-(1 << (F + 1)), 1 << (F + 1))
+        target = a * b - (t << F) * (1 << s) if s >= 0 else a * b - ((t << F) >> -s)
+        c = target // d
+        if not -(8 << F) < c < (8 << F):
+            continue
+        exact = a * b - c * d
+        ex = exact >> (F + s) if F + s >= 0 else exact << -(F + s)
+        out, h = arr(), ctypes.c_int()
+        rc = lib.mpx_probe_fx_upd(L, arr(*_fx_words(a, N)), arr(*_fx_words(b, N)), arr(*_fx_words(c, N)),
+                                  arr(*_fx_words(d, N)), s, out, ctypes.byref(h))
+        if not -(8 << F) <= ex < (8 << F):
+            assert rc == 1
+            continue
+        assert rc == 0, (s, ex)
+        got = _fx_int(list(out), N)
+        bar = 0 if s < -68 else 1
+        assert abs(got - ex) <= bar, (L, s, got - ex)
+        key = "exact" if s < -68 else "truncated"
+        worst[key] = max(worst.get(key, 0), abs(got - ex))
+    print(f"L={L} N={N}: worst |error| in units of 2^-F by tier: {worst}")
