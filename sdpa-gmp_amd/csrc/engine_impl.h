@@ -676,6 +676,60 @@ __device__ __forceinline__ void info_min(int *info, int v) {
     }
 }
 
+// Block b of a Cholesky factor (n x n in A, blocks of NBK) by one workgroup, inf the pivot info seen at its
+// start (the same in every thread): the block's factor L_kk (written by k_potrf_col to Linv at b NBK^2, leading
+// dimension nb) is copied into A, then replaced in Linv by its inverse: with U = L_kk diag(1 / L_cc) (unit lower
+// triangular; reciprocal pivots in rdiag), inv(U) by the row operations Y_i <- Y_i - U_ij Y_j (j = 0 .. nb-2,
+// i > j) on Y = I, then inv(L_kk)_ic = Y_ic / L_ii.  When a pivot failed (inf > 0), only a block before the
+// failing pivot's block is copied, and no inverse is formed.  LDS: U and Y as (L+1) planes of NBK^2 words each.
+template <int L>
+__device__ void blkinv_block(PV A, int64_t off, int ld, int n, int NBK, PV rdiag, int64_t roff, PV Linv, int b,
+                             int inf, uint32_t *shm) {
+    const int tid = threadIdx.x, nt = blockDim.x, k = b * NBK, nb = min(NBK, n - k);
+    if (inf != 0 && k >= ((inf - 1) / NBK) * NBK) return;
+    const int P = NBK * NBK;
+    const int64_t lo = (int64_t)b * P;
+    uint32_t *sU = shm, *sY = shm + (size_t)(L + 1) * P;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, j = t / nb;
+        if (i < j) continue;
+        const mp<L> l = ldm<L>(Linv, lo + t);
+        stm<L>(A, off + (k + i) + (int64_t)ld * (k + j), l);
+        if (inf == 0) lds_st<L>(sU, P, t, i > j ? mul<L>(l, ldm<L>(rdiag, roff + k + j)) : zero<L>());
+    }
+    if (inf != 0) return;
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, j = t / nb;
+        if (i < j) lds_st<L>(sU, P, t, zero<L>());
+        lds_st<L>(sY, P, t, i == j ? from_int<L>(1) : zero<L>());
+    }
+    __syncthreads();
+    for (int j = 0; j < nb - 1; j++) {
+        const int n1 = nb - 1 - j;
+        for (int t = tid; t < n1 * (j + 1); t += nt) {
+            const int i = j + 1 + t % n1, c = t / n1;
+            const mp<L> y = lds_ld<L>(sY, P, i + nb * c);
+            lds_st<L>(sY, P, i + nb * c, sub<L>(y, mul<L>(lds_ld<L>(sU, P, i + nb * j), lds_ld<L>(sY, P, j + nb * c))));
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < nb * nb; t += nt) {
+        const int i = t % nb, c = t / nb;
+        stm<L>(Linv, lo + t, i >= c ? mul<L>(ldm<L>(rdiag, roff + k + i), lds_ld<L>(sY, P, t)) : zero<L>());
+    }
+}
+
+// k_blkinv: blkinv_block for blocks b0, b0 + 1, ... (one workgroup each)
+template <int L>
+__global__ __launch_bounds__(256) void k_blkinv(PV A, int64_t off, int ld, int n, int NBK, PV rdiag, int64_t roff,
+                                                 PV Linv, const int *info, int b0) {
+    extern __shared__ uint32_t shm[];
+    __shared__ int s_inf;
+    if (threadIdx.x == 0) s_inf = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    blkinv_block<L>(A, off, ld, n, NBK, rdiag, roff, Linv, b0 + (int)blockIdx.x, s_inf, shm);
+}
+
 // LDS words of k_potrf_col for a workgroup of R panel rows
 template <int L>
 constexpr size_t potrf_col_words(int nb, int R) {
@@ -704,20 +758,22 @@ constexpr int potrf_col_threads(int nb, int R) {
 // step per pivot, two fixed-point products per entry).  Workgroup w takes the panel rows [w R, w R + R) and
 // runs the diagonal block's elimination itself (the same arithmetic in every workgroup, so they exchange
 // nothing); it writes its panel rows' factor entries over A.  Workgroup 0 also writes the block's factor
-// (lower triangle, leading dimension nb) to Lblk at lbo, not over A (other workgroups may still be reading the
-// block), and the reciprocal pivots 1 / L_jj; k_blkinv moves the block into A.  Each thread holds at most one
-// entry at set-up and one item in the steps (blockDim.x = potrf_col_threads(nb, R)); the last two waves hold no
-// item and form the normalised pivots and their prefix products while the others update.  The reference factors
+// (lower triangle, leading dimension nb) to Lblk at (k / NBK) NBK^2, not over A (other workgroups may still be
+// reading the block), and the reciprocal pivots 1 / L_jj; k_blkinv moves the block into A and inverts it.  Each
+// thread holds at most one entry at set-up and one item in the steps (blockDim.x = potrf_col_threads(nb, R));
+// the last two waves hold no item and form the normalised pivots, their prefix products and the row scales
+// while the others update.  The reference factors
 // the diagonal block with a square root and a division per pivot and solves the panel
 // (mplapack/Rpotrf2.cpp:86-97, mplapack/Rtrsm.cpp:263-285); the factor is the same to rounding.  info: the
 // first failing pivot (1-based, as Rpotrf2's), or for an entry out of the fixed-point range (impossible for a
 // positive definite matrix) the first pivot not yet confirmed; once info is set, later launches do nothing.
 template <int L>
 __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, int k, int nb, int r, int R, PV rdiag,
-                                                    int64_t roff, PV Lblk, int64_t lbo, int *info) {
+                                                    int64_t roff, PV Lblk, int NBK, int *info) {
     extern __shared__ uint32_t shm[];
     __shared__ int s_flag, s_ovf;
     const int tid = threadIdx.x, nt = blockDim.x, wg = blockIdx.x;
+    const int64_t lbo = (int64_t)(k / NBK) * NBK * NBK;
     const int p0 = wg * R, Rw = r > p0 ? min(R, r - p0) : 0;
     const int LDV = nb + Rw, R1 = Rw ? Rw : 1;
     using CE = ColElim<L>;
@@ -820,51 +876,6 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
                 if (i >= c) stm<L>(Lblk, lbo + t, ce.factor_entry(i, c));
             }
         }
-    }
-}
-
-// k_blkinv: after the block columns, one workgroup per diagonal block, all blocks in one launch: block b's
-// factor L_kk (written by k_potrf_col to Linv at b NBK^2, leading dimension nb) is copied into A, then
-// replaced in Linv by its inverse: with U = L_kk diag(1 / L_cc) (unit lower triangular; reciprocal pivots in
-// rdiag), inv(U) by the row operations Y_i <- Y_i - U_ij Y_j (j = 0 .. nb-2, i > j) on Y = I, then
-// inv(L_kk)_ic = Y_ic / L_ii.  When a pivot failed (info > 0), only the blocks before the failing pivot's block
-// are copied, and no inverse is formed.  LDS: U and Y as (L+1) planes of NBK^2 words each.
-template <int L>
-__global__ __launch_bounds__(256) void k_blkinv(PV A, int64_t off, int ld, int n, int NBK, PV rdiag, int64_t roff,
-                                                 PV Linv, const int *info) {
-    extern __shared__ uint32_t shm[];
-    const int inf = __hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int tid = threadIdx.x, nt = blockDim.x, b = blockIdx.x, k = b * NBK, nb = min(NBK, n - k);
-    if (inf != 0 && k >= ((inf - 1) / NBK) * NBK) return;
-    const int P = NBK * NBK;
-    const int64_t lo = (int64_t)b * P;
-    uint32_t *sU = shm, *sY = shm + (size_t)(L + 1) * P;
-    for (int t = tid; t < nb * nb; t += nt) {
-        const int i = t % nb, j = t / nb;
-        if (i < j) continue;
-        const mp<L> l = ldm<L>(Linv, lo + t);
-        stm<L>(A, off + (k + i) + (int64_t)ld * (k + j), l);
-        if (inf == 0) lds_st<L>(sU, P, t, i > j ? mul<L>(l, ldm<L>(rdiag, roff + k + j)) : zero<L>());
-    }
-    if (inf != 0) return;
-    for (int t = tid; t < nb * nb; t += nt) {
-        const int i = t % nb, j = t / nb;
-        if (i < j) lds_st<L>(sU, P, t, zero<L>());
-        lds_st<L>(sY, P, t, i == j ? from_int<L>(1) : zero<L>());
-    }
-    __syncthreads();
-    for (int j = 0; j < nb - 1; j++) {
-        const int n1 = nb - 1 - j;
-        for (int t = tid; t < n1 * (j + 1); t += nt) {
-            const int i = j + 1 + t % n1, c = t / n1;
-            const mp<L> y = lds_ld<L>(sY, P, i + nb * c);
-            lds_st<L>(sY, P, i + nb * c, sub<L>(y, mul<L>(lds_ld<L>(sU, P, i + nb * j), lds_ld<L>(sY, P, j + nb * c))));
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < nb * nb; t += nt) {
-        const int i = t % nb, c = t / nb;
-        stm<L>(Linv, lo + t, i >= c ? mul<L>(ldm<L>(rdiag, roff + k + i), lds_ld<L>(sY, P, t)) : zero<L>());
     }
 }
 
@@ -1475,6 +1486,14 @@ struct Engine : EngineBase {
                         pairs.push_back(Pair{i, j, P.slot_of[i], f});
                 }
             }
+            // k_pairs runs one pair per thread: pairs in 8 x 8 tiles of (i, j), so that a wave's reads of
+            // X / inv(Z) and its updates of B[i, j] and B[j, i] fall in 8 runs of 8 consecutive entries each way
+            // (any order gives the same B: every pair adds to its own entries)
+            std::stable_sort(pairs.begin(), pairs.end(), [](const Pair &x, const Pair &y) {
+                const int64_t kx = ((int64_t)(x.i >> 3) << 32) | ((int64_t)(x.j >> 3) << 6) | ((x.i & 7) << 3) | (x.j & 7);
+                const int64_t ky = ((int64_t)(y.i >> 3) << 32) | ((int64_t)(y.j >> 3) << 6) | ((y.i & 7) << 3) | (y.j & 7);
+                return kx < ky;
+            });
             P.npairs = (int)pairs.size();
             if ((rc = upload(&P.d_pairs, pairs))) return rc;
             P.npairs_fx = (int)pairs_fx.size();
@@ -1843,9 +1862,9 @@ struct Engine : EngineBase {
     // index k / NBK); in the Ozaki form the trailing updates go in pairs of block columns: after the first of a
     // pair only the next block column is updated (a GEMM of NBK columns), after the second the trailing matrix
     // with both panels at once (a lower-triangular GEMM with K = 2 NBK: half as many passes over the trailing
-    // matrix, whose per-output work does not grow with K in that form; the plain form's does);
-    // then k_blkinv moves the blocks' factors into A and leaves their inverses in linv.  Pivot failures set
-    // d_info (checked by the caller).
+    // matrix, whose per-output work does not grow with K in that form; the plain form's does).  Then k_blkinv
+    // moves the blocks' factors into A and leaves their inverses in linv.  Pivot failures set d_info (checked
+    // by the caller).
     int chol(PV A, int64_t off, int n, int ld, PV rd, int64_t roff, Arr &linv) {
         int rc;
         int pend = 0;  // columns of the previous block column not yet applied to the trailing matrix
@@ -1864,8 +1883,7 @@ struct Engine : EngineBase {
             }
             const size_t sh = potrf_col_words<L>(nb, R) * 4;
             hipEvent_t ev = kprof.begin(st);
-            k_potrf_col<L><<<G, nt, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, linv.pv(),
-                                              (int64_t)(k / NBK) * NBK * NBK, d_info);
+            k_potrf_col<L><<<G, nt, sh, st>>>(A, off, ld, k, nb, r, R, rd, roff, linv.pv(), NBK, d_info);
             kprof.end(st, ev, 2, (double)nb * nb * nb / 6.0 + (double)r * nb * nb / 2.0);
             if (r == 0) break;
             const int64_t c22 = off + (k + nb) + (int64_t)ld * (k + nb);
@@ -1890,7 +1908,7 @@ struct Engine : EngineBase {
         // the diagonal blocks into A and their inverses (triangular solves, inverse factors), one launch
         const int nblk = (int)cdiv(n, NBK);
         k_blkinv<L><<<nblk, 256, (size_t)2 * (L + 1) * NBK * NBK * 4, st>>>(A, off, ld, n, NBK, rd, roff, linv.pv(),
-                                                                             d_info);
+                                                                             d_info, 0);
         HIPCHK(hipGetLastError());
         return 0;
     }
