@@ -505,19 +505,37 @@ struct ColElim {
     }
     // the same for the item whose code (item_code) is it
     MPX_HD int update_item(int j, int32_t it, const int32_t (&a)[N], int32_t dj) const {
+        if (!item_active(j, it)) return 0;
+        ItemOps o;
+        item_load(j, it, o);
+        return item_update(j, it, a, dj, o);
+    }
+    // the same in two parts, so that a caller can request the operands with the pivot: is the item updated at
+    // step j; its operands (entry, the two pivot-column entries, the two row exponents); the update
+    struct ItemOps {
+        int32_t b[N], c[N], d[N], hr, hc;
+    };
+    MPX_HD bool item_active(int j, int32_t it) const {
+        const int col = (int)((uint32_t)it >> 16);
+        return col > j && !(col == 0xFFFF && j == nb - 1);
+    }
+    MPX_HD void item_load(int j, int32_t it, ItemOps &o) const {
         const int row = it & 0xFFFF, col = (int)((uint32_t)it >> 16);
-        if (col <= j || (col == 0xFFFF && j == nb - 1)) return 0;
-        const int32_t *hj = hist + j * LDV;
+        const bool pd = col == 0xFFFF;
+        ld(pd ? didx(row - nb) : row + LDV * col, o.b);
+        ld(row + LDV * j, o.c);
+        ld(pd ? row + LDV * j : col + LDV * j, o.d);
+        o.hr = hist[j * LDV + row];
+        o.hc = pd ? o.hr : hist[j * LDV + col];
+    }
+    MPX_HD int item_update(int j, int32_t it, const int32_t (&a)[N], int32_t dj, const ItemOps &o) const {
+        const int row = it & 0xFFFF, col = (int)((uint32_t)it >> 16);
         int32_t *hn = hist + (j + 1) * LDV;
         const bool pd = col == 0xFFFF;
-        const int ib = pd ? didx(row - nb) : row + LDV * col;
-        int32_t b[N], c[N], d[N], r[N];
-        ld(ib, b);
-        ld(row + LDV * j, c);
-        ld(pd ? row + LDV * j : col + LDV * j, d);
-        const int32_t s = 2 * dj + fx_delta(hj[row]) + (pd ? fx_delta(hj[row]) : fx_delta(hj[col]));
-        const bool ovf = fx_upd_tier<L>(uniform_tier(fx_tier<L>(s)), a, b, c, d, s, r);
-        st(ib, r);
+        int32_t r[N];
+        const int32_t s = 2 * dj + fx_delta(o.hr) + fx_delta(o.hc);
+        const bool ovf = fx_upd_tier<L>(uniform_tier(fx_tier<L>(s)), a, o.b, o.c, o.d, s, r);
+        st(pd ? didx(row - nb) : row + LDV * col, r);
         if (pd || row == col)
             fx_amax(hn + row, fx_lead<N>(r));
         else if (row >= nb)

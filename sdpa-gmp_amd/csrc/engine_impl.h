@@ -825,12 +825,14 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
     if (ok) {
         mp<L> g = from_int<L>(1);  // gamma_j in the last wave
         for (int j = 0; j < nb; j++) {
-            // the pivot is the same for every thread (scalar registers), and so is the failure exit
+            // the pivot (the same for every thread: scalar registers, and so is the failure exit), its row
+            // exponent and the flags, read together
             int32_t a[CE::N];
             ce.ld(j + LDV * j, a);
+            const int32_t hjj = hist[j * LDV + j], ovf0 = s_ovf;
 #pragma unroll
             for (int q = 0; q < CE::N; q++) a[q] = __builtin_amdgcn_readfirstlane(a[q]);
-            if ((s_ovf & 1) || mpx::fx_nonpos<CE::N>(a)) {
+            if ((ovf0 & 1) || mpx::fx_nonpos<CE::N>(a)) {
                 if (tid == 0) info_min(info, k + j + 1);
                 ok = false;
                 break;
@@ -844,7 +846,7 @@ __global__ __launch_bounds__(1024) void k_potrf_col(PV A, int64_t off, int ld, i
                 Fpre[j * LDV + vl] = fr;  // f_r^(j)
             }
             if (j == nb - 1) break;
-            const int32_t dj = __builtin_amdgcn_readfirstlane(mpx::fx_delta(hist[j * LDV + j]));
+            const int32_t dj = __builtin_amdgcn_readfirstlane(mpx::fx_delta(hjj));
             int ov = 0;
             if (has_item) ov = ce.update_item(j, item, a, dj);
             if (vwave) {

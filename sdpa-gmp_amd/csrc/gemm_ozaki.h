@@ -3,7 +3,7 @@
 // 1. Block fixed point as in the plain form: row r of op(A) gets the exponent e_r of its
 //    largest element and x becomes the integer Y = round(x * 2^(8S - 2 - e_r)),
 //    |Y| < 2^(8S-2); the same for the columns of op(B).
-// 2. Slicing (k_oz_slice; k_oz_fixslice does exponents and slices in one pass for K <= 32):
+// 2. Slicing (k_oz_slice; k_oz_fixslice does exponents and slices in one pass for K <= 64):
 //    Y = sum_{p<S} y_p 256^(S-1-p) with balanced digits y_p in [-128, 127] (y_0 in [-64, 64]).
 //    The digits are stored as int8 planes s[chunk][p][row][KC]: KC consecutive k of one row for one
 //    slice.
@@ -115,12 +115,13 @@ __global__ void k_oz_slice(GemmArgs g, int which, int R, int Rp, int nch, const 
 // workgroups of 256 threads of k_oz_slice for n slice bytes per plane set (rows x padded K)
 inline unsigned oz_slice_blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// K <= 32 (at most two 16-k chunks, the Cholesky products): row exponents and slices in one pass, the
-// same values as k_fix_max followed by k_oz_slice.  A workgroup covers 32 rows: thread (row rl, k = kt)
-// reads its element (consecutive threads take consecutive rows, which are contiguous for column-major
-// operands), the row's largest exponent is reduced in LDS, each thread writes its S digits into an LDS
-// copy of the output block (per chunk and slice the block's 32 rows x 16 k are contiguous in the slice
-// planes), and the block leaves in 16-byte stores.  blockIdx.z = 1: the second operand (R2, Rp2, ...).
+// K <= 64 (at most four 16-k chunks, the Cholesky products): row exponents and slices in one pass, the
+// same values as k_fix_max followed by k_oz_slice.  A workgroup of 1024 threads covers RW = 32 rows for
+// K <= 32, 16 rows for K <= 64: thread (row rl, k = kt) reads its element (consecutive threads take
+// consecutive rows, which are contiguous for column-major operands), the row's largest exponent is reduced
+// in LDS, each thread writes its S digits into an LDS copy of the output block (per chunk and slice the
+// block's RW rows x 16 k are contiguous in the slice planes), and the block leaves in 16-byte stores.
+// blockIdx.z = 1: the second operand (R2, Rp2, ...).
 template <int L, int S>
 __global__ __launch_bounds__(1024) void k_oz_fixslice(GemmArgs g, int which, int R, int Rp, int nch, int32_t *eo,
                                                       int8_t *out, int R2 = 0, int Rp2 = 0, int32_t *eo2 = nullptr,
@@ -133,20 +134,21 @@ __global__ __launch_bounds__(1024) void k_oz_fixslice(GemmArgs g, int which, int
         eo = eo2;
         out = out2;
     }
-    const int r0 = blockIdx.x * 32, b = blockIdx.y;
+    const int RW = g.K <= 32 ? 32 : 16, KP = 1024 / RW;  // rows per workgroup, k per row
+    const int r0 = blockIdx.x * RW, b = blockIdx.y;
     if (r0 >= Rp) return;  // past this operand's rows (the grid covers the larger operand)
-    __shared__ int32_t red[32][33];
-    __shared__ uint4 sout[2 * S * 32];  // [ch][p][row][16 k] as 16-byte rows
-    const int rl = threadIdx.x & 31, kt = threadIdx.x >> 5, r = r0 + rl;
+    __shared__ int32_t red[32 * 33];      // [row][KP + 1]
+    __shared__ uint4 sout[2 * S * 32];  // [ch][p][row][16 k] as 16-byte rows (nch RW <= 64)
+    const int rl = threadIdx.x % RW, kt = threadIdx.x / RW, r = r0 + rl;
     const bool live = r < R && kt < g.K;
     const mpx::mp<L> x = live ? ldm<L>(which == 0 ? g.A : g.B, fx_at(g, which, b, r, kt)) : mpx::zero<L>();
-    red[rl][kt] = live ? x.e : mpx::EZERO;
+    red[rl * (KP + 1) + kt] = live ? x.e : mpx::EZERO;
     __syncthreads();
     if (kt == 0) {
         int32_t e = mpx::EZERO;
 #pragma unroll 8
-        for (int q = 0; q < 32; q++) e = red[rl][q] > e ? red[rl][q] : e;
-        red[rl][32] = e;
+        for (int q = 0; q < KP; q++) e = red[rl * (KP + 1) + q] > e ? red[rl * (KP + 1) + q] : e;
+        red[rl * (KP + 1) + KP] = e;
         if (r < R) eo[(int64_t)b * R + r] = e;
     }
     __syncthreads();
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(1024) void k_oz_fixslice(GemmArgs g, int which, int
     if (ch < nch) {
         uint32_t zw[S / 4];
         if (live) {
-            oz_digit_words<L, S>(x, red[rl][32], zw);
+            oz_digit_words<L, S>(x, red[rl * (KP + 1) + KP], zw);
         } else {
 #pragma unroll
             for (int w = 0; w < S / 4; w++) zw[w] = 0;
@@ -163,16 +165,16 @@ __global__ __launch_bounds__(1024) void k_oz_fixslice(GemmArgs g, int which, int
 #pragma unroll
         for (int p = 0; p < S; p++) {
             const int u = S - 1 - p;
-            so[((ch * S + p) * 32 + rl) * 16 + kk] = (int8_t)(zw[u >> 2] >> (8 * (u & 3)));
+            so[((ch * S + p) * RW + rl) * 16 + kk] = (int8_t)(zw[u >> 2] >> (8 * (u & 3)));
         }
     }
     __syncthreads();
-    // per (ch, p): 32 rows x 16 bytes at ((((b * nch + ch) * S + p) * Rp + r0) * 16
-    const int nvec = nch * S * 32;
+    // per (ch, p): RW rows x 16 bytes at ((((b * nch + ch) * S + p) * Rp + r0) * 16
+    const int nvec = nch * S * RW;
     for (int v = threadIdx.x; v < nvec; v += blockDim.x) {
-        const int row = v & 31, cp = v >> 5;  // cp = ch * S + p
+        const int row = v % RW, cp = v / RW;  // cp = ch * S + p
         uint4 *dst = (uint4 *)(out + ((((int64_t)b * nch) * S + cp) * Rp + r0 + row) * 16);
-        *dst = sout[cp * 32 + row];
+        *dst = sout[cp * RW + row];
     }
 }
 
@@ -714,16 +716,17 @@ struct OzakiWork {
             qb = eb;
         }
         int64_t sa_n = (int64_t)Mp * nch * KC, sb_n = (int64_t)Np * nch * KC;
-        if (!longk && g.K <= 32 && (fillA || fillB)) {
+        if (!longk && g.K <= 64 && (fillA || fillB)) {
             // exponents and slices in one pass (both operands in one launch when both are needed)
+            const int RW = g.K <= 32 ? 32 : 16;  // rows per workgroup (k_oz_fixslice)
             if (fillA && fillB && nbA == nbB) {
-                k_oz_fixslice<L, S><<<dim3((unsigned)(std::max(Mp, Np) / 32), nbA, 2), 1024, 0, st>>>(
+                k_oz_fixslice<L, S><<<dim3((unsigned)(std::max(Mp, Np) / RW), nbA, 2), 1024, 0, st>>>(
                     g, 0, g.M, Mp, nch, qa, pa, g.N, Np, qb, pb);
             } else {
                 if (fillA)
-                    k_oz_fixslice<L, S><<<dim3((unsigned)(Mp / 32), nbA), 1024, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa);
+                    k_oz_fixslice<L, S><<<dim3((unsigned)(Mp / RW), nbA), 1024, 0, st>>>(g, 0, g.M, Mp, nch, qa, pa);
                 if (fillB)
-                    k_oz_fixslice<L, S><<<dim3((unsigned)(Np / 32), nbB), 1024, 0, st>>>(g, 1, g.N, Np, nch, qb, pb);
+                    k_oz_fixslice<L, S><<<dim3((unsigned)(Np / RW), nbB), 1024, 0, st>>>(g, 1, g.N, Np, nch, qb, pb);
             }
             fillA = fillB = false;
         }
