@@ -1820,6 +1820,7 @@ struct Engine : EngineBase {
     int NBK = 32;
     int col_rmax = 4;
     bool chol_pairs = true;  // Ozaki form: Cholesky trailing updates in pairs of block columns (SDPG_CHOL_PAIRS=0: not)
+    int col_r_fixed = 0;     // panel rows per block-column workgroup when > 0 (else chosen from the panel)
 
     int alloc_factor_space() {
         // diagonal block edge of the blocked Cholesky factorisations (SDPG_NBK = 8, 16 or 32 overrides)
@@ -1831,6 +1832,8 @@ struct Engine : EngineBase {
         while (NBK > 8 && std::max(potrf_col_words<L>(NBK, 4), (size_t)2 * (L + 1) * NBK * NBK) * 4 + 256 > lds_max)
             NBK /= 2;
         if (const char *ev = getenv("SDPG_CHOL_PAIRS")) chol_pairs = atoi(ev) != 0;
+        // panel rows per workgroup of the block-column kernel fixed (SDPG_COL_R; tests of the wide workgroups)
+        if (const char *ev = getenv("SDPG_COL_R")) col_r_fixed = std::max(0, atoi(ev));
         // panel rows per workgroup of the block-column kernel: as many as the LDS holds beside its static words
         col_rmax = 64;
         while (col_rmax > 1 && potrf_col_words<L>(NBK, col_rmax) * 4 + 256 > lds_max) col_rmax--;
@@ -1875,7 +1878,7 @@ struct Engine : EngineBase {
             // panel rows per workgroup: 4 while at most 256 workgroups hold the panel (the diagonal block's
             // elimination is repeated in every workgroup), more for larger panels, within the LDS and with
             // one item per thread
-            int R = r > 0 ? std::min(col_rmax, std::max(4, (int)cdiv(r, 256))) : 0;
+            int R = r > 0 ? std::min(col_rmax, col_r_fixed > 0 ? col_r_fixed : std::max(4, (int)cdiv(r, 256))) : 0;
             while (R > 1 && !potrf_col_threads(nb, R)) R--;
             R = std::min(R, r);
             const int G = r > 0 ? (int)cdiv(r, R) : 1, nt = potrf_col_threads(nb, R);

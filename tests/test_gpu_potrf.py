@@ -99,3 +99,36 @@ def test_potrf_reports_first_failing_pivot(gemm, workload):
         assert r <= 2.0 ** -(P - 16) * m, r
     finally:
         ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,rows", [(256, 12), (256, 7), (448, 5), (512, 0)],
+                         ids=["256-R12", "256-R7", "448-R5", "512"])
+def test_potrf_wide_workgroups_and_precisions(prec, rows, workload, monkeypatch):
+    """the block-column kernel with more panel rows per workgroup than the max-cut sizes use (R = 5 .. 12:
+    up to 880 items, 1024 threads, 44 rows for the row-scale lanes) and at 448 / 512 bits (16 / 18 limbs of
+    32 bits, 18 / 20 of 29 bits in the elimination): info of a failing pivot, the factor before it, and a
+    positive definite factor, exactly"""
+    if rows:
+        monkeypatch.setenv("SDPG_COL_R", str(rows))
+    ctx = sdpg.Context(workload(1), prec, gemm=sdpg.GEMM_PLAIN)  # m = 351
+    try:
+        m, L = ctx.m, ctx.L
+        P = 32 * L
+        fail_at = 100
+        print(f"case prec={prec} R={rows} m={m} fail_at={fail_at}", flush=True)
+        Bp = sdpg.pack(ldl(m, fail_at, seed=7), L)
+        ctx.put(sdpg.SDPG_B, Bp)
+        assert ctx.potrf() == fail_at
+        r = factor_residual(ctx.get(sdpg.SDPG_L), Bp, m, L, 96)
+        print(f"  first 96 columns: scaled residual {r:.3e}", flush=True)
+        assert r <= 2.0 ** -(P - 16) * m, r
+        print(f"case prec={prec} R={rows} m={m} positive definite", flush=True)
+        Bp = sdpg.pack(ldl(m, 0, seed=8), L)
+        ctx.put(sdpg.SDPG_B, Bp)
+        assert ctx.potrf() == 0
+        r = factor_residual(ctx.get(sdpg.SDPG_L), Bp, m, L, m)
+        print(f"  all columns: scaled residual {r:.3e}", flush=True)
+        assert r <= 2.0 ** -(P - 16) * m, r
+    finally:
+        ctx.close()
