@@ -207,3 +207,17 @@ extern "C" int mpx_probe_fx_limbs(int L) {
 #undef C_
 }
 extern "C" int mpx_probe_limbs(int bits) { return mpx::limbs_for_bits(bits); }
+// the block-column kernel's enumerations for nb block columns and R panel rows: items[t] = item_code(t)
+// (t < nitems), entries[2 t], entries[2 t + 1] = the row and column of entry_code(t) (t < nb (nb + 1) / 2 +
+// R (nb + 1)); returns the item count
+extern "C" int mpx_probe_col_enum(int nb, int R, int *items, int *entries) {
+    const mpx::ColElim<10> ce{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nb, R, nb + R};
+    const int n = ce.nitems();
+    for (int t = 0; t < n; t++) items[t] = ce.item_code(t);
+    for (int t = 0, ne = nb * (nb + 1) / 2 + R * (nb + 1); t < ne; t++) {
+        int col = 0;
+        entries[2 * t] = ce.entry_code(t, col);
+        entries[2 * t + 1] = col;
+    }
+    return n;
+}

@@ -216,3 +216,31 @@ def test_fx_update(L):
         key = "exact" if s < -68 else "truncated"
         worst[key] = max(worst.get(key, 0), abs(got - ex))
     print(f"L={L} N={N}: worst |error| in units of 2^-F by tier: {worst}")
+
+
+def test_block_column_enumerations():
+    """the block-column kernel gives each lane one entry at set-up and one item per step by closed-form index
+    arithmetic (chol_fx.h entry_code / item_code): every entry of the block column (the block's lower
+    triangle, the panel entries, the panel diagonals) exactly once; every item exactly once, panel diagonals
+    first, then by non-increasing column, so that the items still active at step j are a prefix"""
+    lib = ctypes.CDLL(LIB)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.mpx_probe_col_enum.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip]
+    for nb in range(1, 33):
+        for R in range(0, 24):
+            ne = nb * (nb + 1) // 2 + R * (nb + 1)
+            items = (ctypes.c_int * max(1, ne))()
+            ents = (ctypes.c_int * max(2, 2 * ne))()
+            n = lib.mpx_probe_col_enum(nb, R, items, ents)
+            assert n == nb * (nb - 1) // 2 + R * nb
+            got = [(items[t] & 0xFFFF, (items[t] >> 16) & 0xFFFF) for t in range(n)]
+            want = {(r, c) for c in range(1, nb) for r in range(c, nb + R)} | {(nb + p, 0xFFFF) for p in range(R)}
+            assert sorted(got) == sorted(want) and len(set(got)) == n, (nb, R)
+            key = [c if c != 0xFFFF else nb for _, c in got]
+            assert all(key[t] >= key[t + 1] for t in range(n - 1)), (nb, R)
+            for j in range(nb - 1):  # the active items at step j are a prefix
+                act = [c > j for _, c in got]
+                assert act == sorted(act, reverse=True), (nb, R, j)
+            e = [(ents[2 * t], ents[2 * t + 1]) for t in range(ne)]
+            want_e = {(r, c) for c in range(nb) for r in range(c, nb + R)} | {(nb + p, -1) for p in range(R)}
+            assert sorted(e) == sorted(want_e) and len(set(e)) == ne, (nb, R)
