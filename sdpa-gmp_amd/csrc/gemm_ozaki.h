@@ -242,8 +242,15 @@ __device__ __forceinline__ void oz_fold(oz_v4i (&acc)[S], uint32_t (&R)[4][S / 4
     for (int d = 0; d < S; d++) acc[d] = oz_v4i{0, 0, 0, 0};
 }
 
+// s_waitcnt vmcnt(n) alone (expcnt and lgkmcnt at their maxima), gfx9 encoding
+constexpr int ozk_wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+// LDS bytes of k_gemm_oz: two buffers of S x OZ_T 16-byte fragments per operand
+constexpr size_t oz_lds_bytes(int S) { return (size_t)4 * S * OZ_T * 16; }
+
 // Short-K product kernel: one workgroup per 32 x 32 output tile, 2 x 2 waves of 16 x 16, the
-// slices of the tile's rows and columns staged in LDS chunk by chunk.
+// slices of the tile's rows and columns staged in LDS chunk by chunk, double-buffered: the next chunk's
+// slices are copied global -> LDS by buffer loads (no registers) while the current chunk's MFMAs run.
 // Pair schedule: the MFMA numbered (u, q') gives lane group g = lane / 16 the slice pair
 // (p, q) = (4u + g, q' - g), so the four groups of a wave all add into the same digit position
 // d = p + q = 4u + q' and that MFMA accumulates into the INT32 digit sum T_d (groups with q < 0 feed a
@@ -259,13 +266,34 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
                                                  int nch) {
     constexpr int NW = S / 4, NU = (S + 3) / 4;
     static_assert(S <= 48, "digit sums of OZ_FOLD chunks must stay below 2^31");
-    __shared__ oz_v4i lA[S * OZ_T], lB[S * OZ_T];
+    constexpr int NF = S * OZ_T;  // 16-byte fragments of one operand and chunk (a multiple of 64)
+    extern __shared__ oz_v4i ozsh[];  // [2][NF] of A, then [2][NF] of B (oz_lds_bytes)
     const int i0 = blockIdx.x * OZ_T, j0 = blockIdx.y * OZ_T, b = blockIdx.z;
     if (g.lower && i0 + OZ_T - 1 < j0) return;  // tile strictly above the diagonal
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = wv & 1, wc = wv >> 1, grp = lane >> 4;
     const int ar = wr * 16 + (lane & 15), bc = wc * 16 + (lane & 15);
     const int bA = nbA > 1 ? b : 0, bB = nbB > 1 ? b : 0;
+    // chunk ch's slices into LDS buffer bf: fragment t of an operand (slice t / OZ_T, tile row t % OZ_T)
+    // at ((ch S + slice) Mp + i0 + row) 16 bytes from this batch entry's planes; a wave moves 64
+    // consecutive fragments (a wave-contiguous LDS block), the chunk offset in the scalar offset
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(as + (int64_t)bA * nch * S * Mp * 16), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(bs + (int64_t)bB * nch * S * Np * 16), (short)0, (int)0xFFFFFFFFu, 0x00020000);
+    auto stage = [&](int ch, int bf) {
+#pragma unroll
+        for (int g0 = 64 * wv; g0 < NF; g0 += 256) {
+            const int t = g0 + lane;
+            const int voA = ((t / OZ_T) * Mp + i0 + t % OZ_T) * 16, voB = ((t / OZ_T) * Np + j0 + t % OZ_T) * 16;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rA, (__attribute__((address_space(3))) void *)&ozsh[bf * NF + g0], 16, voA, ch * S * Mp * 16, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rB, (__attribute__((address_space(3))) void *)&ozsh[(2 + bf) * NF + g0], 16, voB, ch * S * Np * 16, 0,
+                0);
+        }
+    };
+    stage(0, 0);
     // The four outputs of this lane, C[i, j0c + e]: their exponents and, for op != SET, the raw words
     // of their old C values are requested before the products (at clamped, always valid addresses,
     // without a branch, and decoded only at the end) so that the loads' latency hides under them.
@@ -297,30 +325,14 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 #pragma unroll
         for (int w = 0; w < NW; w++) R[e][w] = 0;
     }
-    const oz_v4i *sa = (const oz_v4i *)as, *sb = (const oz_v4i *)bs;
     for (int ch = 0; ch < nch; ch++) {
         if (ch > 0 && ch % OZ_FOLD == 0) oz_fold<S>(acc, R, RI);
-        const oz_v4i *pa = sa + (((int64_t)bA * nch + ch) * S) * Mp + i0;
-        const oz_v4i *pb = sb + (((int64_t)bB * nch + ch) * S) * Np + j0;
-        constexpr int NST = (S * OZ_T + 255) / 256;  // staged fragments per thread and operand
-        oz_v4i sta[NST], stb[NST];
-#pragma unroll
-        for (int k = 0; k < NST; k++) {
-            const int t = tid + 256 * k;
-            if (t < S * OZ_T) {
-                sta[k] = pa[(int64_t)(t / OZ_T) * Mp + t % OZ_T];
-                stb[k] = pb[(int64_t)(t / OZ_T) * Np + t % OZ_T];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < NST; k++) {
-            const int t = tid + 256 * k;
-            if (t < S * OZ_T) {
-                lA[t] = sta[k];
-                lB[t] = stb[k];
-            }
-        }
+        // this chunk's slices have landed (this wave's copies, then every wave's); the other buffer was
+        // last read in the chunk before, which every wave has finished: the next chunk's copies go there
+        __builtin_amdgcn_s_waitcnt(ozk_wait_vm(0));
         __syncthreads();
+        if (ch + 1 < nch) stage(ch + 1, (ch + 1) & 1);
+        const oz_v4i *lA = ozsh + (ch & 1) * NF, *lB = ozsh + (2 + (ch & 1)) * NF;
         oz_v4i af[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) {
@@ -362,7 +374,6 @@ __global__ __launch_bounds__(256) void k_gemm_oz(GemmArgs g, const int8_t *__res
 #pragma unroll
             for (int x = 0; x < OZ_QB; x++) bcur[x] = bnxt[x];
         }
-        __syncthreads();
     }
     oz_fold<S>(acc, R, RI);
 #pragma unroll
@@ -466,8 +477,6 @@ __device__ __forceinline__ void ozk_carry(oz_v4i (&acc)[S], uint32_t (*sR)[64], 
 
 // s_waitcnt lgkmcnt(0) alone (vmcnt and expcnt at their maxima), gfx9 encoding
 constexpr int OZK_WAIT_LGKM0 = 0xC07F;
-// s_waitcnt vmcnt(n) alone (expcnt and lgkmcnt at their maxima), gfx9 encoding
-constexpr int ozk_wait_vm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 // column fragments q < OzkQL of the next chunk are prefetched into LDS (at most 28 KB per wave, so
 // that four waves fit in a CU's LDS with their accumulators); the others, first used by the last
 // batches of the descending order, are loaded at the start of their chunk
@@ -767,8 +776,15 @@ struct OzakiWork {
                 k_gemm_ozk<L, S><<<dim3((unsigned)nwg, 1), 64, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp,
                                                                         same ? Mp : Np, nch);
         } else {
+            // the kernel addresses one batch entry's slice planes with 32-bit buffer offsets
+            if ((int64_t)nch * S * std::max(Mp, Np) * 16 >= ((int64_t)1 << 31)) return -6;
             dim3 grd(Mp / OZ_T, Np / OZ_T, g.batch);
-            k_gemm_oz<L, S><<<grd, 256, 0, st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp, same ? Mp : Np, nch);
+            static const bool lds_set = hipFuncSetAttribute((const void *)k_gemm_oz<L, S>,
+                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                            (int)oz_lds_bytes(S)) == hipSuccess;
+            if (!lds_set) return -2;
+            k_gemm_oz<L, S><<<grd, 256, oz_lds_bytes(S), st>>>(g, pa, pb, qa, qb, nbA, same ? nbA : nbB, Mp,
+                                                                  same ? Mp : Np, nch);
         }
         // algorithmic INT8 operations: 2 per multiply-add, S(S+1)/2 slice pairs per mp multiply-add
         if (prof) prof->end(st, ev, longk ? 3 : 0, gemm_outputs(g) * (double)g.K * 2.0 * (S * (S + 1) / 2));
