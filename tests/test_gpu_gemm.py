@@ -101,8 +101,10 @@ FORM_IDS = ["plain", "element", "ozaki", "ozaki-wide"]
 @pytest.mark.parametrize("shape", [(70, 67, 37, False, True, 0), (33, 129, 16, True, False, 1),
                                    (1, 5, 1, False, False, 2), (96, 96, 32, False, True, 1),
                                    (50, 40, 300, False, True, 1), (17, 20, 2100, True, False, 2),
-                                   (40, 36, 150, True, True, 1)],
-                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub", "NT-sub-K300", "TN-add-K2100", "TT-sub-K150"])
+                                   (40, 36, 150, True, True, 1), (65, 97, 64, False, True, 1),
+                                   (130, 33, 49, True, False, 2)],
+                         ids=["NT-set", "TN-sub", "tiny-add", "NT-sub", "NT-sub-K300", "TN-add-K2100", "TT-sub-K150",
+                              "NT-sub-K64", "TN-add-K49"])
 def test_rgemm_random(form, shape):
     mode, slices, bits_lost = form
     m, n, k, ta, tb, op = shape
