@@ -307,11 +307,12 @@ def main():
         kr["k_gemm_oz"] = mfma_roofline(kst["ozaki_mfma"], "k_gemm_oz (Ozaki INT8 slice products, short K)",
                                         load_traffic("k_gemm_oz_maxcut1000_traffic.json"),
                                         f"INT8 ops: 2 S(S+1)/2 per mp multiply-add, S = {tuned[256]['slices']}; "
-                                        "Cholesky panel and trailing-update products (K = 32)")
+                                        "Cholesky trailing updates in pairs of block columns: the next block column "
+                                        "(K = 32), then the trailing matrix (K = 64)")
     if "plain" in res:
         kr["k_gemm_plain"] = valu_roofline(res["plain"]["_kst"]["plain_limb"], "k_gemm_plain (plain limb products)",
                                            peak, limbs_req * limbs_req, None,
-                                           "plain form, max-cut Cholesky products (K = 32)")
+                                           "plain form, max-cut Cholesky trailing updates (K = 32)")
     out["step_breakdown"] = {"share_of_step": shares, "dominant": dom}
     for f in res.values():
         f.pop("_kst", None)
